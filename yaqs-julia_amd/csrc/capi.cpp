@@ -1,0 +1,441 @@
+// C-ABI of the TJM engine (include/yaqs_hip.h): contexts, the job queue with
+// admission by projected HBM footprint, and worker threads that run admitted
+// jobs concurrently on their own HIP streams.
+#include "../../include/yaqs_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "engine.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+yq::c64 mkc(double r, double i) {
+    yq::c64 z;
+    z.re = r;
+    z.im = i;
+    return z;
+}
+
+// Julia column-major (Dl, d, Dr) -> engine row-major [a][s][b]
+void load_mps(yq::JobIn &J, int64_t L, const yaqs_mps_tensor *mps) {
+    if (!mps) throw std::invalid_argument("mps is NULL");
+    J.mps.resize(L);
+    J.mps_dl.resize(L);
+    J.mps_dr.resize(L);
+    for (int64_t i = 0; i < L; ++i) {
+        const yaqs_mps_tensor &t = mps[i];
+        if (t.dl < 1 || t.dr < 1 || t.d != 2 || !t.data) throw std::invalid_argument("bad MPS tensor (physical dim must be 2)");
+        J.mps_dl[i] = (int)t.dl;
+        J.mps_dr[i] = (int)t.dr;
+        J.mps[i].resize((size_t)t.dl * 2 * t.dr);
+        for (int64_t a = 0; a < t.dl; ++a)
+            for (int64_t s = 0; s < 2; ++s)
+                for (int64_t b = 0; b < t.dr; ++b) {
+                    const size_t src = (size_t)(a + t.dl * (s + 2 * b));
+                    J.mps[i][(size_t)(a * 2 + s) * t.dr + b] = mkc(t.data[2 * src], t.data[2 * src + 1]);
+                }
+    }
+}
+
+void load_noise(yq::JobIn &J, int32_t has, int64_t n, const yaqs_noise_process *p, int64_t L) {
+    J.has_noise = has != 0;
+    if (!has) return;
+    if (n > 0 && !p) throw std::invalid_argument("noise is NULL");
+    for (int64_t i = 0; i < n; ++i) {
+        yq::ProcIn q;
+        q.nsites = p[i].nsites;
+        if (q.nsites != 1 && q.nsites != 2) throw std::invalid_argument("noise process must act on 1 or 2 sites");
+        q.s0 = (int)p[i].site0 - 1;
+        q.s1 = q.nsites == 2 ? (int)p[i].site1 - 1 : q.s0;
+        if (q.s0 < 0 || q.s0 >= L || q.s1 < 0 || q.s1 >= L) throw std::invalid_argument("noise site out of range");
+        q.gamma = p[i].strength;
+        q.pauli_name = p[i].pauli_name != 0;
+        const int d = q.nsites == 1 ? 2 : 4;
+        for (int k = 0; k < 16; ++k) q.m[k] = mkc(0, 0);
+        for (int k = 0; k < d * d; ++k) q.m[k] = mkc(p[i].matrix[2 * k], p[i].matrix[2 * k + 1]);
+        J.noise.push_back(q);
+    }
+}
+
+void load_obs(yq::JobIn &J, int64_t n, const yaqs_observable *o) {
+    if (n > 0 && !o) throw std::invalid_argument("obs is NULL");
+    for (int64_t i = 0; i < n; ++i) {
+        yq::ObsIn q;
+        q.nsites = o[i].nsites;
+        if (q.nsites != 1 && q.nsites != 2) throw std::invalid_argument("observables act on 1 or 2 sites");
+        q.s0 = (int)o[i].site0 - 1;
+        q.s1 = q.nsites == 2 ? (int)o[i].site1 - 1 : q.s0;
+        const int d = q.nsites == 1 ? 2 : 4;
+        for (int k = 0; k < 16; ++k) q.op[k] = mkc(0, 0);
+        for (int k = 0; k < d * d; ++k) q.op[k] = mkc(o[i].op[2 * k], o[i].op[2 * k + 1]);
+        J.obs.push_back(q);
+    }
+}
+
+yq::JobIn convert_analog(const yaqs_analog_job *job) {
+    if (!job) throw std::invalid_argument("job is NULL");
+    yq::JobIn J;
+    J.digital = false;
+    J.L = job->L;
+    if (J.L < 1) throw std::invalid_argument("L must be >= 1");
+    load_mps(J, J.L, job->mps);
+    if (!job->mpo) throw std::invalid_argument("mpo is NULL");
+    J.mpo.resize(J.L);
+    J.mpo_dl.resize(J.L);
+    J.mpo_dr.resize(J.L);
+    for (int64_t i = 0; i < J.L; ++i) {
+        const yaqs_mpo_tensor &w = job->mpo[i];
+        if (w.dout != 2 || w.din != 2 || w.dl < 1 || w.dr < 1 || !w.data) throw std::invalid_argument("bad MPO tensor");
+        J.mpo_dl[i] = (int)w.dl;
+        J.mpo_dr[i] = (int)w.dr;
+        J.mpo[i].resize((size_t)w.dl * 4 * w.dr);
+        for (int64_t m = 0; m < w.dl; ++m)
+            for (int64_t so = 0; so < 2; ++so)
+                for (int64_t si = 0; si < 2; ++si)
+                    for (int64_t n = 0; n < w.dr; ++n) {
+                        const size_t src = (size_t)(m + w.dl * (so + 2 * (si + 2 * n)));
+                        J.mpo[i][(size_t)((m * 2 + so) * 2 + si) * w.dr + n] = mkc(w.data[2 * src], w.data[2 * src + 1]);
+                    }
+    }
+    load_noise(J, job->has_noise_model, job->n_noise, job->noise, J.L);
+    load_obs(J, job->n_obs, job->obs);
+    J.dt = job->dt;
+    J.num_times = job->num_times;
+    J.num_traj = job->num_traj;
+    J.max_bond = job->max_bond_dim;
+    J.thr = job->truncation_threshold;
+    J.order = job->order;
+    J.sample_timesteps = job->sample_timesteps;
+    J.kdim = job->krylov_dim > 0 ? job->krylov_dim : 25;
+    J.ktol = job->krylov_tol > 0 ? job->krylov_tol : 1e-10;
+    J.precision = job->precision;
+    J.seed = job->seed;
+    J.traj_offset = job->traj_offset;
+    if (J.precision != YAQS_PREC_FP64 && J.precision != YAQS_PREC_MIXED) throw std::invalid_argument("bad precision");
+    return J;
+}
+
+yq::JobIn convert_digital(const yaqs_digital_job *job) {
+    if (!job) throw std::invalid_argument("job is NULL");
+    yq::JobIn J;
+    J.digital = true;
+    J.L = job->L;
+    if (J.L < 1) throw std::invalid_argument("L must be >= 1");
+    load_mps(J, J.L, job->mps);
+    if (job->n_gates > 0 && !job->gates) throw std::invalid_argument("gates is NULL");
+    for (int64_t g = 0; g < job->n_gates; ++g) {
+        const yaqs_gate &G = job->gates[g];
+        yq::GateIn q;
+        q.kind = G.kind;
+        if (q.kind < 1 || q.kind > 3) throw std::invalid_argument("bad gate kind");
+        q.s0 = (int)G.site0 - 1;
+        q.s1 = q.kind == 2 ? (int)G.site1 - 1 : q.s0;
+        const int d = q.kind == 1 ? 2 : 4;
+        for (int k = 0; k < 16; ++k) q.m[k] = mkc(0, 0);
+        if (q.kind != 3)
+            for (int k = 0; k < d * d; ++k) q.m[k] = mkc(G.matrix[2 * k], G.matrix[2 * k + 1]);
+        J.gates.push_back(q);
+    }
+    load_noise(J, job->has_noise_model, job->n_noise, job->noise, J.L);
+    load_obs(J, job->n_obs, job->obs);
+    J.num_traj = job->num_traj;
+    J.max_bond = job->max_bond_dim;
+    J.thr = job->truncation_threshold;
+    J.precision = job->precision;
+    J.seed = job->seed;
+    J.traj_offset = job->traj_offset;
+    if (J.precision != YAQS_PREC_FP64 && J.precision != YAQS_PREC_MIXED) throw std::invalid_argument("bad precision");
+    return J;
+}
+
+struct Job {
+    uint64_t id;
+    yq::JobIn in;
+    yq::JobResult out;
+    int status = -1;  // -1 queued/running, else yaqs_status
+    std::string err;
+    double t_submit = 0, t_start = 0, t_end = 0;
+    int64_t batch = 0, bytes = 0;
+};
+
+}  // namespace
+
+struct yaqs_ctx {
+    int device = 0;
+    int64_t budget = 0;   // HBM bytes the context may use for jobs
+    int64_t in_use = 0;
+    int max_batch = 0;
+    int verbose = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Job>> queue;
+    std::map<uint64_t, std::shared_ptr<Job>> jobs;
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::vector<std::thread> workers;
+};
+
+namespace {
+
+// Batch size and bytes for a job under the remaining budget; 0 if it cannot
+// start now.
+int64_t choose_batch(yaqs_ctx *c, const yq::JobIn &in, int64_t avail, int64_t *bytes) {
+    const int64_t per = std::max<int64_t>(1, yq::footprint_bytes(in, 1));
+    int64_t b = std::min<int64_t>(in.num_traj, c->max_batch > 0 ? c->max_batch : 4096);
+    b = std::min<int64_t>(b, avail / per);
+    if (b < 1) return 0;
+    // balance waves: same number of waves, fewer padded slots
+    const int64_t waves = (in.num_traj + b - 1) / b;
+    b = (in.num_traj + waves - 1) / waves;
+    *bytes = yq::footprint_bytes(in, b);
+    return b;
+}
+
+void worker_main(yaqs_ctx *c) {
+    if (hipSetDevice(c->device) != hipSuccess) return;
+    hipStream_t stream = nullptr;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return;
+    for (;;) {
+        std::shared_ptr<Job> job;
+        {
+            std::unique_lock<std::mutex> lk(c->mu);
+            for (;;) {
+                if (c->stop) break;
+                if (!c->queue.empty()) {
+                    auto &head = c->queue.front();
+                    int64_t bytes = 0;
+                    const int64_t b = choose_batch(c, head->in, c->budget - c->in_use, &bytes);
+                    if (b > 0) {
+                        job = head;
+                        c->queue.pop_front();
+                        job->batch = b;
+                        job->bytes = bytes;
+                        c->in_use += bytes;
+                        break;
+                    }
+                    if (c->in_use == 0) {  // cannot fit even alone: fail it
+                        head->status = YAQS_ENOMEM_DEVICE;
+                        head->err = "job footprint exceeds the HBM budget";
+                        head->t_end = now_s();
+                        c->queue.pop_front();
+                        c->cv_done.notify_all();
+                        continue;
+                    }
+                }
+                c->cv_work.wait(lk);
+            }
+        }
+        if (!job) break;
+        job->t_start = now_s();
+        int st = YAQS_OK;
+        std::string err;
+        try {
+            yq::run_job(job->in, job->batch, stream, job->out);
+        } catch (const yq::HipError &e) {
+            st = YAQS_EDEVICE;
+            err = e.what();
+        } catch (const std::invalid_argument &e) {
+            st = YAQS_EINVAL;
+            err = e.what();
+        } catch (const std::bad_alloc &e) {
+            st = YAQS_ENOMEM_DEVICE;
+            err = e.what();
+        } catch (const std::exception &e) {
+            st = YAQS_EDEVICE;
+            err = e.what();
+        }
+        {
+            std::lock_guard<std::mutex> lk(c->mu);
+            job->t_end = now_s();
+            job->status = st;
+            job->err = err;
+            c->in_use -= job->bytes;
+        }
+        c->cv_done.notify_all();
+        c->cv_work.notify_all();
+    }
+    (void)hipStreamDestroy(stream);
+}
+
+int submit(yaqs_ctx *ctx, yq::JobIn &&in, uint64_t *job_id) {
+    yq::validate_job(in);
+    auto job = std::make_shared<Job>();
+    job->in = std::move(in);
+    job->t_submit = now_s();
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        job->id = ctx->next_id++;
+        ctx->jobs[job->id] = job;
+        ctx->queue.push_back(job);
+    }
+    ctx->cv_work.notify_all();
+    if (job_id) *job_id = job->id;
+    return YAQS_OK;
+}
+
+int collect(yaqs_ctx *ctx, uint64_t id, double *traj_out, double *mean_out, int64_t *bond_out, yaqs_job_stats *stats,
+            bool block) {
+    std::shared_ptr<Job> job;
+    {
+        std::unique_lock<std::mutex> lk(ctx->mu);
+        auto it = ctx->jobs.find(id);
+        if (it == ctx->jobs.end()) return fail(YAQS_ENOTFOUND, "unknown job id");
+        job = it->second;
+        if (job->status < 0) {
+            if (!block) return YAQS_EAGAIN;
+            ctx->cv_done.wait(lk, [&] { return job->status >= 0; });
+        }
+        ctx->jobs.erase(id);
+    }
+    if (job->status != YAQS_OK) return fail(job->status, job->err);
+    const yq::JobResult &r = job->out;
+    const int64_t nobs = (int64_t)job->in.obs.size();
+    if (traj_out && nobs) std::copy(r.traj.begin(), r.traj.begin() + nobs * r.ncol * job->in.num_traj, traj_out);
+    if (mean_out && nobs) std::copy(r.mean.begin(), r.mean.begin() + nobs * r.ncol, mean_out);
+    if (bond_out && !r.bonds.empty()) std::copy(r.bonds.begin(), r.bonds.end(), bond_out);
+    if (stats) {
+        stats->wall_s = job->t_end - job->t_submit;
+        stats->queue_s = job->t_start - job->t_submit;
+        stats->flop_model = r.flop;
+        stats->matvecs = r.matvecs;
+        stats->jumps = r.jumps;
+        stats->max_bond = r.max_bond;
+        stats->num_samples = r.ncol;
+    }
+    return YAQS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int yaqs_abi_version(void) { return YAQS_ABI_VERSION; }
+
+const char *yaqs_last_error(void) { return g_last_error.c_str(); }
+
+int yaqs_create(yaqs_ctx **out, int device, const yaqs_ctx_opts *opts) {
+    if (!out) return fail(YAQS_EINVAL, "out is NULL");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(YAQS_EDEVICE, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(YAQS_EINVAL, "device index out of range");
+    if (hipSetDevice(device) != hipSuccess) return fail(YAQS_EDEVICE, "hipSetDevice failed");
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return fail(YAQS_EDEVICE, "hipMemGetInfo failed");
+    auto *c = new yaqs_ctx();
+    c->device = device;
+    c->budget = (opts && opts->hbm_budget_bytes > 0) ? opts->hbm_budget_bytes : (int64_t)(0.9 * (double)freeb);
+    c->max_batch = opts ? opts->max_batch : 0;
+    c->verbose = opts ? opts->verbose : 0;
+    const int nworkers = 2;
+    for (int i = 0; i < nworkers; ++i) c->workers.emplace_back(worker_main, c);
+    *out = c;
+    return YAQS_OK;
+}
+
+void yaqs_destroy(yaqs_ctx *ctx) {
+    if (!ctx) return;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->stop = true;
+    }
+    ctx->cv_work.notify_all();
+    for (auto &t : ctx->workers) t.join();
+    delete ctx;
+}
+
+int yaqs_submit_analog(yaqs_ctx *ctx, const yaqs_analog_job *job, uint64_t *job_id) {
+    if (!ctx) return fail(YAQS_EINVAL, "ctx is NULL");
+    try {
+        return submit(ctx, convert_analog(job), job_id);
+    } catch (const std::exception &e) {
+        return fail(YAQS_EINVAL, e.what());
+    }
+}
+
+int yaqs_submit_digital(yaqs_ctx *ctx, const yaqs_digital_job *job, uint64_t *job_id) {
+    if (!ctx) return fail(YAQS_EINVAL, "ctx is NULL");
+    try {
+        return submit(ctx, convert_digital(job), job_id);
+    } catch (const std::exception &e) {
+        return fail(YAQS_EINVAL, e.what());
+    }
+}
+
+int yaqs_wait(yaqs_ctx *ctx, uint64_t job_id, double *traj_out, double *mean_out, int64_t *bond_out,
+              yaqs_job_stats *stats) {
+    if (!ctx) return fail(YAQS_EINVAL, "ctx is NULL");
+    return collect(ctx, job_id, traj_out, mean_out, bond_out, stats, true);
+}
+
+int yaqs_poll(yaqs_ctx *ctx, uint64_t job_id, double *traj_out, double *mean_out, int64_t *bond_out,
+              yaqs_job_stats *stats) {
+    if (!ctx) return fail(YAQS_EINVAL, "ctx is NULL");
+    return collect(ctx, job_id, traj_out, mean_out, bond_out, stats, false);
+}
+
+int yaqs_run_analog(yaqs_ctx *ctx, const yaqs_analog_job *job, double *traj_out, double *mean_out,
+                    yaqs_job_stats *stats) {
+    uint64_t id = 0;
+    int rc = yaqs_submit_analog(ctx, job, &id);
+    if (rc != YAQS_OK) return rc;
+    return yaqs_wait(ctx, id, traj_out, mean_out, nullptr, stats);
+}
+
+int yaqs_run_digital(yaqs_ctx *ctx, const yaqs_digital_job *job, double *traj_out, double *mean_out,
+                     int64_t *bond_out, yaqs_job_stats *stats) {
+    uint64_t id = 0;
+    int rc = yaqs_submit_digital(ctx, job, &id);
+    if (rc != YAQS_OK) return rc;
+    return yaqs_wait(ctx, id, traj_out, mean_out, bond_out, stats);
+}
+
+int64_t yaqs_analog_num_samples(const yaqs_analog_job *job) {
+    if (!job) return -1;
+    return job->sample_timesteps ? job->num_times : 1;
+}
+
+int64_t yaqs_digital_num_samples(const yaqs_digital_job *job) {
+    try {
+        return yq::num_columns(convert_digital(job));
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return -1;
+    }
+}
+
+int64_t yaqs_job_footprint_bytes(const yaqs_analog_job *job) {
+    try {
+        yq::JobIn in = convert_analog(job);
+        return yq::footprint_bytes(in, std::max<int64_t>(1, in.num_traj));
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return -1;
+    }
+}
+
+double yaqs_rng_uniform(uint64_t seed, uint64_t traj, uint64_t k) { return yq::rng_uniform_host(seed, traj, k); }
+
+}  // extern "C"

@@ -1,0 +1,1312 @@
+// Host orchestration of the batched TJM engine.
+//
+// All trajectories of a wave advance in lockstep through the same sequence of
+// batched kernels; per-trajectory differences (jumps, Krylov convergence,
+// truncation ranks) live in device-side masks and scalars.  Bond dimensions
+// are zero-padded to the wave's maximum at each bond (`wd`), which leaves
+// every contraction exact.  Conventions (engine layout, all row-major):
+//   site tensor   A_i[a][s][b]           (wd[i] x 2 x wd[i+1])
+//   MPO           W_i[m][s'][s][n]       (Dm[i] x 2 x 2 x Dm[i+1])
+//   left env      EL[b][bra][mpo][ket]   (wd[b] x Dm[b] x wd[b])
+//   right env     ER[b][mpo][ket][bra]   (Dm[b] x wd[b] x wd[b])
+//   two-site Θ    [a][2 s1 + s2][b]
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+
+#include "kernels.h"
+
+namespace yq {
+
+namespace {
+
+typedef std::complex<double> cd;
+
+// ------------------------------------------------------------ host utils ---
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() {}
+    explicit DevBuf(size_t b) { alloc(b); }
+    void alloc(size_t b) {
+        release();
+        bytes = b ? b : 16;
+        YQ_HIP(hipMalloc(&p, bytes));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    template <typename X>
+    X *as() const { return static_cast<X *>(p); }
+};
+
+template <typename X>
+void upload(DevBuf &b, const std::vector<X> &v, hipStream_t s) {
+    b.alloc(std::max<size_t>(1, v.size()) * sizeof(X));
+    if (!v.empty()) YQ_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice, s));
+}
+
+c64 mk(double r, double i) {
+    c64 z;
+    z.re = r;
+    z.im = i;
+    return z;
+}
+cd toc(const c64 &z) { return cd(z.re, z.im); }
+c64 fromc(const cd &z) { return mk(z.real(), z.imag()); }
+
+// 2x2 reference-order (column-major) -> engine row-major M[r][c]
+void op2_engine(const c64 *ref, c64 *out) {
+    for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c) out[r * 2 + c] = ref[r + 2 * c];
+}
+// 4x4 reference-order (column-major, index p1 + 2 p2) -> engine row-major on 2 s1 + s2
+void op4_engine(const c64 *ref, c64 *out) {
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            const int R = 2 * (r & 1) + (r >> 1), C = 2 * (c & 1) + (c >> 1);
+            out[R * 4 + C] = ref[r + 4 * c];
+        }
+}
+
+// exp(-scale * L^†L) for a 2x2 L (engine row-major), closed form for the
+// Hermitian PSD 2x2 (the role of exp(::Matrix), src/Dissipation.jl:63-64).
+void damp2(const c64 *Le, double scale, c64 *out) {
+    cd L[4] = {toc(Le[0]), toc(Le[1]), toc(Le[2]), toc(Le[3])};
+    cd M[4];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) M[i * 2 + j] = std::conj(L[0 * 2 + i]) * L[0 * 2 + j] + std::conj(L[1 * 2 + i]) * L[1 * 2 + j];
+    const double a = M[0].real(), d = M[3].real();
+    const cd b = M[1];
+    const double mid = 0.5 * (a + d), rad = std::sqrt(0.25 * (a - d) * (a - d) + std::norm(b));
+    const double l1 = mid + rad, l2 = mid - rad;
+    const double e1 = std::exp(-scale * l1), e2 = std::exp(-scale * l2);
+    // f(M) = e2 I + (e1 - e2)/(l1 - l2) (M - l2 I)   (rad > 0), else e1 I
+    cd R[4];
+    if (rad < 1e-300) {
+        R[0] = e1; R[1] = 0; R[2] = 0; R[3] = e1;
+    } else {
+        const double f = (e1 - e2) / (l1 - l2);
+        R[0] = e2 + f * (M[0] - l2);
+        R[1] = f * M[1];
+        R[2] = f * M[2];
+        R[3] = e2 + f * (M[3] - l2);
+    }
+    for (int i = 0; i < 4; ++i) out[i] = fromc(R[i]);
+}
+
+void matmul2(const c64 *A, const c64 *B, c64 *C) {  // C = A B (2x2 row-major)
+    c64 r[4];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) r[i * 2 + j] = fromc(toc(A[i * 2]) * toc(B[j]) + toc(A[i * 2 + 1]) * toc(B[2 + j]));
+    std::memcpy(C, r, sizeof(r));
+}
+
+bool is_unitary2(const c64 *m) {  // engine or reference order: U^†U = I is order-invariant
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            cd acc = 0;
+            for (int k = 0; k < 2; ++k) acc += std::conj(toc(m[k * 2 + i])) * toc(m[k * 2 + j]);
+            if (std::abs(acc - (i == j ? 1.0 : 0.0)) > 1e-10) return false;
+        }
+    return true;
+}
+
+// Factor a 4x4 (engine order, index 2 s1 + s2) as kron(A, B): M[(2a+b)][(2c+e)] = A[a][c] B[b][e].
+bool kron_factor(const c64 *M, c64 *A, c64 *B) {
+    int bi = 0;
+    double best = -1;
+    for (int i = 0; i < 16; ++i) {
+        double v = std::abs(toc(M[i]));
+        if (v > best) { best = v; bi = i; }
+    }
+    if (best <= 0) return false;
+    const int R = bi / 4, C = bi % 4;
+    const int a0 = R >> 1, b0 = R & 1, c0 = C >> 1, e0 = C & 1;
+    // B[b][e] = M[(2a0+b)][(2c0+e)], A[a][c] = M[(2a+b0)][(2c+e0)] / B[b0][e0]
+    for (int b = 0; b < 2; ++b)
+        for (int e = 0; e < 2; ++e) B[b * 2 + e] = M[(2 * a0 + b) * 4 + (2 * c0 + e)];
+    const cd piv = toc(B[b0 * 2 + e0]);
+    for (int a = 0; a < 2; ++a)
+        for (int c = 0; c < 2; ++c) A[a * 2 + c] = fromc(toc(M[(2 * a + b0) * 4 + (2 * c + e0)]) / piv);
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c)
+                for (int e = 0; e < 2; ++e)
+                    if (std::abs(toc(A[a * 2 + c]) * toc(B[b * 2 + e]) - toc(M[(2 * a + b) * 4 + (2 * c + e)])) > 1e-12)
+                        return false;
+    return true;
+}
+
+// CSR of the effective-Hamiltonian MPO step Y[p][j][q] = Σ_i W[j][i] X[p][i][q].
+struct Csr {
+    std::vector<int> rowptr, col;
+    std::vector<c64> val;
+    int I = 0, J = 0;
+};
+struct CsrDev {
+    DevBuf rowptr, col, val;
+    int I = 0, J = 0;
+};
+void upload_csr(CsrDev &d, const Csr &c, hipStream_t s) {
+    upload(d.rowptr, c.rowptr, s);
+    upload(d.col, c.col, s);
+    upload(d.val, c.val, s);
+    d.I = c.I;
+    d.J = c.J;
+}
+// Left form (matvec and left env): i = m d + s, j = s' Dr + n.
+Csr csr_left(const std::vector<c64> &W, int Dl, int d, int Dr) {
+    Csr c;
+    c.I = Dl * d;
+    c.J = d * Dr;
+    c.rowptr.push_back(0);
+    for (int sp = 0; sp < d; ++sp)
+        for (int n = 0; n < Dr; ++n) {
+            for (int m = 0; m < Dl; ++m)
+                for (int s = 0; s < d; ++s) {
+                    const c64 w = W[((m * d + sp) * d + s) * Dr + n];
+                    if (w.re != 0.0 || w.im != 0.0) {
+                        c.col.push_back(m * d + s);
+                        c.val.push_back(w);
+                    }
+                }
+            c.rowptr.push_back((int)c.col.size());
+        }
+    return c;
+}
+// Right form (right env): i = s Dr + n, j = n' d + s' with n' the left MPO index.
+Csr csr_right(const std::vector<c64> &W, int Dl, int d, int Dr) {
+    Csr c;
+    c.I = d * Dr;
+    c.J = Dl * d;
+    c.rowptr.push_back(0);
+    for (int np = 0; np < Dl; ++np)
+        for (int sp = 0; sp < d; ++sp) {
+            for (int s = 0; s < d; ++s)
+                for (int n = 0; n < Dr; ++n) {
+                    const c64 w = W[((np * d + sp) * d + s) * Dr + n];
+                    if (w.re != 0.0 || w.im != 0.0) {
+                        c.col.push_back(s * Dr + n);
+                        c.val.push_back(w);
+                    }
+                }
+            c.rowptr.push_back((int)c.col.size());
+        }
+    return c;
+}
+// Two-site MPO W2[m][S'][S][n] with S = 2 s1 + s2 (src/Algorithms.jl:639-643).
+std::vector<c64> merge_mpo(const std::vector<c64> &W1, const std::vector<c64> &W2, int Dl, int Dk, int Dr) {
+    std::vector<c64> G((size_t)Dl * 16 * Dr, mk(0, 0));
+    for (int m = 0; m < Dl; ++m)
+        for (int k = 0; k < Dk; ++k)
+            for (int n = 0; n < Dr; ++n)
+                for (int a = 0; a < 2; ++a)
+                    for (int b = 0; b < 2; ++b) {
+                        const cd w1 = toc(W1[((m * 2 + a) * 2 + b) * Dk + k]);
+                        if (w1 == 0.0) continue;
+                        for (int c = 0; c < 2; ++c)
+                            for (int e = 0; e < 2; ++e) {
+                                const cd w2 = toc(W2[((k * 2 + c) * 2 + e) * Dr + n]);
+                                const int So = 2 * a + c, Si = 2 * b + e;
+                                c64 &g = G[((m * 4 + So) * 4 + Si) * Dr + n];
+                                g = fromc(toc(g) + w1 * w2);
+                            }
+                    }
+    return G;
+}
+
+GemmDesc gd(int M, int N, int K, int opA, int opB, int64_t lda, int64_t ldb, int64_t ldc, int64_t sA, int64_t sB,
+            int64_t sC, int outer, const int *active = nullptr) {
+    GemmDesc d;
+    d.M = M; d.N = N; d.K = K; d.opA = opA; d.opB = opB;
+    d.lda = lda; d.ldb = ldb; d.ldc = ldc;
+    d.sA_outer = sA; d.sB_outer = sB; d.sC_outer = sC;
+    d.sA_inner = d.sB_inner = d.sC_inner = 0;
+    d.inner = 1; d.outer = outer; d.accumulate = 0; d.active = active;
+    return d;
+}
+
+template <typename T>
+void gemmT(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s) {
+    if (!gemm_mfma<T>(d, A, B, C, s)) gemm<T, T, T, T>(d, A, B, C, s);
+}
+
+// --------------------------------------------------------------- engine ----
+struct State {
+    std::vector<DevBuf> A;  // per site, B slots
+    std::vector<int> wd;    // working dims per bond
+    DevBuf dims;            // [B][L+1] true dims
+};
+
+struct PairCand {  // jump candidates of the local noise model of a gate pair
+    DevBuf site, op, site2, op2, LdL, gamma;
+    int n = 0;
+};
+
+template <typename T>
+class Engine {
+  public:
+    typedef cplx<T> CT;
+
+    Engine(const JobIn &job, int B, hipStream_t s) : J(job), B(B), s(s) { setup(); }
+
+    void run_analog(int64_t t0, int nb, double *d_out, JobResult &res);
+    void run_digital(int64_t t0, int nb, double *d_out, int64_t *h_bonds, JobResult &res);
+
+    static int64_t footprint(const JobIn &job, int B);
+
+  private:
+
+    const JobIn &J;
+    int B;      // slots allocated
+    int nb = 0; // trajectories in the current wave
+    hipStream_t s;
+    int L = 0, kdim = 25;
+    std::vector<int> cap, Dm;
+    std::vector<int64_t> slotA, slotE;
+    int64_t nmax = 0, slotS = 0, gdim = 0, slotG = 0, capmax = 0, slotX = 0;
+    State PHI, PSI;
+    std::vector<DevBuf> EL, ER;
+    std::vector<CsrDev> csrL, csrR, csr2;
+    DevBuf theta, V, T1, T2, G, U, evals, perm, keep, Cf, absorb;
+    DevBuf Ebuf0, Ebuf1, Xbuf, rho1, rho2;
+    DevBuf beta0, galpha, lscale, alpha, beta, coef, nit, active, part, partn, nactive;
+    DevBuf norm2v, fac, jstate, chosen, opidx, mask, rngk, wbuf, count;
+    DevBuf diss_half, diss_full, diss_one;  // per-site 2x2 dissipation operators
+    std::vector<int> diss_site_active;
+    DevBuf cand_site, cand_op, cand_site2, cand_op2, cand_LdL, cand_gamma, jump_ops;
+    int ncand = 0;
+    DevBuf obs_site, obs_n, obs_op;
+    bool any_obs2 = false;
+    std::vector<char> obs2_at;
+    int *h_pinned = nullptr;
+    uint64_t traj0 = 0;
+    int64_t nchunk_max = 0;
+    std::map<int, PairCand> pair_cand;  // digital candidate tables per gate pair
+    DevBuf gate_ops;  // digital: gate matrices (engine order), 16 c64 per gate
+    std::vector<int64_t> gate_off;
+    DevBuf dissd_ops;  // digital dissipation (dt = 1) per gate pair and site: [(2 a + r) * 4]
+    std::vector<int> dgt_active;
+
+    void setup();
+    void alloc_state(State &S);
+    void init_state(State &S);
+    void copy_state(const State &S, State &D);
+
+    CT *A(State &S, int i) { return S.A[i].template as<CT>(); }
+    CT *Vq(int q) { return V.as<CT>() + (int64_t)q * nmax; }
+    int64_t sV() const { return (int64_t)(kdim + 1) * nmax; }
+    int sync_count(int *dcount);
+
+    // primitives
+    void matvec(State &S, bool two, int i, const CT *x, int64_t sx, CT *y, int64_t sy, const int *act);
+    void expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, JobResult &res);
+    void merge(State &S, int i);
+    void split(State &S, int i, bool right, int rule, double thr);
+    void env_left(State &S, int i);
+    void env_right(State &S, int i);
+    void tdvp(State &S, JobResult &res);
+    void lq_sweep(State &S, int from, const int *msk);
+    void qr_step(State &S, int i, const int *msk);
+    void dissipate(State &S, const DevBuf &ops);
+    void transfer(State &S, const int *msk);
+    void measure(State &S, int col, int ncol, double *d_out);
+    void stochastic(State &S, double dt, JobResult &res);
+    void absorb_left(State &S, int i, const int *msk);
+};
+
+template <typename T>
+int64_t Engine<T>::footprint(const JobIn &job, int B) {
+    const int L = (int)job.L;
+    std::vector<int> cap(L + 1, 1);
+    for (int b = 1; b < L; ++b) {
+        int e = std::min(b, L - b);
+        int64_t c = e >= 30 ? (int64_t)1 << 30 : (int64_t)1 << e;
+        cap[b] = (int)std::min<int64_t>(c, job.max_bond);
+        cap[b] = std::max(cap[b], std::max(job.mps_dr[b - 1], job.mps_dl[b]));
+    }
+    int Dmax = 1;
+    for (int i = 0; i < L; ++i) Dmax = std::max(Dmax, std::max(job.mpo_dl[i], job.mpo_dr[i]));
+    int64_t per = 0, nmax = 0, smax = 0, cm = 0;
+    for (int i = 0; i < L; ++i) per += 2 * (int64_t)cap[i] * 2 * cap[i + 1];  // PHI + PSI
+    for (int b = 0; b <= L; ++b) per += 2 * (int64_t)cap[b] * cap[b] * Dmax, cm = std::max<int64_t>(cm, cap[b]);
+    for (int i = 0; i + 1 < L; ++i) {
+        nmax = std::max<int64_t>(nmax, (int64_t)cap[i] * 4 * cap[i + 2]);
+        smax = std::max<int64_t>(smax, (int64_t)cap[i] * Dmax * 4 * cap[i + 2]);
+    }
+    nmax = std::max<int64_t>(nmax, 2 * cm * cm);
+    smax = std::max<int64_t>(smax, 2 * cm * cm * Dmax);
+    per += (job.kdim + 2) * nmax + 2 * smax + cm * cm + 2 * cm * cm;
+    int64_t bytes = per * (int64_t)sizeof(cplx<T>);
+    const int64_t g = 4 * cm * cm;
+    bytes += 2 * g * (int64_t)sizeof(c64) + 3 * cm * cm * 4 * (int64_t)sizeof(c64);
+    bytes += (int64_t)L * 20 * sizeof(c64) + (job.kdim + 2) * 64 * 16;
+    return bytes * B;
+}
+
+template <typename T>
+void Engine<T>::setup() {
+    L = (int)J.L;
+    kdim = J.kdim;
+    cap.assign(L + 1, 1);
+    for (int b = 1; b < L; ++b) {
+        int e = std::min(b, L - b);
+        int64_t c = e >= 30 ? (int64_t)1 << 30 : (int64_t)1 << e;
+        cap[b] = (int)std::min<int64_t>(c, J.max_bond);
+        cap[b] = std::max(cap[b], std::max(J.mps_dr[b - 1], J.mps_dl[b]));
+    }
+    cap[0] = J.mps_dl[0];
+    cap[L] = J.mps_dr[L - 1];
+    Dm.assign(L + 1, 1);
+    Dm[0] = J.mpo.empty() ? 1 : J.mpo_dl[0];
+    for (int i = 0; i < L; ++i) Dm[i + 1] = J.mpo.empty() ? 1 : J.mpo_dr[i];
+    int Dmax = *std::max_element(Dm.begin(), Dm.end());
+    slotA.resize(L);
+    for (int i = 0; i < L; ++i) slotA[i] = (int64_t)cap[i] * 2 * cap[i + 1];
+    slotE.resize(L + 1);
+    for (int b = 0; b <= L; ++b) slotE[b] = (int64_t)cap[b] * cap[b] * Dm[b];
+    capmax = *std::max_element(cap.begin(), cap.end());
+    nmax = 2 * capmax * capmax;
+    slotS = (int64_t)capmax * Dmax * 2 * capmax;
+    gdim = 2;
+    for (int i = 0; i + 1 < L; ++i) {
+        nmax = std::max<int64_t>(nmax, (int64_t)cap[i] * 4 * cap[i + 2]);
+        slotS = std::max<int64_t>(slotS, (int64_t)cap[i] * Dmax * 4 * cap[i + 2]);
+        gdim = std::max<int64_t>(gdim, 2 * std::max(cap[i], cap[i + 2]));
+    }
+    slotG = gdim * gdim;
+    slotX = std::max<int64_t>(nmax, 2 * capmax * capmax);
+    alloc_state(PHI);
+    if (!J.digital) alloc_state(PSI);
+    EL.resize(L + 1);
+    ER.resize(L + 1);
+    if (!J.digital) {
+        for (int b = 0; b <= L; ++b) {
+            EL[b].alloc(slotE[b] * B * sizeof(CT));
+            ER[b].alloc(slotE[b] * B * sizeof(CT));
+        }
+        csrL.resize(L);
+        csrR.resize(L);
+        csr2.resize(std::max(0, L - 1));
+        for (int i = 0; i < L; ++i) {
+            upload_csr(csrL[i], csr_left(J.mpo[i], Dm[i], 2, Dm[i + 1]), s);
+            upload_csr(csrR[i], csr_right(J.mpo[i], Dm[i], 2, Dm[i + 1]), s);
+        }
+        for (int i = 0; i + 1 < L; ++i)
+            upload_csr(csr2[i], csr_left(merge_mpo(J.mpo[i], J.mpo[i + 1], Dm[i], Dm[i + 1], Dm[i + 2]), Dm[i], 4,
+                                         Dm[i + 2]), s);
+        V.alloc((size_t)B * sV() * sizeof(CT));
+        T1.alloc((size_t)B * slotS * sizeof(CT));
+        T2.alloc((size_t)B * slotS * sizeof(CT));
+    }
+    theta.alloc((size_t)B * nmax * sizeof(CT));
+    G.alloc((size_t)B * slotG * sizeof(c64));
+    U.alloc((size_t)B * slotG * sizeof(c64));
+    evals.alloc((size_t)B * (gdim + 2) * sizeof(double));
+    perm.alloc((size_t)B * (gdim + 2) * sizeof(int));
+    keep.alloc((size_t)B * sizeof(int));
+    Cf.alloc((size_t)B * capmax * capmax * sizeof(CT));
+    int64_t amax = 0;
+    for (int i = 0; i < L; ++i) amax = std::max(amax, slotA[i]);
+    absorb.alloc((size_t)B * amax * sizeof(CT));
+    Ebuf0.alloc((size_t)B * capmax * capmax * sizeof(c64));
+    Ebuf1.alloc((size_t)B * capmax * capmax * sizeof(c64));
+    Xbuf.alloc((size_t)B * slotX * sizeof(c64));
+    rho1.alloc((size_t)B * L * 4 * sizeof(c64));
+    rho2.alloc((size_t)B * L * 16 * sizeof(c64));
+    nchunk_max = (nmax + LZ_CHUNK_ELEMS - 1) / LZ_CHUNK_ELEMS;
+    beta0.alloc(B * sizeof(double));
+    galpha.alloc(B * sizeof(double));
+    lscale.alloc(B * sizeof(double));
+    alpha.alloc((size_t)B * kdim * sizeof(double));
+    beta.alloc((size_t)B * (kdim + 1) * sizeof(double));
+    coef.alloc((size_t)B * kdim * sizeof(c64));
+    nit.alloc(B * sizeof(int));
+    active.alloc(B * sizeof(int));
+    part.alloc((size_t)B * (kdim + 1) * nchunk_max * sizeof(c64));
+    partn.alloc((size_t)B * nchunk_max * sizeof(double));
+    nactive.alloc(sizeof(int));
+    norm2v.alloc(B * sizeof(double));
+    fac.alloc(B * sizeof(double));
+    jstate.alloc(B * sizeof(int));
+    chosen.alloc(B * sizeof(int));
+    opidx.alloc(B * sizeof(int));
+    mask.alloc(B * sizeof(int));
+    rngk.alloc(B * sizeof(uint64_t));
+    count.alloc(sizeof(int));
+    YQ_HIP(hipHostMalloc((void **)&h_pinned, 16 * sizeof(int), hipHostMallocDefault));
+
+    // ---- noise: dissipation operators per site and jump candidates
+    auto site_ops = [&](double dt, std::vector<c64> &ops, std::vector<int> *act) {
+        ops.assign((size_t)L * 4, mk(0, 0));
+        if (act) act->assign(L, 0);
+        for (int i = 0; i < L; ++i) { ops[i * 4 + 0] = mk(1, 0); ops[i * 4 + 3] = mk(1, 0); }
+        bool allzero = true;
+        for (auto &p : J.noise) allzero = allzero && p.gamma == 0.0;
+        if (!J.has_noise || allzero) return;
+        for (int i = 0; i < L; ++i) {
+            for (auto &p : J.noise) {
+                if (p.nsites == 1 && p.s0 == i) {
+                    c64 M[4];
+                    if (p.pauli_name) {
+                        const double f = std::exp(-0.5 * dt * p.gamma);
+                        M[0] = mk(f, 0); M[1] = mk(0, 0); M[2] = mk(0, 0); M[3] = mk(f, 0);
+                    } else {
+                        c64 Le[4];
+                        op2_engine(p.m, Le);
+                        damp2(Le, 0.5 * dt * p.gamma, M);
+                    }
+                    matmul2(M, &ops[i * 4], &ops[i * 4]);
+                    if (act) (*act)[i] = 1;
+                }
+            }
+            for (auto &p : J.noise) {  // 2-site Pauli processes scale site max(s0, s1)
+                if (p.nsites == 2 && std::max(p.s0, p.s1) == i && p.pauli_name) {
+                    const double f = std::exp(-0.5 * dt * p.gamma);
+                    for (int k = 0; k < 4; ++k) ops[i * 4 + k] = mk(ops[i * 4 + k].re * f, ops[i * 4 + k].im * f);
+                    if (act) (*act)[i] = 1;
+                }
+            }
+        }
+    };
+    std::vector<c64> hops;
+    site_ops(J.dt / 2, hops, &diss_site_active);
+    upload(diss_half, hops, s);
+    site_ops(J.dt, hops, nullptr);
+    upload(diss_full, hops, s);
+    site_ops(1.0, hops, nullptr);
+    upload(diss_one, hops, s);
+
+    // jump operator table: per process (and second factor for 2-site Pauli)
+    std::vector<c64> jops;
+    std::vector<int> proc_op(J.noise.size(), -1), proc_op2(J.noise.size(), -1), proc_site2(J.noise.size(), -1);
+    for (size_t q = 0; q < J.noise.size(); ++q) {
+        const ProcIn &p = J.noise[q];
+        if (p.nsites == 1) {
+            c64 e[4];
+            op2_engine(p.m, e);
+            proc_op[q] = (int)(jops.size() / 4);
+            jops.insert(jops.end(), e, e + 4);
+        } else {
+            c64 e4[16], Af[4], Bf[4];
+            op4_engine(p.m, e4);
+            if (!kron_factor(e4, Af, Bf)) throw std::invalid_argument("non-product 2-site jump operator");
+            // engine index 2 s1 + s2: A acts on site min, B on site max
+            proc_op[q] = (int)(jops.size() / 4);
+            jops.insert(jops.end(), Af, Af + 4);
+            proc_op2[q] = (int)(jops.size() / 4);
+            jops.insert(jops.end(), Bf, Bf + 4);
+            proc_site2[q] = std::max(p.s0, p.s1);
+        }
+    }
+    upload(jump_ops, jops, s);
+    // analog candidates in the reference's order (src/StochasticProcess.jl:97-167)
+    {
+        std::vector<int> cs, co, cs2, co2;
+        std::vector<c64> ldl;
+        std::vector<double> cg;
+        auto add = [&](size_t q, int site, bool two) {
+            const ProcIn &p = J.noise[q];
+            cs.push_back(site);
+            co.push_back(proc_op[q]);
+            cs2.push_back(two ? proc_site2[q] : -1);
+            co2.push_back(two ? proc_op2[q] : -1);
+            c64 M[4];
+            if (!two) {
+                c64 e[4];
+                op2_engine(p.m, e);
+                for (int i = 0; i < 2; ++i)
+                    for (int j = 0; j < 2; ++j) M[i * 2 + j] = fromc(std::conj(toc(e[0 * 2 + i])) * toc(e[0 * 2 + j]) +
+                                                                     std::conj(toc(e[1 * 2 + i])) * toc(e[1 * 2 + j]));
+            } else {  // Pauli product: weight = dt γ ‖ψ‖² = dt γ Tr ρ_site
+                M[0] = mk(1, 0); M[1] = mk(0, 0); M[2] = mk(0, 0); M[3] = mk(1, 0);
+            }
+            ldl.insert(ldl.end(), M, M + 4);
+            cg.push_back(p.gamma);
+        };
+        for (int site = 0; site < L; ++site) {
+            for (size_t q = 0; q < J.noise.size(); ++q)
+                if (J.noise[q].nsites == 1 && J.noise[q].s0 == site) add(q, site, false);
+            for (size_t q = 0; q < J.noise.size(); ++q) {
+                const ProcIn &p = J.noise[q];
+                if (p.nsites == 2 && std::min(p.s0, p.s1) == site && p.s0 == site && site + 1 < L) add(q, site, true);
+            }
+        }
+        ncand = (int)cs.size();
+        upload(cand_site, cs, s);
+        upload(cand_op, co, s);
+        upload(cand_site2, cs2, s);
+        upload(cand_op2, co2, s);
+        upload(cand_LdL, ldl, s);
+        upload(cand_gamma, cg, s);
+        wbuf.alloc((size_t)B * std::max(1, ncand) * sizeof(double));
+        if (J.digital) {
+            // per adjacent pair (s1, s1+1): processes on s1, pair processes, processes on s2
+            for (int s1 = 0; s1 + 1 < L; ++s1) {
+                const int s2 = s1 + 1;
+                std::vector<int> ps, po, ps2, po2;
+                std::vector<c64> pl;
+                std::vector<double> pg;
+                cs.clear(); co.clear(); cs2.clear(); co2.clear(); ldl.clear(); cg.clear();
+                for (int site : {s1, s2}) {
+                    for (size_t q = 0; q < J.noise.size(); ++q)
+                        if (J.noise[q].nsites == 1 && J.noise[q].s0 == site) add(q, site, false);
+                    for (size_t q = 0; q < J.noise.size(); ++q) {
+                        const ProcIn &p = J.noise[q];
+                        if (site == s1 && p.nsites == 2 && p.s0 == s1 && p.s1 == s2) add(q, site, true);
+                    }
+                }
+                PairCand &pc = pair_cand[s1];
+                pc.n = (int)cs.size();
+                upload(pc.site, cs, s);
+                upload(pc.op, co, s);
+                upload(pc.site2, cs2, s);
+                upload(pc.op2, co2, s);
+                upload(pc.LdL, ldl, s);
+                upload(pc.gamma, cg, s);
+            }
+            // local dissipation operators of a gate on (a, a+1) (create_local_noise_model,
+            // src/DigitalTJM.jl:172-188, then apply_dissipation with dt = 1)
+            std::vector<c64> dops((size_t)std::max(1, 2 * L) * 4, mk(0, 0));
+            dgt_active.assign((size_t)2 * L, 0);
+            bool allzero = true;
+            for (auto &p : J.noise) allzero = allzero && p.gamma == 0.0;
+            for (int a = 0; a + 1 < L; ++a)
+                for (int r = 0; r < 2; ++r) {
+                    const int q = a + r;
+                    c64 *M = &dops[(size_t)(2 * a + r) * 4];
+                    M[0] = mk(1, 0);
+                    M[3] = mk(1, 0);
+                    if (!J.has_noise || allzero) continue;
+                    for (auto &p : J.noise) {
+                        if (p.nsites == 1 && p.s0 == q) {
+                            c64 D[4];
+                            if (p.pauli_name) {
+                                const double f = std::exp(-0.5 * p.gamma);
+                                D[0] = mk(f, 0); D[1] = mk(0, 0); D[2] = mk(0, 0); D[3] = mk(f, 0);
+                            } else {
+                                c64 Le[4];
+                                op2_engine(p.m, Le);
+                                damp2(Le, 0.5 * p.gamma, D);
+                            }
+                            matmul2(D, M, M);
+                            dgt_active[2 * a + r] = 1;
+                        }
+                        if (r == 1 && p.nsites == 2 && p.pauli_name && std::min(p.s0, p.s1) == a &&
+                            std::max(p.s0, p.s1) == a + 1) {
+                            const double f = std::exp(-0.5 * p.gamma);
+                            for (int k = 0; k < 4; ++k) M[k] = mk(M[k].re * f, M[k].im * f);
+                            dgt_active[2 * a + r] = 1;
+                        }
+                    }
+                }
+            upload(dissd_ops, dops, s);
+            std::vector<c64> gops;
+            gate_off.assign(J.gates.size(), -1);
+            for (size_t g = 0; g < J.gates.size(); ++g) {
+                const GateIn &G2 = J.gates[g];
+                if (G2.kind == 1) {
+                    c64 e[4];
+                    op2_engine(G2.m, e);
+                    gate_off[g] = (int64_t)gops.size();
+                    gops.insert(gops.end(), e, e + 4);
+                } else if (G2.kind == 2) {
+                    c64 e[16];
+                    op4_engine(G2.m, e);
+                    gate_off[g] = (int64_t)gops.size();
+                    gops.insert(gops.end(), e, e + 16);
+                }
+            }
+            upload(gate_ops, gops, s);
+        }
+    }
+    // observables
+    {
+        std::vector<int> os, on;
+        std::vector<c64> oo;
+        obs2_at.assign(L, 0);
+        for (auto &o : J.obs) {
+            os.push_back(std::min(o.s0, o.s1));
+            on.push_back(o.nsites);
+            c64 e[16];
+            for (auto &z : e) z = mk(0, 0);
+            if (o.nsites == 1) op2_engine(o.op, e);
+            else {
+                op4_engine(o.op, e);
+                obs2_at[std::min(o.s0, o.s1)] = 1;
+                any_obs2 = true;
+            }
+            oo.insert(oo.end(), e, e + 16);
+        }
+        upload(obs_site, os, s);
+        upload(obs_n, on, s);
+        upload(obs_op, oo, s);
+    }
+    YQ_HIP(hipStreamSynchronize(s));
+}
+
+template <typename T>
+void Engine<T>::alloc_state(State &S) {
+    S.A.resize(L);
+    for (int i = 0; i < L; ++i) S.A[i].alloc((size_t)B * slotA[i] * sizeof(CT));
+    S.dims.alloc((size_t)B * (L + 1) * sizeof(int));
+    S.wd.assign(L + 1, 1);
+}
+
+template <typename T>
+void Engine<T>::init_state(State &S) {
+    // broadcast the (normalized) initial MPS into every slot
+    for (int i = 0; i < L; ++i) {
+        const int n = J.mps_dl[i] * 2 * J.mps_dr[i];
+        DevBuf tmp;
+        upload(tmp, J.mps[i], s);
+        copy2d<double, T>(tmp.as<c64>(), 0, n, A(S, i), slotA[i], n, 1, n, nullptr, nb, s);
+        YQ_HIP(hipStreamSynchronize(s));
+    }
+    S.wd[0] = J.mps_dl[0];
+    for (int i = 0; i < L; ++i) S.wd[i + 1] = J.mps_dr[i];
+    std::vector<int> hd((size_t)B * (L + 1));
+    for (int t = 0; t < B; ++t)
+        for (int b = 0; b <= L; ++b) hd[(size_t)t * (L + 1) + b] = S.wd[b];
+    YQ_HIP(hipMemcpyAsync(S.dims.p, hd.data(), hd.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    YQ_HIP(hipStreamSynchronize(s));
+}
+
+template <typename T>
+void Engine<T>::copy_state(const State &S, State &D) {
+    for (int i = 0; i < L; ++i)
+        YQ_HIP(hipMemcpyAsync(D.A[i].p, S.A[i].p, (size_t)nb * slotA[i] * sizeof(CT), hipMemcpyDeviceToDevice, s));
+    YQ_HIP(hipMemcpyAsync(D.dims.p, S.dims.p, (size_t)B * (L + 1) * sizeof(int), hipMemcpyDeviceToDevice, s));
+    D.wd = S.wd;
+}
+
+template <typename T>
+int Engine<T>::sync_count(int *dcount) {
+    YQ_HIP(hipMemcpyAsync(h_pinned, dcount, sizeof(int), hipMemcpyDeviceToHost, s));
+    YQ_HIP(hipStreamSynchronize(s));
+    return h_pinned[0];
+}
+
+// y = H_eff x (src/Algorithms.jl:330-361): two GEMMs around the MPO step.
+template <typename T>
+void Engine<T>::matvec(State &S, bool two, int i, const CT *x, int64_t sx, CT *y, int64_t sy, const int *act) {
+    const int d = two ? 4 : 2;
+    const int bl = i, br = two ? i + 2 : i + 1;
+    const int cl = S.wd[bl], cr = S.wd[br], Dl = Dm[bl], Dr = Dm[br];
+    const CsrDev &W = two ? csr2[i] : csrL[i];
+    CT *t1 = T1.as<CT>(), *t2 = T2.as<CT>();
+    gemmT<T>(gd(cl * Dl, d * cr, cl, OP_N, OP_N, cl, d * cr, d * cr, slotE[bl], sx, slotS, nb, act),
+             EL[bl].as<CT>(), x, t1, s);
+    wstep<T>(t1, t2, cl, Dl * d, d * Dr, cr, slotS, slotS, W.rowptr.as<int>(), W.col.as<int>(), W.val.as<c64>(), act,
+             nb, s);
+    gemmT<T>(gd(cl * d, cr, Dr * cr, OP_N, OP_N, Dr * cr, cr, cr, slotS, slotE[br], sy, nb, act), t2,
+             ER[br].as<CT>(), y, s);
+}
+
+// x <- exp(-i dt H_eff) x by batched Lanczos (src/Algorithms.jl:144-187).
+template <typename T>
+void Engine<T>::expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, JobResult &res) {
+    const int d = two ? 4 : 2;
+    const int cl = S.wd[i], cr = S.wd[two ? i + 2 : i + 1];
+    const int64_t n = (int64_t)cl * d * cr;
+    const double fl = two ? 8.0 * (4.0 * Dm[i] * cl * cr * (double)(cl + cr) + 16.0 * Dm[i] * Dm[i] * (double)cl * cr)
+                          : 8.0 * (2.0 * Dm[i] * cl * cr * (double)(cl + cr) + 4.0 * Dm[i] * Dm[i] * (double)cl * cr);
+    lz_init<T>(x, sx, V.as<CT>(), sV(), n, beta0.as<double>(), active.as<int>(), nit.as<int>(), nb, s);
+    int nact = nb;
+    for (int j = 0; j < kdim; ++j) {
+        matvec(S, two, i, Vq(j), sV(), Vq(j + 1), sV(), active.as<int>());
+        res.matvecs += nact;
+        res.flop += fl * nact;
+        lz_dots<T>(V.as<CT>(), sV(), nmax, n, j, part.as<c64>(), kdim + 1, active.as<int>(), nb, s);
+        lz_update<T>(V.as<CT>(), sV(), nmax, n, j, part.as<c64>(), kdim + 1, partn.as<double>(), galpha.as<double>(),
+                     active.as<int>(), nb, s);
+        fill_int(nactive.as<int>(), 0, 1, s);
+        lz_finalize(j, kdim, dt, J.ktol, partn.as<double>(), n, galpha.as<double>(), alpha.as<double>(),
+                    beta.as<double>(), beta0.as<double>(), coef.as<c64>(), nit.as<int>(), active.as<int>(),
+                    lscale.as<double>(), nactive.as<int>(), nb, s);
+        lz_scale<T>(V.as<CT>(), sV(), nmax, n, j + 1, lscale.as<double>(), active.as<int>(), nb, s);
+        nact = sync_count(nactive.as<int>());
+        if (nact == 0) break;
+    }
+    lz_combine<T>(x, sx, V.as<CT>(), sV(), nmax, n, beta0.as<double>(), coef.as<c64>(), nit.as<int>(), kdim, nullptr,
+                  nb, s);
+}
+
+// Θ = A_i · A_{i+1} -> theta
+template <typename T>
+void Engine<T>::merge(State &S, int i) {
+    const int cl = S.wd[i], cm = S.wd[i + 1], cr = S.wd[i + 2];
+    gemmT<T>(gd(2 * cl, 2 * cr, cm, OP_N, OP_N, cm, 2 * cr, 2 * cr, slotA[i], slotA[i + 1], nmax, nb), A(S, i),
+             A(S, i + 1), theta.as<CT>(), s);
+}
+
+// SVD split of theta (split_mps_tensor_svd, src/Algorithms.jl:409-445) via
+// the FP64 Gram matrix and a Jacobi eigensolver.  right = centre moves right
+// (A_i = U, A_{i+1} = U^† Θ); else A_{i+1} = V^†, A_i = Θ V.
+template <typename T>
+void Engine<T>::split(State &S, int i, bool right, int rule, double thr) {
+    const int cl = S.wd[i], cr = S.wd[i + 2];
+    const int m = 2 * cl, n = 2 * cr;
+    const int ns = std::min(m, n);
+    const int gn = right ? m : n;
+    if (right)
+        gemm<T, T, double, double>(gd(m, m, n, OP_N, OP_C, n, n, m, nmax, nmax, slotG, nb), theta.as<CT>(),
+                                   theta.as<CT>(), G.as<c64>(), s);
+    else
+        gemm<T, T, double, double>(gd(n, n, m, OP_C, OP_N, n, n, n, nmax, nmax, slotG, nb), theta.as<CT>(),
+                                   theta.as<CT>(), G.as<c64>(), s);
+    jacobi_eig(G.as<c64>(), slotG, U.as<c64>(), slotG, gn, ns, evals.as<double>(), perm.as<int>(), gdim + 2,
+               keep.as<int>(), rule, thr, (int)std::min<int64_t>(J.max_bond, 1 << 30), nullptr, nb, s);
+    // new working dimension = max kept rank over the wave
+    std::vector<int> hk(nb);
+    YQ_HIP(hipMemcpyAsync(hk.data(), keep.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+    YQ_HIP(hipStreamSynchronize(s));
+    int kw = 1;
+    for (int v : hk) kw = std::max(kw, v);
+    if (kw > cap[i + 1]) throw std::runtime_error("bond dimension exceeds capacity");
+    if (right) {
+        write_iso<T>(U.as<c64>(), slotG, perm.as<int>(), gdim + 2, keep.as<int>(), m, kw, 0, A(S, i), slotA[i], nullptr,
+                     nb, s);
+        gemmT<T>(gd(kw, n, m, OP_C, OP_N, kw, n, n, slotA[i], nmax, slotA[i + 1], nb), A(S, i), theta.as<CT>(),
+                 A(S, i + 1), s);
+    } else {
+        write_iso<T>(U.as<c64>(), slotG, perm.as<int>(), gdim + 2, keep.as<int>(), n, kw, 1, A(S, i + 1), slotA[i + 1],
+                     nullptr, nb, s);
+        gemm<T, T, T, T>(gd(m, kw, n, OP_N, OP_C, n, n, kw, nmax, slotA[i + 1], slotA[i], nb), theta.as<CT>(),
+                         A(S, i + 1), A(S, i), s);
+    }
+    S.wd[i + 1] = kw;
+    set_dims(S.dims.as<int>(), L + 1, i + 1, keep.as<int>(), 0, nullptr, nb, s);
+}
+
+// E_left[i+1] from E_left[i] and A_i (update_left_environment, src/Algorithms.jl:199-204)
+template <typename T>
+void Engine<T>::env_left(State &S, int i) {
+    const int cl = S.wd[i], cr = S.wd[i + 1], Dl = Dm[i], Dr = Dm[i + 1];
+    CT *t1 = T1.as<CT>(), *t2 = T2.as<CT>();
+    gemmT<T>(gd(cl * Dl, 2 * cr, cl, OP_N, OP_N, cl, 2 * cr, 2 * cr, slotE[i], slotA[i], slotS, nb), EL[i].as<CT>(),
+             A(S, i), t1, s);
+    wstep<T>(t1, t2, cl, Dl * 2, 2 * Dr, cr, slotS, slotS, csrL[i].rowptr.as<int>(), csrL[i].col.as<int>(),
+             csrL[i].val.as<c64>(), nullptr, nb, s);
+    gemmT<T>(gd(cr, Dr * cr, 2 * cl, OP_C, OP_N, cr, Dr * cr, Dr * cr, slotA[i], slotS, slotE[i + 1], nb), A(S, i), t2,
+             EL[i + 1].as<CT>(), s);
+}
+
+// E_right[i] from E_right[i+1] and A_i (update_right_environment, src/Algorithms.jl:206-211)
+template <typename T>
+void Engine<T>::env_right(State &S, int i) {
+    const int cl = S.wd[i], cr = S.wd[i + 1], Dl = Dm[i], Dr = Dm[i + 1];
+    CT *t1 = T1.as<CT>(), *t2 = T2.as<CT>();
+    GemmDesc g1 = gd(2 * cl, cr, cr, OP_N, OP_N, cr, cr, (int64_t)Dr * cr, slotA[i], slotE[i + 1], slotS, nb);
+    g1.inner = Dr;
+    g1.sB_inner = (int64_t)cr * cr;
+    g1.sC_inner = cr;
+    gemmT<T>(g1, A(S, i), ER[i + 1].as<CT>(), t1, s);
+    wstep<T>(t1, t2, cl, 2 * Dr, Dl * 2, cr, slotS, slotS, csrR[i].rowptr.as<int>(), csrR[i].col.as<int>(),
+             csrR[i].val.as<c64>(), nullptr, nb, s);
+    GemmDesc g3 = gd(cl, cl, 2 * cr, OP_N, OP_C, (int64_t)Dl * 2 * cr, 2 * cr, cl, slotS, slotA[i], slotE[i], nb);
+    g3.inner = Dl;
+    g3.sA_inner = 2 * cr;
+    g3.sC_inner = (int64_t)cl * cl;
+    gemmT<T>(g3, t2, A(S, i), ER[i].as<CT>(), s);
+}
+
+// One symmetric 2-site TDVP sweep (src/Algorithms.jl:580-603, 631-760).
+template <typename T>
+void Engine<T>::tdvp(State &S, JobResult &res) {
+    if (L < 2) return;
+    const double dt = J.dt;
+    identity_env<T>(ER[L].as<CT>(), slotE[L], S.wd[L], Dm[L], 1, nb, s);
+    for (int i = L - 1; i >= 1; --i) env_right(S, i);
+    identity_env<T>(EL[0].as<CT>(), slotE[0], S.wd[0], Dm[0], 0, nb, s);
+    for (int i = 0; i + 2 < L; ++i) {  // forward (src/Algorithms.jl:631-672)
+        merge(S, i);
+        expm(S, true, i, theta.as<CT>(), nmax, dt / 2, res);
+        split(S, i, true, 0, J.thr);
+        env_left(S, i);
+        expm(S, false, i + 1, A(S, i + 1), slotA[i + 1], -dt / 2, res);
+    }
+    {  // edge (src/Algorithms.jl:674-711)
+        const int i = L - 2;
+        merge(S, i);
+        expm(S, true, i, theta.as<CT>(), nmax, dt, res);
+        split(S, i, false, 0, J.thr);
+        env_right(S, i + 1);
+    }
+    for (int i = L - 3; i >= 0; --i) {  // backward (src/Algorithms.jl:713-760)
+        expm(S, false, i + 1, A(S, i + 1), slotA[i + 1], -dt / 2, res);
+        merge(S, i);
+        expm(S, true, i, theta.as<CT>(), nmax, dt / 2, res);
+        split(S, i, false, 0, J.thr);
+        env_right(S, i + 1);
+    }
+}
+
+// A_{i-1} <- A_{i-1} · Cf (absorb the L factor of an LQ)
+template <typename T>
+void Engine<T>::absorb_left(State &S, int i, const int *msk) {
+    const int c0 = S.wd[i - 1], c1 = S.wd[i];
+    gemmT<T>(gd(2 * c0, c1, c1, OP_N, OP_N, c1, c1, c1, slotA[i - 1], (int64_t)capmax * capmax, slotA[i - 1], nb, msk),
+             A(S, i - 1), Cf.as<CT>(), absorb.as<CT>(), s);
+    copy2d<T, T>(absorb.as<CT>(), slotA[i - 1], 2 * c1, A(S, i - 1), slotA[i - 1], 2 * c1, c0, 2 * c1, msk, nb, s);
+}
+
+// Right-canonicalise sites from..1 by LQ (CGS2 on rows), leaving the norm in
+// site 0 (normalize! without the final scaling, src/MPS.jl:274-311).
+template <typename T>
+void Engine<T>::lq_sweep(State &S, int from, const int *msk) {
+    for (int i = from; i >= 1; --i) {
+        const int c0 = S.wd[i], c1 = S.wd[i + 1];
+        cgs2<T>(A(S, i), slotA[i], c0, 2 * c1, 2 * c1, 1, Cf.as<CT>(), (int64_t)capmax * capmax, msk, nb, s);
+        absorb_left(S, i, msk);
+    }
+}
+
+// Move the centre i -> i+1 by QR (CGS2 on columns) of A_i.
+template <typename T>
+void Engine<T>::qr_step(State &S, int i, const int *msk) {
+    const int c0 = S.wd[i], c1 = S.wd[i + 1], c2 = S.wd[i + 2];
+    cgs2<T>(A(S, i), slotA[i], c1, 2 * c0, 1, c1, Cf.as<CT>(), (int64_t)capmax * capmax, msk, nb, s);
+    // A_{i+1} <- R A_{i+1} with R[j][k] = Cf[k][j]
+    gemmT<T>(gd(c1, 2 * c2, c1, OP_T, OP_N, c1, 2 * c2, 2 * c2, (int64_t)capmax * capmax, slotA[i + 1], slotA[i + 1],
+                nb, msk),
+             Cf.as<CT>(), A(S, i + 1), absorb.as<CT>(), s);
+    copy2d<T, T>(absorb.as<CT>(), slotA[i + 1], 2 * c2, A(S, i + 1), slotA[i + 1], 2 * c2, c1, 2 * c2, msk, nb, s);
+}
+
+template <typename T>
+void Engine<T>::dissipate(State &S, const DevBuf &ops) {
+    for (int i = 0; i < L; ++i)
+        if (diss_site_active[i])
+            local_op<T>(A(S, i), slotA[i], S.wd[i], S.wd[i + 1], ops.as<c64>() + 4 * (int64_t)i, nullptr, nullptr, nb, s);
+}
+
+// Left transfer sweep in FP64 giving every site's reduced matrix ρ_s (and
+// 4x4 ρ for 2-site observables) of a right-canonical state.
+template <typename T>
+void Engine<T>::transfer(State &S, const int *msk) {
+    c64 *E0 = Ebuf0.as<c64>(), *E1 = Ebuf1.as<c64>();
+    const int64_t sE = (int64_t)capmax * capmax;
+    identity_env<double>(E0, sE, S.wd[0], 1, 0, nb, s);
+    for (int i = 0; i < L; ++i) {
+        const int cl = S.wd[i], cr = S.wd[i + 1];
+        gemm<double, T, double, double>(gd(cl, 2 * cr, cl, OP_N, OP_N, cl, 2 * cr, 2 * cr, sE, slotA[i], slotX, nb, msk),
+                                        E0, A(S, i), Xbuf.as<c64>(), s);
+        rho<T>(Xbuf.as<c64>(), slotX, A(S, i), slotA[i], cl, 2, cr, rho1.as<c64>() + (int64_t)i * 4, (int64_t)L * 4,
+               msk, nb, s);
+        if (obs2_at[i] && i + 1 < L) {
+            merge(S, i);
+            const int c2 = S.wd[i + 2];
+            gemm<double, T, double, double>(gd(cl, 4 * c2, cl, OP_N, OP_N, cl, 4 * c2, 4 * c2, sE, nmax, slotX, nb, msk),
+                                            E0, theta.as<CT>(), Xbuf.as<c64>(), s);
+            rho<T>(Xbuf.as<c64>(), slotX, theta.as<CT>(), nmax, cl, 4, c2, rho2.as<c64>() + (int64_t)i * 16,
+                   (int64_t)L * 16, msk, nb, s);
+            gemm<double, T, double, double>(gd(cl, 2 * cr, cl, OP_N, OP_N, cl, 2 * cr, 2 * cr, sE, slotA[i], slotX, nb,
+                                               msk),
+                                            E0, A(S, i), Xbuf.as<c64>(), s);
+        }
+        gemm<T, double, double, double>(gd(cr, cr, 2 * cl, OP_C, OP_N, cr, cr, cr, slotA[i], slotX, sE, nb, msk), A(S, i),
+                                        Xbuf.as<c64>(), E1, s);
+        std::swap(E0, E1);
+    }
+}
+
+template <typename T>
+void Engine<T>::measure(State &S, int col, int ncol, double *d_out) {
+    if (J.obs.empty()) return;
+    transfer(S, nullptr);
+    observe(rho1.as<c64>(), (int64_t)L * 4, rho2.as<c64>(), (int64_t)L * 16, (int)J.obs.size(), obs_site.as<int>(),
+            obs_n.as<int>(), obs_op.as<c64>(), col, ncol, J.num_traj, (int64_t)traj0 - (int64_t)J.traj_offset, d_out,
+            nb, s);
+}
+
+// stochastic_process! (src/StochasticProcess.jl:219-330) for a whole wave.
+template <typename T>
+void Engine<T>::stochastic(State &S, double dt, JobResult &res) {
+    lq_sweep(S, L - 1, nullptr);
+    norm2<T>(A(S, 0), slotA[0], (int64_t)S.wd[0] * 2 * S.wd[1], norm2v.as<double>(), nullptr, nb, s);
+    jump_decide(norm2v.as<double>(), J.seed, traj0, rngk.as<uint64_t>(), J.noise.empty() ? 0 : 1, jstate.as<int>(),
+                fac.as<double>(), nb, s);
+    mask_from_state(jstate.as<int>(), 0, mask.as<int>(), nullptr, nb, s);
+    scale<T>(A(S, 0), slotA[0], (int64_t)S.wd[0] * 2 * S.wd[1], fac.as<double>(), mask.as<int>(), nb, s);
+    fill_int(count.as<int>(), 0, 1, s);
+    mask_from_state(jstate.as<int>(), 1, mask.as<int>(), count.as<int>(), nb, s);
+    const int njump = sync_count(count.as<int>());
+    if (njump == 0) return;
+    transfer(S, mask.as<int>());
+    jump_choose(rho1.as<c64>(), (int64_t)L * 4, L, ncand, cand_site.as<int>(), cand_LdL.as<c64>(),
+                cand_gamma.as<double>(), dt, J.seed, traj0, rngk.as<uint64_t>(), jstate.as<int>(), chosen.as<int>(),
+                wbuf.as<double>(), nb, s);
+    for (int i = 0; i < L; ++i) {
+        jump_opidx(jstate.as<int>(), chosen.as<int>(), cand_site.as<int>(), cand_op.as<int>(), cand_site2.as<int>(),
+                   cand_op2.as<int>(), i, opidx.as<int>(), nb, s);
+        local_op<T>(A(S, i), slotA[i], S.wd[i], S.wd[i + 1], jump_ops.as<c64>(), opidx.as<int>(), nullptr, nb, s);
+    }
+    fill_int(count.as<int>(), 0, 1, s);
+    mask_from_state(jstate.as<int>(), 1, mask.as<int>(), count.as<int>(), nb, s);
+    res.jumps += sync_count(count.as<int>());
+    mask_from_state(jstate.as<int>(), -1, mask.as<int>(), nullptr, nb, s);
+    lq_sweep(S, L - 1, mask.as<int>());
+    norm2<T>(A(S, 0), slotA[0], (int64_t)S.wd[0] * 2 * S.wd[1], norm2v.as<double>(), mask.as<int>(), nb, s);
+    jump_decide(norm2v.as<double>(), 0, 0, rngk.as<uint64_t>(), 0, opidx.as<int>(), fac.as<double>(), nb, s);
+    scale<T>(A(S, 0), slotA[0], (int64_t)S.wd[0] * 2 * S.wd[1], fac.as<double>(), mask.as<int>(), nb, s);
+}
+
+// analog_tjm_2 / analog_tjm_1 (src/AnalogTJM.jl:104-176) for one wave.
+template <typename T>
+void Engine<T>::run_analog(int64_t t0, int nb_, double *d_out, JobResult &res) {
+    nb = nb_;
+    traj0 = J.traj_offset + (uint64_t)t0;
+    YQ_HIP(hipMemsetAsync(rngk.p, 0, (size_t)B * sizeof(uint64_t), s));
+    init_state(PHI);
+    const int nt = (int)J.num_times;
+    const int ncol = J.sample_timesteps ? nt : 1;
+    if (J.sample_timesteps) measure(PHI, 0, ncol, d_out);
+    const bool noise = J.has_noise;
+    auto record_bond = [&](State &S) {
+        for (int b = 0; b <= L; ++b) res.max_bond = std::max<int64_t>(res.max_bond, S.wd[b]);
+    };
+    if (J.order == 1) {
+        for (int j = 2; j <= nt; ++j) {
+            tdvp(PHI, res);
+            if (noise) {
+                dissipate(PHI, diss_full);
+                stochastic(PHI, J.dt, res);
+            }
+            record_bond(PHI);
+            if (J.sample_timesteps) measure(PHI, j - 1, ncol, d_out);
+            else if (j == nt) measure(PHI, 0, ncol, d_out);
+        }
+        return;
+    }
+    if (noise) {  // initialize (src/AnalogTJM.jl:21-33)
+        dissipate(PHI, diss_half);
+        stochastic(PHI, J.dt, res);
+    }
+    auto sample = [&](int j) {  // src/AnalogTJM.jl:62-81
+        copy_state(PHI, PSI);
+        tdvp(PSI, res);
+        if (noise) {
+            dissipate(PSI, diss_half);
+            stochastic(PSI, J.dt, res);
+        }
+        record_bond(PSI);
+        measure(PSI, J.sample_timesteps ? j - 1 : 0, ncol, d_out);
+    };
+    if (J.sample_timesteps && nt > 1) sample(2);
+    for (int j = 3; j <= nt; ++j) {
+        tdvp(PHI, res);  // step_through (src/AnalogTJM.jl:40-55)
+        if (noise) {
+            dissipate(PHI, diss_full);
+            stochastic(PHI, J.dt, res);
+        }
+        if (J.sample_timesteps || j == nt) sample(j);
+    }
+}
+
+// run_digital_tjm with TEBD gates (src/DigitalTJM.jl:363-450) for one wave.
+template <typename T>
+void Engine<T>::run_digital(int64_t t0, int nb_, double *d_out, int64_t *h_bonds, JobResult &res) {
+    nb = nb_;
+    traj0 = J.traj_offset + (uint64_t)t0;
+    YQ_HIP(hipMemsetAsync(rngk.p, 0, (size_t)B * sizeof(uint64_t), s));
+    init_state(PHI);
+    // greedy layering (src/DigitalTJM.jl:86-145)
+    std::vector<std::vector<int>> layers;
+    std::vector<int> barrier_layers;
+    {
+        std::vector<int> cur, busy;
+        for (size_t g = 0; g < J.gates.size(); ++g) {
+            const GateIn &G2 = J.gates[g];
+            if (G2.kind == 3) {
+                if (!cur.empty()) { layers.push_back(cur); cur.clear(); busy.clear(); }
+                barrier_layers.push_back((int)layers.size());
+                continue;
+            }
+            std::vector<int> sites = {G2.s0};
+            if (G2.kind == 2) sites.push_back(G2.s1);
+            bool ov = false;
+            for (int q : sites) ov = ov || std::find(busy.begin(), busy.end(), q) != busy.end();
+            if (ov) { layers.push_back(cur); cur.clear(); busy.clear(); }
+            cur.push_back((int)g);
+            for (int q : sites) busy.push_back(q);
+        }
+        if (!cur.empty()) layers.push_back(cur);
+    }
+    bool at_start = false;
+    std::vector<int> after;
+    for (int b : barrier_layers) {
+        if (b == 0) at_start = true;
+        else if (std::find(after.begin(), after.end(), b) == after.end()) after.push_back(b);
+    }
+    std::sort(after.begin(), after.end());
+    int nsamp = (at_start ? 1 : 0) + (int)after.size();
+    if (nsamp == 0) nsamp = 1;
+    int centre = 0;
+    int idx = 0;
+    std::vector<int> hd((size_t)nb * (L + 1));
+    auto move_centre = [&](int target) {
+        while (centre < target) { qr_step(PHI, centre, nullptr); ++centre; }
+        if (centre > target) { lq_sweep(PHI, centre, nullptr); centre = 0; }
+        while (centre < target) { qr_step(PHI, centre, nullptr); ++centre; }
+    };
+    auto sample = [&](int col) {
+        move_centre(0);
+        measure(PHI, col, nsamp, d_out);
+        YQ_HIP(hipMemcpyAsync(hd.data(), PHI.dims.p, hd.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+        YQ_HIP(hipStreamSynchronize(s));
+        for (int t = 0; t < nb; ++t) {
+            int m = 0;
+            for (int b = 0; b <= L; ++b) m = std::max(m, hd[(size_t)t * (L + 1) + b]);
+            h_bonds[(size_t)(t0 + t) * nsamp + col] = m;
+            res.max_bond = std::max<int64_t>(res.max_bond, m);
+        }
+    };
+    if (at_start) sample(idx++);
+    const c64 *gops = gate_ops.as<c64>();
+    for (size_t li = 0; li < layers.size(); ++li) {
+        for (int g : layers[li])
+            if (J.gates[g].kind == 1) {
+                const int q = J.gates[g].s0;
+                local_op<T>(A(PHI, q), slotA[q], PHI.wd[q], PHI.wd[q + 1], gops + gate_off[g], nullptr, nullptr, nb, s);
+            }
+        for (int g : layers[li]) {
+            const GateIn &G2 = J.gates[g];
+            if (G2.kind != 2) continue;
+            const int a = std::min(G2.s0, G2.s1);
+            move_centre(a);
+            merge(PHI, a);
+            local_op4<T>(theta.as<CT>(), nmax, PHI.wd[a], PHI.wd[a + 2], gops + gate_off[g], nb, s);
+            split(PHI, a, true, 2, J.thr);  // rule T-D
+            centre = a + 1;
+            PairCand &pc = pair_cand[a];
+            const bool noisy = J.has_noise && !J.noise.empty() && pc.n > 0;
+            if (noisy) {
+                // dissipation on the gate's sites (dt = 1), then restore the centre at a+1
+                for (int r = 0; r < 2; ++r) {
+                    const int q = a + r;
+                    if (!dgt_active[2 * a + r]) continue;
+                    local_op<T>(A(PHI, q), slotA[q], PHI.wd[q], PHI.wd[q + 1], dissd_ops.as<c64>() + (2 * a + r) * 4,
+                                nullptr, nullptr, nb, s);
+                }
+                qr_step(PHI, a, nullptr);  // A_a isometric again, norm in A_{a+1}
+                const int64_t nc = (int64_t)PHI.wd[a + 1] * 2 * PHI.wd[a + 2];
+                norm2<T>(A(PHI, a + 1), slotA[a + 1], nc, norm2v.as<double>(), nullptr, nb, s);
+                jump_decide(norm2v.as<double>(), J.seed, traj0, rngk.as<uint64_t>(), 1, jstate.as<int>(),
+                            fac.as<double>(), nb, s);
+                mask_from_state(jstate.as<int>(), 0, mask.as<int>(), nullptr, nb, s);
+                scale<T>(A(PHI, a + 1), slotA[a + 1], nc, fac.as<double>(), mask.as<int>(), nb, s);
+                fill_int(count.as<int>(), 0, 1, s);
+                mask_from_state(jstate.as<int>(), 1, mask.as<int>(), count.as<int>(), nb, s);
+                if (sync_count(count.as<int>()) > 0) {
+                    // ρ of both sites from the 2-site block (left part isometric, right part canonical)
+                    merge(PHI, a);
+                    const int cl = PHI.wd[a], c2 = PHI.wd[a + 2];
+                    copy2d<T, double>(theta.as<CT>(), nmax, 4 * c2, Xbuf.as<c64>(), slotX, 4 * c2, cl, 4 * c2,
+                                      mask.as<int>(), nb, s);
+                    rho<T>(Xbuf.as<c64>(), slotX, theta.as<CT>(), nmax, cl, 4, c2, rho2.as<c64>(), (int64_t)L * 16,
+                           mask.as<int>(), nb, s);
+                    // partial traces of ρ2 into the 1-site matrices of sites a and a+1:
+                    // site a: ρ_a[s][s'] = Σ_{s2} ρ2[(2s+s2)][(2s'+s2)]
+                    // site a+1: ρ_{a+1}[s][s'] = Σ_{s1} ρ2[(2s1+s)][(2s1+s')]
+                    std::vector<c64> h2((size_t)nb * 16);
+                    YQ_HIP(hipMemcpy2DAsync(h2.data(), 16 * sizeof(c64), rho2.p, (size_t)L * 16 * sizeof(c64),
+                                            16 * sizeof(c64), nb, hipMemcpyDeviceToHost, s));
+                    YQ_HIP(hipStreamSynchronize(s));
+                    std::vector<c64> h1((size_t)nb * L * 4, mk(0, 0));
+                    for (int t = 0; t < nb; ++t)
+                        for (int x = 0; x < 2; ++x)
+                            for (int y = 0; y < 2; ++y) {
+                                cd ra = 0, rb = 0;
+                                for (int z = 0; z < 2; ++z) {
+                                    ra += toc(h2[(size_t)t * 16 + (2 * x + z) * 4 + (2 * y + z)]);
+                                    rb += toc(h2[(size_t)t * 16 + (2 * z + x) * 4 + (2 * z + y)]);
+                                }
+                                h1[((size_t)t * L + a) * 4 + x * 2 + y] = fromc(ra);
+                                h1[((size_t)t * L + a + 1) * 4 + x * 2 + y] = fromc(rb);
+                            }
+                    YQ_HIP(hipMemcpyAsync(rho1.p, h1.data(), h1.size() * sizeof(c64), hipMemcpyHostToDevice, s));
+                    jump_choose(rho1.as<c64>(), (int64_t)L * 4, L, pc.n, pc.site.as<int>(), pc.LdL.as<c64>(),
+                                pc.gamma.as<double>(), 1.0, J.seed, traj0, rngk.as<uint64_t>(), jstate.as<int>(),
+                                chosen.as<int>(), wbuf.as<double>(), nb, s);
+                    for (int q : {a, a + 1}) {
+                        jump_opidx(jstate.as<int>(), chosen.as<int>(), pc.site.as<int>(), pc.op.as<int>(),
+                                   pc.site2.as<int>(), pc.op2.as<int>(), q, opidx.as<int>(), nb, s);
+                        local_op<T>(A(PHI, q), slotA[q], PHI.wd[q], PHI.wd[q + 1], jump_ops.as<c64>(), opidx.as<int>(),
+                                    nullptr, nb, s);
+                    }
+                    fill_int(count.as<int>(), 0, 1, s);
+                    mask_from_state(jstate.as<int>(), 1, mask.as<int>(), count.as<int>(), nb, s);
+                    res.jumps += sync_count(count.as<int>());
+                    mask_from_state(jstate.as<int>(), -1, mask.as<int>(), nullptr, nb, s);
+                    qr_step(PHI, a, mask.as<int>());
+                    norm2<T>(A(PHI, a + 1), slotA[a + 1], nc, norm2v.as<double>(), mask.as<int>(), nb, s);
+                    jump_decide(norm2v.as<double>(), 0, 0, rngk.as<uint64_t>(), 0, opidx.as<int>(), fac.as<double>(),
+                                nb, s);
+                    scale<T>(A(PHI, a + 1), slotA[a + 1], nc, fac.as<double>(), mask.as<int>(), nb, s);
+                }
+            } else {
+                // normalize! after a noise-free gate (src/DigitalTJM.jl:415-418)
+                const int64_t nc = (int64_t)PHI.wd[a + 1] * 2 * PHI.wd[a + 2];
+                norm2<T>(A(PHI, a + 1), slotA[a + 1], nc, norm2v.as<double>(), nullptr, nb, s);
+                jump_decide(norm2v.as<double>(), 0, 0, rngk.as<uint64_t>(), 0, opidx.as<int>(), fac.as<double>(), nb,
+                            s);
+                scale<T>(A(PHI, a + 1), slotA[a + 1], nc, fac.as<double>(), nullptr, nb, s);
+            }
+        }
+        const int lnum = (int)li + 1;
+        if (std::find(after.begin(), after.end(), lnum) != after.end()) sample(idx++);
+    }
+    if (!at_start && after.empty() && !J.obs.empty()) sample(0);
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- API -----
+double rng_uniform_host(uint64_t seed, uint64_t traj, uint64_t k) {
+    uint32_t c[4] = {(uint32_t)k, (uint32_t)(k >> 32), (uint32_t)traj, (uint32_t)(traj >> 32)};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t o0 = (uint32_t)(p1 >> 32) ^ c[1] ^ key[0], o2 = (uint32_t)(p0 >> 32) ^ c[3] ^ key[1];
+        c[0] = o0;
+        c[1] = (uint32_t)p1;
+        c[2] = o2;
+        c[3] = (uint32_t)p0;
+        key[0] += 0x9E3779B9u;
+        key[1] += 0xBB67AE85u;
+    }
+    const uint64_t bits = ((uint64_t)(c[0] >> 5) << 26) | (uint64_t)(c[1] >> 6);
+    return (double)bits * (1.0 / 9007199254740992.0);
+}
+
+void validate_job(const JobIn &job) {
+    if (job.L < 1) throw std::invalid_argument("L must be >= 1");
+    if ((int64_t)job.mps.size() != job.L) throw std::invalid_argument("MPS length mismatch");
+    for (int64_t i = 0; i < job.L; ++i) {
+        if ((int64_t)job.mps[i].size() != (int64_t)job.mps_dl[i] * 2 * job.mps_dr[i])
+            throw std::invalid_argument("MPS tensors must have physical dimension 2");
+        if (i + 1 < job.L && job.mps_dr[i] != job.mps_dl[i + 1]) throw std::invalid_argument("MPS bond mismatch");
+    }
+    if (job.mps_dl[0] != 1 || job.mps_dr[job.L - 1] != 1) throw std::invalid_argument("MPS boundary bonds must be 1");
+    if (!job.digital) {
+        if ((int64_t)job.mpo.size() != job.L) throw std::invalid_argument("MPO length mismatch");
+        for (int64_t i = 0; i + 1 < job.L; ++i)
+            if (job.mpo_dr[i] != job.mpo_dl[i + 1]) throw std::invalid_argument("MPO bond mismatch");
+        if (job.num_times < 1) throw std::invalid_argument("num_times must be >= 1");
+        if (job.kdim < 1 || job.kdim > 60) throw std::invalid_argument("krylov_dim must be in [1, 60]");
+        int Dmax = 1;
+        for (int64_t i = 0; i < job.L; ++i) Dmax = std::max(Dmax, std::max(job.mpo_dl[i], job.mpo_dr[i]));
+        if (Dmax > 16) throw std::invalid_argument("MPO bond dimension > 16 is not supported");
+    }
+    if (job.num_traj < 1) throw std::invalid_argument("num_traj must be >= 1");
+    if (job.max_bond < 1) throw std::invalid_argument("max_bond_dim must be >= 1");
+    if (job.max_bond > 512) throw std::invalid_argument("max_bond_dim > 512 is not supported by the GPU path");
+    for (auto &p : job.noise) {
+        if (p.s0 < 0 || p.s0 >= job.L || p.s1 < 0 || p.s1 >= job.L) throw std::invalid_argument("noise site out of range");
+        if (p.nsites == 2) {
+            if (std::abs(p.s1 - p.s0) != 1) throw std::invalid_argument("long-range noise processes are not supported");
+            if (!p.pauli_name)
+                throw std::invalid_argument("adjacent 2-site non-Pauli noise processes are not supported by the GPU path");
+        }
+    }
+    for (auto &o : job.obs) {
+        if (o.s0 < 0 || o.s0 >= job.L || o.s1 < 0 || o.s1 >= job.L) throw std::invalid_argument("observable site out of range");
+        if (o.nsites == 2 && std::abs(o.s1 - o.s0) != 1) throw std::invalid_argument("2-site observables must be adjacent");
+    }
+    for (auto &g : job.gates) {
+        if (g.kind == 3) continue;
+        if (g.s0 < 0 || g.s0 >= job.L || g.s1 < 0 || g.s1 >= job.L) throw std::invalid_argument("gate site out of range");
+        if (g.kind == 2 && std::abs(g.s1 - g.s0) != 1)
+            throw std::invalid_argument("2-qubit gates must act on adjacent sites (TEBD)");
+    }
+}
+
+int64_t num_columns(const JobIn &job) {
+    if (!job.digital) return job.sample_timesteps ? job.num_times : 1;
+    std::vector<int> busy;
+    int nlayers = 0, cur = 0;
+    bool at_start = false;
+    std::vector<int> after;
+    for (auto &g : job.gates) {
+        if (g.kind == 3) {
+            if (cur) { ++nlayers; cur = 0; busy.clear(); }
+            if (nlayers == 0) at_start = true;
+            else if (std::find(after.begin(), after.end(), nlayers) == after.end()) after.push_back(nlayers);
+            continue;
+        }
+        std::vector<int> sites = {g.s0};
+        if (g.kind == 2) sites.push_back(g.s1);
+        bool ov = false;
+        for (int q : sites) ov = ov || std::find(busy.begin(), busy.end(), q) != busy.end();
+        if (ov) { ++nlayers; busy.clear(); cur = 0; }
+        ++cur;
+        for (int q : sites) busy.push_back(q);
+    }
+    int64_t n = (at_start ? 1 : 0) + (int64_t)after.size();
+    return n == 0 ? 1 : n;
+}
+
+int64_t footprint_bytes(const JobIn &job, int64_t batch) {
+    return job.precision == 1 ? Engine<float>::footprint(job, (int)batch) : Engine<double>::footprint(job, (int)batch);
+}
+
+void run_job(const JobIn &job, int64_t batch, hipStream_t stream, JobResult &out, std::atomic<bool> *cancel) {
+    validate_job(job);
+    const int64_t ntraj = job.num_traj;
+    const int64_t ncol = num_columns(job);
+    const int64_t nobs = (int64_t)job.obs.size();
+    out.ncol = ncol;
+    out.traj.assign((size_t)std::max<int64_t>(1, nobs * ncol * ntraj), 0.0);
+    out.mean.assign((size_t)std::max<int64_t>(1, nobs * ncol), 0.0);
+    if (job.digital) out.bonds.assign((size_t)ntraj * ncol, 0);
+    const int64_t B = std::max<int64_t>(1, std::min(batch, ntraj));
+    double *d_out = nullptr;
+    YQ_HIP(hipMalloc(&d_out, out.traj.size() * sizeof(double)));
+    try {
+        YQ_HIP(hipMemsetAsync(d_out, 0, out.traj.size() * sizeof(double), stream));
+        auto go = [&](auto *tag) {
+            typedef std::remove_pointer_t<decltype(tag)> TT;
+            Engine<TT> eng(job, (int)B, stream);
+            for (int64_t t0 = 0; t0 < ntraj; t0 += B) {
+                if (cancel && cancel->load()) throw std::runtime_error("cancelled");
+                const int nb = (int)std::min<int64_t>(B, ntraj - t0);
+                if (job.digital) eng.run_digital(t0, nb, d_out, out.bonds.data(), out);
+                else eng.run_analog(t0, nb, d_out, out);
+            }
+        };
+        if (job.precision == 1) go((float *)nullptr);
+        else go((double *)nullptr);
+        YQ_HIP(hipMemcpyAsync(out.traj.data(), d_out, out.traj.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        YQ_HIP(hipStreamSynchronize(stream));
+    } catch (...) {
+        (void)hipFree(d_out);
+        throw;
+    }
+    (void)hipFree(d_out);
+    for (int64_t q = 0; q < nobs; ++q)
+        for (int64_t c = 0; c < ncol; ++c) {
+            double sum = 0;
+            for (int64_t t = 0; t < ntraj; ++t) sum += out.traj[((size_t)q * ncol + c) * ntraj + t];
+            out.mean[(size_t)q * ncol + c] = sum / (double)ntraj;
+        }
+}
+
+}  // namespace yq

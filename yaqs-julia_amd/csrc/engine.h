@@ -1,0 +1,73 @@
+// Batched TJM engine: job description (deep copy of a C-ABI job) and the
+// entry points that run all trajectories of a job on one device.
+#pragma once
+
+#include <atomic>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace yq {
+
+struct ProcIn {
+    int nsites;
+    int s0, s1;       // 0-based sites (s1 == s0 for 1-site processes)
+    double gamma;
+    bool pauli_name;  // name-based Pauli test (src/Dissipation.jl:14-21)
+    c64 m[16];        // reference order: column-major, 4x4 on p1 + 2 p2
+};
+
+struct ObsIn {
+    int nsites;
+    int s0, s1;
+    c64 op[16];  // reference order
+};
+
+struct GateIn {
+    int kind;  // 1: 1q, 2: 2q, 3: sample barrier
+    int s0, s1;
+    c64 m[16];  // reference order
+};
+
+struct JobIn {
+    bool digital = false;
+    int64_t L = 0;
+    // MPS in engine layout: site i row-major [a][s][b]
+    std::vector<std::vector<c64>> mps;
+    std::vector<int> mps_dl, mps_dr;
+    // MPO in engine layout: site i row-major [m][so][si][n]
+    std::vector<std::vector<c64>> mpo;
+    std::vector<int> mpo_dl, mpo_dr;
+    bool has_noise = false;
+    std::vector<ProcIn> noise;
+    std::vector<ObsIn> obs;
+    std::vector<GateIn> gates;
+    double dt = 0.1;
+    int64_t num_times = 1, num_traj = 1, max_bond = 4096;
+    double thr = 1e-9;
+    int order = 2, sample_timesteps = 1, kdim = 25, precision = 0;
+    double ktol = 1e-10;
+    uint64_t seed = 0, traj_offset = 0;
+};
+
+struct JobResult {
+    int64_t ncol = 0;
+    std::vector<double> traj;     // [obs][col][traj]
+    std::vector<double> mean;     // [obs][col]
+    std::vector<int64_t> bonds;   // [traj][col] (digital)
+    double flop = 0;
+    int64_t matvecs = 0, jumps = 0, max_bond = 0;
+};
+
+// Throws std::invalid_argument for malformed jobs, HipError for device errors.
+void validate_job(const JobIn &job);
+int64_t num_columns(const JobIn &job);
+int64_t footprint_bytes(const JobIn &job, int64_t batch);
+// Runs every trajectory of `job` in waves of at most `batch` trajectories.
+void run_job(const JobIn &job, int64_t batch, hipStream_t stream, JobResult &out, std::atomic<bool> *cancel = nullptr);
+
+double rng_uniform_host(uint64_t seed, uint64_t traj, uint64_t k);
+
+}  // namespace yq

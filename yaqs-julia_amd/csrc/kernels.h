@@ -1,0 +1,85 @@
+// Host-side launchers of the TJM engine kernels (kernels.hip, gemm_mfma.hip).
+#pragma once
+
+#include "common.h"
+
+namespace yq {
+
+template <typename TA, typename TB, typename TC, typename Tacc>
+void gemm(const GemmDesc &d, const cplx<TA> *A, const cplx<TB> *B, cplx<TC> *C, hipStream_t s);
+
+// MFMA GEMM for op(A) in {N, C}, op(B) = N, same precision in and out
+// (float: v_mfma_f32_16x16x4f32, double: v_mfma_f64_16x16x4f64).  Returns
+// false when the descriptor is outside what the kernel covers.
+template <typename T>
+bool gemm_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s);
+
+template <typename T>
+void wstep(const cplx<T> *X, cplx<T> *Y, int P, int I, int J, int Q, int64_t sX, int64_t sY, const int *rowptr,
+           const int *col, const c64 *val, const int *active, int B, hipStream_t s);
+
+template <typename T>
+void lz_init(const cplx<T> *x, int64_t sx, cplx<T> *V, int64_t sV, int64_t n, double *beta0, int *active, int *nit,
+             int B, hipStream_t s);
+template <typename T>
+void lz_dots(const cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, c64 *part, int kslots, const int *active,
+             int B, hipStream_t s);
+template <typename T>
+void lz_update(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, const c64 *part, int kslots, double *partn,
+               double *galpha, const int *active, int B, hipStream_t s);
+void lz_finalize(int j, int kmax, double dt, double tol, const double *partn, int64_t n, const double *galpha,
+                 double *alpha, double *beta, const double *beta0, c64 *coef, int *nit, int *active, double *scale,
+                 int *nactive, int B, hipStream_t s);
+template <typename T>
+void lz_scale(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int q, const double *scale, const int *active, int B,
+              hipStream_t s);
+template <typename T>
+void lz_combine(cplx<T> *x, int64_t sx, const cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, const double *beta0,
+                const c64 *coef, const int *nit, int kmax, const int *mask, int B, hipStream_t s);
+constexpr int LZ_CHUNK_ELEMS = 2048;
+
+void jacobi_eig(c64 *G, int64_t sG, c64 *U, int64_t sU, int n, int ns, double *evals, int *perm, int64_t sE, int *keep,
+                int rule, double thr, int max_bond, const int *mask, int B, hipStream_t s);
+template <typename T>
+void write_iso(const c64 *U, int64_t sU, const int *perm, int64_t sE, const int *keep, int n, int kw, int mode,
+               cplx<T> *Out, int64_t sO, const int *mask, int B, hipStream_t s);
+
+template <typename T>
+void cgs2(cplx<T> *A, int64_t sA, int nvec, int len, int64_t vs, int64_t es, cplx<T> *Cf, int64_t sC, const int *mask,
+          int B, hipStream_t s);
+
+template <typename T>
+void local_op(cplx<T> *A, int64_t sA, int dl, int dr, const c64 *ops, const int *op_idx, const int *mask, int B,
+              hipStream_t s);
+template <typename T>
+void local_op4(cplx<T> *A, int64_t sA, int dl, int dr, const c64 *M, int B, hipStream_t s);
+template <typename T>
+void norm2(const cplx<T> *A, int64_t sA, int64_t n, double *out, const int *mask, int B, hipStream_t s);
+template <typename T>
+void scale(cplx<T> *A, int64_t sA, int64_t n, const double *f, const int *mask, int B, hipStream_t s);
+template <typename TS, typename TD>
+void copy2d(const cplx<TS> *S, int64_t sS, int64_t lds, cplx<TD> *D, int64_t sD, int64_t ldd, int rows, int cols,
+            const int *mask, int B, hipStream_t s);
+template <typename T>
+void fill_zero(cplx<T> *A, int64_t sA, int64_t n, int B, hipStream_t s);
+template <typename T>
+void identity_env(cplx<T> *E, int64_t sE, int dim, int D, int right, int B, hipStream_t s);
+template <typename T>
+void rho(const c64 *X, int64_t sX, const cplx<T> *A, int64_t sA, int dl, int d, int dr, c64 *out, int64_t sR,
+         const int *mask, int B, hipStream_t s);
+
+void jump_decide(const double *norm2v, uint64_t seed, uint64_t traj0, uint64_t *rng_k, int has_procs, int *state,
+                 double *fac, int B, hipStream_t s);
+void jump_choose(const c64 *rhov, int64_t sR, int L, int ncand, const int *cand_site, const c64 *cand_LdL,
+                 const double *cand_gamma, double dt, uint64_t seed, uint64_t traj0, uint64_t *rng_k, int *state,
+                 int *chosen, double *wbuf, int B, hipStream_t s);
+void jump_opidx(const int *state, const int *chosen, const int *cand_site, const int *cand_op, const int *cand_site2,
+                const int *cand_op2, int site, int *opidx, int B, hipStream_t s);
+void mask_from_state(const int *state, int v, int *mask, int *count, int B, hipStream_t s);
+void observe(const c64 *rho1, int64_t sR1, const c64 *rho2, int64_t sR2, int nobs, const int *obs_site,
+             const int *obs_n, const c64 *obs_op, int col, int ncol, int64_t ntraj, int64_t t0, double *out, int B,
+             hipStream_t s);
+void set_dims(int *dims, int ld, int b, const int *keep, int val, const int *mask, int B, hipStream_t s);
+void fill_int(int *p, int v, int n, hipStream_t s);
+
+}  // namespace yq

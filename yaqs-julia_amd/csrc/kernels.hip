@@ -1,0 +1,1136 @@
+// Device kernels of the batched TJM engine (gfx950).
+//
+// Every kernel works on a batch of trajectories laid out slot by slot:
+// trajectory t of a buffer starts at base + t * stride.  Storage precision T
+// is float (mixed mode) or double; norms, Lanczos scalars, Gram matrices,
+// eigen-decompositions, transfer matrices and jump probabilities are always
+// computed in double.
+#include "kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace yq {
+
+// ---------------------------------------------------------------- helpers --
+template <typename T>
+__device__ __forceinline__ double2 ld2(const cplx<T> &z) {
+    return make_double2((double)z.re, (double)z.im);
+}
+template <typename T>
+__device__ __forceinline__ void st2(cplx<T> &z, double2 v) {
+    z.re = (T)v.x;
+    z.im = (T)v.y;
+}
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // conj(a) * b
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block-wide sum of a double (blockDim.x multiple of 64, <= 1024).
+__device__ double block_sum(double v, double *sh) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    for (int i = 0; i < nw; ++i) r += sh[i];
+    __syncthreads();
+    return r;
+}
+
+__device__ double2 block_sum2(double2 v, double *sh) {
+    double a = block_sum(v.x, sh);
+    double b = block_sum(v.y, sh);
+    return make_double2(a, b);
+}
+
+// ------------------------------------------------------------------ GEMM ---
+// Generic batched complex GEMM, 64x64 tile per 256-thread workgroup, each
+// thread accumulating a 4x4 block in Tacc.  Used for every contraction whose
+// shape or precision mix the MFMA kernels do not cover.
+template <typename TA, typename TB, typename TC, typename Tacc>
+__global__ __launch_bounds__(256) void gemm_generic_kernel(GemmDesc d, const cplx<TA> *__restrict__ A,
+                                                           const cplx<TB> *__restrict__ B, cplx<TC> *__restrict__ C) {
+    constexpr int BM = 64, BN = 64, BK = 16;
+    const int t = blockIdx.z, jb = blockIdx.y;
+    if (d.active && !d.active[t]) return;
+    const int tilesN = (d.N + BN - 1) / BN;
+    const int m0 = (blockIdx.x / tilesN) * BM, n0 = (blockIdx.x % tilesN) * BN;
+    const cplx<TA> *Ab = A + t * d.sA_outer + jb * d.sA_inner;
+    const cplx<TB> *Bb = B + t * d.sB_outer + jb * d.sB_inner;
+    cplx<TC> *Cb = C + t * d.sC_outer + jb * d.sC_inner;
+    __shared__ Tacc As_r[BK][BM + 1], As_i[BK][BM + 1], Bs_r[BK][BN + 1], Bs_i[BK][BN + 1];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    Tacc acc_r[4][4], acc_i[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc_r[i][j] = acc_i[i][j] = (Tacc)0;
+    for (int k0 = 0; k0 < d.K; k0 += BK) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int e = threadIdx.x + 256 * r;
+            // A tile: element (row, kk)
+            int row, kk;
+            if (d.opA == OP_N) { row = e / BK; kk = e % BK; }
+            else { kk = e / BM; row = e % BM; }
+            const int gi = m0 + row, gk = k0 + kk;
+            Tacc vr = 0, vi = 0;
+            if (gi < d.M && gk < d.K) {
+                cplx<TA> z = (d.opA == OP_N) ? Ab[(int64_t)gi * d.lda + gk] : Ab[(int64_t)gk * d.lda + gi];
+                vr = (Tacc)z.re;
+                vi = (d.opA == OP_C) ? -(Tacc)z.im : (Tacc)z.im;
+            }
+            As_r[kk][row] = vr;
+            As_i[kk][row] = vi;
+            // B tile: element (kk, col)
+            int col;
+            if (d.opB == OP_N) { kk = e / BN; col = e % BN; }
+            else { col = e / BK; kk = e % BK; }
+            const int gj = n0 + col, gk2 = k0 + kk;
+            Tacc wr = 0, wi = 0;
+            if (gj < d.N && gk2 < d.K) {
+                cplx<TB> z = (d.opB == OP_N) ? Bb[(int64_t)gk2 * d.ldb + gj] : Bb[(int64_t)gj * d.ldb + gk2];
+                wr = (Tacc)z.re;
+                wi = (d.opB == OP_C) ? -(Tacc)z.im : (Tacc)z.im;
+            }
+            Bs_r[kk][col] = wr;
+            Bs_i[kk][col] = wi;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < BK; ++kk) {
+            Tacc ar[4], ai[4], br[4], bi[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                ar[i] = As_r[kk][ty * 4 + i];
+                ai[i] = As_i[kk][ty * 4 + i];
+                br[i] = Bs_r[kk][tx * 4 + i];
+                bi[i] = Bs_i[kk][tx * 4 + i];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    acc_r[i][j] += ar[i] * br[j] - ai[i] * bi[j];
+                    acc_i[i][j] += ar[i] * bi[j] + ai[i] * br[j];
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int gi = m0 + ty * 4 + i;
+        if (gi >= d.M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gj = n0 + tx * 4 + j;
+            if (gj >= d.N) continue;
+            cplx<TC> &c = Cb[(int64_t)gi * d.ldc + gj];
+            if (d.accumulate) {
+                c.re = (TC)((Tacc)c.re + acc_r[i][j]);
+                c.im = (TC)((Tacc)c.im + acc_i[i][j]);
+            } else {
+                c.re = (TC)acc_r[i][j];
+                c.im = (TC)acc_i[i][j];
+            }
+        }
+    }
+}
+
+template <typename TA, typename TB, typename TC, typename Tacc>
+void gemm(const GemmDesc &d, const cplx<TA> *A, const cplx<TB> *B, cplx<TC> *C, hipStream_t s) {
+    if (d.M <= 0 || d.N <= 0 || d.outer <= 0 || d.inner <= 0) return;
+    if (d.K <= 0) throw HipError("gemm with K = 0");
+    const int tiles = ((d.M + 63) / 64) * ((d.N + 63) / 64);
+    dim3 grid(tiles, d.inner, d.outer);
+    hipLaunchKernelGGL((gemm_generic_kernel<TA, TB, TC, Tacc>), grid, dim3(256), 0, s, d, A, B, C);
+    YQ_HIP(hipGetLastError());
+}
+
+template void gemm<float, float, float, float>(const GemmDesc &, const c32 *, const c32 *, c32 *, hipStream_t);
+template void gemm<double, double, double, double>(const GemmDesc &, const c64 *, const c64 *, c64 *, hipStream_t);
+template void gemm<float, float, double, double>(const GemmDesc &, const c32 *, const c32 *, c64 *, hipStream_t);
+template void gemm<double, float, double, double>(const GemmDesc &, const c64 *, const c32 *, c64 *, hipStream_t);
+template void gemm<float, double, double, double>(const GemmDesc &, const c32 *, const c64 *, c64 *, hipStream_t);
+template void gemm<double, double, float, double>(const GemmDesc &, const c64 *, const c64 *, c32 *, hipStream_t);
+
+// ------------------------------------------------------------- W step ------
+// Y[t][p][j][q] = Σ_i W[j][i] X[t][p][i][q] with W in CSR (rows j).  The MPO
+// contraction of the effective Hamiltonian; memory-bound, coalesced along q.
+template <typename T>
+__global__ __launch_bounds__(256) void wstep_kernel(const cplx<T> *__restrict__ X, cplx<T> *__restrict__ Y, int P, int I,
+                                                    int J, int Q, int64_t sX, int64_t sY, const int *__restrict__ rowptr,
+                                                    const int *__restrict__ col, const c64 *__restrict__ val,
+                                                    const int *active) {
+    const int t = blockIdx.y;
+    if (active && !active[t]) return;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)P * Q) return;
+    const int p = (int)(idx / Q), q = (int)(idx % Q);
+    const cplx<T> *x = X + t * sX + (int64_t)p * I * Q + q;
+    cplx<T> *y = Y + t * sY + (int64_t)p * J * Q + q;
+    for (int j = 0; j < J; ++j) {
+        double2 acc = make_double2(0, 0);
+        for (int z = rowptr[j]; z < rowptr[j + 1]; ++z) {
+            const c64 w = val[z];
+            const double2 xv = ld2(x[(int64_t)col[z] * Q]);
+            acc.x += w.re * xv.x - w.im * xv.y;
+            acc.y += w.re * xv.y + w.im * xv.x;
+        }
+        st2(y[(int64_t)j * Q], acc);
+    }
+}
+
+template <typename T>
+void wstep(const cplx<T> *X, cplx<T> *Y, int P, int I, int J, int Q, int64_t sX, int64_t sY, const int *rowptr,
+           const int *col, const c64 *val, const int *active, int B, hipStream_t s) {
+    const int64_t n = (int64_t)P * Q;
+    dim3 grid((unsigned)((n + 255) / 256), B);
+    hipLaunchKernelGGL(wstep_kernel<T>, grid, dim3(256), 0, s, X, Y, P, I, J, Q, sX, sY, rowptr, col, val, active);
+    YQ_HIP(hipGetLastError());
+}
+template void wstep<float>(const c32 *, c32 *, int, int, int, int, int64_t, int64_t, const int *, const int *,
+                           const c64 *, const int *, int, hipStream_t);
+template void wstep<double>(const c64 *, c64 *, int, int, int, int, int64_t, int64_t, const int *, const int *,
+                            const c64 *, const int *, int, hipStream_t);
+
+// ------------------------------------------------------------- Lanczos -----
+// lz_init: β0 = ‖x‖, V0 = x/β0, active = β0 > 0.
+template <typename T>
+__global__ __launch_bounds__(256) void lz_init_kernel(const cplx<T> *x, int64_t sx, cplx<T> *V, int64_t sV, int64_t n,
+                                                      double *beta0, int *active, int *nit) {
+    __shared__ double sh[16];
+    const int t = blockIdx.x;
+    const cplx<T> *xt = x + t * sx;
+    cplx<T> *v = V + t * sV;
+    double s = 0;
+    for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+        double2 z = ld2(xt[e]);
+        s += z.x * z.x + z.y * z.y;
+    }
+    s = block_sum(s, sh);
+    const double b0 = sqrt(s);
+    if (threadIdx.x == 0) {
+        beta0[t] = b0;
+        active[t] = b0 > 0.0 ? 1 : 0;
+        nit[t] = 0;
+    }
+    if (b0 > 0.0) {
+        const double inv = 1.0 / b0;
+        for (int64_t e = threadIdx.x; e < n; e += blockDim.x) st2(v[e], cscale(ld2(xt[e]), inv));
+    }
+}
+
+// lz_dots: part[t][q][c] = Σ_{e in chunk c} conj(V_q[e]) w[e] for q <= j,
+// w = V_{j+1} (the raw matvec output).
+constexpr int LZ_CHUNK = 2048;  // elements per block (256 threads x 8)
+
+template <typename T>
+__global__ __launch_bounds__(256) void lz_dots_kernel(const cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j,
+                                                      c64 *part, int nchunk, int kslots, const int *active) {
+    __shared__ double sh[16];
+    const int t = blockIdx.y, c = blockIdx.x;
+    if (!active[t]) return;
+    const cplx<T> *Vt = V + t * sV;
+    const cplx<T> *w = Vt + (int64_t)(j + 1) * nmax;
+    double2 wv[8];
+    const int64_t base = (int64_t)c * LZ_CHUNK;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int64_t e = base + threadIdx.x + 256 * r;
+        wv[r] = e < n ? ld2(w[e]) : make_double2(0, 0);
+    }
+    for (int q = 0; q <= j; ++q) {
+        const cplx<T> *vq = Vt + (int64_t)q * nmax;
+        double2 acc = make_double2(0, 0);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int64_t e = base + threadIdx.x + 256 * r;
+            if (e < n) acc = cadd(acc, cmulc(ld2(vq[e]), wv[r]));
+        }
+        acc = block_sum2(acc, sh);
+        if (threadIdx.x == 0) {
+            c64 o;
+            o.re = acc.x;
+            o.im = acc.y;
+            part[((int64_t)t * kslots + q) * nchunk + c] = o;
+        }
+    }
+}
+
+// lz_update: g_q = Σ_c part; w -= Σ_q g_q V_q; partn[t][c] = ‖w_chunk‖².
+template <typename T>
+__global__ __launch_bounds__(256) void lz_update_kernel(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j,
+                                                        const c64 *part, int nchunk, int kslots, double *partn,
+                                                        double *galpha, const int *active) {
+    __shared__ double sh[16];
+    __shared__ double2 g[64];
+    const int t = blockIdx.y, c = blockIdx.x;
+    if (!active[t]) return;
+    if (threadIdx.x <= j) {
+        double2 s = make_double2(0, 0);
+        const c64 *p = part + ((int64_t)t * kslots + threadIdx.x) * nchunk;
+        for (int i = 0; i < nchunk; ++i) s = cadd(s, make_double2(p[i].re, p[i].im));
+        g[threadIdx.x] = s;
+    }
+    __syncthreads();
+    cplx<T> *Vt = V + t * sV;
+    cplx<T> *w = Vt + (int64_t)(j + 1) * nmax;
+    double nrm = 0;
+    const int64_t base = (int64_t)c * LZ_CHUNK;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int64_t e = base + threadIdx.x + 256 * r;
+        if (e >= n) continue;
+        double2 acc = ld2(w[e]);
+        for (int q = 0; q <= j; ++q) acc = csub(acc, cmul(g[q], ld2(Vt[(int64_t)q * nmax + e])));
+        st2(w[e], acc);
+        const double2 back = ld2(w[e]);
+        nrm += back.x * back.x + back.y * back.y;
+    }
+    nrm = block_sum(nrm, sh);
+    if (threadIdx.x == 0) {
+        partn[(int64_t)t * nchunk + c] = nrm;
+        if (c == 0) galpha[t] = g[j].x;
+    }
+}
+
+// exp(-i tau T) e_0 for the symmetric tridiagonal T (k <= 64) by Taylor
+// sub-steps with |h|·‖T‖∞ <= 1/2, one lane per component.
+__device__ double2 tridiag_exp_lane(const double *alpha, const double *beta, int k, double tau, int lane,
+                                    double a_last) {
+    double a = lane < k ? (lane == k - 1 ? a_last : alpha[lane]) : 0.0;
+    double bl = (lane >= 1 && lane < k) ? beta[lane] : 0.0;          // coupling to lane-1
+    double br = (lane + 1 < k) ? beta[lane + 1] : 0.0;              // coupling to lane+1
+    double rown = fabs(a) + fabs(bl) + fabs(br);
+    for (int o = 32; o > 0; o >>= 1) rown = fmax(rown, __shfl_xor(rown, o, 64));
+    int s = (int)ceil(fabs(tau) * rown / 0.5);
+    if (s < 1) s = 1;
+    const double h = tau / s;
+    double2 v = make_double2(lane == 0 ? 1.0 : 0.0, 0.0);
+    for (int st = 0; st < s; ++st) {
+        double2 term = v, acc = v;
+        for (int m = 1; m <= 40; ++m) {
+            // term <- (-i h T) term / m
+            const double tlx = __shfl(term.x, lane - 1 < 0 ? 0 : lane - 1, 64);
+            const double tly = __shfl(term.y, lane - 1 < 0 ? 0 : lane - 1, 64);
+            const double trx = __shfl(term.x, lane + 1 > 63 ? 63 : lane + 1, 64);
+            const double try_ = __shfl(term.y, lane + 1 > 63 ? 63 : lane + 1, 64);
+            double ux = a * term.x + bl * tlx + br * trx;
+            double uy = a * term.y + bl * tly + br * try_;
+            if (lane >= k) { ux = 0; uy = 0; }
+            // (-i h / m) * (ux + i uy) = (h/m) (uy - i ux)
+            const double f = h / m;
+            term = make_double2(f * uy, -f * ux);
+            acc = cadd(acc, term);
+            double mag = fabs(term.x) + fabs(term.y);
+            for (int o = 32; o > 0; o >>= 1) mag = fmax(mag, __shfl_xor(mag, o, 64));
+            if (mag < 1e-20) break;
+        }
+        v = acc;
+    }
+    return lane < k ? v : make_double2(0, 0);
+}
+
+// lz_finalize: one wavefront per trajectory.  β = ‖w‖, c = exp(-i dt T) e0,
+// stopping rule β0·β·|c_j| <= tol (shared with the oracle), else prepare
+// V_{j+1} = w/β.
+__global__ __launch_bounds__(256) void lz_finalize_kernel(int j, int kmax, double dt, double tol, const double *partn,
+                                                          int nchunk, const double *galpha, double *alpha, double *beta,
+                                                          const double *beta0, c64 *coef, int *nit, int *active,
+                                                          double *scale, int *nactive, int B) {
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (t >= B || !active[t]) return;
+    double s = 0;
+    for (int i = lane; i < nchunk; i += 64) s += partn[(int64_t)t * nchunk + i];
+    s = wave_sum(s);
+    const double b = sqrt(s);
+    double *al = alpha + (int64_t)t * kmax;
+    double *be = beta + (int64_t)t * (kmax + 1);
+    const double aj = galpha[t];
+    if (lane == 0) al[j] = aj;
+    const double2 cv = tridiag_exp_lane(al, be, j + 1, dt, lane, aj);
+    const double cjx = __shfl(cv.x, j, 64), cjy = __shfl(cv.y, j, 64);
+    const double err = beta0[t] * b * sqrt(cjx * cjx + cjy * cjy);
+    const bool done = (err <= tol) || (b == 0.0) || (j + 1 == kmax);
+    if (done) {
+        if (lane <= j) {
+            c64 o;
+            o.re = cv.x;
+            o.im = cv.y;
+            coef[(int64_t)t * kmax + lane] = o;
+        }
+        if (lane == 0) {
+            nit[t] = j + 1;
+            active[t] = 0;
+        }
+    } else if (lane == 0) {
+        be[j + 1] = b;
+        scale[t] = 1.0 / b;
+        atomicAdd(nactive, 1);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void lz_scale_kernel(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int q,
+                                                       const double *scale, const int *active) {
+    const int t = blockIdx.y;
+    if (!active[t]) return;
+    cplx<T> *v = V + t * sV + (int64_t)q * nmax;
+    const double f = scale[t];
+    const int64_t base = (int64_t)blockIdx.x * LZ_CHUNK;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int64_t e = base + threadIdx.x + 256 * r;
+        if (e < n) st2(v[e], cscale(ld2(v[e]), f));
+    }
+}
+
+// x = β0 Σ_{q<nit} c_q V_q  (trajectories with β0 > 0)
+template <typename T>
+__global__ __launch_bounds__(256) void lz_combine_kernel(cplx<T> *x, int64_t sx, const cplx<T> *V, int64_t sV,
+                                                         int64_t nmax, int64_t n, const double *beta0, const c64 *coef,
+                                                         const int *nit, int kmax, const int *mask) {
+    const int t = blockIdx.y;
+    if (mask && !mask[t]) return;
+    const double b0 = beta0[t];
+    if (!(b0 > 0.0)) return;
+    const int k = nit[t];
+    const cplx<T> *Vt = V + t * sV;
+    const c64 *ct = coef + (int64_t)t * kmax;
+    const int64_t base = (int64_t)blockIdx.x * LZ_CHUNK;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int64_t e = base + threadIdx.x + 256 * r;
+        if (e >= n) continue;
+        double2 acc = make_double2(0, 0);
+        for (int q = 0; q < k; ++q) acc = cadd(acc, cmul(make_double2(ct[q].re, ct[q].im), ld2(Vt[(int64_t)q * nmax + e])));
+        st2(x[t * sx + e], cscale(acc, b0));
+    }
+}
+
+template <typename T>
+void lz_init(const cplx<T> *x, int64_t sx, cplx<T> *V, int64_t sV, int64_t n, double *beta0, int *active, int *nit,
+             int B, hipStream_t s) {
+    hipLaunchKernelGGL(lz_init_kernel<T>, dim3(B), dim3(256), 0, s, x, sx, V, sV, n, beta0, active, nit);
+    YQ_HIP(hipGetLastError());
+}
+template <typename T>
+void lz_dots(const cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, c64 *part, int kslots, const int *active,
+             int B, hipStream_t s) {
+    const int nchunk = (int)((n + LZ_CHUNK - 1) / LZ_CHUNK);
+    hipLaunchKernelGGL(lz_dots_kernel<T>, dim3(nchunk, B), dim3(256), 0, s, V, sV, nmax, n, j, part, nchunk, kslots,
+                       active);
+    YQ_HIP(hipGetLastError());
+}
+template <typename T>
+void lz_update(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, const c64 *part, int kslots, double *partn,
+               double *galpha, const int *active, int B, hipStream_t s) {
+    const int nchunk = (int)((n + LZ_CHUNK - 1) / LZ_CHUNK);
+    hipLaunchKernelGGL(lz_update_kernel<T>, dim3(nchunk, B), dim3(256), 0, s, V, sV, nmax, n, j, part, nchunk, kslots,
+                       partn, galpha, active);
+    YQ_HIP(hipGetLastError());
+}
+void lz_finalize(int j, int kmax, double dt, double tol, const double *partn, int64_t n, const double *galpha,
+                 double *alpha, double *beta, const double *beta0, c64 *coef, int *nit, int *active, double *scale,
+                 int *nactive, int B, hipStream_t s) {
+    const int nchunk = (int)((n + LZ_CHUNK - 1) / LZ_CHUNK);
+    hipLaunchKernelGGL(lz_finalize_kernel, dim3((B + 3) / 4), dim3(256), 0, s, j, kmax, dt, tol, partn, nchunk, galpha,
+                       alpha, beta, beta0, coef, nit, active, scale, nactive, B);
+    YQ_HIP(hipGetLastError());
+}
+template <typename T>
+void lz_scale(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int q, const double *scale, const int *active, int B,
+              hipStream_t s) {
+    const int nchunk = (int)((n + LZ_CHUNK - 1) / LZ_CHUNK);
+    hipLaunchKernelGGL(lz_scale_kernel<T>, dim3(nchunk, B), dim3(256), 0, s, V, sV, nmax, n, q, scale, active);
+    YQ_HIP(hipGetLastError());
+}
+template <typename T>
+void lz_combine(cplx<T> *x, int64_t sx, const cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, const double *beta0,
+                const c64 *coef, const int *nit, int kmax, const int *mask, int B, hipStream_t s) {
+    const int nchunk = (int)((n + LZ_CHUNK - 1) / LZ_CHUNK);
+    hipLaunchKernelGGL(lz_combine_kernel<T>, dim3(nchunk, B), dim3(256), 0, s, x, sx, V, sV, nmax, n, beta0, coef, nit,
+                       kmax, mask);
+    YQ_HIP(hipGetLastError());
+}
+
+#define YQ_LZ_INST(T)                                                                                              \
+    template void lz_init<T>(const cplx<T> *, int64_t, cplx<T> *, int64_t, int64_t, double *, int *, int *, int,    \
+                             hipStream_t);                                                                         \
+    template void lz_dots<T>(const cplx<T> *, int64_t, int64_t, int64_t, int, c64 *, int, const int *, int,         \
+                             hipStream_t);                                                                         \
+    template void lz_update<T>(cplx<T> *, int64_t, int64_t, int64_t, int, const c64 *, int, double *, double *,     \
+                               const int *, int, hipStream_t);                                                     \
+    template void lz_scale<T>(cplx<T> *, int64_t, int64_t, int64_t, int, const double *, const int *, int,          \
+                              hipStream_t);                                                                        \
+    template void lz_combine<T>(cplx<T> *, int64_t, const cplx<T> *, int64_t, int64_t, int64_t, const double *,     \
+                                const c64 *, const int *, int, const int *, int, hipStream_t);
+YQ_LZ_INST(float)
+YQ_LZ_INST(double)
+
+// ----------------------------------------------------- Jacobi eigensolver --
+// Cyclic Jacobi on a batch of Hermitian n x n FP64 matrices (one workgroup
+// per matrix, matrices in global memory), parallel rounds of the round-robin
+// tournament so the n/2 rotations of a round touch disjoint index pairs.
+// Output: eigenvector matrix U (columns, unsorted), eigenvalues sorted
+// descending with their column permutation, and the kept rank under the
+// requested truncation rule.
+__global__ __launch_bounds__(512) void jacobi_kernel(c64 *G, int64_t sG, c64 *U, int64_t sU, int n, int ns,
+                                                     double *evals, int *perm, int64_t sE, int *keep, int rule,
+                                                     double thr, int max_bond, int max_sweeps, const int *mask) {
+    extern __shared__ double smem[];
+    const int t = blockIdx.x;
+    if (mask && !mask[t]) return;
+    c64 *A = G + t * sG;
+    c64 *V = U + t * sU;
+    const int m = (n + 1) & ~1;  // even number of players (index n is a dummy when n is odd)
+    const int half = m / 2;
+    // LDS: rotation parameters per pair
+    double *rc = smem;                 // c
+    double *rsr = rc + half;           // Re(sp)
+    double *rsi = rsr + half;          // Im(sp)
+    int *rp = (int *)(rsi + half);     // p
+    int *rq = rp + half;               // q
+    int *flag = rq + half;             // [0]: any rotation this sweep
+    for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += blockDim.x) {
+        const int r = (int)(e / n), c = (int)(e % n);
+        V[e].re = (r == c) ? 1.0 : 0.0;
+        V[e].im = 0.0;
+    }
+    __syncthreads();
+    const double eps = 2.220446049250313e-16;
+    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+        if (threadIdx.x == 0) flag[0] = 0;
+        __syncthreads();
+        for (int round = 0; round < m - 1; ++round) {
+            for (int k = threadIdx.x; k < half; k += blockDim.x) {
+                const int i0 = k, i1 = m - 1 - k;
+                const int p0 = (i0 == 0) ? 0 : ((i0 - 1 + round) % (m - 1)) + 1;
+                const int q0 = (i1 == 0) ? 0 : ((i1 - 1 + round) % (m - 1)) + 1;
+                const int p = p0 < q0 ? p0 : q0, q = p0 < q0 ? q0 : p0;
+                rp[k] = p;
+                rq[k] = q;
+                rc[k] = 1.0;
+                rsr[k] = 0.0;
+                rsi[k] = 0.0;
+                if (q >= n) continue;
+                const double a = A[(int64_t)p * n + p].re, b = A[(int64_t)q * n + q].re;
+                const c64 g = A[(int64_t)p * n + q];
+                const double ag = sqrt(g.re * g.re + g.im * g.im);
+                if (ag == 0.0 || ag <= eps * sqrt(fabs(a * b))) continue;
+                const double zeta = (b - a) / (2.0 * ag);
+                const double tt = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / sqrt(1.0 + tt * tt), s = c * tt;
+                rc[k] = c;
+                rsr[k] = s * g.re / ag;
+                rsi[k] = s * g.im / ag;
+                flag[0] = 1;
+            }
+            __syncthreads();
+            // rows: row_p' = c row_p - sp row_q ; row_q' = conj(sp) row_p + c row_q
+            for (int64_t e = threadIdx.x; e < (int64_t)half * n; e += blockDim.x) {
+                const int k = (int)(e / n), col = (int)(e % n);
+                const int q = rq[k];
+                if (q >= n || (rsr[k] == 0.0 && rsi[k] == 0.0)) continue;
+                const int p = rp[k];
+                const double c = rc[k];
+                const double2 sp = make_double2(rsr[k], rsi[k]);
+                const double2 xp = make_double2(A[(int64_t)p * n + col].re, A[(int64_t)p * n + col].im);
+                const double2 xq = make_double2(A[(int64_t)q * n + col].re, A[(int64_t)q * n + col].im);
+                const double2 np_ = csub(cscale(xp, c), cmul(sp, xq));
+                const double2 nq_ = cadd(cmulc(sp, xp), cscale(xq, c));
+                A[(int64_t)p * n + col].re = np_.x;
+                A[(int64_t)p * n + col].im = np_.y;
+                A[(int64_t)q * n + col].re = nq_.x;
+                A[(int64_t)q * n + col].im = nq_.y;
+            }
+            __syncthreads();
+            // columns (A and V): col_p' = c col_p - conj(sp) col_q ; col_q' = sp col_p + c col_q
+            for (int64_t e = threadIdx.x; e < (int64_t)half * n * 2; e += blockDim.x) {
+                const int64_t ee = e % ((int64_t)half * n);
+                const int which = (int)(e / ((int64_t)half * n));
+                const int k = (int)(ee / n), row = (int)(ee % n);
+                const int q = rq[k];
+                if (q >= n || (rsr[k] == 0.0 && rsi[k] == 0.0)) continue;
+                const int p = rp[k];
+                const double c = rc[k];
+                const double2 sp = make_double2(rsr[k], rsi[k]);
+                c64 *M = which == 0 ? A : V;
+                const double2 xp = make_double2(M[(int64_t)row * n + p].re, M[(int64_t)row * n + p].im);
+                const double2 xq = make_double2(M[(int64_t)row * n + q].re, M[(int64_t)row * n + q].im);
+                const double2 np_ = csub(cscale(xp, c), cmulc(sp, xq));
+                const double2 nq_ = cadd(cmul(sp, xp), cscale(xq, c));
+                M[(int64_t)row * n + p].re = np_.x;
+                M[(int64_t)row * n + p].im = np_.y;
+                M[(int64_t)row * n + q].re = nq_.x;
+                M[(int64_t)row * n + q].im = nq_.y;
+            }
+            __syncthreads();
+        }
+        const int any = flag[0];
+        __syncthreads();
+        if (!any) break;
+    }
+    // sort eigenvalues descending (rank sort, ties by index)
+    double *ev = evals + t * sE;
+    int *pm = perm + t * sE;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double di = A[(int64_t)i * n + i].re;
+        int rank = 0;
+        for (int j2 = 0; j2 < n; ++j2) {
+            const double dj = A[(int64_t)j2 * n + j2].re;
+            if (dj > di || (dj == di && j2 < i)) ++rank;
+        }
+        ev[rank] = di;
+        pm[rank] = i;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // truncation rules on s² = max(λ, 0) over the top ns values
+        int kp = ns;
+        if (rule == 0) {  // T-A (TDVP split, src/Algorithms.jl:422-438)
+            double disc = 0;
+            for (int k = ns; k >= 1; --k) {
+                disc += fmax(ev[k - 1], 0.0);
+                if (disc >= thr) { kp = k > 2 ? k : 2; break; }
+            }
+        } else if (rule == 1) {  // T-C (relative, src/Dissipation.jl:123-136)
+            double tot = 0;
+            for (int k = 0; k < ns; ++k) tot += fmax(ev[k], 0.0);
+            double cur = 0;
+            for (int k = ns; k >= 1; --k) {
+                cur += fmax(ev[k - 1], 0.0);
+                if (cur > thr * tot) { kp = k; break; }
+            }
+        } else {  // T-D (TEBD, src/DigitalTJM.jl:283-292)
+            double cur = 0;
+            for (int k = ns; k >= 1; --k) {
+                const double s2 = fmax(ev[k - 1], 0.0);
+                if (cur + s2 > thr) break;
+                cur += s2;
+                kp -= 1;
+            }
+        }
+        if (kp > max_bond) kp = max_bond;
+        if (kp < 1) kp = 1;
+        if (kp > ns) kp = ns;
+        keep[t] = kp;
+    }
+}
+
+void jacobi_eig(c64 *G, int64_t sG, c64 *U, int64_t sU, int n, int ns, double *evals, int *perm, int64_t sE, int *keep,
+                int rule, double thr, int max_bond, const int *mask, int B, hipStream_t s) {
+    const int half = ((n + 1) & ~1) / 2;
+    const size_t shmem = (size_t)half * 3 * sizeof(double) + (size_t)half * 2 * sizeof(int) + 16;
+    hipLaunchKernelGGL(jacobi_kernel, dim3(B), dim3(512), shmem, s, G, sG, U, sU, n, ns, evals, perm, sE, keep, rule, thr,
+                       max_bond, 30, mask);
+    YQ_HIP(hipGetLastError());
+}
+
+// Write the kept eigenvectors as an isometry in storage precision.
+//   left  (mode 0): Out[r][k] = U[r][perm[k]]        (r < n, k < kw), 0 for k >= keep
+//   right (mode 1): Out[k][r] = conj(U[r][perm[k]])  (k < kw, r < n), 0 for k >= keep
+template <typename T>
+__global__ void write_iso_kernel(const c64 *U, int64_t sU, const int *perm, int64_t sE, const int *keep, int n, int kw,
+                                 int mode, cplx<T> *Out, int64_t sO, const int *mask) {
+    const int t = blockIdx.y;
+    if (mask && !mask[t]) return;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * kw) return;
+    int r, k;
+    if (mode == 0) { r = (int)(e / kw); k = (int)(e % kw); }
+    else { k = (int)(e / n); r = (int)(e % n); }
+    double2 v = make_double2(0, 0);
+    if (k < keep[t]) {
+        const c64 u = U[t * sU + (int64_t)r * n + perm[t * sE + k]];
+        v = make_double2(u.re, mode == 0 ? u.im : -u.im);
+    }
+    st2(Out[t * sO + e], v);
+}
+
+template <typename T>
+void write_iso(const c64 *U, int64_t sU, const int *perm, int64_t sE, const int *keep, int n, int kw, int mode,
+               cplx<T> *Out, int64_t sO, const int *mask, int B, hipStream_t s) {
+    const int64_t tot = (int64_t)n * kw;
+    hipLaunchKernelGGL(write_iso_kernel<T>, dim3((unsigned)((tot + 255) / 256), B), dim3(256), 0, s, U, sU, perm, sE,
+                       keep, n, kw, mode, Out, sO, mask);
+    YQ_HIP(hipGetLastError());
+}
+template void write_iso<float>(const c64 *, int64_t, const int *, int64_t, const int *, int, int, int, c32 *, int64_t,
+                               const int *, int, hipStream_t);
+template void write_iso<double>(const c64 *, int64_t, const int *, int64_t, const int *, int, int, int, c64 *, int64_t,
+                                const int *, int, hipStream_t);
+
+// ------------------------------------------------- CGS2 orthonormalisation --
+// Orthonormalises nvec vectors v_k[e] = A[k*vs + e*es] (e < len) in order by
+// classical Gram-Schmidt applied twice (FP64 arithmetic), writing the
+// coefficients Cf[k][j] = <q_j, a_k> (j <= k, row-major nvec x nvec).  A
+// vector whose remaining norm is <= 1e-14 ‖A‖_F becomes exactly zero (a dead
+// direction, e.g. a zero-padded bond index).  Rows give LQ, columns QR.
+template <typename T>
+__global__ __launch_bounds__(256) void cgs2_kernel(cplx<T> *A, int64_t sA, int nvec, int len, int64_t vs, int64_t es,
+                                                   cplx<T> *Cf, int64_t sC, const int *mask) {
+    __shared__ double sh[16];
+    __shared__ double2 h[1024];
+    const int t = blockIdx.x;
+    if (mask && !mask[t]) return;
+    cplx<T> *M = A + t * sA;
+    cplx<T> *C = Cf + t * sC;
+    for (int64_t e = threadIdx.x; e < (int64_t)nvec * nvec; e += blockDim.x) C[e].re = C[e].im = (T)0;
+    double fro = 0;
+    for (int64_t e = threadIdx.x; e < (int64_t)nvec * len; e += blockDim.x) {
+        const int k = (int)(e / len), i = (int)(e % len);
+        double2 z = ld2(M[k * vs + i * es]);
+        fro += z.x * z.x + z.y * z.y;
+    }
+    fro = sqrt(block_sum(fro, sh));
+    const double dead = 1e-14 * fro;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int k = 0; k < nvec; ++k) {
+        for (int pass = 0; pass < 2; ++pass) {
+            if (k > 0) {
+                for (int j = wid; j < k; j += nw) {
+                    double2 acc = make_double2(0, 0);
+                    for (int i = lane; i < len; i += 64)
+                        acc = cadd(acc, cmulc(ld2(M[j * vs + i * es]), ld2(M[k * vs + i * es])));
+                    acc.x = wave_sum(acc.x);
+                    acc.y = wave_sum(acc.y);
+                    if (lane == 0) h[j] = acc;
+                }
+                __syncthreads();
+                for (int i = threadIdx.x; i < len; i += blockDim.x) {
+                    double2 v = ld2(M[k * vs + i * es]);
+                    for (int j = 0; j < k; ++j) v = csub(v, cmul(h[j], ld2(M[j * vs + i * es])));
+                    st2(M[k * vs + i * es], v);
+                }
+                for (int j = threadIdx.x; j < k; j += blockDim.x) {
+                    double2 c = ld2(C[(int64_t)k * nvec + j]);
+                    st2(C[(int64_t)k * nvec + j], cadd(c, h[j]));
+                }
+                __syncthreads();
+            }
+        }
+        double nr = 0;
+        for (int i = threadIdx.x; i < len; i += blockDim.x) {
+            double2 z = ld2(M[k * vs + i * es]);
+            nr += z.x * z.x + z.y * z.y;
+        }
+        nr = sqrt(block_sum(nr, sh));
+        const bool live = nr > dead && nr > 0.0;
+        const double inv = live ? 1.0 / nr : 0.0;
+        for (int i = threadIdx.x; i < len; i += blockDim.x) st2(M[k * vs + i * es], cscale(ld2(M[k * vs + i * es]), inv));
+        if (threadIdx.x == 0) st2(C[(int64_t)k * nvec + k], make_double2(live ? nr : 0.0, 0.0));
+        __syncthreads();
+    }
+}
+
+template <typename T>
+void cgs2(cplx<T> *A, int64_t sA, int nvec, int len, int64_t vs, int64_t es, cplx<T> *Cf, int64_t sC, const int *mask,
+          int B, hipStream_t s) {
+    if (nvec > 1024) throw HipError("cgs2: more than 1024 vectors");
+    hipLaunchKernelGGL(cgs2_kernel<T>, dim3(B), dim3(256), 0, s, A, sA, nvec, len, vs, es, Cf, sC, mask);
+    YQ_HIP(hipGetLastError());
+}
+template void cgs2<float>(c32 *, int64_t, int, int, int64_t, int64_t, c32 *, int64_t, const int *, int, hipStream_t);
+template void cgs2<double>(c64 *, int64_t, int, int, int64_t, int64_t, c64 *, int64_t, const int *, int, hipStream_t);
+
+// ------------------------------------------------------- local operators --
+// A[a][s][b] <- Σ_s' M[s][s'] A[a][s'][b]; M given per trajectory through
+// op_idx (index into ops, -1 = skip) or shared (op_idx == nullptr -> ops[0]).
+template <typename T>
+__global__ void local_op_kernel(cplx<T> *A, int64_t sA, int dl, int dr, const c64 *ops, const int *op_idx,
+                                const int *mask) {
+    const int t = blockIdx.y;
+    if (mask && !mask[t]) return;
+    int oi = 0;
+    if (op_idx) {
+        oi = op_idx[t];
+        if (oi < 0) return;
+    }
+    const c64 *M = ops + (int64_t)oi * 4;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)dl * dr) return;
+    const int a = (int)(e / dr), b = (int)(e % dr);
+    cplx<T> *p0 = A + t * sA + ((int64_t)a * 2 + 0) * dr + b;
+    cplx<T> *p1 = A + t * sA + ((int64_t)a * 2 + 1) * dr + b;
+    const double2 x0 = ld2(*p0), x1 = ld2(*p1);
+    const double2 m00 = make_double2(M[0].re, M[0].im), m01 = make_double2(M[1].re, M[1].im);
+    const double2 m10 = make_double2(M[2].re, M[2].im), m11 = make_double2(M[3].re, M[3].im);
+    st2(*p0, cadd(cmul(m00, x0), cmul(m01, x1)));
+    st2(*p1, cadd(cmul(m10, x0), cmul(m11, x1)));
+}
+
+template <typename T>
+void local_op(cplx<T> *A, int64_t sA, int dl, int dr, const c64 *ops, const int *op_idx, const int *mask, int B,
+              hipStream_t s) {
+    const int64_t tot = (int64_t)dl * dr;
+    hipLaunchKernelGGL(local_op_kernel<T>, dim3((unsigned)((tot + 255) / 256), B), dim3(256), 0, s, A, sA, dl, dr, ops,
+                       op_idx, mask);
+    YQ_HIP(hipGetLastError());
+}
+template void local_op<float>(c32 *, int64_t, int, int, const c64 *, const int *, const int *, int, hipStream_t);
+template void local_op<double>(c64 *, int64_t, int, int, const c64 *, const int *, const int *, int, hipStream_t);
+
+// Θ[a][p][b] <- Σ_p' M[p][p'] Θ[a][p'][b] for a 4x4 M on the combined
+// physical index p = 2 s1 + s2.
+template <typename T>
+__global__ void local_op4_kernel(cplx<T> *A, int64_t sA, int dl, int dr, const c64 *M) {
+    const int t = blockIdx.y;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)dl * dr) return;
+    const int a = (int)(e / dr), b = (int)(e % dr);
+    double2 x[4];
+    for (int p = 0; p < 4; ++p) x[p] = ld2(A[t * sA + ((int64_t)a * 4 + p) * dr + b]);
+    for (int p = 0; p < 4; ++p) {
+        double2 acc = make_double2(0, 0);
+        for (int q = 0; q < 4; ++q) acc = cadd(acc, cmul(make_double2(M[p * 4 + q].re, M[p * 4 + q].im), x[q]));
+        st2(A[t * sA + ((int64_t)a * 4 + p) * dr + b], acc);
+    }
+}
+template <typename T>
+void local_op4(cplx<T> *A, int64_t sA, int dl, int dr, const c64 *M, int B, hipStream_t s) {
+    const int64_t tot = (int64_t)dl * dr;
+    hipLaunchKernelGGL(local_op4_kernel<T>, dim3((unsigned)((tot + 255) / 256), B), dim3(256), 0, s, A, sA, dl, dr, M);
+    YQ_HIP(hipGetLastError());
+}
+template void local_op4<float>(c32 *, int64_t, int, int, const c64 *, int, hipStream_t);
+template void local_op4<double>(c64 *, int64_t, int, int, const c64 *, int, hipStream_t);
+
+// ‖A_t‖² in FP64 (one workgroup per trajectory).
+template <typename T>
+__global__ __launch_bounds__(256) void norm2_kernel(const cplx<T> *A, int64_t sA, int64_t n, double *out, const int *mask) {
+    __shared__ double sh[16];
+    const int t = blockIdx.x;
+    if (mask && !mask[t]) return;
+    double s = 0;
+    for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+        const double2 z = ld2(A[t * sA + e]);
+        s += z.x * z.x + z.y * z.y;
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[t] = s;
+}
+template <typename T>
+void norm2(const cplx<T> *A, int64_t sA, int64_t n, double *out, const int *mask, int B, hipStream_t s) {
+    hipLaunchKernelGGL(norm2_kernel<T>, dim3(B), dim3(256), 0, s, A, sA, n, out, mask);
+    YQ_HIP(hipGetLastError());
+}
+template void norm2<float>(const c32 *, int64_t, int64_t, double *, const int *, int, hipStream_t);
+template void norm2<double>(const c64 *, int64_t, int64_t, double *, const int *, int, hipStream_t);
+
+template <typename T>
+__global__ void scale_kernel(cplx<T> *A, int64_t sA, int64_t n, const double *f, const int *mask) {
+    const int t = blockIdx.y;
+    if (mask && !mask[t]) return;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    st2(A[t * sA + e], cscale(ld2(A[t * sA + e]), f[t]));
+}
+template <typename T>
+void scale(cplx<T> *A, int64_t sA, int64_t n, const double *f, const int *mask, int B, hipStream_t s) {
+    hipLaunchKernelGGL(scale_kernel<T>, dim3((unsigned)((n + 255) / 256), B), dim3(256), 0, s, A, sA, n, f, mask);
+    YQ_HIP(hipGetLastError());
+}
+template void scale<float>(c32 *, int64_t, int64_t, const double *, const int *, int, hipStream_t);
+template void scale<double>(c64 *, int64_t, int64_t, const double *, const int *, int, hipStream_t);
+
+// Strided 2D copy with precision conversion: Dst[t][r][c] = Src[t][r][c].
+template <typename TS, typename TD>
+__global__ void copy2d_kernel(const cplx<TS> *S, int64_t sS, int64_t lds, cplx<TD> *D, int64_t sD, int64_t ldd, int rows,
+                              int cols, const int *mask) {
+    const int t = blockIdx.y;
+    if (mask && !mask[t]) return;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)rows * cols) return;
+    const int r = (int)(e / cols), c = (int)(e % cols);
+    st2(D[t * sD + r * ldd + c], ld2(S[t * sS + r * lds + c]));
+}
+template <typename TS, typename TD>
+void copy2d(const cplx<TS> *S, int64_t sS, int64_t lds, cplx<TD> *D, int64_t sD, int64_t ldd, int rows, int cols,
+            const int *mask, int B, hipStream_t s) {
+    const int64_t tot = (int64_t)rows * cols;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL((copy2d_kernel<TS, TD>), dim3((unsigned)((tot + 255) / 256), B), dim3(256), 0, s, S, sS, lds, D, sD,
+                       ldd, rows, cols, mask);
+    YQ_HIP(hipGetLastError());
+}
+template void copy2d<float, float>(const c32 *, int64_t, int64_t, c32 *, int64_t, int64_t, int, int, const int *, int,
+                                   hipStream_t);
+template void copy2d<double, double>(const c64 *, int64_t, int64_t, c64 *, int64_t, int64_t, int, int, const int *, int,
+                                     hipStream_t);
+template void copy2d<float, double>(const c32 *, int64_t, int64_t, c64 *, int64_t, int64_t, int, int, const int *, int,
+                                    hipStream_t);
+template void copy2d<double, float>(const c64 *, int64_t, int64_t, c32 *, int64_t, int64_t, int, int, const int *, int,
+                                    hipStream_t);
+
+// Fill with zero / identity-environment patterns.
+template <typename T>
+__global__ void fill_kernel(cplx<T> *A, int64_t sA, int64_t n) {
+    const int t = blockIdx.y;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) A[t * sA + e].re = A[t * sA + e].im = (T)0;
+}
+template <typename T>
+void fill_zero(cplx<T> *A, int64_t sA, int64_t n, int B, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(fill_kernel<T>, dim3((unsigned)((n + 255) / 256), B), dim3(256), 0, s, A, sA, n);
+    YQ_HIP(hipGetLastError());
+}
+template void fill_zero<float>(c32 *, int64_t, int64_t, int, hipStream_t);
+template void fill_zero<double>(c64 *, int64_t, int64_t, int, hipStream_t);
+
+// E[i][m0][i] = 1 for i < dim (left env [bra][mpo][ket] or right env
+// [mpo][ket][bra] with the identity on mpo index 0); the rest zero.
+template <typename T>
+__global__ void identity_env_kernel(cplx<T> *E, int64_t sE, int dim, int D, int right) {
+    const int t = blockIdx.y;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t tot = (int64_t)dim * D * dim;
+    if (e >= tot) return;
+    int i, m, k;
+    if (!right) { i = (int)(e / ((int64_t)D * dim)); m = (int)((e / dim) % D); k = (int)(e % dim); }
+    else { m = (int)(e / ((int64_t)dim * dim)); i = (int)((e / dim) % dim); k = (int)(e % dim); }
+    E[t * sE + e].re = (i == k && m == 0) ? (T)1 : (T)0;
+    E[t * sE + e].im = (T)0;
+}
+template <typename T>
+void identity_env(cplx<T> *E, int64_t sE, int dim, int D, int right, int B, hipStream_t s) {
+    const int64_t tot = (int64_t)dim * D * dim;
+    hipLaunchKernelGGL(identity_env_kernel<T>, dim3((unsigned)((tot + 255) / 256), B), dim3(256), 0, s, E, sE, dim, D,
+                       right);
+    YQ_HIP(hipGetLastError());
+}
+template void identity_env<float>(c32 *, int64_t, int, int, int, int, hipStream_t);
+template void identity_env<double>(c64 *, int64_t, int, int, int, int, hipStream_t);
+
+// ρ[t][s][s'] = Σ_{a',b} X[a'][s][b] conj(A[a'][s'][b]) (site reduced matrix,
+// d = 2 or 4), one workgroup per trajectory.
+template <typename T>
+__global__ __launch_bounds__(256) void rho_kernel(const c64 *X, int64_t sX, const cplx<T> *A, int64_t sA, int dl, int d,
+                                                  int dr, c64 *rho, int64_t sR, const int *mask) {
+    __shared__ double sh[16];
+    const int t = blockIdx.x;
+    if (mask && !mask[t]) return;
+    for (int s = 0; s < d; ++s)
+        for (int s2 = 0; s2 < d; ++s2) {
+            double2 acc = make_double2(0, 0);
+            for (int64_t e = threadIdx.x; e < (int64_t)dl * dr; e += blockDim.x) {
+                const int a = (int)(e / dr), b = (int)(e % dr);
+                const c64 x = X[t * sX + ((int64_t)a * d + s) * dr + b];
+                const double2 y = ld2(A[t * sA + ((int64_t)a * d + s2) * dr + b]);
+                acc = cadd(acc, cmulc(y, make_double2(x.re, x.im)));
+            }
+            acc = block_sum2(acc, sh);
+            if (threadIdx.x == 0) {
+                c64 o;
+                o.re = acc.x;
+                o.im = acc.y;
+                rho[t * sR + s * d + s2] = o;
+            }
+        }
+}
+template <typename T>
+void rho(const c64 *X, int64_t sX, const cplx<T> *A, int64_t sA, int dl, int d, int dr, c64 *out, int64_t sR,
+         const int *mask, int B, hipStream_t s) {
+    hipLaunchKernelGGL(rho_kernel<T>, dim3(B), dim3(256), 0, s, X, sX, A, sA, dl, d, dr, out, sR, mask);
+    YQ_HIP(hipGetLastError());
+}
+template void rho<float>(const c64 *, int64_t, const c32 *, int64_t, int, int, int, c64 *, int64_t, const int *, int,
+                         hipStream_t);
+template void rho<double>(const c64 *, int64_t, const c64 *, int64_t, int, int, int, c64 *, int64_t, const int *, int,
+                          hipStream_t);
+
+// -------------------------------------------------------------- RNG + jumps
+__device__ __forceinline__ void philox_round(uint32_t c[4], const uint32_t k[2]) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    const uint32_t o0 = hi1 ^ c[1] ^ k[0], o2 = hi0 ^ c[3] ^ k[1];
+    c[0] = o0;
+    c[1] = lo1;
+    c[2] = o2;
+    c[3] = lo0;
+}
+
+__device__ double philox_uniform_dev(uint64_t seed, uint64_t traj, uint64_t kk) {
+    uint32_t c[4] = {(uint32_t)kk, (uint32_t)(kk >> 32), (uint32_t)traj, (uint32_t)(traj >> 32)};
+    uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int r = 0; r < 10; ++r) {
+        philox_round(c, k);
+        k[0] += 0x9E3779B9u;
+        k[1] += 0xBB67AE85u;
+    }
+    const uint64_t bits = ((uint64_t)(c[0] >> 5) << 26) | (uint64_t)(c[1] >> 6);
+    return (double)bits * (1.0 / 9007199254740992.0);
+}
+
+// Jump decision (src/StochasticProcess.jl:219-229): dp = 1 - ‖ψ‖², r1 draw.
+// state[t]: 0 = no jump (scale factor written), 1 = jump candidate.
+__global__ void jump_decide_kernel(const double *norm2, uint64_t seed, uint64_t traj0, uint64_t *rng_k, int has_procs,
+                                   int *state, double *fac, int B) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B) return;
+    const double n2 = norm2[t];
+    const double dp = 1.0 - n2;
+    bool jump = false;
+    if (has_procs) {
+        const double r1 = philox_uniform_dev(seed, traj0 + t, rng_k[t]);
+        rng_k[t] += 1;
+        jump = !(r1 >= dp);
+    }
+    state[t] = jump ? 1 : 0;
+    const double nrm = sqrt(n2);
+    fac[t] = nrm > 1e-13 ? 1.0 / nrm : 1.0;
+}
+
+// Jump choice (src/StochasticProcess.jl:89-212, 233-256) from the site
+// reduced matrices ρ_s of the unnormalised state.  cand_* list the
+// candidates in the reference's order; weight = dt γ Tr(L†L ρ_site).
+__global__ void jump_choose_kernel(const c64 *rho, int64_t sR, int L, int ncand, const int *cand_site,
+                                   const c64 *cand_LdL, const double *cand_gamma, double dt, uint64_t seed, uint64_t traj0,
+                                   uint64_t *rng_k, int *state, int *chosen, double *wbuf, int B) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B || state[t] != 1) return;
+    double tot = 0;
+    double *w = wbuf + (int64_t)t * ncand;
+    for (int c = 0; c < ncand; ++c) {
+        const c64 *r = rho + t * sR + (int64_t)cand_site[c] * 4;
+        const c64 *M = cand_LdL + (int64_t)c * 4;
+        // Tr(M ρ) = Σ_{s,s'} M[s'][s] ρ[s][s']
+        double v = 0;
+        for (int s = 0; s < 2; ++s)
+            for (int s2 = 0; s2 < 2; ++s2) {
+                const c64 m = M[s2 * 2 + s], p = r[s * 2 + s2];
+                v += m.re * p.re - m.im * p.im;
+            }
+        w[c] = dt * cand_gamma[c] * v;
+        tot += w[c];
+    }
+    if (tot == 0.0) {  // empty distribution: normalise only
+        state[t] = 2;
+        chosen[t] = -1;
+        return;
+    }
+    const double r2 = philox_uniform_dev(seed, traj0 + t, rng_k[t]);
+    rng_k[t] += 1;
+    double acc = 0;
+    int ch = 0;
+    for (int c = 0; c < ncand; ++c) {
+        acc += w[c] / tot;
+        if (r2 <= acc) { ch = c; break; }
+    }
+    chosen[t] = ch;
+}
+
+void jump_decide(const double *norm2v, uint64_t seed, uint64_t traj0, uint64_t *rng_k, int has_procs, int *state,
+                 double *fac, int B, hipStream_t s) {
+    hipLaunchKernelGGL(jump_decide_kernel, dim3((B + 255) / 256), dim3(256), 0, s, norm2v, seed, traj0, rng_k, has_procs,
+                       state, fac, B);
+    YQ_HIP(hipGetLastError());
+}
+void jump_choose(const c64 *rhov, int64_t sR, int L, int ncand, const int *cand_site, const c64 *cand_LdL,
+                 const double *cand_gamma, double dt, uint64_t seed, uint64_t traj0, uint64_t *rng_k, int *state,
+                 int *chosen, double *wbuf, int B, hipStream_t s) {
+    hipLaunchKernelGGL(jump_choose_kernel, dim3((B + 255) / 256), dim3(256), 0, s, rhov, sR, L, ncand, cand_site, cand_LdL,
+                       cand_gamma, dt, seed, traj0, rng_k, state, chosen, wbuf, B);
+    YQ_HIP(hipGetLastError());
+}
+
+// Per-site op index for the chosen jumps: opidx[t] = op of candidate
+// chosen[t] if it acts on `site` (first or second factor), else -1.
+__global__ void jump_opidx_kernel(const int *state, const int *chosen, const int *cand_site, const int *cand_op,
+                                  const int *cand_site2, const int *cand_op2, int site, int *opidx, int B) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B) return;
+    int o = -1;
+    if (state[t] == 1 && chosen[t] >= 0) {
+        const int c = chosen[t];
+        if (cand_site[c] == site) o = cand_op[c];
+        else if (cand_site2[c] == site) o = cand_op2[c];
+    }
+    opidx[t] = o;
+}
+void jump_opidx(const int *state, const int *chosen, const int *cand_site, const int *cand_op, const int *cand_site2,
+                const int *cand_op2, int site, int *opidx, int B, hipStream_t s) {
+    hipLaunchKernelGGL(jump_opidx_kernel, dim3((B + 255) / 256), dim3(256), 0, s, state, chosen, cand_site, cand_op,
+                       cand_site2, cand_op2, site, opidx, B);
+    YQ_HIP(hipGetLastError());
+}
+
+// mask[t] = (state[t] == v) or, with v < 0, state[t] != 0.
+__global__ void mask_from_state_kernel(const int *state, int v, int *mask, int *count, int B) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B) return;
+    const int m = (v >= 0) ? (state[t] == v) : (state[t] != 0);
+    mask[t] = m;
+    if (m && count) atomicAdd(count, 1);
+}
+void mask_from_state(const int *state, int v, int *mask, int *count, int B, hipStream_t s) {
+    hipLaunchKernelGGL(mask_from_state_kernel, dim3((B + 255) / 256), dim3(256), 0, s, state, v, mask, count, B);
+    YQ_HIP(hipGetLastError());
+}
+
+// Observables: out[q][col][t0+t] = Re Tr(O_q ρ_{site_q}) (2x2 or 4x4).
+__global__ void observe_kernel(const c64 *rho1, int64_t sR1, const c64 *rho2, int64_t sR2, int nobs, const int *obs_site,
+                               const int *obs_n, const c64 *obs_op, int col, int ncol, int64_t ntraj, int64_t t0,
+                               double *out, int B) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B) return;
+    for (int q = 0; q < nobs; ++q) {
+        const int d = obs_n[q] == 1 ? 2 : 4;
+        const c64 *r = (d == 2) ? rho1 + t * sR1 + (int64_t)obs_site[q] * 4 : rho2 + t * sR2 + (int64_t)obs_site[q] * 16;
+        const c64 *O = obs_op + (int64_t)q * 16;
+        double v = 0;
+        for (int s = 0; s < d; ++s)
+            for (int s2 = 0; s2 < d; ++s2) {
+                const c64 o = O[s2 * d + s], p = r[s * d + s2];
+                v += o.re * p.re - o.im * p.im;
+            }
+        out[((int64_t)q * ncol + col) * ntraj + t0 + t] = v;
+    }
+}
+void observe(const c64 *rho1, int64_t sR1, const c64 *rho2, int64_t sR2, int nobs, const int *obs_site, const int *obs_n,
+             const c64 *obs_op, int col, int ncol, int64_t ntraj, int64_t t0, double *out, int B, hipStream_t s) {
+    hipLaunchKernelGGL(observe_kernel, dim3((B + 255) / 256), dim3(256), 0, s, rho1, sR1, rho2, sR2, nobs, obs_site, obs_n,
+                       obs_op, col, ncol, ntraj, t0, out, B);
+    YQ_HIP(hipGetLastError());
+}
+
+// dims[t][b] bookkeeping of each trajectory's true bond dimension.
+__global__ void set_dims_kernel(int *dims, int ld, int b, const int *keep, int val, const int *mask, int B) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B) return;
+    if (mask && !mask[t]) return;
+    dims[(int64_t)t * ld + b] = keep ? keep[t] : val;
+}
+void set_dims(int *dims, int ld, int b, const int *keep, int val, const int *mask, int B, hipStream_t s) {
+    hipLaunchKernelGGL(set_dims_kernel, dim3((B + 255) / 256), dim3(256), 0, s, dims, ld, b, keep, val, mask, B);
+    YQ_HIP(hipGetLastError());
+}
+
+__global__ void fill_int_kernel(int *p, int v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+void fill_int(int *p, int v, int n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
+    YQ_HIP(hipGetLastError());
+}
+
+}  // namespace yq
