@@ -1,0 +1,171 @@
+"""Throughput benchmark of the batched TJM engine on one MI355X.
+
+Workload (the headline configuration): Heisenberg XXX chain L=30 from the
+Neel state, lowering + pauli_z noise gamma=0.01 on every site, 2-site TDVP
+dt=0.05, chi_max=64, threshold 1e-9, Krylov dim <= 25, observables Z_i and
+X_i, 1000 trajectories, complex-FP32 contractions with FP64 Lanczos / Gram /
+norms ("mixed").  One step = one TJM time step of every trajectory (the
+step_through and sample of analog_tjm_2: two TDVP sweeps, dissipation,
+jumps, 60 observables).  The job starts at t=0; the first W steps are the
+untimed warm-up, the next K steps are timed (stream-synchronised at both
+ends).  Metric: trajectory-site-steps per second = ntraj * L * K / time.
+
+Also reported on the same window: the plain per-trajectory FP64 GPU path
+(one trajectory per batch, on a trajectory subset) and the FP64 CPU oracle
+(on a bounded trajectory subset, OpenMP, one trajectory per thread), plus the
+roofline of the dominant kernel (the complex-FP32 MFMA GEMM) from HIP events
+around every launch in the window.
+"""
+import argparse
+import ctypes as ct
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "yaqs-julia_amd")
+FP32_MFMA_PEAK = 157.3e12  # MI355X dense FP32 matrix peak, FLOP/s
+
+
+def _pkg():
+    if "yaqs_julia_amd" in sys.modules:
+        return sys.modules["yaqs_julia_amd"]
+    spec = importlib.util.spec_from_file_location("yaqs_julia_amd", os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["yaqs_julia_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def c2_job(Y, ntraj, nsteps, precision, L=30, chi=64):
+    obs = [Y.Observable(f"z{i}", Y.gates.ZGate(), i) for i in range(1, L + 1)] + \
+          [Y.Observable(f"x{i}", Y.gates.XGate(), i) for i in range(1, L + 1)]
+    cfg = Y.TimeEvolutionConfig(obs, nsteps * 0.05, dt=0.05, num_traj=ntraj, max_bond_dim=chi,
+                                truncation_threshold=1e-9)
+    assert len(cfg.times) == nsteps + 1
+    H = Y.init_general_hamiltonian(L, 1, 1, 1, 0, 0, 0)
+    noise = Y.relaxation_dephasing(L, 0.01)
+    mps = Y.MPS(L, state="Neel")
+    mps.normalize()
+    prec = Y._abi.PREC_MIXED if precision == "mixed" else Y._abi.PREC_FP64
+    return Y._abi.pack_analog(mps, H, cfg, noise, precision=prec, seed=0), cfg
+
+
+def gpu_window(Y, lib, ctx, ntraj, warmup, steps, precision):
+    h, cfg = c2_job(Y, ntraj, warmup + steps, precision)
+    nobs, ncol = len(cfg.observables), len(cfg.times)
+    traj = np.zeros(nobs * ncol * ntraj)
+    mean = np.zeros(nobs * ncol)
+    rep = Y._abi.BenchReport()
+    rc = lib.yaqs_bench_analog(ctx.handle, ct.byref(h.job), warmup, traj.ctypes.data_as(Y._abi.c_dp),
+                               mean.ctypes.data_as(Y._abi.c_dp), ct.byref(rep))
+    if rc != 0:
+        raise RuntimeError(f"yaqs_bench_analog failed ({rc}): {lib.yaqs_last_error().decode()}")
+    return rep.as_dict(), mean.reshape(nobs, ncol)
+
+
+def cpu_window(Y, ntraj, warmup, steps, nthreads):
+    """FP64 oracle on the same job restricted to `ntraj` trajectories: time of
+    the first warmup+steps steps minus the first warmup steps."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    O.lib()
+    times = []
+    for n in (warmup, warmup + steps):
+        if n == 0:
+            times.append(0.0)
+            continue
+        h, _ = c2_job(Y, ntraj, n, "fp64")
+        t0 = time.perf_counter()
+        O.run_analog(h, nthreads=nthreads)
+        times.append(time.perf_counter() - t0)
+    return times[1] - times[0]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--ntraj", type=int, default=1000)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--plain-traj", type=int, default=2, help="trajectories for the plain FP64 per-trajectory path")
+    ap.add_argument("--cpu-traj", type=int, default=0, help="oracle trajectories (0 = host threads, max 16)")
+    ap.add_argument("--cpu-steps", type=int, default=-1, help="oracle timed steps (-1 = same as --steps)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); trajectories shard across GPUs "
+              "with a single reduction at the end, which this single-GPU build does not launch.", file=sys.stderr)
+        sys.exit(2)
+    Y = _pkg()
+    lib = Y._abi.load_library()
+    ctx = Y.Context(0)
+    print(f"[bench] C2 window: warmup={args.warmup} steps={args.steps} ntraj={args.ntraj} {args.precision}",
+          flush=True)
+    rep, mean = gpu_window(Y, lib, ctx, args.ntraj, args.warmup, args.steps, args.precision)
+    L = 30
+    value = args.ntraj * L * rep["timed_steps"] / rep["timed_s"]
+    print(f"[bench] GPU window {rep['timed_s']:.2f}s -> {value:.4g} traj-site-steps/s, max bond {rep['max_bond']}",
+          flush=True)
+    achieved = rep["gemm_flop"] / (rep["gemm_ms"] * 1e-3) if rep["gemm_ms"] > 0 else 0.0
+    out = {
+        "metric": "traj-site-steps/s (L=30, chi=64, <Z> err<=1e-4) vs plain GPU; p50/p99 job latency",
+        "value": value,
+        "unit": "traj-site-steps/s",
+        "n_gpus": 1,
+        "steps": int(rep["timed_steps"]),
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * rep["timed_s"] / rep["timed_steps"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 contractions + f64 Lanczos/Gram/norms" if args.precision == "mixed" else "c128",
+        "data": "synthetic (Neel initial state, seeded jumps)",
+        "config": {
+            "workload": "C2: Heisenberg XXX L=30 Neel, lowering+pauli_z gamma=0.01, 2-site TDVP dt=0.05, "
+                        "chi_max=64, thr 1e-9, Krylov<=25, 60 observables",
+            "ntraj": args.ntraj, "L": L, "chi_max": 64, "precision": args.precision,
+            "window_t": [args.warmup * 0.05, (args.warmup + args.steps) * 0.05],
+            "max_bond_reached": int(rep["max_bond"]),
+            "flop_model_per_traj_site_step": rep["flop_model"] / (args.ntraj * L * rep["timed_steps"]),
+            "matvecs": int(rep["matvecs"]),
+        },
+        "roofline": {
+            "bound": "mfma", "kernel": "gemm_mfma_kernel<float> (complex-FP32 real-split MFMA)",
+            "achieved": achieved / 1e12, "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / FP32_MFMA_PEAK, "traffic": None,
+            "launches": int(rep["gemm_launches"]),
+            "avg_launch_us": 1e3 * rep["gemm_ms"] / max(1, rep["gemm_launches"]),
+            "gemm_share_of_window": rep["gemm_ms"] * 1e-3 / rep["timed_s"],
+        },
+    }
+    if not args.no_plain and args.plain_traj > 0:
+        pctx = Y.Context(0, max_batch=1)
+        prep, _ = gpu_window(Y, lib, pctx, args.plain_traj, args.warmup, args.steps, "fp64")
+        pctx.close()
+        pv = args.plain_traj * L * prep["timed_steps"] / prep["timed_s"]
+        out["plain_fp64_gpu"] = {"value": pv, "unit": "traj-site-steps/s", "ntraj": args.plain_traj,
+                                 "batch": 1, "speedup_of_batched": value / pv}
+        print(f"[bench] plain FP64 per-trajectory GPU: {pv:.4g} traj-site-steps/s", flush=True)
+    if not args.no_cpu:
+        nthreads = args.cpu_traj or min(16, os.cpu_count() or 1)
+        csteps = args.steps if args.cpu_steps < 0 else args.cpu_steps
+        t = cpu_window(Y, nthreads, args.warmup, csteps, nthreads)
+        cv = nthreads * L * csteps / t
+        out["cpu_baseline"] = {"value": cv, "unit": "traj-site-steps/s", "cores": nthreads, "kind": "port",
+                               "sample": f"FP64 oracle, {nthreads} trajectories (one per thread), steps "
+                                         f"{args.warmup + 1}..{args.warmup + csteps} of the same job"}
+        print(f"[bench] CPU oracle: {cv:.4g} traj-site-steps/s on {nthreads} threads", flush=True)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -220,6 +220,26 @@ int64_t yaqs_digital_num_samples(const yaqs_digital_job *job);
 /* Projected device footprint of a job in bytes (the admission model). */
 int64_t yaqs_job_footprint_bytes(const yaqs_analog_job *job);
 
+/* Measurement entry point (no counterpart in the reference, whose timing is
+ * stdout-only, experiments/benchmark_julia.jl:66-69): runs an analog job
+ * synchronously on the calling thread, treats its first `warmup_steps` time
+ * steps as untimed warm-up and reports the stream-synchronised wall time of
+ * the remaining steps, plus HIP-event timing of every matrix-core GEMM
+ * launched inside that window. */
+typedef struct {
+    double timed_s;          /* wall time of the timed steps                      */
+    int64_t timed_steps;     /* time steps inside the window                      */
+    double gemm_ms;          /* summed device time of the MFMA GEMM launches      */
+    int64_t gemm_launches;
+    double gemm_flop;        /* 8 FLOP per complex MAC of those launches          */
+    double flop_model;       /* reference-order H_eff FLOP in the window          */
+    int64_t matvecs;         /* H_eff applies in the window                       */
+    int64_t max_bond;
+} yaqs_bench_report;
+
+int yaqs_bench_analog(yaqs_ctx *ctx, const yaqs_analog_job *job, int64_t warmup_steps, double *traj_out,
+                      double *mean_out, yaqs_bench_report *report);
+
 /* Counter-based RNG of the trajectory seeding contract: draw k of global
  * trajectory t under `seed` is a pure function of (seed, t, k).  The
  * reference draws Random.default_rng() at src/StochasticProcess.jl:221-244;

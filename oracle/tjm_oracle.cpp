@@ -1059,6 +1059,12 @@ int stochastic_process(MPSState &s, const std::vector<Proc> &noise, double dt, d
     }
     const Candidate &c = cands[chosen];
     const Proc &p = noise[c.proc];
+    if (std::getenv("ORACLE_DEBUG_JUMPS")) {
+        fprintf(stderr, "oracle jump traj=%llu k=%llu cand=%zu site=%d dp=%.17g r2=%.17g probs:",
+                (unsigned long long)rng.traj, (unsigned long long)rng.k, chosen, c.site, dp, r);
+        for (double x : probs) fprintf(stderr, " %.15g", x);
+        fprintf(stderr, "\n");
+    }
     if (c.type == 1) {
         shift_oc(s, c.site);
         apply_1site_op(s.t[c.site - 1], p.m);

@@ -77,6 +77,15 @@ class JobStats(ct.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class BenchReport(ct.Structure):
+    _fields_ = [("timed_s", ct.c_double), ("timed_steps", ct.c_int64), ("gemm_ms", ct.c_double),
+                ("gemm_launches", ct.c_int64), ("gemm_flop", ct.c_double), ("flop_model", ct.c_double),
+                ("matvecs", ct.c_int64), ("max_bond", ct.c_int64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class CtxOpts(ct.Structure):
     _fields_ = [("hbm_budget_bytes", ct.c_int64), ("max_batch", ct.c_int32), ("verbose", ct.c_int32)]
 
@@ -247,6 +256,8 @@ def load_library(path: Optional[str] = None):
     lib.yaqs_digital_num_samples.restype = ct.c_int64
     lib.yaqs_job_footprint_bytes.argtypes = [ct.POINTER(AnalogJob)]
     lib.yaqs_job_footprint_bytes.restype = ct.c_int64
+    lib.yaqs_bench_analog.argtypes = [ct.c_void_p, ct.POINTER(AnalogJob), ct.c_int64, c_dp, c_dp,
+                                      ct.POINTER(BenchReport)]
     lib.yaqs_rng_uniform.argtypes = [ct.c_uint64, ct.c_uint64, ct.c_uint64]
     lib.yaqs_rng_uniform.restype = ct.c_double
     if path is None:
@@ -257,5 +268,6 @@ def load_library(path: Optional[str] = None):
 EXPORTED_SYMBOLS = [
     "yaqs_abi_version", "yaqs_last_error", "yaqs_create", "yaqs_destroy", "yaqs_submit_analog",
     "yaqs_submit_digital", "yaqs_wait", "yaqs_poll", "yaqs_run_analog", "yaqs_run_digital",
-    "yaqs_analog_num_samples", "yaqs_digital_num_samples", "yaqs_job_footprint_bytes", "yaqs_rng_uniform",
+    "yaqs_analog_num_samples", "yaqs_digital_num_samples", "yaqs_job_footprint_bytes", "yaqs_bench_analog",
+    "yaqs_rng_uniform",
 ]

@@ -438,4 +438,50 @@ int64_t yaqs_job_footprint_bytes(const yaqs_analog_job *job) {
 
 double yaqs_rng_uniform(uint64_t seed, uint64_t traj, uint64_t k) { return yq::rng_uniform_host(seed, traj, k); }
 
+int yaqs_bench_analog(yaqs_ctx *ctx, const yaqs_analog_job *job, int64_t warmup_steps, double *traj_out,
+                      double *mean_out, yaqs_bench_report *report) {
+    if (!ctx) return fail(YAQS_EINVAL, "ctx is NULL");
+    hipStream_t stream = nullptr;
+    try {
+        yq::JobIn in = convert_analog(job);
+        if (warmup_steps < 0 || warmup_steps + 2 > in.num_times)
+            throw std::invalid_argument("warmup_steps must leave at least one timed step");
+        in.bench_warmup = warmup_steps;
+        yq::validate_job(in);
+        int64_t bytes = 0;
+        const int64_t b = choose_batch(ctx, in, ctx->budget, &bytes);
+        if (b < 1) return fail(YAQS_ENOMEM_DEVICE, "job footprint exceeds the HBM budget");
+        if (hipSetDevice(ctx->device) != hipSuccess) return fail(YAQS_EDEVICE, "hipSetDevice failed");
+        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
+            return fail(YAQS_EDEVICE, "hipStreamCreate failed");
+        yq::JobResult r;
+        yq::run_job(in, b, stream, r);
+        (void)hipStreamDestroy(stream);
+        stream = nullptr;
+        const int64_t nobs = (int64_t)in.obs.size();
+        if (traj_out && nobs) std::copy(r.traj.begin(), r.traj.begin() + nobs * r.ncol * in.num_traj, traj_out);
+        if (mean_out && nobs) std::copy(r.mean.begin(), r.mean.begin() + nobs * r.ncol, mean_out);
+        if (report) {
+            report->timed_s = r.timed_s;
+            report->timed_steps = r.timed_steps;
+            report->gemm_ms = r.gemm_ms;
+            report->gemm_launches = r.gemm_launches;
+            report->gemm_flop = r.gemm_flop;
+            report->flop_model = r.win_flop;
+            report->matvecs = r.win_matvecs;
+            report->max_bond = r.max_bond;
+        }
+        return YAQS_OK;
+    } catch (const yq::HipError &e) {
+        if (stream) (void)hipStreamDestroy(stream);
+        return fail(YAQS_EDEVICE, e.what());
+    } catch (const std::invalid_argument &e) {
+        if (stream) (void)hipStreamDestroy(stream);
+        return fail(YAQS_EINVAL, e.what());
+    } catch (const std::exception &e) {
+        if (stream) (void)hipStreamDestroy(stream);
+        return fail(YAQS_EDEVICE, e.what());
+    }
+}
+
 }  // extern "C"

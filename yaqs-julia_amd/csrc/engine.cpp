@@ -13,8 +13,11 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <stdexcept>
@@ -26,6 +29,10 @@ namespace yq {
 namespace {
 
 typedef std::complex<double> cd;
+
+double host_now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 // ------------------------------------------------------------ host utils ---
 struct DevBuf {
@@ -242,9 +249,66 @@ GemmDesc gd(int M, int N, int K, int opA, int opB, int64_t lda, int64_t ldb, int
     return d;
 }
 
+// HIP-event timing of matrix-core GEMM launches inside a measurement window.
+struct GemmProf {
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<double> fl;
+    size_t used = 0;
+    double ms = 0, flop = 0;
+    int64_t n = 0;
+    hipStream_t s = nullptr;
+    void start(hipStream_t st) {
+        s = st;
+        if (ev.empty()) {
+            ev.resize(4096);
+            for (auto &e : ev) YQ_HIP(hipEventCreate(&e));
+            fl.resize(2048);
+        }
+        used = 0;
+        ms = flop = 0;
+        n = 0;
+        on = true;
+    }
+    void flush() {
+        if (!used) return;
+        YQ_HIP(hipEventSynchronize(ev[used - 1]));
+        for (size_t i = 0; i < used; i += 2) {
+            float x = 0;
+            YQ_HIP(hipEventElapsedTime(&x, ev[i], ev[i + 1]));
+            ms += x;
+            flop += fl[i / 2];
+            ++n;
+        }
+        used = 0;
+    }
+    void stop() {
+        flush();
+        on = false;
+    }
+    ~GemmProf() {
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+};
+thread_local GemmProf *g_prof = nullptr;
+
 template <typename T>
 void gemmT(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s) {
-    if (!gemm_mfma<T>(d, A, B, C, s)) gemm<T, T, T, T>(d, A, B, C, s);
+    GemmProf *p = g_prof;
+    const bool prof = p && p->on;
+    if (prof) {
+        if (p->used + 2 > p->ev.size()) p->flush();
+        YQ_HIP(hipEventRecord(p->ev[p->used], s));
+    }
+    const bool mf = gemm_mfma<T>(d, A, B, C, s);
+    if (!mf) gemm<T, T, T, T>(d, A, B, C, s);
+    if (prof) {
+        if (mf) {
+            YQ_HIP(hipEventRecord(p->ev[p->used + 1], s));
+            p->fl[p->used / 2] = 8.0 * d.M * (double)d.N * d.K * d.inner * d.outer;
+            p->used += 2;
+        }
+    }
 }
 
 // --------------------------------------------------------------- engine ----
@@ -284,12 +348,17 @@ class Engine {
     State PHI, PSI;
     std::vector<DevBuf> EL, ER;
     std::vector<CsrDev> csrL, csrR, csr2;
-    DevBuf theta, V, T1, T2, G, U, evals, perm, keep, Cf, absorb;
+    DevBuf theta, V, T1, T2, G, U, evals, perm, keep, Cf, absorb, Zbuf, Pbuf;
+    int64_t slotZ = 0, slotP = 0;
     DevBuf Ebuf0, Ebuf1, Xbuf, rho1, rho2;
     DevBuf beta0, galpha, lscale, alpha, beta, coef, nit, active, part, partn, nactive;
     DevBuf norm2v, fac, jstate, chosen, opidx, mask, rngk, wbuf, count;
     DevBuf diss_half, diss_full, diss_one;  // per-site 2x2 dissipation operators
+    DevBuf invL_half, invL_full;            // inverse of the last site's operator
+    bool last_site_damped = false;
     std::vector<int> diss_site_active;
+    State SCR;       // scratch state for the reference's jump weights (see stochastic)
+    DevBuf rho1b;
     DevBuf cand_site, cand_op, cand_site2, cand_op2, cand_LdL, cand_gamma, jump_ops;
     int ncand = 0;
     DevBuf obs_site, obs_n, obs_op;
@@ -325,14 +394,15 @@ class Engine {
     void lq_sweep(State &S, int from, const int *msk);
     void qr_step(State &S, int i, const int *msk);
     void dissipate(State &S, const DevBuf &ops);
-    void transfer(State &S, const int *msk);
+    void transfer(State &S, const int *msk, c64 *rho_out);
     void measure(State &S, int col, int ncol, double *d_out);
-    void stochastic(State &S, double dt, JobResult &res);
+    void stochastic(State &S, double dt, const DevBuf *invL, JobResult &res);
     void absorb_left(State &S, int i, const int *msk);
 };
 
 template <typename T>
 int64_t Engine<T>::footprint(const JobIn &job, int B) {
+    // Device bytes of an engine with B trajectory slots (mirrors setup()).
     const int L = (int)job.L;
     std::vector<int> cap(L + 1, 1);
     for (int b = 1; b < L; ++b) {
@@ -342,21 +412,29 @@ int64_t Engine<T>::footprint(const JobIn &job, int B) {
         cap[b] = std::max(cap[b], std::max(job.mps_dr[b - 1], job.mps_dl[b]));
     }
     int Dmax = 1;
-    for (int i = 0; i < L; ++i) Dmax = std::max(Dmax, std::max(job.mpo_dl[i], job.mpo_dr[i]));
-    int64_t per = 0, nmax = 0, smax = 0, cm = 0;
-    for (int i = 0; i < L; ++i) per += 2 * (int64_t)cap[i] * 2 * cap[i + 1];  // PHI + PSI
-    for (int b = 0; b <= L; ++b) per += 2 * (int64_t)cap[b] * cap[b] * Dmax, cm = std::max<int64_t>(cm, cap[b]);
+    if (!job.digital)
+        for (int i = 0; i < L; ++i) Dmax = std::max(Dmax, std::max(job.mpo_dl[i], job.mpo_dr[i]));
+    int64_t cm = 1, sites = 0, envs = 0, nmax = 0, smax = 0, gd = 2;
+    for (int b = 0; b <= L; ++b) cm = std::max<int64_t>(cm, cap[b]);
+    for (int i = 0; i < L; ++i) sites += (int64_t)cap[i] * 2 * cap[i + 1];
+    for (int b = 0; b <= L; ++b) envs += (int64_t)cap[b] * cap[b] * Dmax;
+    nmax = 2 * cm * cm;
+    smax = 2 * cm * cm * Dmax;
     for (int i = 0; i + 1 < L; ++i) {
         nmax = std::max<int64_t>(nmax, (int64_t)cap[i] * 4 * cap[i + 2]);
         smax = std::max<int64_t>(smax, (int64_t)cap[i] * Dmax * 4 * cap[i + 2]);
+        gd = std::max<int64_t>(gd, 2 * std::max(cap[i], cap[i + 2]));
     }
-    nmax = std::max<int64_t>(nmax, 2 * cm * cm);
-    smax = std::max<int64_t>(smax, 2 * cm * cm * Dmax);
-    per += (job.kdim + 2) * nmax + 2 * smax + cm * cm + 2 * cm * cm;
-    int64_t bytes = per * (int64_t)sizeof(cplx<T>);
-    const int64_t g = 4 * cm * cm;
-    bytes += 2 * g * (int64_t)sizeof(c64) + 3 * cm * cm * 4 * (int64_t)sizeof(c64);
-    bytes += (int64_t)L * 20 * sizeof(c64) + (job.kdim + 2) * 64 * 16;
+    const bool noise = job.has_noise;
+    int64_t elems = sites * (job.digital ? 1 : (noise ? 3 : 2));  // PHI (+ PSI, + scratch)
+    if (!job.digital) elems += 2 * envs + (int64_t)(job.kdim + 1) * nmax + 2 * smax;
+    elems += nmax + cm * cm + sites;  // theta, Cf, absorb (bounded by the state size)
+    int64_t bytes = elems * (int64_t)sizeof(cplx<T>);
+    bytes += 2 * gd * gd * (int64_t)sizeof(c64);                       // Gram + eigenvectors
+    bytes += (gd * gd + 2 * gd + 4) * (int64_t)sizeof(double);         // eigensolver scratch
+    if (gd > 128) bytes += gd * (gd + 1) / 2 * (int64_t)sizeof(c64);
+    bytes += (2 * cm * cm + std::max<int64_t>(nmax, 2 * cm * cm)) * (int64_t)sizeof(c64);  // transfer
+    bytes += (int64_t)L * 20 * sizeof(c64) * 2 + (job.kdim + 2) * 64 * 16 + 256;
     return bytes * B;
 }
 
@@ -418,6 +496,12 @@ void Engine<T>::setup() {
     theta.alloc((size_t)B * nmax * sizeof(CT));
     G.alloc((size_t)B * slotG * sizeof(c64));
     U.alloc((size_t)B * slotG * sizeof(c64));
+    slotZ = gdim * gdim + 2 * gdim + 4;
+    Zbuf.alloc((size_t)B * slotZ * sizeof(double));
+    if (gdim > 128) {
+        slotP = gdim * (gdim + 1) / 2;
+        Pbuf.alloc((size_t)B * slotP * sizeof(double2));
+    }
     evals.alloc((size_t)B * (gdim + 2) * sizeof(double));
     perm.alloc((size_t)B * (gdim + 2) * sizeof(int));
     keep.alloc((size_t)B * sizeof(int));
@@ -486,10 +570,24 @@ void Engine<T>::setup() {
         }
     };
     std::vector<c64> hops;
+    auto inv_last = [&](const std::vector<c64> &ops, DevBuf &dst) {
+        const c64 *M = &ops[(size_t)(L - 1) * 4];
+        const cd a = toc(M[0]), b = toc(M[1]), c = toc(M[2]), d = toc(M[3]);
+        const cd det = a * d - b * c;
+        std::vector<c64> inv = {fromc(d / det), fromc(-b / det), fromc(-c / det), fromc(a / det)};
+        upload(dst, inv, s);
+    };
     site_ops(J.dt / 2, hops, &diss_site_active);
     upload(diss_half, hops, s);
+    inv_last(hops, invL_half);
+    last_site_damped = L >= 2 && diss_site_active[L - 1];
     site_ops(J.dt, hops, nullptr);
     upload(diss_full, hops, s);
+    inv_last(hops, invL_full);
+    if (!J.digital && J.has_noise && last_site_damped) {
+        alloc_state(SCR);
+        rho1b.alloc((size_t)B * L * 4 * sizeof(c64));
+    }
     site_ops(1.0, hops, nullptr);
     upload(diss_one, hops, s);
 
@@ -771,8 +869,9 @@ void Engine<T>::split(State &S, int i, bool right, int rule, double thr) {
     else
         gemm<T, T, double, double>(gd(n, n, m, OP_C, OP_N, n, n, n, nmax, nmax, slotG, nb), theta.as<CT>(),
                                    theta.as<CT>(), G.as<c64>(), s);
-    jacobi_eig(G.as<c64>(), slotG, U.as<c64>(), slotG, gn, ns, evals.as<double>(), perm.as<int>(), gdim + 2,
-               keep.as<int>(), rule, thr, (int)std::min<int64_t>(J.max_bond, 1 << 30), nullptr, nb, s);
+    heev(G.as<c64>(), slotG, U.as<c64>(), slotG, Zbuf.as<double>(), slotZ, Pbuf.p ? Pbuf.as<double2>() : nullptr,
+         slotP, gn, ns, evals.as<double>(), perm.as<int>(), gdim + 2, keep.as<int>(), rule, thr,
+         (int)std::min<int64_t>(J.max_bond, 1 << 30), nullptr, nb, s);
     // new working dimension = max kept rank over the wave
     std::vector<int> hk(nb);
     YQ_HIP(hipMemcpyAsync(hk.data(), keep.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -900,7 +999,7 @@ void Engine<T>::dissipate(State &S, const DevBuf &ops) {
 // Left transfer sweep in FP64 giving every site's reduced matrix ρ_s (and
 // 4x4 ρ for 2-site observables) of a right-canonical state.
 template <typename T>
-void Engine<T>::transfer(State &S, const int *msk) {
+void Engine<T>::transfer(State &S, const int *msk, c64 *rho_out) {
     c64 *E0 = Ebuf0.as<c64>(), *E1 = Ebuf1.as<c64>();
     const int64_t sE = (int64_t)capmax * capmax;
     identity_env<double>(E0, sE, S.wd[0], 1, 0, nb, s);
@@ -908,8 +1007,8 @@ void Engine<T>::transfer(State &S, const int *msk) {
         const int cl = S.wd[i], cr = S.wd[i + 1];
         gemm<double, T, double, double>(gd(cl, 2 * cr, cl, OP_N, OP_N, cl, 2 * cr, 2 * cr, sE, slotA[i], slotX, nb, msk),
                                         E0, A(S, i), Xbuf.as<c64>(), s);
-        rho<T>(Xbuf.as<c64>(), slotX, A(S, i), slotA[i], cl, 2, cr, rho1.as<c64>() + (int64_t)i * 4, (int64_t)L * 4,
-               msk, nb, s);
+        rho<T>(Xbuf.as<c64>(), slotX, A(S, i), slotA[i], cl, 2, cr, rho_out + (int64_t)i * 4, (int64_t)L * 4, msk, nb,
+               s);
         if (obs2_at[i] && i + 1 < L) {
             merge(S, i);
             const int c2 = S.wd[i + 2];
@@ -930,15 +1029,28 @@ void Engine<T>::transfer(State &S, const int *msk) {
 template <typename T>
 void Engine<T>::measure(State &S, int col, int ncol, double *d_out) {
     if (J.obs.empty()) return;
-    transfer(S, nullptr);
+    transfer(S, nullptr, rho1.as<c64>());
     observe(rho1.as<c64>(), (int64_t)L * 4, rho2.as<c64>(), (int64_t)L * 16, (int)J.obs.size(), obs_site.as<int>(),
             obs_n.as<int>(), obs_op.as<c64>(), col, ncol, J.num_traj, (int64_t)traj0 - (int64_t)J.traj_offset, d_out,
             nb, s);
 }
 
 // stochastic_process! (src/StochasticProcess.jl:219-330) for a whole wave.
+//
+// Jump weights follow the reference exactly, including one property of its
+// gauge bookkeeping: apply_dissipation (src/Dissipation.jl:37-159) enters
+// with the orthogonality-centre field at site 1 after a TDVP sweep, so its
+// first shift (to L-1) runs QR from the left and never re-orthogonalises site
+// L, which keeps its damping factor D_L.  create_probability_distribution
+// then reads ‖L_k A_s‖² at the centre (src/StochasticProcess.jl:100-131),
+// which for s < L equals <ψ~|L_k†L_k|ψ~> with ψ~ = D_L^{-1} ψ; only site L
+// sees the true state.  `quirk` is false when the field was at L on entry
+// (then site L is re-orthogonalised by the first LQ), which happens in
+// initialize() right after the t=0 measurement of an observable on site L.
+// invL: inverse of the last site's dissipation operator just applied, or
+// nullptr when the weights see the true state everywhere.
 template <typename T>
-void Engine<T>::stochastic(State &S, double dt, JobResult &res) {
+void Engine<T>::stochastic(State &S, double dt, const DevBuf *invL, JobResult &res) {
     lq_sweep(S, L - 1, nullptr);
     norm2<T>(A(S, 0), slotA[0], (int64_t)S.wd[0] * 2 * S.wd[1], norm2v.as<double>(), nullptr, nb, s);
     jump_decide(norm2v.as<double>(), J.seed, traj0, rngk.as<uint64_t>(), J.noise.empty() ? 0 : 1, jstate.as<int>(),
@@ -949,10 +1061,38 @@ void Engine<T>::stochastic(State &S, double dt, JobResult &res) {
     mask_from_state(jstate.as<int>(), 1, mask.as<int>(), count.as<int>(), nb, s);
     const int njump = sync_count(count.as<int>());
     if (njump == 0) return;
-    transfer(S, mask.as<int>());
+    transfer(S, mask.as<int>(), rho1.as<c64>());
+    if (invL && last_site_damped) {
+        copy_state(S, SCR);
+        local_op<T>(A(SCR, L - 1), slotA[L - 1], SCR.wd[L - 1], SCR.wd[L], invL->as<c64>(), nullptr, mask.as<int>(), nb,
+                    s);
+        lq_sweep(SCR, L - 1, mask.as<int>());
+        transfer(SCR, mask.as<int>(), rho1b.as<c64>());
+        copy2d<double, double>(rho1b.as<c64>(), (int64_t)L * 4, 4 * (L - 1), rho1.as<c64>(), (int64_t)L * 4,
+                               4 * (L - 1), 1, 4 * (L - 1), mask.as<int>(), nb, s);
+    }
     jump_choose(rho1.as<c64>(), (int64_t)L * 4, L, ncand, cand_site.as<int>(), cand_LdL.as<c64>(),
                 cand_gamma.as<double>(), dt, J.seed, traj0, rngk.as<uint64_t>(), jstate.as<int>(), chosen.as<int>(),
                 wbuf.as<double>(), nb, s);
+    if (std::getenv("YAQS_DEBUG_JUMPS")) {
+        std::vector<int> hs(nb), hc(nb);
+        std::vector<uint64_t> hk(nb);
+        YQ_HIP(hipMemcpyAsync(hs.data(), jstate.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        YQ_HIP(hipMemcpyAsync(hc.data(), chosen.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        YQ_HIP(hipMemcpyAsync(hk.data(), rngk.p, nb * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        YQ_HIP(hipStreamSynchronize(s));
+        std::vector<double> hw((size_t)nb * std::max(1, ncand));
+        YQ_HIP(hipMemcpy(hw.data(), wbuf.p, hw.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int t = 0; t < nb; ++t)
+            if (hs[t] == 1) {
+                double tot = 0;
+                for (int c = 0; c < ncand; ++c) tot += hw[(size_t)t * ncand + c];
+                fprintf(stderr, "gpu jump traj=%llu k=%llu cand=%d probs:", (unsigned long long)(traj0 + t),
+                        (unsigned long long)hk[t], hc[t]);
+                for (int c = 0; c < ncand; ++c) fprintf(stderr, " %.15g", hw[(size_t)t * ncand + c] / tot);
+                fprintf(stderr, "\n");
+            }
+    }
     for (int i = 0; i < L; ++i) {
         jump_opidx(jstate.as<int>(), chosen.as<int>(), cand_site.as<int>(), cand_op.as<int>(), cand_site2.as<int>(),
                    cand_op2.as<int>(), i, opidx.as<int>(), nb, s);
@@ -987,7 +1127,7 @@ void Engine<T>::run_analog(int64_t t0, int nb_, double *d_out, JobResult &res) {
             tdvp(PHI, res);
             if (noise) {
                 dissipate(PHI, diss_full);
-                stochastic(PHI, J.dt, res);
+                stochastic(PHI, J.dt, &invL_full, res);
             }
             record_bond(PHI);
             if (J.sample_timesteps) measure(PHI, j - 1, ncol, d_out);
@@ -996,28 +1136,75 @@ void Engine<T>::run_analog(int64_t t0, int nb_, double *d_out, JobResult &res) {
         return;
     }
     if (noise) {  // initialize (src/AnalogTJM.jl:21-33)
+        // centre field after the t=0 measurement: site of the last observable
+        int field = 0;
+        if (J.sample_timesteps && !J.obs.empty()) field = std::min(J.obs.back().s0, J.obs.back().s1);
         dissipate(PHI, diss_half);
-        stochastic(PHI, J.dt, res);
+        stochastic(PHI, J.dt, field == L - 1 ? nullptr : &invL_half, res);
     }
     auto sample = [&](int j) {  // src/AnalogTJM.jl:62-81
         copy_state(PHI, PSI);
         tdvp(PSI, res);
         if (noise) {
             dissipate(PSI, diss_half);
-            stochastic(PSI, J.dt, res);
+            stochastic(PSI, J.dt, &invL_half, res);
         }
         record_bond(PSI);
         measure(PSI, J.sample_timesteps ? j - 1 : 0, ncol, d_out);
     };
+    // measurement window: steps j > bench_warmup + 1 (one step = one time index)
+    GemmProf prof;
+    double t_win = 0, flop0 = 0;
+    int64_t mv0 = 0;
+    auto window = [&](int j) {
+        if (J.bench_warmup < 0) return;
+        if (j == J.bench_warmup + 1) {
+            YQ_HIP(hipStreamSynchronize(s));
+            t_win = host_now();
+            flop0 = res.flop;
+            mv0 = res.matvecs;
+            g_prof = &prof;
+            prof.start(s);
+        }
+        if (j == nt && j > J.bench_warmup + 1) {
+            YQ_HIP(hipStreamSynchronize(s));
+            res.timed_s += host_now() - t_win;
+            res.timed_steps += nt - 1 - J.bench_warmup;
+            prof.stop();
+            g_prof = nullptr;
+            res.gemm_ms += prof.ms;
+            res.gemm_flop += prof.flop;
+            res.gemm_launches += prof.n;
+            res.win_flop += res.flop - flop0;
+            res.win_matvecs += res.matvecs - mv0;
+        }
+    };
+    static const bool progress = std::getenv("YAQS_PROGRESS") != nullptr;
+    const double t_run = host_now();
+    auto report = [&](int j) {
+        if (!progress) return;
+        YQ_HIP(hipStreamSynchronize(s));
+        int mb = 0;
+        for (int b = 0; b <= L; ++b) mb = std::max(mb, PHI.wd[b]);
+        fprintf(stderr, "[yaqs] traj %lld+%d step %d/%d  %.2fs  max bond %d  matvecs %lld\n", (long long)t0, nb,
+                j - 1, nt - 1, host_now() - t_run, mb, (long long)res.matvecs);
+        fflush(stderr);
+    };
+    window(1);
     if (J.sample_timesteps && nt > 1) sample(2);
+    window(2);
+    report(2);
     for (int j = 3; j <= nt; ++j) {
         tdvp(PHI, res);  // step_through (src/AnalogTJM.jl:40-55)
         if (noise) {
             dissipate(PHI, diss_full);
-            stochastic(PHI, J.dt, res);
+            stochastic(PHI, J.dt, &invL_full, res);
         }
         if (J.sample_timesteps || j == nt) sample(j);
+        window(j);
+        report(j);
     }
+    g_prof = nullptr;
 }
 
 // run_digital_tjm with TEBD gates (src/DigitalTJM.jl:363-450) for one wave.

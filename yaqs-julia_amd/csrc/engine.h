@@ -50,6 +50,7 @@ struct JobIn {
     int order = 2, sample_timesteps = 1, kdim = 25, precision = 0;
     double ktol = 1e-10;
     uint64_t seed = 0, traj_offset = 0;
+    int64_t bench_warmup = -1;  // >= 0: time the steps after this many (analog)
 };
 
 struct JobResult {
@@ -59,6 +60,9 @@ struct JobResult {
     std::vector<int64_t> bonds;   // [traj][col] (digital)
     double flop = 0;
     int64_t matvecs = 0, jumps = 0, max_bond = 0;
+    // measurement window (bench_warmup >= 0)
+    double timed_s = 0, gemm_ms = 0, gemm_flop = 0, win_flop = 0;
+    int64_t timed_steps = 0, gemm_launches = 0, win_matvecs = 0;
 };
 
 // Throws std::invalid_argument for malformed jobs, HipError for device errors.

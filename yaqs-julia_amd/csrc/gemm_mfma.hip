@@ -23,12 +23,17 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 template <typename T>
 struct Mfma;
+// Operand layout (both): lane l holds A[l % 16][l / 16] and B[l / 16][l % 16].
+// Output register r of lane l holds D[row][l % 16] with
+//   f32: row = 4 (l / 16) + r        f64: row = 4 r + l / 16
+// (measured on gfx950 with one-hot operands).
 template <>
 struct Mfma<float> {
     typedef f32x4 V;
     static __device__ __forceinline__ V mma(float a, float b, V c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lk, int r) { return lk * 4 + r; }
 };
 template <>
 struct Mfma<double> {
@@ -36,6 +41,7 @@ struct Mfma<double> {
     static __device__ __forceinline__ V mma(double a, double b, V c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lk, int r) { return r * 4 + lk; }
 };
 
 template <typename T>
@@ -121,7 +127,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gi = m0 + wm + i * 16 + lk * 4 + r;
+                const int gi = m0 + wm + i * 16 + Mfma<T>::row(lk, r);
                 const int gj = n0 + wn + j * 16 + li;
                 if (gi < d.M && gj < d.N) {
                     cplx<T> &c = Cb[(int64_t)gi * d.ldc + gj];

@@ -482,165 +482,6 @@ void lz_combine(cplx<T> *x, int64_t sx, const cplx<T> *V, int64_t sV, int64_t nm
 YQ_LZ_INST(float)
 YQ_LZ_INST(double)
 
-// ----------------------------------------------------- Jacobi eigensolver --
-// Cyclic Jacobi on a batch of Hermitian n x n FP64 matrices (one workgroup
-// per matrix, matrices in global memory), parallel rounds of the round-robin
-// tournament so the n/2 rotations of a round touch disjoint index pairs.
-// Output: eigenvector matrix U (columns, unsorted), eigenvalues sorted
-// descending with their column permutation, and the kept rank under the
-// requested truncation rule.
-__global__ __launch_bounds__(512) void jacobi_kernel(c64 *G, int64_t sG, c64 *U, int64_t sU, int n, int ns,
-                                                     double *evals, int *perm, int64_t sE, int *keep, int rule,
-                                                     double thr, int max_bond, int max_sweeps, const int *mask) {
-    extern __shared__ double smem[];
-    const int t = blockIdx.x;
-    if (mask && !mask[t]) return;
-    c64 *A = G + t * sG;
-    c64 *V = U + t * sU;
-    const int m = (n + 1) & ~1;  // even number of players (index n is a dummy when n is odd)
-    const int half = m / 2;
-    // LDS: rotation parameters per pair
-    double *rc = smem;                 // c
-    double *rsr = rc + half;           // Re(sp)
-    double *rsi = rsr + half;          // Im(sp)
-    int *rp = (int *)(rsi + half);     // p
-    int *rq = rp + half;               // q
-    int *flag = rq + half;             // [0]: any rotation this sweep
-    for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += blockDim.x) {
-        const int r = (int)(e / n), c = (int)(e % n);
-        V[e].re = (r == c) ? 1.0 : 0.0;
-        V[e].im = 0.0;
-    }
-    __syncthreads();
-    const double eps = 2.220446049250313e-16;
-    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-        if (threadIdx.x == 0) flag[0] = 0;
-        __syncthreads();
-        for (int round = 0; round < m - 1; ++round) {
-            for (int k = threadIdx.x; k < half; k += blockDim.x) {
-                const int i0 = k, i1 = m - 1 - k;
-                const int p0 = (i0 == 0) ? 0 : ((i0 - 1 + round) % (m - 1)) + 1;
-                const int q0 = (i1 == 0) ? 0 : ((i1 - 1 + round) % (m - 1)) + 1;
-                const int p = p0 < q0 ? p0 : q0, q = p0 < q0 ? q0 : p0;
-                rp[k] = p;
-                rq[k] = q;
-                rc[k] = 1.0;
-                rsr[k] = 0.0;
-                rsi[k] = 0.0;
-                if (q >= n) continue;
-                const double a = A[(int64_t)p * n + p].re, b = A[(int64_t)q * n + q].re;
-                const c64 g = A[(int64_t)p * n + q];
-                const double ag = sqrt(g.re * g.re + g.im * g.im);
-                if (ag == 0.0 || ag <= eps * sqrt(fabs(a * b))) continue;
-                const double zeta = (b - a) / (2.0 * ag);
-                const double tt = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                const double c = 1.0 / sqrt(1.0 + tt * tt), s = c * tt;
-                rc[k] = c;
-                rsr[k] = s * g.re / ag;
-                rsi[k] = s * g.im / ag;
-                flag[0] = 1;
-            }
-            __syncthreads();
-            // rows: row_p' = c row_p - sp row_q ; row_q' = conj(sp) row_p + c row_q
-            for (int64_t e = threadIdx.x; e < (int64_t)half * n; e += blockDim.x) {
-                const int k = (int)(e / n), col = (int)(e % n);
-                const int q = rq[k];
-                if (q >= n || (rsr[k] == 0.0 && rsi[k] == 0.0)) continue;
-                const int p = rp[k];
-                const double c = rc[k];
-                const double2 sp = make_double2(rsr[k], rsi[k]);
-                const double2 xp = make_double2(A[(int64_t)p * n + col].re, A[(int64_t)p * n + col].im);
-                const double2 xq = make_double2(A[(int64_t)q * n + col].re, A[(int64_t)q * n + col].im);
-                const double2 np_ = csub(cscale(xp, c), cmul(sp, xq));
-                const double2 nq_ = cadd(cmulc(sp, xp), cscale(xq, c));
-                A[(int64_t)p * n + col].re = np_.x;
-                A[(int64_t)p * n + col].im = np_.y;
-                A[(int64_t)q * n + col].re = nq_.x;
-                A[(int64_t)q * n + col].im = nq_.y;
-            }
-            __syncthreads();
-            // columns (A and V): col_p' = c col_p - conj(sp) col_q ; col_q' = sp col_p + c col_q
-            for (int64_t e = threadIdx.x; e < (int64_t)half * n * 2; e += blockDim.x) {
-                const int64_t ee = e % ((int64_t)half * n);
-                const int which = (int)(e / ((int64_t)half * n));
-                const int k = (int)(ee / n), row = (int)(ee % n);
-                const int q = rq[k];
-                if (q >= n || (rsr[k] == 0.0 && rsi[k] == 0.0)) continue;
-                const int p = rp[k];
-                const double c = rc[k];
-                const double2 sp = make_double2(rsr[k], rsi[k]);
-                c64 *M = which == 0 ? A : V;
-                const double2 xp = make_double2(M[(int64_t)row * n + p].re, M[(int64_t)row * n + p].im);
-                const double2 xq = make_double2(M[(int64_t)row * n + q].re, M[(int64_t)row * n + q].im);
-                const double2 np_ = csub(cscale(xp, c), cmulc(sp, xq));
-                const double2 nq_ = cadd(cmul(sp, xp), cscale(xq, c));
-                M[(int64_t)row * n + p].re = np_.x;
-                M[(int64_t)row * n + p].im = np_.y;
-                M[(int64_t)row * n + q].re = nq_.x;
-                M[(int64_t)row * n + q].im = nq_.y;
-            }
-            __syncthreads();
-        }
-        const int any = flag[0];
-        __syncthreads();
-        if (!any) break;
-    }
-    // sort eigenvalues descending (rank sort, ties by index)
-    double *ev = evals + t * sE;
-    int *pm = perm + t * sE;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const double di = A[(int64_t)i * n + i].re;
-        int rank = 0;
-        for (int j2 = 0; j2 < n; ++j2) {
-            const double dj = A[(int64_t)j2 * n + j2].re;
-            if (dj > di || (dj == di && j2 < i)) ++rank;
-        }
-        ev[rank] = di;
-        pm[rank] = i;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        // truncation rules on s² = max(λ, 0) over the top ns values
-        int kp = ns;
-        if (rule == 0) {  // T-A (TDVP split, src/Algorithms.jl:422-438)
-            double disc = 0;
-            for (int k = ns; k >= 1; --k) {
-                disc += fmax(ev[k - 1], 0.0);
-                if (disc >= thr) { kp = k > 2 ? k : 2; break; }
-            }
-        } else if (rule == 1) {  // T-C (relative, src/Dissipation.jl:123-136)
-            double tot = 0;
-            for (int k = 0; k < ns; ++k) tot += fmax(ev[k], 0.0);
-            double cur = 0;
-            for (int k = ns; k >= 1; --k) {
-                cur += fmax(ev[k - 1], 0.0);
-                if (cur > thr * tot) { kp = k; break; }
-            }
-        } else {  // T-D (TEBD, src/DigitalTJM.jl:283-292)
-            double cur = 0;
-            for (int k = ns; k >= 1; --k) {
-                const double s2 = fmax(ev[k - 1], 0.0);
-                if (cur + s2 > thr) break;
-                cur += s2;
-                kp -= 1;
-            }
-        }
-        if (kp > max_bond) kp = max_bond;
-        if (kp < 1) kp = 1;
-        if (kp > ns) kp = ns;
-        keep[t] = kp;
-    }
-}
-
-void jacobi_eig(c64 *G, int64_t sG, c64 *U, int64_t sU, int n, int ns, double *evals, int *perm, int64_t sE, int *keep,
-                int rule, double thr, int max_bond, const int *mask, int B, hipStream_t s) {
-    const int half = ((n + 1) & ~1) / 2;
-    const size_t shmem = (size_t)half * 3 * sizeof(double) + (size_t)half * 2 * sizeof(int) + 16;
-    hipLaunchKernelGGL(jacobi_kernel, dim3(B), dim3(512), shmem, s, G, sG, U, sU, n, ns, evals, perm, sE, keep, rule, thr,
-                       max_bond, 30, mask);
-    YQ_HIP(hipGetLastError());
-}
-
 // Write the kept eigenvectors as an isometry in storage precision.
 //   left  (mode 0): Out[r][k] = U[r][perm[k]]        (r < n, k < kw), 0 for k >= keep
 //   right (mode 1): Out[k][r] = conj(U[r][perm[k]])  (k < kw, r < n), 0 for k >= keep
