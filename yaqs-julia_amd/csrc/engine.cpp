@@ -348,8 +348,8 @@ class Engine {
     State PHI, PSI;
     std::vector<DevBuf> EL, ER;
     std::vector<CsrDev> csrL, csrR, csr2;
-    DevBuf theta, V, T1, T2, G, U, evals, perm, keep, Cf, absorb, Zbuf, Pbuf;
-    int64_t slotZ = 0, slotP = 0;
+    DevBuf theta, V, T1, T2, G, U, evals, perm, keep, Cf, absorb, Zbuf, Pbuf, Hwork, Hfb;
+    int64_t slotZ = 0, slotP = 0, slotH = 0;
     DevBuf Ebuf0, Ebuf1, Xbuf, rho1, rho2;
     DevBuf beta0, galpha, lscale, alpha, beta, coef, nit, active, part, partn, nactive;
     DevBuf norm2v, fac, jstate, chosen, opidx, mask, rngk, wbuf, count;
@@ -433,6 +433,7 @@ int64_t Engine<T>::footprint(const JobIn &job, int B) {
     bytes += 2 * gd * gd * (int64_t)sizeof(c64);                       // Gram + eigenvectors
     bytes += (gd * gd + 2 * gd + 4) * (int64_t)sizeof(double);         // eigensolver scratch
     if (gd > 128) bytes += gd * (gd + 1) / 2 * (int64_t)sizeof(c64);
+    bytes += heev_work_bytes((int)gd) + (int64_t)sizeof(int);           // fast eigensolver work area
     bytes += (2 * cm * cm + std::max<int64_t>(nmax, 2 * cm * cm)) * (int64_t)sizeof(c64);  // transfer
     bytes += (int64_t)L * 20 * sizeof(c64) * 2 + (job.kdim + 2) * 64 * 16 + 256;
     return bytes * B;
@@ -501,6 +502,11 @@ void Engine<T>::setup() {
     if (gdim > 128) {
         slotP = gdim * (gdim + 1) / 2;
         Pbuf.alloc((size_t)B * slotP * sizeof(double2));
+    }
+    slotH = heev_work_bytes((int)gdim);
+    if (slotH) {
+        Hwork.alloc((size_t)B * slotH);
+        Hfb.alloc((size_t)B * sizeof(int));
     }
     evals.alloc((size_t)B * (gdim + 2) * sizeof(double));
     perm.alloc((size_t)B * (gdim + 2) * sizeof(int));
@@ -870,7 +876,7 @@ void Engine<T>::split(State &S, int i, bool right, int rule, double thr) {
         gemm<T, T, double, double>(gd(n, n, m, OP_C, OP_N, n, n, n, nmax, nmax, slotG, nb), theta.as<CT>(),
                                    theta.as<CT>(), G.as<c64>(), s);
     heev(G.as<c64>(), slotG, U.as<c64>(), slotG, Zbuf.as<double>(), slotZ, Pbuf.p ? Pbuf.as<double2>() : nullptr,
-         slotP, gn, ns, evals.as<double>(), perm.as<int>(), gdim + 2, keep.as<int>(), rule, thr,
+         slotP, Hwork.p, slotH, Hfb.p ? Hfb.as<int>() : nullptr, gn, ns, evals.as<double>(), perm.as<int>(), gdim + 2, keep.as<int>(), rule, thr,
          (int)std::min<int64_t>(J.max_bond, 1 << 30), nullptr, nb, s);
     // new working dimension = max kept rank over the wave
     std::vector<int> hk(nb);
