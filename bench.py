@@ -70,36 +70,51 @@ def gpu_window(Y, lib, ctx, ntraj, warmup, steps, precision):
 
 
 def cpu_window(Y, ntraj, warmup, steps, nthreads):
-    """FP64 oracle on the same job restricted to `ntraj` trajectories: time of
-    the first warmup+steps steps minus the first warmup steps."""
-    sys.path.insert(0, ROOT)
+    """FP64 oracle on the same job restricted to `ntraj` trajectories (one per
+    OpenMP thread): wall time of steps warmup+1 .. warmup+steps, measured
+    inside the oracle for every trajectory (longest one reported)."""
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
     from oracle import oracle as O
-    O.lib()
-    times = []
-    for n in (warmup, warmup + steps):
-        if n == 0:
-            times.append(0.0)
-            continue
-        h, _ = c2_job(Y, ntraj, n, "fp64")
-        t0 = time.perf_counter()
-        O.run_analog(h, nthreads=nthreads)
-        times.append(time.perf_counter() - t0)
-    return times[1] - times[0]
+    h, _ = c2_job(Y, ntraj, warmup + steps, "fp64")
+    return O.run_analog_window(h, warmup, nthreads=nthreads)
+
+
+def trace_summary(path, last, kernel="gemm_mfma_kernel<float>"):
+    """Average duration of the last `last` dispatches of `kernel` in a
+    rocprofv3 --kernel-trace CSV (the timed window is the end of the run)."""
+    import csv
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if kernel in r["Kernel_Name"]:
+                rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    rows.sort()
+    sel = rows[-last:] if last > 0 else rows
+    dur = [e - s for s, e in sel]
+    return {"kernel": kernel, "dispatches": len(rows), "window_dispatches": len(sel),
+            "avg_us_window": sum(dur) / max(1, len(dur)) / 1e3,
+            "avg_us_all": sum(e - s for s, e in rows) / max(1, len(rows)) / 1e3}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16)
-    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=19, help="untimed steps from t=0 (19 reaches saturated chi=64)")
     ap.add_argument("--ntraj", type=int, default=1000)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--plain-traj", type=int, default=2, help="trajectories for the plain FP64 per-trajectory path")
     ap.add_argument("--cpu-traj", type=int, default=0, help="oracle trajectories (0 = host threads, max 16)")
-    ap.add_argument("--cpu-steps", type=int, default=-1, help="oracle timed steps (-1 = same as --steps)")
+    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle timed steps (bounded CPU sample)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--trace-summary", metavar="CSV", help="summarise a rocprofv3 kernel trace and exit")
+    ap.add_argument("--trace-last", type=int, default=0, help="dispatches of the timed window (from the bench line)")
     args = ap.parse_args()
+    if args.trace_summary:
+        print(json.dumps(trace_summary(args.trace_summary, args.trace_last)))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); trajectories shard across GPUs "
               "with a single reduction at the end, which this single-GPU build does not launch.", file=sys.stderr)
@@ -111,7 +126,7 @@ def main():
           flush=True)
     rep, mean = gpu_window(Y, lib, ctx, args.ntraj, args.warmup, args.steps, args.precision)
     L = 30
-    value = args.ntraj * L * rep["timed_steps"] / rep["timed_s"]
+    value = args.ntraj * L * args.steps / rep["timed_s"]
     print(f"[bench] GPU window {rep['timed_s']:.2f}s -> {value:.4g} traj-site-steps/s, max bond {rep['max_bond']}",
           flush=True)
     achieved = rep["gemm_flop"] / (rep["gemm_ms"] * 1e-3) if rep["gemm_ms"] > 0 else 0.0
@@ -120,9 +135,9 @@ def main():
         "value": value,
         "unit": "traj-site-steps/s",
         "n_gpus": 1,
-        "steps": int(rep["timed_steps"]),
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * rep["timed_s"] / rep["timed_steps"],
+        "ms_per_step": 1e3 * rep["timed_s"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -134,7 +149,7 @@ def main():
             "ntraj": args.ntraj, "L": L, "chi_max": 64, "precision": args.precision,
             "window_t": [args.warmup * 0.05, (args.warmup + args.steps) * 0.05],
             "max_bond_reached": int(rep["max_bond"]),
-            "flop_model_per_traj_site_step": rep["flop_model"] / (args.ntraj * L * rep["timed_steps"]),
+            "flop_model_per_traj_site_step": rep["flop_model"] / (args.ntraj * L * args.steps),
             "matvecs": int(rep["matvecs"]),
         },
         "roofline": {
@@ -148,15 +163,18 @@ def main():
     }
     if not args.no_plain and args.plain_traj > 0:
         pctx = Y.Context(0, max_batch=1)
+        print(f"[bench] plain FP64 per-trajectory GPU path, {args.plain_traj} trajectories", flush=True)
         prep, _ = gpu_window(Y, lib, pctx, args.plain_traj, args.warmup, args.steps, "fp64")
         pctx.close()
-        pv = args.plain_traj * L * prep["timed_steps"] / prep["timed_s"]
+        pv = args.plain_traj * L * args.steps / prep["timed_s"]
         out["plain_fp64_gpu"] = {"value": pv, "unit": "traj-site-steps/s", "ntraj": args.plain_traj,
                                  "batch": 1, "speedup_of_batched": value / pv}
         print(f"[bench] plain FP64 per-trajectory GPU: {pv:.4g} traj-site-steps/s", flush=True)
     if not args.no_cpu:
         nthreads = args.cpu_traj or min(16, os.cpu_count() or 1)
-        csteps = args.steps if args.cpu_steps < 0 else args.cpu_steps
+        csteps = args.cpu_steps
+        print(f"[bench] CPU oracle, {nthreads} trajectories on {nthreads} threads, {args.warmup}+{csteps} steps",
+              flush=True)
         t = cpu_window(Y, nthreads, args.warmup, csteps, nthreads)
         cv = nthreads * L * csteps / t
         out["cpu_baseline"] = {"value": cv, "unit": "traj-site-steps/s", "cores": nthreads, "kind": "port",

@@ -20,7 +20,8 @@
  *
  * Ownership: inputs are caller-owned and deep-copied at submit; outputs are
  * caller-owned and written only inside yaqs_wait / yaqs_run_*.
- * Threading: a context owns one device worker thread; every call is
+ * Threading: a context owns two device worker threads (one HIP stream
+ * each) that run admitted trajectory waves concurrently; every call is
  * thread-safe.  Errors: every call returns a yaqs_status; the message of the
  * last failure on the calling thread is yaqs_last_error().
  */
@@ -180,7 +181,7 @@ typedef struct yaqs_ctx yaqs_ctx;
 int yaqs_abi_version(void);
 const char *yaqs_last_error(void);
 
-/* Context: binds one HIP device, starts its worker thread. */
+/* Context: binds one HIP device, starts its worker threads. */
 int yaqs_create(yaqs_ctx **out, int device, const yaqs_ctx_opts *opts);
 void yaqs_destroy(yaqs_ctx *ctx);
 
@@ -228,10 +229,10 @@ int64_t yaqs_job_footprint_bytes(const yaqs_analog_job *job);
  * launched inside that window. */
 typedef struct {
     double timed_s;          /* wall time of the timed steps                      */
-    int64_t timed_steps;     /* time steps inside the window                      */
+    int64_t timed_steps;     /* time steps inside the window, summed over waves   */
     double gemm_ms;          /* summed device time of the MFMA GEMM launches      */
     int64_t gemm_launches;
-    double gemm_flop;        /* 8 FLOP per complex MAC of those launches          */
+    double gemm_flop;        /* 8 FLOP per complex MAC, active trajectories only  */
     double flop_model;       /* reference-order H_eff FLOP in the window          */
     int64_t matvecs;         /* H_eff applies in the window                       */
     int64_t max_bond;

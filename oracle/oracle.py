@@ -48,6 +48,8 @@ def lib():
         L.oracle_stochastic_factor.argtypes = [ct.c_int64, ct.c_void_p]
         L.oracle_prob_distribution.argtypes = [ct.c_int64, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_double,
                                                c_dp, ct.POINTER(ct.c_int)]
+        L.oracle_set_window.argtypes = [ct.c_int]
+        L.oracle_last_window_seconds.restype = ct.c_double
         L.oracle_tdvp_evolve.argtypes = [ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_int, ct.c_int,
                                          ct.c_double, ct.c_int, ct.c_void_p, c_dp]
         _lib = L
@@ -81,6 +83,18 @@ def run_analog(holder, nthreads: int = 0):
         raise RuntimeError(f"oracle error {rc}")
     stats = dict(zip(["flop", "matvec2", "matvec1", "svd", "env", "jumps"], st.tolist()))
     return traj[: nobs * ncol * ntraj].reshape(nobs, ncol, ntraj), mean[: nobs * ncol].reshape(nobs, ncol), stats
+
+
+def run_analog_window(holder, warmup: int, nthreads: int = 0) -> float:
+    """Runs the job and returns the wall time of steps warmup+1.. of its
+    slowest trajectory (trajectories run concurrently, one per thread)."""
+    L = lib()
+    L.oracle_set_window(int(warmup))
+    try:
+        run_analog(holder, nthreads=nthreads)
+    finally:
+        L.oracle_set_window(-1)
+    return float(L.oracle_last_window_seconds())
 
 
 def run_digital(holder, nthreads: int = 0):

@@ -1145,7 +1145,19 @@ struct TrajOut {
     std::vector<double> res;  // [obs][time]
     int64_t jumps = 0;
     Counters cnt;
+    double w0 = 0.0, w1 = 0.0;  // wall clock at the start of the timed window and at the end
 };
+
+// Timed window for the CPU baseline: steps after the first g_window_warmup
+// time steps of each trajectory (negative: whole trajectory).
+int g_window_warmup = -1;
+double wall_now() {
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
+}
 
 // analog_tjm_2 / analog_tjm_1 (src/AnalogTJM.jl:104-176) for one trajectory.
 void run_trajectory(const yaqs_analog_job *job, const MPSState &psi0, const std::vector<T4> &H,
@@ -1164,12 +1176,14 @@ void run_trajectory(const yaqs_analog_job *job, const MPSState &psi0, const std:
     const int mb = (int)job->max_bond_dim;
     const double thr = job->truncation_threshold;
     MPSState state = psi0;
+    out.w0 = wall_now();
     if (job->sample_timesteps) measure(state, 0);
     auto stoch = [&](MPSState &st, double dts) {
         if (noise) out.jumps += stochastic_process(st, *noise, dts, thr, mb, rng);
     };
     if (job->order == 1) {
         for (int j = 2; j <= nt; ++j) {
+            if (j == g_window_warmup + 2) out.w0 = wall_now();
             two_site_tdvp(state, C);
             if (noise) {
                 apply_dissipation(state, noise, dt, thr, mb);
@@ -1178,6 +1192,7 @@ void run_trajectory(const yaqs_analog_job *job, const MPSState &psi0, const std:
             if (job->sample_timesteps) measure(state, j - 1);
             else if (j == nt) measure(state, 0);
         }
+        out.w1 = wall_now();
         return;
     }
     // initialize (src/AnalogTJM.jl:21-33)
@@ -1192,11 +1207,13 @@ void run_trajectory(const yaqs_analog_job *job, const MPSState &psi0, const std:
     };
     if (job->sample_timesteps && nt > 1) sample(2);
     for (int j = 3; j <= nt; ++j) {
+        if (j == g_window_warmup + 2) out.w0 = wall_now();
         two_site_tdvp(state, C);  // step_through (src/AnalogTJM.jl:40-55)
         apply_dissipation(state, noise, dt, thr, mb);
         stoch(state, dt);
         if (job->sample_timesteps || j == nt) sample(j);
     }
+    out.w1 = wall_now();
 }
 
 // ----------------------------------------------------------- digital ------
@@ -1326,6 +1343,7 @@ void run_digital_trajectory(const yaqs_digital_job *job, const MPSState &psi0, c
 }
 
 thread_local std::string g_err;
+double g_last_window = 0.0;
 
 }  // namespace
 
@@ -1398,8 +1416,17 @@ int oracle_run_analog(const yaqs_analog_job *job, double *traj_out, double *mean
         st[3] += o.cnt.svd; st[4] += o.cnt.env; st[5] += o.jumps;
     }
     if (stats_out) std::copy(st, st + 6, stats_out);
+    g_last_window = 0.0;
+    for (auto &o : outs) g_last_window = std::max(g_last_window, o.w1 - o.w0);
     return YAQS_OK;
 }
+
+// CPU-baseline timing: time steps warmup+1 .. end of every trajectory of the
+// next oracle_run_analog call (warmup < 0: whole trajectories); the window
+// reported is the longest over trajectories (they run concurrently, one per
+// thread).
+void oracle_set_window(int warmup_steps) { g_window_warmup = warmup_steps; }
+double oracle_last_window_seconds(void) { return g_last_window; }
 
 int64_t oracle_digital_num_samples(const yaqs_digital_job *job) {
     std::vector<Gate> gates(job->n_gates);

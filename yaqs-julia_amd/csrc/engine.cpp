@@ -291,6 +291,9 @@ struct GemmProf {
     }
 };
 thread_local GemmProf *g_prof = nullptr;
+// Trajectories active in the current masked launch (Lanczos), -1 = all: the
+// profiler counts algorithmic FLOP of active trajectories only.
+thread_local int g_nact = -1;
 
 template <typename T>
 void gemmT(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s) {
@@ -305,7 +308,8 @@ void gemmT(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hi
     if (prof) {
         if (mf) {
             YQ_HIP(hipEventRecord(p->ev[p->used + 1], s));
-            p->fl[p->used / 2] = 8.0 * d.M * (double)d.N * d.K * d.inner * d.outer;
+            const int units = (d.active && g_nact >= 0) ? g_nact : d.outer;
+            p->fl[p->used / 2] = 8.0 * d.M * (double)d.N * d.K * d.inner * units;
             p->used += 2;
         }
     }
@@ -834,7 +838,9 @@ void Engine<T>::expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, Jo
     lz_init<T>(x, sx, V.as<CT>(), sV(), n, beta0.as<double>(), active.as<int>(), nit.as<int>(), nb, s);
     int nact = nb;
     for (int j = 0; j < kdim; ++j) {
+        g_nact = nact;
         matvec(S, two, i, Vq(j), sV(), Vq(j + 1), sV(), active.as<int>());
+        g_nact = -1;
         res.matvecs += nact;
         res.flop += fl * nact;
         lz_dots<T>(V.as<CT>(), sV(), nmax, n, j, part.as<c64>(), kdim + 1, active.as<int>(), nb, s);
