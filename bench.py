@@ -80,21 +80,86 @@ def cpu_window(Y, ntraj, warmup, steps, nthreads):
     return O.run_analog_window(h, warmup, nthreads=nthreads)
 
 
-def trace_summary(path, last, kernel="gemm_mfma_kernel<float>"):
+def serve_stream(Y, njobs, seed=0, mean_gap_s=0.25):
+    """Serving stream shaped like the mixed-request workload, scaled to finish
+    in about a minute: Poisson arrivals of analog jobs drawn from the seed
+    (Ising g in [0.2, 1.5], XY or Heisenberg; L in 10..32; chi_max in
+    {16, 32, 64}; relaxation + dephasing gamma log-uniform in [1e-3, 1e-1];
+    64..256 trajectories; T=0.5, dt=0.05; mixed precision).  Latency is
+    submit -> done as measured by the library."""
+    rng = np.random.default_rng(seed)
+    ctx = Y.Context(0)
+    specs = []
+    for _ in range(njobs):
+        L = int(rng.integers(10, 33))
+        model = int(rng.integers(0, 3))
+        if model == 0:
+            H = Y.init_ising(L, 1.0, float(rng.uniform(0.2, 1.5)))
+        elif model == 1:
+            H = Y.init_general_hamiltonian(L, 1, 1, 0, 0, 0, 0)
+        else:
+            H = Y.init_general_hamiltonian(L, 1, 1, 1, 0, 0, 0)
+        chi = int(rng.choice([16, 32, 64]))
+        gamma = float(10 ** rng.uniform(-3, -1))
+        ntraj = int(rng.integers(64, 257))
+        obs = [Y.Observable(f"z{i}", Y.gates.ZGate(), i) for i in range(1, L + 1)]
+        cfg = Y.TimeEvolutionConfig(obs, 0.5, dt=0.05, num_traj=ntraj, max_bond_dim=chi, truncation_threshold=1e-9)
+        mps = Y.MPS(L, state="Neel" if model == 2 else "zeros")
+        mps.normalize()
+        h = Y._abi.pack_analog(mps, H, cfg, Y.relaxation_dephasing(L, gamma), precision=Y._abi.PREC_MIXED,
+                               seed=int(rng.integers(0, 2 ** 31)))
+        specs.append((h, cfg, L, ntraj))
+    gaps = rng.exponential(mean_gap_s, njobs)
+    t0 = time.perf_counter()
+    ids = []
+    for (h, cfg, L, ntraj), gap in zip(specs, gaps):
+        time.sleep(gap)
+        ids.append(ctx.submit_analog(h))
+    lat, work = [], 0
+    for (h, cfg, L, ntraj), jid in zip(specs, ids):
+        _, _, _, st = ctx.wait(jid, len(cfg.observables), len(cfg.times), ntraj)
+        lat.append(st["wall_s"])
+        work += ntraj * L * (len(cfg.times) - 1)
+    span = time.perf_counter() - t0
+    ctx.close()
+    lat = np.array(lat)
+    return {"jobs": njobs, "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "max_s": float(lat.max()), "throughput": work / span, "unit": "traj-site-steps/s",
+            "makespan_s": span, "mean_interarrival_s": mean_gap_s,
+            "stream": "Poisson; Ising/XY/Heisenberg, L 10-32, chi 16/32/64, gamma 1e-3..1e-1, 64-256 traj, "
+                      "T=0.5 dt=0.05, mixed"}
+
+
+def trace_summary(path, last, kernel="gemm_mfma_kernel<float"):
     """Average duration of the last `last` dispatches of `kernel` in a
     rocprofv3 --kernel-trace CSV (the timed window is the end of the run)."""
     import csv
-    rows = []
+    rows, allk = [], []
     with open(path) as f:
         for r in csv.DictReader(f):
+            s, e = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+            name = r["Kernel_Name"].split("(")[0]
+            allk.append((s, e, name))
             if kernel in r["Kernel_Name"]:
-                rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+                rows.append((s, e))
     rows.sort()
     sel = rows[-last:] if last > 0 else rows
     dur = [e - s for s, e in sel]
+    # every kernel dispatched from the first GEMM of the window on
+    t0 = sel[0][0] if sel else 0
+    per = {}
+    for s, e, name in allk:
+        if s >= t0:
+            c = per.setdefault(name, [0, 0])
+            c[0] += 1
+            c[1] += e - s
+    span = (max(e for s, e, _ in allk) - t0) if allk else 0
+    top = sorted(per.items(), key=lambda kv: -kv[1][1])[:20]
     return {"kernel": kernel, "dispatches": len(rows), "window_dispatches": len(sel),
             "avg_us_window": sum(dur) / max(1, len(dur)) / 1e3,
-            "avg_us_all": sum(e - s for s, e in rows) / max(1, len(rows)) / 1e3}
+            "avg_us_all": sum(e - s for s, e in rows) / max(1, len(rows)) / 1e3,
+            "window_span_s": span / 1e9,
+            "window_kernels": [{"name": k, "calls": v[0], "total_s": v[1] / 1e9} for k, v in top]}
 
 
 def main():
@@ -109,6 +174,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle timed steps (bounded CPU sample)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=12, help="jobs of the serving stream (0 = skip)")
     ap.add_argument("--trace-summary", metavar="CSV", help="summarise a rocprofv3 kernel trace and exit")
     ap.add_argument("--trace-last", type=int, default=0, help="dispatches of the timed window (from the bench line)")
     args = ap.parse_args()
@@ -153,7 +219,7 @@ def main():
             "matvecs": int(rep["matvecs"]),
         },
         "roofline": {
-            "bound": "mfma", "kernel": "gemm_mfma_kernel<float> (complex-FP32 real-split MFMA)",
+            "bound": "mfma", "kernel": "gemm_mfma_kernel<float, opA, opB> (complex-FP32 real-split MFMA)",
             "achieved": achieved / 1e12, "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
             "frac": achieved / FP32_MFMA_PEAK, "traffic": None,
             "launches": int(rep["gemm_launches"]),
@@ -170,6 +236,11 @@ def main():
         out["plain_fp64_gpu"] = {"value": pv, "unit": "traj-site-steps/s", "ntraj": args.plain_traj,
                                  "batch": 1, "speedup_of_batched": value / pv}
         print(f"[bench] plain FP64 per-trajectory GPU: {pv:.4g} traj-site-steps/s", flush=True)
+    if args.serve_jobs > 0:
+        print(f"[bench] serving stream, {args.serve_jobs} jobs", flush=True)
+        out["serving"] = serve_stream(Y, args.serve_jobs)
+        print(f"[bench] serving p50 {out['serving']['p50_s']:.2f}s p99 {out['serving']['p99_s']:.2f}s, "
+              f"{out['serving']['throughput']:.4g} traj-site-steps/s", flush=True)
     if not args.no_cpu:
         nthreads = args.cpu_traj or min(16, os.cpu_count() or 1)
         csteps = args.cpu_steps

@@ -37,7 +37,7 @@ def _run(prec, opA, opB, M, N, K, mfma, rng):
 def test_gemm_forms(prec, tol, mfma):
     print(f"case: gemm prec={prec} mfma={mfma}", flush=True)
     rng = np.random.default_rng(7)
-    for opA, opB in [(0, 0), (2, 0), (0, 2), (1, 0), (0, 1), (2, 2)]:
+    for opA, opB in [(a, b) for a in range(3) for b in range(3)]:
         for M, N, K in [(64, 64, 16), (70, 33, 21), (5, 130, 67), (320, 256, 64), (1, 1, 1)]:
             err = _run(prec, opA, opB, M, N, K, mfma, rng)
             assert err < tol, (opA, opB, M, N, K, err)

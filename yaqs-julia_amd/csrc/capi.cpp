@@ -178,7 +178,12 @@ struct Job {
     int status = -1;  // -1 queued/running, else yaqs_status
     std::string err;
     double t_submit = 0, t_start = 0, t_end = 0;
-    int64_t batch = 0, bytes = 0;
+    // wave schedule, fixed when the job is admitted
+    int64_t batch = 0;  // trajectories per wave (0: not admitted yet)
+    int64_t nwaves = 0, next_wave = 0, done_waves = 0, running = 0;
+    double *d_out = nullptr;  // [obs][col][traj] on the device, shared by the waves
+    bool failed = false;
+    int status_code_pending = YAQS_OK;
 };
 
 }  // namespace
@@ -191,7 +196,7 @@ struct yaqs_ctx {
     int verbose = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    std::deque<std::shared_ptr<Job>> queue;
+    std::deque<std::shared_ptr<Job>> queue;  // jobs not finished yet, in submission order
     std::map<uint64_t, std::shared_ptr<Job>> jobs;
     uint64_t next_id = 1;
     bool stop = false;
@@ -214,46 +219,125 @@ int64_t choose_batch(yaqs_ctx *c, const yq::JobIn &in, int64_t avail, int64_t *b
     return b;
 }
 
+int64_t output_elems(const yq::JobIn &in) {
+    return std::max<int64_t>(1, (int64_t)in.obs.size() * yq::num_columns(in) * in.num_traj);
+}
+
+void finish_locked(yaqs_ctx *c, const std::shared_ptr<Job> &job, int status, const std::string &err) {
+    job->status = status;
+    job->err = err;
+    job->t_end = now_s();
+    for (auto it = c->queue.begin(); it != c->queue.end(); ++it)
+        if (*it == job) {
+            c->queue.erase(it);
+            break;
+        }
+    c->cv_done.notify_all();
+}
+
+struct Wave {
+    std::shared_ptr<Job> job;
+    int64_t t0 = 0;
+    int nb = 0;
+    int64_t bytes = 0;
+};
+
+// Continuous batching (called with the lock held).  Among admitted jobs with
+// pending waves whose footprint fits, the one with the fewest running waves
+// (earliest submitted among equals) gets the worker, so a job that arrives
+// behind a long one starts on the next free worker; the oldest job not yet
+// admitted competes as a job with no running wave, and is admitted when its
+// first wave fits (admission stays in submission order).
+bool pick_wave(yaqs_ctx *c, Wave &w) {
+    std::shared_ptr<Job> best;
+    int64_t best_nb = 0, best_bytes = 0;
+    const int64_t avail = c->budget - c->in_use;
+    for (size_t qi = 0; qi < c->queue.size();) {
+        std::shared_ptr<Job> j = c->queue[qi];
+        if (j->failed) {
+            ++qi;
+            continue;
+        }
+        if (j->batch == 0) {
+            // oldest job not admitted yet
+            if (best && best->running == 0) break;
+            int64_t bytes = 0;
+            const int64_t b = choose_batch(c, j->in, avail, &bytes);
+            if (b < 1) {
+                if (c->in_use == 0) {  // cannot fit even alone
+                    finish_locked(c, j, YAQS_ENOMEM_DEVICE, "job footprint exceeds the HBM budget");
+                    continue;  // the queue shifted: same qi
+                }
+                break;
+            }
+            try {
+                const int64_t n = output_elems(j->in);
+                if (hipMalloc(&j->d_out, n * sizeof(double)) != hipSuccess) throw std::bad_alloc();
+                if (hipMemset(j->d_out, 0, n * sizeof(double)) != hipSuccess) throw std::bad_alloc();
+                j->out.ncol = yq::num_columns(j->in);
+                j->out.traj.assign((size_t)n, 0.0);
+                j->out.mean.assign((size_t)std::max<int64_t>(1, (int64_t)j->in.obs.size() * j->out.ncol), 0.0);
+                if (j->in.digital) j->out.bonds.assign((size_t)j->in.num_traj * j->out.ncol, 0);
+            } catch (const std::exception &) {
+                if (j->d_out) (void)hipFree(j->d_out);
+                j->d_out = nullptr;
+                finish_locked(c, j, YAQS_ENOMEM_DEVICE, "cannot allocate the job output on the device");
+                continue;
+            }
+            j->batch = b;
+            j->nwaves = (j->in.num_traj + b - 1) / b;
+            j->t_start = now_s();
+            best = j;
+            best_nb = (int)std::min<int64_t>(b, j->in.num_traj);
+            best_bytes = bytes;
+            break;
+        }
+        if (j->next_wave < j->nwaves && (!best || j->running < best->running)) {
+            const int64_t t0 = j->next_wave * j->batch;
+            const int64_t nb = std::min<int64_t>(j->batch, j->in.num_traj - t0);
+            const int64_t bytes = yq::footprint_bytes(j->in, nb);
+            if (bytes <= avail) {
+                best = j;
+                best_nb = nb;
+                best_bytes = bytes;
+            }
+        }
+        ++qi;
+    }
+    if (!best) return false;
+    w.job = best;
+    w.t0 = best->next_wave * best->batch;
+    w.nb = (int)best_nb;
+    w.bytes = best_bytes;
+    best->next_wave++;
+    best->running++;
+    c->in_use += best_bytes;
+    return true;
+}
+
 void worker_main(yaqs_ctx *c) {
     if (hipSetDevice(c->device) != hipSuccess) return;
     hipStream_t stream = nullptr;
     if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return;
     for (;;) {
-        std::shared_ptr<Job> job;
+        Wave w;
         {
             std::unique_lock<std::mutex> lk(c->mu);
             for (;;) {
                 if (c->stop) break;
-                if (!c->queue.empty()) {
-                    auto &head = c->queue.front();
-                    int64_t bytes = 0;
-                    const int64_t b = choose_batch(c, head->in, c->budget - c->in_use, &bytes);
-                    if (b > 0) {
-                        job = head;
-                        c->queue.pop_front();
-                        job->batch = b;
-                        job->bytes = bytes;
-                        c->in_use += bytes;
-                        break;
-                    }
-                    if (c->in_use == 0) {  // cannot fit even alone: fail it
-                        head->status = YAQS_ENOMEM_DEVICE;
-                        head->err = "job footprint exceeds the HBM budget";
-                        head->t_end = now_s();
-                        c->queue.pop_front();
-                        c->cv_done.notify_all();
-                        continue;
-                    }
-                }
+                if (pick_wave(c, w)) break;
                 c->cv_work.wait(lk);
             }
         }
-        if (!job) break;
-        job->t_start = now_s();
+        if (!w.job) break;
+        Job &job = *w.job;
         int st = YAQS_OK;
         std::string err;
+        yq::JobResult wr;
         try {
-            yq::run_job(job->in, job->batch, stream, job->out);
+            auto eng = yq::make_engine(job.in, w.nb, stream);
+            eng->run(w.t0, w.nb, job.d_out, job.in.digital ? job.out.bonds.data() : nullptr, wr);
+            YQ_HIP(hipStreamSynchronize(stream));
         } catch (const yq::HipError &e) {
             st = YAQS_EDEVICE;
             err = e.what();
@@ -267,14 +351,47 @@ void worker_main(yaqs_ctx *c) {
             st = YAQS_EDEVICE;
             err = e.what();
         }
+        bool last = false;
         {
             std::lock_guard<std::mutex> lk(c->mu);
-            job->t_end = now_s();
-            job->status = st;
-            job->err = err;
-            c->in_use -= job->bytes;
+            c->in_use -= w.bytes;
+            job.running--;
+            job.done_waves++;
+            job.out.flop += wr.flop;
+            job.out.matvecs += wr.matvecs;
+            job.out.jumps += wr.jumps;
+            job.out.max_bond = std::max(job.out.max_bond, wr.max_bond);
+            if (st != YAQS_OK && !job.failed) {
+                job.failed = true;
+                job.status_code_pending = st;
+                job.err = err;
+            }
+            last = job.running == 0 && (job.failed || job.done_waves == job.nwaves);
         }
-        c->cv_done.notify_all();
+        if (last) {
+            // only this worker can reach here for the job: no wave is running or pending
+            int fst = job.failed ? job.status_code_pending : YAQS_OK;
+            std::string ferr = job.err;
+            if (!job.failed) {
+                if (hipMemcpy(job.out.traj.data(), job.d_out, job.out.traj.size() * sizeof(double),
+                              hipMemcpyDeviceToHost) != hipSuccess) {
+                    fst = YAQS_EDEVICE;
+                    ferr = "copy of the job output failed";
+                } else {
+                    const int64_t nobs = (int64_t)job.in.obs.size(), ncol = job.out.ncol, ntraj = job.in.num_traj;
+                    for (int64_t q = 0; q < nobs; ++q)
+                        for (int64_t col = 0; col < ncol; ++col) {
+                            double sum = 0;
+                            for (int64_t t = 0; t < ntraj; ++t) sum += job.out.traj[((size_t)q * ncol + col) * ntraj + t];
+                            job.out.mean[(size_t)q * ncol + col] = sum / (double)ntraj;
+                        }
+                }
+            }
+            (void)hipFree(job.d_out);
+            job.d_out = nullptr;
+            std::lock_guard<std::mutex> lk(c->mu);
+            finish_locked(c, w.job, fst, ferr);
+        }
         c->cv_work.notify_all();
     }
     (void)hipStreamDestroy(stream);

@@ -1467,6 +1467,28 @@ int64_t footprint_bytes(const JobIn &job, int64_t batch) {
     return job.precision == 1 ? Engine<float>::footprint(job, (int)batch) : Engine<double>::footprint(job, (int)batch);
 }
 
+namespace {
+template <typename T>
+class WaveEngineT : public WaveEngine {
+  public:
+    WaveEngineT(const JobIn &job, int batch, hipStream_t s) : eng(job, batch, s), digital(job.digital) {}
+    void run(int64_t t0, int nb, double *d_out, int64_t *h_bonds, JobResult &res) override {
+        if (digital) eng.run_digital(t0, nb, d_out, h_bonds, res);
+        else eng.run_analog(t0, nb, d_out, res);
+    }
+
+  private:
+    Engine<T> eng;
+    bool digital;
+};
+}  // namespace
+
+std::unique_ptr<WaveEngine> make_engine(const JobIn &job, int batch, hipStream_t stream) {
+    validate_job(job);
+    if (job.precision == 1) return std::unique_ptr<WaveEngine>(new WaveEngineT<float>(job, batch, stream));
+    return std::unique_ptr<WaveEngine>(new WaveEngineT<double>(job, batch, stream));
+}
+
 void run_job(const JobIn &job, int64_t batch, hipStream_t stream, JobResult &out, std::atomic<bool> *cancel) {
     validate_job(job);
     const int64_t ntraj = job.num_traj;

@@ -4,6 +4,7 @@
 
 #include <atomic>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -71,6 +72,18 @@ int64_t num_columns(const JobIn &job);
 int64_t footprint_bytes(const JobIn &job, int64_t batch);
 // Runs every trajectory of `job` in waves of at most `batch` trajectories.
 void run_job(const JobIn &job, int64_t batch, hipStream_t stream, JobResult &out, std::atomic<bool> *cancel = nullptr);
+
+// One engine of `batch` trajectory slots on `stream`; run() executes the
+// trajectories [t0, t0 + nb) of the job, writing their observables into the
+// job's device output d_out ([obs][col][num_traj], FP64) and, for digital
+// jobs, their bond dimensions into h_bonds ([num_traj][col], host).  Waves of
+// one job on different engines write disjoint slices.
+class WaveEngine {
+  public:
+    virtual ~WaveEngine() = default;
+    virtual void run(int64_t t0, int nb, double *d_out, int64_t *h_bonds, JobResult &res) = 0;
+};
+std::unique_ptr<WaveEngine> make_engine(const JobIn &job, int batch, hipStream_t stream);
 
 double rng_uniform_host(uint64_t seed, uint64_t traj, uint64_t k);
 

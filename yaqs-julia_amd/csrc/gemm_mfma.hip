@@ -6,8 +6,8 @@
 // wavefronts owns a 32x32 quarter as 2x2 blocks of 16x16 MFMA outputs
 // (v_mfma_f32_16x16x4f32 for complex-FP32, v_mfma_f64_16x16x4f64 for
 // complex-FP64).  Tiles of A and B are staged through LDS split into real
-// and imaginary planes, K-major, so the 16 lanes that share a k read 16
-// consecutive LDS words.
+// and imaginary planes; the op (N, T, C) of each operand is a template
+// parameter.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -44,10 +44,27 @@ struct Mfma<double> {
     static __device__ __forceinline__ int row(int lk, int r) { return r * 4 + lk; }
 };
 
-template <typename T>
+// Element (m, k) of op(A) and (k, n) of op(B) in row-major storage.
+template <int OP>
+__device__ __forceinline__ int64_t a_index(int m, int k, int64_t ld) {
+    return OP == OP_N ? (int64_t)m * ld + k : (int64_t)k * ld + m;
+}
+template <int OP>
+__device__ __forceinline__ int64_t b_index(int k, int n, int64_t ld) {
+    return OP == OP_N ? (int64_t)k * ld + n : (int64_t)n * ld + k;
+}
+
+// Tile 64 x 64 of C per 256-thread workgroup, K in steps of 16.  LDS rows
+// hold one row of op(A) (or one column of op(B)) with k permuted as
+// (k % 4) * 4 + k / 4, so the four k values a lane feeds to the four MFMAs
+// of a 16-wide k step are one contiguous 4-element read.  The next k tile is
+// loaded into registers while the current one is multiplied (two LDS
+// buffers, one barrier per tile); loads outside the matrix read element 0
+// and are zeroed, so there is no branch between the loads.
+template <typename T, int OPA, int OPB>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T> *__restrict__ A,
                                                         const cplx<T> *__restrict__ B, cplx<T> *__restrict__ C) {
-    constexpr int BM = 64, BN = 64, BK = 16, PAD = 4;
+    constexpr int BM = 64, BN = 64, BK = 16, LDK = BK + 4;
     typedef typename Mfma<T>::V V;
     const int t = blockIdx.z, jb = blockIdx.y;
     if (d.active && !d.active[t]) return;
@@ -56,10 +73,49 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
     const cplx<T> *Ab = A + t * d.sA_outer + jb * d.sA_inner;
     const cplx<T> *Bb = B + t * d.sB_outer + jb * d.sB_inner;
     cplx<T> *Cb = C + t * d.sC_outer + jb * d.sC_inner;
-    __shared__ T As_r[BK][BM + PAD], As_i[BK][BM + PAD], Bs_r[BK][BN + PAD], Bs_i[BK][BN + PAD];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ __align__(16) T As[2][2][BM][LDK];  // [buffer][re/im][m][k']
+    __shared__ __align__(16) T Bs[2][2][BN][LDK];  // [buffer][re/im][n][k']
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
+    // the four (row, k) pairs this thread loads per tile, in coalescing order
+    int am[4], ak[4], bn[4], bk[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int e = tid + 256 * r;
+        if (OPA == OP_N) { am[r] = e / BK; ak[r] = e % BK; }
+        else { ak[r] = e / BM; am[r] = e % BM; }
+        if (OPB == OP_N) { bk[r] = e / BN; bn[r] = e % BN; }
+        else { bn[r] = e / BK; bk[r] = e % BK; }
+    }
+    cplx<T> ra[4], rb[4];
+    unsigned okm = 0;  // in-bounds flags of the pending loads, applied at the LDS store
+    auto load = [&](int k0) {
+        okm = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int gm = m0 + am[r], gk = k0 + ak[r];
+            const bool ok = gm < d.M && gk < d.K;
+            ra[r] = Ab[ok ? a_index<OPA>(gm, gk, d.lda) : 0];
+            const int gn = n0 + bn[r], gk2 = k0 + bk[r];
+            const bool ok2 = gn < d.N && gk2 < d.K;
+            rb[r] = Bb[ok2 ? b_index<OPB>(gk2, gn, d.ldb) : 0];
+            okm |= (ok ? 1u : 0u) << r;
+            okm |= (ok2 ? 1u : 0u) << (r + 4);
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const bool ok = (okm >> r) & 1u, ok2 = (okm >> (r + 4)) & 1u;
+            const int ka = (ak[r] & 3) * 4 + (ak[r] >> 2);
+            As[buf][0][am[r]][ka] = ok ? ra[r].re : (T)0;
+            As[buf][1][am[r]][ka] = ok ? (OPA == OP_C ? -ra[r].im : ra[r].im) : (T)0;
+            const int kb = (bk[r] & 3) * 4 + (bk[r] >> 2);
+            Bs[buf][0][bn[r]][kb] = ok2 ? rb[r].re : (T)0;
+            Bs[buf][1][bn[r]][kb] = ok2 ? (OPB == OP_C ? -rb[r].im : rb[r].im) : (T)0;
+        }
+    };
     V accr[2][2], acci[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -67,58 +123,42 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) accr[i][j][r] = acci[i][j][r] = (T)0;
-    for (int k0 = 0; k0 < d.K; k0 += BK) {
+    const int nk = (d.K + BK - 1) / BK;
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) load((kt + 1) * BK);
+        T ar[2][4], ai[2][4], br[2][4], bi[2][4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int e = threadIdx.x + 256 * r;
-            int row, kk;
-            if (d.opA == OP_N) { row = e / BK; kk = e % BK; }
-            else { kk = e / BM; row = e % BM; }
-            const int gi = m0 + row, gk = k0 + kk;
-            T vr = 0, vi = 0;
-            if (gi < d.M && gk < d.K) {
-                const cplx<T> z = (d.opA == OP_N) ? Ab[(int64_t)gi * d.lda + gk] : Ab[(int64_t)gk * d.lda + gi];
-                vr = z.re;
-                vi = (d.opA == OP_C) ? -z.im : z.im;
-            }
-            As_r[kk][row] = vr;
-            As_i[kk][row] = vi;
-            int col;
-            if (d.opB == OP_N) { kk = e / BN; col = e % BN; }
-            else { col = e / BK; kk = e % BK; }
-            const int gj = n0 + col, gk2 = k0 + kk;
-            T wr = 0, wi = 0;
-            if (gj < d.N && gk2 < d.K) {
-                const cplx<T> z = (d.opB == OP_N) ? Bb[(int64_t)gk2 * d.ldb + gj] : Bb[(int64_t)gj * d.ldb + gk2];
-                wr = z.re;
-                wi = (d.opB == OP_C) ? -z.im : z.im;
-            }
-            Bs_r[kk][col] = wr;
-            Bs_i[kk][col] = wi;
-        }
-        __syncthreads();
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int ks = 0; ks < BK / 4; ++ks) {
-            const int kk = ks * 4 + lk;
-            T ar[2], ai[2], nai[2], br[2], bi[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                ar[i] = As_r[kk][wm + i * 16 + li];
-                ai[i] = As_i[kk][wm + i * 16 + li];
-                nai[i] = -ai[i];
-                br[i] = Bs_r[kk][wn + i * 16 + li];
-                bi[i] = Bs_i[kk][wn + i * 16 + li];
+            for (int q = 0; q < 4; ++q) {
+                ar[i][q] = As[buf][0][wm + i * 16 + li][lk * 4 + q];
+                ai[i][q] = As[buf][1][wm + i * 16 + li][lk * 4 + q];
+                br[i][q] = Bs[buf][0][wn + i * 16 + li][lk * 4 + q];
+                bi[i][q] = Bs[buf][1][wn + i * 16 + li][lk * 4 + q];
             }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
-                    accr[i][j] = Mfma<T>::mma(ar[i], br[j], accr[i][j]);
-                    accr[i][j] = Mfma<T>::mma(nai[i], bi[j], accr[i][j]);
-                    acci[i][j] = Mfma<T>::mma(ar[i], bi[j], acci[i][j]);
-                    acci[i][j] = Mfma<T>::mma(ai[i], br[j], acci[i][j]);
+                    accr[i][j] = Mfma<T>::mma(ar[i][q], br[j][q], accr[i][j]);
+                    acci[i][j] = Mfma<T>::mma(ar[i][q], bi[j][q], acci[i][j]);
                 }
-        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    accr[i][j] = Mfma<T>::mma(-ai[i][q], bi[j][q], accr[i][j]);
+                    acci[i][j] = Mfma<T>::mma(ai[i][q], br[j][q], acci[i][j]);
+                }
+        if (kt + 1 < nk) store(buf ^ 1);
         __syncthreads();
     }
 #pragma unroll
@@ -157,9 +197,23 @@ bool gemm_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C
     if (d.M <= 0 || d.N <= 0 || d.K <= 0 || d.outer <= 0 || d.inner <= 0) return false;
     const int tiles = ((d.M + 63) / 64) * ((d.N + 63) / 64);
     dim3 grid(tiles, d.inner, d.outer);
-    hipLaunchKernelGGL(gemm_mfma_kernel<T>, grid, dim3(256), 0, s, d, A, B, C);
-    YQ_HIP(hipGetLastError());
-    return true;
+#define YQ_GEMM_CASE(OA, OB)                                                                          \
+    if (d.opA == OA && d.opB == OB) {                                                                \
+        hipLaunchKernelGGL((gemm_mfma_kernel<T, OA, OB>), grid, dim3(256), 0, s, d, A, B, C);       \
+        YQ_HIP(hipGetLastError());                                                                   \
+        return true;                                                                                 \
+    }
+    YQ_GEMM_CASE(OP_N, OP_N)
+    YQ_GEMM_CASE(OP_N, OP_T)
+    YQ_GEMM_CASE(OP_N, OP_C)
+    YQ_GEMM_CASE(OP_T, OP_N)
+    YQ_GEMM_CASE(OP_T, OP_T)
+    YQ_GEMM_CASE(OP_T, OP_C)
+    YQ_GEMM_CASE(OP_C, OP_N)
+    YQ_GEMM_CASE(OP_C, OP_T)
+    YQ_GEMM_CASE(OP_C, OP_C)
+#undef YQ_GEMM_CASE
+    return false;
 }
 template bool gemm_mfma<float>(const GemmDesc &, const c32 *, const c32 *, c32 *, hipStream_t);
 template bool gemm_mfma<double>(const GemmDesc &, const c64 *, const c64 *, c64 *, hipStream_t);
