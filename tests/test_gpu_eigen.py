@@ -92,11 +92,11 @@ def test_batched_fast_path(n, max_bond):
         ref = np.sort(np.linalg.eigvalsh(G))[::-1]
         assert np.abs(ev[b] - ref).max() < 1e-13 * n
         assert np.abs(ev[b] - ev1[b]).max() < 1e-13 * n
-        Uk = U[b][:, :kout]
-        assert np.abs(Uk.conj().T @ Uk - np.eye(kout)).max() < 1e-12 * n
-        assert np.abs(G @ Uk - Uk * ev[b][:kout]).max() < 1e-13 * n
+        k = keep[b]  # the columns the split uses
+        Uk = U[b][:, :k]
+        assert np.abs(Uk.conj().T @ Uk - np.eye(k)).max() < 1e-12 * n
+        assert np.abs(G @ Uk - Uk * ev[b][:k]).max() < 1e-13 * n
         # same kept subspace as the single-workgroup solver where the cut is in a gap
-        k = keep[b]
         if k < n and ev[b][k - 1] - ev[b][k] > 1e-6:
             Pk = U[b][:, :k] @ U[b][:, :k].conj().T
             P1 = U1[b][:, :k] @ U1[b][:, :k].conj().T
@@ -105,12 +105,13 @@ def test_batched_fast_path(n, max_bond):
 
 def _check(G, tol=1e-12):
     n = G.shape[0]
-    ev, U, _ = _heev(G)
+    ev, U, k = _heev(G)
     ref = np.sort(np.linalg.eigvalsh(G))[::-1]
     scale = max(1.0, np.abs(ref).max())
     assert np.abs(ev - ref).max() < tol * scale * n
-    assert np.abs(U.conj().T @ U - np.eye(n)).max() < 1e-11 * n
-    assert np.abs(G @ U - U * ev).max() < tol * scale * n
+    U = U[:, :k]  # the kept columns
+    assert np.abs(U.conj().T @ U - np.eye(k)).max() < 1e-11 * n
+    assert np.abs(G @ U - U * ev[:k]).max() < tol * scale * n
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 8, 33, 64, 100, 128, 160, 256])

@@ -9,23 +9,23 @@
 //      column: the Hermitian product p = tau A22 v (two column halves per
 //      row), the reduction of p^H v, and the rank-2 update, which also
 //      extracts the next column and its norm.
-//   2. tql_kernel: one LANE per matrix runs implicit QL with Wilkinson shifts
-//      on (d, e) in LDS and records every Givens rotation (c, s, i) to global
-//      memory; the serial rotation chain of 64 matrices shares one wavefront,
-//      so the latency of 1000 chains overlaps.  It then sorts the
-//      eigenvalues, applies the truncation rule and writes the kept rank.
-//   3. tqlvec_kernel: one wavefront per matrix rebuilds only the eigenvector
-//      columns that can be kept, lane = column: x = G_1 ... G_K e_order[c],
-//      applying the recorded rotations in reverse order to an LDS-resident
-//      n x 64 block.
+//   2. stebz_kernel: every eigenvalue of T by bisection on Sturm counts, one
+//      thread per eigenvalue (no serial chain across eigenvalues), to an
+//      absolute width of 2 eps ||T||; sorted output and the kept rank.
+//   3. stein_kernel: the kept eigenvectors of T by inverse iteration, one
+//      lane per eigenvector (clusters closer than 1e-9 ||T|| handled in order
+//      by one lane with re-orthogonalisation), then modified Gram-Schmidt of
+//      the kept columns in LDS.
 //   4. ormtr_kernel: u = H_0 ... H_{n-2} x for the kept columns, with the
 //      n x 64 complex block held in registers (thread = (row residue, column))
 //      and one barrier per reflector.
-// Matrices whose QL does not converge within the rotation budget, and every
-// matrix with n > 128, go through heev_lds_kernel: one workgroup per matrix
-// doing all stages (slower, same conventions).
-// Output: U[:, r] for r < min(n, max_bond) in sorted order (row stride n),
-// evals (sorted descending), perm[r] = r, keep.
+// Matrices whose inverse iteration breaks down (zero or non-finite vector),
+// and every matrix with n > 128 or more than 64 kept columns, go through
+// heev_lds_kernel: one workgroup per matrix doing Householder + implicit QL
+// + back-transform (slower, same conventions).
+// Output: U[:, r] for r < keep (eigenvalues sorted descending, row stride
+// n; the single-workgroup solver also fills the columns up to
+// min(n, max_bond)), evals (sorted descending), perm[r] = r, keep.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -63,7 +63,6 @@ __device__ double bsum(double v, double *sh) {
 
 constexpr int LDS_MAX_N = 128;
 constexpr double EPS = 2.220446049250313e-16;
-constexpr int QL_MAX_ITER = 60;
 
 // Kept rank from eigenvalues sorted descending (ev[0..ns)).
 __device__ int truncation_rank(const double *ev, int ns, int rule, double thr, int max_bond) {
@@ -105,13 +104,11 @@ __device__ int truncation_rank(const double *ev, int ns, int rule, double thr, i
 
 // Per-matrix work area of the fast path (byte offsets).
 struct HeevLay {
-    int64_t V, tau, d, e, rot, ri, order, nrot, X, total;
-    int cap;
+    int64_t V, tau, d, e, lu, X, total;
 };
 __host__ __device__ inline int64_t align256(int64_t x) { return (x + 255) & ~(int64_t)255; }
 __host__ __device__ inline HeevLay heev_lay(int n) {
     HeevLay l;
-    l.cap = 3 * n * n + 64;
     int64_t o = 0;
     l.V = o;
     o = align256(o + (int64_t)n * n * 16);
@@ -121,14 +118,8 @@ __host__ __device__ inline HeevLay heev_lay(int n) {
     o = align256(o + (int64_t)n * 8);
     l.e = o;
     o = align256(o + (int64_t)n * 8);
-    l.rot = o;
-    o = align256(o + (int64_t)l.cap * 16);
-    l.ri = o;
-    o = align256(o + (int64_t)l.cap * 4);
-    l.order = o;
-    o = align256(o + (int64_t)n * 4);
-    l.nrot = o;
-    o = align256(o + 16);
+    l.lu = o;
+    o = align256(o + (int64_t)4 * n * 64 * 8);
     l.X = o;
     o = align256(o + (int64_t)n * 64 * 8);
     l.total = o;
@@ -569,213 +560,254 @@ __global__ __launch_bounds__(TRD_WAVES * 64) void hetrd_kernel(const c64 *__rest
     }
 }
 
-// 2. Implicit QL on (d, e), one lane per matrix; rotations recorded.
-__global__ __launch_bounds__(64) void tql_kernel(char *__restrict__ Wk, int64_t sW, int n, int ns, double *evals,
-                                                 int *perm, int64_t sE, int *keep, int rule, double thr, int max_bond,
-                                                 const int *__restrict__ mask, int *__restrict__ fallback, int B) {
-    extern __shared__ double lds_de[];
-    const int lane = threadIdx.x;
-    const int t = blockIdx.x * 64 + lane;
-    if (t >= B) return;
+// 2. Eigenvalues of T by bisection on Sturm counts, one thread per
+// eigenvalue (ascending index k = thread), to an absolute width of
+// 2 eps ||T||; then sorted output, identity perm and the kept rank.
+__device__ __forceinline__ double fast_rcp(double q) {
+    double r = __builtin_amdgcn_rcp(q);  // hardware estimate, refined twice
+    r = r * fma(-q, r, 2.0);
+    return r * fma(-q, r, 2.0);
+}
+
+__global__ __launch_bounds__(LDS_MAX_N) void stebz_kernel(char *__restrict__ Wk, int64_t sW, int n, int ns,
+                                                          double *evals, int *perm, int64_t sE, int *keep, int rule,
+                                                          double thr, int max_bond, const int *__restrict__ mask) {
+    const int t = blockIdx.x;
     if (mask && !mask[t]) return;
+    const int k = threadIdx.x;
+    __shared__ double dd[LDS_MAX_N], e2[LDS_MAX_N], red[3][2];
+    const HeevLay lay = heev_lay(n);
+    const char *base = Wk + t * sW;
+    const double *dg = (const double *)(base + lay.d);
+    const double *eg = (const double *)(base + lay.e);
+    double lo = 0, hi = 0, ee = 0;
+    if (k < n) {
+        const double di = dg[k], ei = k + 1 < n ? fabs(eg[k]) : 0.0, ep = k > 0 ? fabs(eg[k - 1]) : 0.0;
+        dd[k] = di;
+        e2[k] = ei * ei;
+        lo = di - ei - ep;
+        hi = di + ei + ep;
+        ee = ei * ei;
+    } else {
+        lo = 1e300;
+        hi = -1e300;
+    }
+    // Gershgorin interval and max e^2 over the block (two wavefronts)
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, o, 64));
+        hi = fmax(hi, __shfl_xor(hi, o, 64));
+        ee = fmax(ee, __shfl_xor(ee, o, 64));
+    }
+    if ((k & 63) == 0) {
+        red[0][k >> 6] = lo;
+        red[1][k >> 6] = hi;
+        red[2][k >> 6] = ee;
+    }
+    __syncthreads();
+    const int nw = (blockDim.x + 63) >> 6;
+    lo = red[0][0];
+    hi = red[1][0];
+    ee = red[2][0];
+    for (int i = 1; i < nw; ++i) {
+        lo = fmin(lo, red[0][i]);
+        hi = fmax(hi, red[1][i]);
+        ee = fmax(ee, red[2][i]);
+    }
+    const double tn = fmax(fabs(lo), fabs(hi));
+    const double pivmin = 2.2250738585072014e-308 * fmax(1.0, ee);
+    const double width = 2.0 * EPS * fmax(tn, 1e-300);
+    lo -= width;
+    hi += width;
+    if (k < n) {
+        // invariant: count(lo) <= k < count(hi); the width is reached in
+        // about 53 halvings, the cap only guards against a stuck interval
+        for (int it = 0; it < 128 && hi - lo > width; ++it) {
+            const double x = 0.5 * (lo + hi);
+            int cnt = 0;
+            double q = dd[0] - x;
+            if (fabs(q) <= pivmin) q = -pivmin;
+            cnt += q < 0.0;
+            for (int i = 1; i < n; ++i) {
+                q = (dd[i] - x) - e2[i - 1] * fast_rcp(q);
+                if (fabs(q) <= pivmin) q = -pivmin;
+                cnt += q < 0.0;
+            }
+            if (cnt <= k) lo = x;
+            else hi = x;
+        }
+        evals[t * sE + (n - 1 - k)] = 0.5 * (lo + hi);
+        perm[t * sE + k] = k;
+    }
+    __syncthreads();
+    if (k == 0) keep[t] = truncation_rank(evals + t * sE, ns, rule, thr, max_bond);
+}
+
+// 3. Kept eigenvectors of T by inverse iteration, one wavefront per matrix,
+// lane c = the c-th largest eigenvalue, c < keep.  T - lambda I is factored
+// with partial pivoting (U with two superdiagonals, stored as reciprocal
+// diagonal and superdiagonals, plus the L multipliers; arrays in global
+// memory laid out [i][lane]); pivots below eps ||T|| are replaced by
+// +-eps ||T||; three solves from a fixed pseudo-random start.  Eigenvalues
+// closer than 1e-9 ||T|| form a cluster whose vectors the first lane of the
+// cluster computes in order, orthogonalising each against the previous ones
+// after every solve.  Inverse iteration leaves vectors of eigenvalues g
+// apart orthogonal to about eps ||T|| / g; one final classical Gram-Schmidt
+// pass (largest eigenvalue first) makes them orthonormal.
+__global__ __launch_bounds__(64) void stein_kernel(char *__restrict__ Wk, int64_t sW, int n, int kout,
+                                                   const double *__restrict__ evals, int64_t sE,
+                                                   const int *__restrict__ keep, const int *__restrict__ mask,
+                                                   int *__restrict__ fallback) {
+    extern __shared__ double lds_x[];
+    const int t = blockIdx.x;
+    if (mask && !mask[t]) return;
+    const int c = threadIdx.x;
+    const int kc = min(keep[t], kout);
     const HeevLay lay = heev_lay(n);
     char *base = Wk + t * sW;
     const double *dg = (const double *)(base + lay.d);
     const double *eg = (const double *)(base + lay.e);
-    double2 *rot = (double2 *)(base + lay.rot);
-    int *ri = (int *)(base + lay.ri);
-    double *dd = lds_de + lane;          // dd[i * 64]
-    double *ee = lds_de + n * 64 + lane;  // ee[i * 64]
-#define D(i) dd[(i) * 64]
-#define E(i) ee[(i) * 64]
+    double *ui = (double *)(base + lay.lu);  // [i][64]: 1 / U diagonal
+    double *ub = ui + n * 64;                // U first superdiagonal
+    double *uc = ub + n * 64;                // U second superdiagonal
+    double *ml = uc + n * 64;                // L multipliers
+    double *X = lds_x;                       // X[row * 64 + col]
+    double *dd = lds_x + n * 64, *es = dd + n;
+    __shared__ int bad;
+    for (int i = c; i < n; i += 64) {
+        dd[i] = dg[i];
+        es[i] = i + 1 < n ? eg[i] : 0.0;
+    }
+    if (c == 0) bad = 0;
+    __syncthreads();
+    const double *ev = evals + t * sE;
     double tn = 0;
+    for (int i = 0; i < n; ++i) tn = fmax(tn, fabs(dd[i]) + fabs(es[i]) + (i ? fabs(es[i - 1]) : 0.0));
+    const double tiny = EPS * fmax(tn, 1e-300);
+    const double ctol = 1e-9 * tn;
+    // cluster of lane c: [c, c1) when lane c leads it
+    const bool lead = c < kc && (c == 0 || ev[c - 1] - ev[c] > ctol);
+    int c1 = c + 1;
+    if (lead)
+        while (c1 < kc && ev[c1 - 1] - ev[c1] <= ctol) ++c1;
+    if (lead) {
+        for (int j = c; j < c1; ++j) {
+            const double lam = ev[j];
+            // factor T - lam I = P L U
+            uint64_t piv[2] = {0, 0};
+            double a_cur = dd[0] - lam, b_cur = n > 1 ? es[0] : 0.0;  // current row: diagonal, superdiagonal
+            auto put_diag = [&](int i, double u) {
+                if (fabs(u) < tiny) u = u < 0.0 ? -tiny : tiny;
+                ui[i * 64 + c] = 1.0 / u;
+            };
+            for (int i = 0; i + 1 < n; ++i) {
+                const double ci = es[i];  // subdiagonal below the current row
+                const double a1 = dd[i + 1] - lam;
+                const double b1 = i + 2 < n ? es[i + 1] : 0.0;
+                if (fabs(a_cur) >= fabs(ci)) {
+                    const double l = a_cur != 0.0 ? ci / a_cur : 0.0;
+                    put_diag(i, a_cur);
+                    ub[i * 64 + c] = b_cur;
+                    uc[i * 64 + c] = 0.0;
+                    ml[i * 64 + c] = l;
+                    a_cur = a1 - l * b_cur;
+                    b_cur = b1;
+                } else {  // swap rows i and i+1
+                    const double l = a_cur / ci;
+                    put_diag(i, ci);
+                    ub[i * 64 + c] = a1;
+                    uc[i * 64 + c] = b1;
+                    ml[i * 64 + c] = l;
+                    piv[i >> 6] |= 1ull << (i & 63);
+                    a_cur = b_cur - l * a1;
+                    b_cur = -l * b1;
+                }
+            }
+            put_diag(n - 1, a_cur);
+            double *x = X + j;  // column j, stride 64
+            for (int i = 0; i < n; ++i) {
+                const uint32_t h = (uint32_t)(i * 2654435761u) ^ (uint32_t)((j + 1) * 2246822519u);
+                x[i * 64] = 0.5 + (double)((h >> 8) & 0xffff) * (1.0 / 65536.0);
+            }
+            for (int it = 0; it < 3; ++it) {
+                // forward: apply P and L (r0 carries row i)
+                double r0 = x[0];
+#pragma unroll 8
+                for (int i = 0; i + 1 < n; ++i) {
+                    const double l = ml[i * 64 + c];
+                    double r1 = x[(i + 1) * 64];
+                    if ((piv[i >> 6] >> (i & 63)) & 1ull) {
+                        const double tmp = r0;
+                        r0 = r1;
+                        r1 = tmp;
+                    }
+                    x[i * 64] = r0;
+                    r0 = r1 - l * r0;
+                }
+                x[(n - 1) * 64] = r0;
+                // back substitution with U
+                double y2 = 0.0, y1 = 0.0;
+#pragma unroll 8
+                for (int i = n - 1; i >= 0; --i) {
+                    const double y = (x[i * 64] - ub[i * 64 + c] * y1 - uc[i * 64 + c] * y2) * ui[i * 64 + c];
+                    x[i * 64] = y;
+                    y2 = y1;
+                    y1 = y;
+                }
+                // against the earlier vectors of the cluster
+                for (int p = c; p < j; ++p) {
+                    double s = 0;
+                    for (int i = 0; i < n; ++i) s += X[i * 64 + p] * x[i * 64];
+                    for (int i = 0; i < n; ++i) x[i * 64] -= s * X[i * 64 + p];
+                }
+                double nr = 0;
+                for (int i = 0; i < n; ++i) nr += x[i * 64] * x[i * 64];
+                if (!(nr > 0.0) || !isfinite(nr)) {
+                    bad = 1;
+                    break;
+                }
+                const double inv = 1.0 / sqrt(nr);
+                for (int i = 0; i < n; ++i) x[i * 64] *= inv;
+            }
+        }
+    }
+    if (c >= kc)
+        for (int i = 0; i < n; ++i) X[i * 64 + c] = 0.0;
+    __syncthreads();
+    // One classical Gram-Schmidt pass against all earlier (larger-eigenvalue)
+    // columns, every projection computed from the inputs: with unit columns
+    // whose pairwise products are at most ~eps ||T|| / gap, the result is
+    // orthogonal to second order in those products.
+    // (branch-free: all 64 products are formed, those with p >= c dropped)
+    double sp[64];
+#pragma unroll
+    for (int p = 0; p < 64; ++p) sp[p] = 0.0;
     for (int i = 0; i < n; ++i) {
-        const double di = dg[i], ei = eg[i];
-        D(i) = di;
-        E(i) = ei;
-        tn = fmax(tn, fabs(di) + fabs(ei));
+        const double xc = X[i * 64 + c];
+#pragma unroll
+        for (int p = 0; p < 64; ++p) sp[p] += X[i * 64 + p] * xc;
     }
-    const double abstol = EPS * tn;
-    // e_j negligible: relative to its diagonal neighbours or absolutely (eps ||T||)
-    auto negl = [&](int j) {
-        const double ej = fabs(E(j));
-        return ej <= EPS * (fabs(D(j)) + fabs(D(j + 1))) || ej <= abstol;
-    };
-    auto scan = [&](int from) {
-        int m = from;
-        while (m < n - 1 && !negl(m)) ++m;
-        return m;
-    };
-    int q = 0, l = 0, iter = 0;
-    bool fail = false;
-    int m = scan(0);
-    for (;;) {
-        // d_l has converged while the unreduced block starting at l is 1 x 1
-        bool done = false;
-        while (m == l) {
-            if (++l >= n) {
-                done = true;
-                break;
-            }
-            iter = 0;
-            if (m < l) m = scan(l);
-        }
-        if (done) break;
-        if (iter >= QL_MAX_ITER || q + (m - l) > lay.cap) {
-            fail = true;
-            break;
-        }
-        ++iter;
-        // Wilkinson-type shift from the leading 2x2 of the block [l, m]
-        double g = (D(l + 1) - D(l)) / (2.0 * E(l));
-        const double rr = hypot(g, 1.0);
-        g = D(m) - D(l) + E(l) / (g + (g >= 0 ? rr : -rr));
-        double s = 1.0, c = 1.0, pp = 0.0;
-        int mn = m;  // smallest negligible index in (l, m] after the sweep
-        bool deflated = false;
-        // registers: ei = e_i, di = d_i (prefetched), di1 = d_{i+1} (original),
-        // dn1 = d_{i+2} after this sweep
-        double ei = E(m - 1), di = D(m - 1), di1 = D(m), dn1 = 0.0;
-        for (int i = m - 1; i >= l; --i) {
-            const double f = s * ei, b = c * ei;
-            const double x = f * f + g * g;
-            if (x == 0.0) {  // early deflation: the block splits at i + 1
-                E(i + 1) = 0.0;
-                D(i + 1) = di1 - pp;
-                E(m) = 0.0;
-                mn = i + 1;
-                deflated = true;
-                break;
-            }
-            // 1/sqrt(x): hardware estimate plus two Newton steps
-            double ir = __builtin_amdgcn_rsq(x);
-            const double hx = 0.5 * x;
-            ir = ir * fma(-hx, ir * ir, 1.5);
-            ir = ir * fma(-hx, ir * ir, 1.5);
-            const double r = x * ir;
-            E(i + 1) = r;
-            s = f * ir;
-            c = g * ir;
-            const double ein = i > l ? E(i - 1) : 0.0, din = i > l ? D(i - 1) : 0.0;
-            g = di1 - pp;
-            const double r2 = (di - g) * s + 2.0 * c * b;
-            pp = s * r2;
-            const double dnew = g + pp;
-            D(i + 1) = dnew;
-            g = c * r2 - b;
-            if (i + 1 < m && (r <= EPS * (fabs(dnew) + fabs(dn1)) || r <= abstol)) mn = i + 1;
-            rot[q] = make_double2(c, s);
-            ri[q] = i;
-            ++q;
-            dn1 = dnew;
-            di1 = di;
-            ei = ein;
-            di = din;
-        }
-        if (!deflated) {
-            D(l) = di1 - pp;
-            E(l) = g;
-            E(m) = 0.0;
-        }
-        m = negl(l) ? l : mn;
-        if (m == l) {
-            // converged at l: the next block starts at l + 1 and ends at mn
-            if (++l >= n) break;
-            iter = 0;
-            m = mn;
-        }
-    }
-    *(int *)(base + lay.nrot) = q;
-    fallback[t] = fail ? 1 : 0;
-    if (fail) return;
-    // sort descending (stable by index), rule
-    double *ev = evals + t * sE;
-    int *pm = perm + t * sE;
-    int *order = (int *)(base + lay.order);
+#pragma unroll
+    for (int p = 0; p < 64; ++p) sp[p] = p < c ? sp[p] : 0.0;
+    // in place, row by row: the 64 lanes are one wavefront, so every lane
+    // reads row i before any lane writes it
+    double nr = 0;
     for (int i = 0; i < n; ++i) {
-        const double di = D(i);
-        int rank = 0;
-        for (int j = 0; j < n; ++j) {
-            const double dj = D(j);
-            rank += (dj > di || (dj == di && j < i)) ? 1 : 0;
+        double y0 = X[i * 64 + c], y1 = 0.0, y2 = 0.0, y3 = 0.0;
+#pragma unroll
+        for (int p = 0; p < 64; p += 4) {
+            y0 -= sp[p] * X[i * 64 + p];
+            y1 -= sp[p + 1] * X[i * 64 + p + 1];
+            y2 -= sp[p + 2] * X[i * 64 + p + 2];
+            y3 -= sp[p + 3] * X[i * 64 + p + 3];
         }
-        order[rank] = i;
-        ev[rank] = di;
-        pm[i] = i;
+        const double y = (y0 + y1) + (y2 + y3);
+        X[i * 64 + c] = y;
+        nr += y * y;
     }
-    keep[t] = truncation_rank(ev, ns, rule, thr, max_bond);
-#undef D
-#undef E
-}
-
-// 3. Kept eigenvector columns of T, one wavefront per matrix, lane = column.
-__global__ __launch_bounds__(64) void tqlvec_kernel(char *__restrict__ Wk, int64_t sW, int n, int kout,
-                                                    const int *__restrict__ mask, const int *__restrict__ fallback) {
-    extern __shared__ double lds_x[];
-    const int t = blockIdx.x;
-    if (mask && !mask[t]) return;
-    if (fallback[t]) return;
-    const int c = threadIdx.x;
-    const HeevLay lay = heev_lay(n);
-    char *base = Wk + t * sW;
-    const double2 *__restrict__ rot = (const double2 *)(base + lay.rot);
-    const int *__restrict__ ri = (const int *)(base + lay.ri);
-    const int K = *(const int *)(base + lay.nrot);
-    const int oc = c < kout ? ((const int *)(base + lay.order))[c] : -1;
-    double *X = lds_x + c;  // X[row * 64]
-    for (int row = 0; row < n; ++row) X[row * 64] = row == oc ? 1.0 : 0.0;
-    // Reverse order, G(i): x_i' = c x_i + s x_{i+1}, x_{i+1}' = -s x_i + c x_{i+1}.
-    // Blocks of 64 rotations are fetched one block ahead, lane j holding
-    // rotation q0 - j, and broadcast with readlane.  Within a QL sweep the
-    // indices ascend by one, so x_{i+1} is carried in a register and
-    // x_{i+2} is read one rotation ahead.
-    int ci = -2;
-    double carry = 0.0, pre = 0.0;
-    auto fetch = [&](int q0, double &cx, double &sx, int &ix) {
-        const int qj = q0 - c;
-        if (qj >= 0) {
-            const double2 v = rot[qj];
-            cx = v.x;
-            sx = v.y;
-            ix = ri[qj];
-        } else {
-            cx = 1.0;
-            sx = 0.0;
-            ix = 0;
-        }
-    };
-    double cxn, sxn;
-    int ixn;
-    fetch(K - 1, cxn, sxn, ixn);
-    for (int q0 = K - 1; q0 >= 0; q0 -= 64) {
-        const double cxb = cxn, sxb = sxn;
-        const int ixb = ixn;
-        if (q0 - 64 >= 0) fetch(q0 - 64, cxn, sxn, ixn);
-        const int nb = q0 + 1 < 64 ? q0 + 1 : 64;
-        for (int j = 0; j < nb; ++j) {
-            const int i = __builtin_amdgcn_readlane(ixb, j);
-            const double cc = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(cxb), j),
-                                               __builtin_amdgcn_readlane(__double2loint(cxb), j));
-            const double ss = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(sxb), j),
-                                               __builtin_amdgcn_readlane(__double2loint(sxb), j));
-            double xa, xb;
-            if (i == ci) {
-                xa = carry;
-                xb = pre;
-            } else {
-                if (ci >= 0) X[ci * 64] = carry;
-                xa = X[i * 64];
-                xb = X[(i + 1) * 64];
-            }
-            pre = i + 2 < n ? X[(i + 2) * 64] : 0.0;
-            X[i * 64] = cc * xa + ss * xb;
-            carry = cc * xb - ss * xa;
-            ci = i + 1;
-        }
-    }
-    if (ci >= 0) X[ci * 64] = carry;
+    const double inv = (c < kc && nr > 0.0) ? 1.0 / sqrt(nr) : 0.0;
     double *Xg = (double *)(base + lay.X);
-    for (int row = 0; row < n; ++row) Xg[row * 64 + c] = X[row * 64];
+    for (int i = 0; i < n; ++i) Xg[i * 64 + c] = X[i * 64 + c] * inv;
+    if (c == 0) fallback[t] = bad;
 }
 
 // 4. u = H_0 ... H_{n-2} x for the kept columns (n <= 128, kout <= 64).
@@ -865,11 +897,11 @@ void heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, d
     const size_t sh1 = (size_t)(np + (4 + TRD_PARTS) * n) * sizeof(double2);
     hipLaunchKernelGGL(hetrd_kernel, dim3(B), dim3(TRD_WAVES * 64), sh1, s, G, sG, W, sW, n, mask);
     YQ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(tql_kernel, dim3((B + 63) / 64), dim3(64), (size_t)2 * n * 64 * sizeof(double), s, W, sW, n,
-                       ns, evals, perm, sE, keep, rule, thr, max_bond, mask, fallback, B);
+    hipLaunchKernelGGL(stebz_kernel, dim3(B), dim3(LDS_MAX_N), 0, s, W, sW, n, ns, evals, perm, sE, keep, rule, thr,
+                       max_bond, mask);
     YQ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(tqlvec_kernel, dim3(B), dim3(64), (size_t)n * 64 * sizeof(double), s, W, sW, n, kout, mask,
-                       (const int *)fallback);
+    hipLaunchKernelGGL(stein_kernel, dim3(B), dim3(64), (size_t)(n * 64 + 2 * n) * sizeof(double), s, W, sW, n, kout,
+                       (const double *)evals, sE, (const int *)keep, mask, fallback);
     YQ_HIP(hipGetLastError());
     hipLaunchKernelGGL(ormtr_kernel, dim3(B), dim3(256), 0, s, W, sW, n, kout, U, sU, mask, (const int *)fallback);
     YQ_HIP(hipGetLastError());

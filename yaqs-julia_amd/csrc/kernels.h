@@ -40,7 +40,7 @@ constexpr int LZ_CHUNK_ELEMS = 2048;
 
 // Hermitian eigen-decomposition of a batch of FP64 Gram matrices (eigen.hip):
 // sorted eigenvalues, kept rank under rule (0: T-A, 1: T-C, 2: T-D) and the
-// first min(n, max_bond) eigenvectors as columns of U (row stride n).
+// eigenvectors of the kept rank as the first columns of U (row stride n).
 // work / sW: per-matrix work area of heev_work_bytes(n) bytes (null when that
 // is 0), fallback: B ints; Z / sZ: n*n + 2n + 4 doubles per matrix; Pg / sP:
 // n(n+1)/2 double2 per matrix when n > 128.
