@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
+#include "reduce.h"
 
 namespace yq {
 
@@ -44,10 +45,7 @@ __device__ __forceinline__ double2 c_conj(double2 a) { return make_double2(a.x, 
 
 __host__ __device__ __forceinline__ int pidx(int i, int j) { return i * (i + 1) / 2 + j; }  // j <= i
 
-__device__ double wsum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
+__device__ __forceinline__ double wsum(double v) { return wave_sum(v); }
 
 __device__ double bsum(double v, double *sh) {
     v = wsum(v);
@@ -396,7 +394,7 @@ __global__ __launch_bounds__(256) void heev_lds_kernel(const c64 *G, int64_t sG,
 
 // ------------------------------------------------------------------ fast path
 // 1. Householder tridiagonalisation, one workgroup per matrix (n <= 128).
-constexpr int TRD_WAVES = 8;                  // 512 threads: two waves per SIMD hide LDS latency
+constexpr int TRD_WAVES = 8;  // 512 threads: two waves per SIMD
 constexpr int TRD_PARTS = TRD_WAVES * 64 / 128;  // column parts of the Hermitian product
 __global__ __launch_bounds__(TRD_WAVES * 64) void hetrd_kernel(const c64 *__restrict__ G, int64_t sG,
                                                                char *__restrict__ Wk, int64_t sW, int n,
@@ -466,34 +464,39 @@ __global__ __launch_bounds__(TRD_WAVES * 64) void hetrd_kernel(const c64 *__rest
             const int part = (m + TRD_PARTS - 1) / TRD_PARTS;
             const int c0 = min(m, h * part), c1 = min(m, c0 + part);
             for (int r = tid & 127; r < m; r += 128) {
-                double2 acc0 = make_double2(0, 0), acc1 = make_double2(0, 0);
+                double2 acc[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) acc[u] = make_double2(0, 0);
                 const int rowb = pidx(k + 1 + r, k + 1);  // A22(r, c) = P[rowb + c] for c <= r
                 int colb = pidx(k + 1 + c0, k + 1 + r);    // A22(c, r) = P[colb] for c > r
                 int c = c0;
-                for (; c + 1 < c1; c += 2) {
-                    const bool lo0 = c <= r, lo1 = c + 1 <= r;
-                    const int colb1 = colb + k + 2 + c;
-                    double2 a0 = P[lo0 ? rowb + c : colb];
-                    double2 a1 = P[lo1 ? rowb + c + 1 : colb1];
-                    if (!lo0) a0.y = -a0.y;
-                    if (!lo1) a1.y = -a1.y;
-                    const double2 y0c = c == 0 ? y0 : x[c];
-                    const double2 y1c = x[c + 1];
-                    acc0.x += a0.x * y0c.x - a0.y * y0c.y;
-                    acc0.y += a0.x * y0c.y + a0.y * y0c.x;
-                    acc1.x += a1.x * y1c.x - a1.y * y1c.y;
-                    acc1.y += a1.x * y1c.y + a1.y * y1c.x;
-                    colb = colb1 + k + 3 + c;  // two rows further down the packed triangle
+                for (; c + 3 < c1; c += 4) {
+                    double2 a[4], yv[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const bool lo = c + u <= r;
+                        a[u] = P[lo ? rowb + c + u : colb];
+                        if (!lo) a[u].y = -a[u].y;
+                        yv[u] = c + u == 0 ? y0 : x[c + u];
+                        colb += k + 2 + c + u;  // next row of the packed triangle
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        acc[u].x += a[u].x * yv[u].x - a[u].y * yv[u].y;
+                        acc[u].y += a[u].x * yv[u].y + a[u].y * yv[u].x;
+                    }
                 }
-                if (c < c1) {
+                for (; c < c1; ++c) {
                     const bool lo = c <= r;
                     double2 a = P[lo ? rowb + c : colb];
                     if (!lo) a.y = -a.y;
                     const double2 yc = c == 0 ? y0 : x[c];
-                    acc0.x += a.x * yc.x - a.y * yc.y;
-                    acc0.y += a.x * yc.y + a.y * yc.x;
+                    acc[0].x += a.x * yc.x - a.y * yc.y;
+                    acc[0].y += a.x * yc.y + a.y * yc.x;
+                    colb += k + 2 + c;
                 }
-                pb[h * n + r] = make_double2(acc0.x + acc1.x, acc0.y + acc1.y);
+                pb[h * n + r] = make_double2(acc[0].x + acc[1].x + acc[2].x + acc[3].x,
+                                             acc[0].y + acc[1].y + acc[2].y + acc[3].y);
             }
         }
         __syncthreads();
@@ -530,28 +533,45 @@ __global__ __launch_bounds__(TRD_WAVES * 64) void hetrd_kernel(const c64 *__rest
         const double2 al = c_mul(make_double2(-0.5 * tk.x, -0.5 * tk.y), pv);
         double2 *xn = xb + (cur ^ 1) * n;
         double xsn = 0;
-        for (int r = w; r < m; r += TRD_WAVES) {
-            const double2 vr = vf[r];
-            const double2 pr = pf[r];
+        // two rows per pass (r and r + TRD_WAVES), sharing the column loads
+        for (int r = w; r < m; r += 2 * TRD_WAVES) {
+            const int r2 = r + TRD_WAVES;
+            const bool two = r2 < m;
+            const double2 vr = vf[r], pr = pf[r];
             const double2 wr = make_double2(pr.x + al.x * vr.x - al.y * vr.y, pr.y + al.x * vr.y + al.y * vr.x);
-            const int rowb = pidx(k + 1 + r, k + 1);
-            for (int c = lane; c <= r; c += 64) {
+            const double2 vr2 = two ? vf[r2] : make_double2(0, 0), pr2 = two ? pf[r2] : make_double2(0, 0);
+            const double2 wr2 =
+                make_double2(pr2.x + al.x * vr2.x - al.y * vr2.y, pr2.y + al.x * vr2.y + al.y * vr2.x);
+            const int rowb = pidx(k + 1 + r, k + 1), rowb2 = pidx(k + 1 + r2, k + 1);
+            const int rmax = two ? r2 : r;
+            for (int c = lane; c <= rmax; c += 64) {
                 const double2 vc = vf[c];
                 const double2 pc = pf[c];
                 const double2 wc = make_double2(pc.x + al.x * vc.x - al.y * vc.y, pc.y + al.x * vc.y + al.y * vc.x);
-                double2 a = P[rowb + c];
-                // a -= vr conj(wc) + wr conj(vc)
-                a.x -= vr.x * wc.x + vr.y * wc.y + wr.x * vc.x + wr.y * vc.y;
-                a.y -= vr.y * wc.x - vr.x * wc.y + wr.y * vc.x - wr.x * vc.y;
-                P[rowb + c] = a;
-                if (c == 0 && r >= 1) {
-                    xn[r - 1] = a;
-                    if (r >= 2) xsn += a.x * a.x + a.y * a.y;
+                if (c <= r) {
+                    double2 a = P[rowb + c];
+                    // a -= vr conj(wc) + wr conj(vc)
+                    a.x -= vr.x * wc.x + vr.y * wc.y + wr.x * vc.x + wr.y * vc.y;
+                    a.y -= vr.y * wc.x - vr.x * wc.y + wr.y * vc.x - wr.x * vc.y;
+                    P[rowb + c] = a;
+                    if (c == 0 && r >= 1) {
+                        xn[r - 1] = a;
+                        if (r >= 2) xsn += a.x * a.x + a.y * a.y;
+                    }
+                }
+                if (two) {
+                    double2 a = P[rowb2 + c];
+                    a.x -= vr2.x * wc.x + vr2.y * wc.y + wr2.x * vc.x + wr2.y * vc.y;
+                    a.y -= vr2.y * wc.x - vr2.x * wc.y + wr2.y * vc.x - wr2.x * vc.y;
+                    P[rowb2 + c] = a;
+                    if (c == 0) {
+                        xn[r2 - 1] = a;
+                        if (r2 >= 2) xsn += a.x * a.x + a.y * a.y;
+                    }
                 }
             }
         }
-        xsn = wsum(xsn);
-        if (lane == 0) sxs[cur ^ 1][w] = xsn;
+        if (lane == 0) sxs[cur ^ 1][w] = xsn;  // only lane 0 (column 0) contributes
         __syncthreads();
     }
     if (tid == 0) {
@@ -593,11 +613,9 @@ __global__ __launch_bounds__(LDS_MAX_N) void stebz_kernel(char *__restrict__ Wk,
         hi = -1e300;
     }
     // Gershgorin interval and max e^2 over the block (two wavefronts)
-    for (int o = 32; o > 0; o >>= 1) {
-        lo = fmin(lo, __shfl_xor(lo, o, 64));
-        hi = fmax(hi, __shfl_xor(hi, o, 64));
-        ee = fmax(ee, __shfl_xor(ee, o, 64));
-    }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    ee = wave_max(ee);
     if ((k & 63) == 0) {
         red[0][k >> 6] = lo;
         red[1][k >> 6] = hi;

@@ -54,17 +54,19 @@ __device__ __forceinline__ int64_t b_index(int k, int n, int64_t ld) {
     return OP == OP_N ? (int64_t)k * ld + n : (int64_t)n * ld + k;
 }
 
-// Tile 64 x 64 of C per 256-thread workgroup, K in steps of 16.  LDS rows
-// hold one row of op(A) (or one column of op(B)) with k permuted as
-// (k % 4) * 4 + k / 4, so the four k values a lane feeds to the four MFMAs
-// of a 16-wide k step are one contiguous 4-element read.  The next k tile is
-// loaded into registers while the current one is multiplied (two LDS
-// buffers, one barrier per tile); loads outside the matrix read element 0
-// and are zeroed, so there is no branch between the loads.
-template <typename T, int OPA, int OPB>
+// Tile BM x BN of C per 256-thread workgroup (2 x 2 wavefronts, each
+// (BM/2) x (BN/2) as 16 x 16 MFMA blocks), K in steps of BK.  LDS rows hold
+// one row of op(A) (or one column of op(B)) with k permuted as
+// (k % 4) * (BK / 4) + k / 4, so the BK / 4 values a lane feeds to the MFMAs
+// of consecutive k steps are contiguous.  The next k tile is loaded into
+// registers while the current one is multiplied (two LDS buffers, one
+// barrier per tile); loads outside the matrix read element 0 and are
+// zeroed at the LDS store, so there is no branch between the loads.
+template <typename T, int OPA, int OPB, int BM, int BN, int BK>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T> *__restrict__ A,
                                                         const cplx<T> *__restrict__ B, cplx<T> *__restrict__ C) {
-    constexpr int BM = 64, BN = 64, BK = 16, LDK = BK + 4;
+    constexpr int LDK = BK + 4, KQ = BK / 4;  // k values per lane and tile
+    constexpr int IM = BM / 32, JN = BN / 32;  // 16 x 16 blocks per wave
     typedef typename Mfma<T>::V V;
     const int t = blockIdx.z, jb = blockIdx.y;
     if (d.active && !d.active[t]) return;
@@ -76,51 +78,60 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
     __shared__ __align__(16) T As[2][2][BM][LDK];  // [buffer][re/im][m][k']
     __shared__ __align__(16) T Bs[2][2][BN][LDK];  // [buffer][re/im][n][k']
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
     const int li = lane & 15, lk = lane >> 4;
-    // the four (row, k) pairs this thread loads per tile, in coalescing order
-    int am[4], ak[4], bn[4], bk[4];
+    // Two load mappings, chosen per operand by which index is contiguous in
+    // memory.  Row-contiguous (op(A) with A stored K x M, op(B) = B): thread t
+    // owns row (t % BM) and k group g = t / BM, i.e. the four values
+    // k = g + 4 q that sit contiguously in its permuted LDS row, so loads stay
+    // coalesced along the rows and each plane is one conflict-free 16-byte
+    // store.  K-contiguous (op(A) = A, op(B) with B stored N x K): element
+    // e = t + 256 r is (row e / BK, k e % BK), loads coalesced along k and
+    // scalar LDS stores.
+    static_assert(BM == 64 && BN == 64 && BK == 16, "tile shape outside the load mappings");
+    constexpr bool ROWA = OPA != OP_N, ROWB = OPB == OP_N;
+    int ar_[4], ak_[4], br_[4], bk_[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int e = tid + 256 * r;
-        if (OPA == OP_N) { am[r] = e / BK; ak[r] = e % BK; }
-        else { ak[r] = e / BM; am[r] = e % BM; }
-        if (OPB == OP_N) { bk[r] = e / BN; bn[r] = e % BN; }
-        else { bn[r] = e / BK; bk[r] = e % BK; }
+    for (int q = 0; q < 4; ++q) {
+        const int e = tid + 256 * q;
+        ar_[q] = ROWA ? tid % BM : e / BK;
+        ak_[q] = ROWA ? tid / BM + 4 * q : e % BK;
+        br_[q] = ROWB ? tid % BN : e / BK;
+        bk_[q] = ROWB ? tid / BN + 4 * q : e % BK;
     }
     cplx<T> ra[4], rb[4];
-    unsigned okm = 0;  // in-bounds flags of the pending loads, applied at the LDS store
+    unsigned oka = 0, okb = 0;  // in-bounds flags of the pending loads, applied at the LDS store
     auto load = [&](int k0) {
-        okm = 0;
+        oka = okb = 0;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int gm = m0 + am[r], gk = k0 + ak[r];
+        for (int q = 0; q < 4; ++q) {
+            const int gm = m0 + ar_[q], gk = k0 + ak_[q];
             const bool ok = gm < d.M && gk < d.K;
-            ra[r] = Ab[ok ? a_index<OPA>(gm, gk, d.lda) : 0];
-            const int gn = n0 + bn[r], gk2 = k0 + bk[r];
+            ra[q] = Ab[ok ? a_index<OPA>(gm, gk, d.lda) : 0];
+            oka |= (ok ? 1u : 0u) << q;
+            const int gn = n0 + br_[q], gk2 = k0 + bk_[q];
             const bool ok2 = gn < d.N && gk2 < d.K;
-            rb[r] = Bb[ok2 ? b_index<OPB>(gk2, gn, d.ldb) : 0];
-            okm |= (ok ? 1u : 0u) << r;
-            okm |= (ok2 ? 1u : 0u) << (r + 4);
+            rb[q] = Bb[ok2 ? b_index<OPB>(gk2, gn, d.ldb) : 0];
+            okb |= (ok2 ? 1u : 0u) << q;
         }
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const bool ok = (okm >> r) & 1u, ok2 = (okm >> (r + 4)) & 1u;
-            const int ka = (ak[r] & 3) * 4 + (ak[r] >> 2);
-            As[buf][0][am[r]][ka] = ok ? ra[r].re : (T)0;
-            As[buf][1][am[r]][ka] = ok ? (OPA == OP_C ? -ra[r].im : ra[r].im) : (T)0;
-            const int kb = (bk[r] & 3) * 4 + (bk[r] >> 2);
-            Bs[buf][0][bn[r]][kb] = ok2 ? rb[r].re : (T)0;
-            Bs[buf][1][bn[r]][kb] = ok2 ? (OPB == OP_C ? -rb[r].im : rb[r].im) : (T)0;
+        for (int q = 0; q < 4; ++q) {
+            const bool ok = (oka >> q) & 1u, ok2 = (okb >> q) & 1u;
+            const int ka = (ak_[q] & 3) * KQ + (ak_[q] >> 2);
+            As[buf][0][ar_[q]][ka] = ok ? ra[q].re : (T)0;
+            As[buf][1][ar_[q]][ka] = ok ? (OPA == OP_C ? -ra[q].im : ra[q].im) : (T)0;
+            const int kb = (bk_[q] & 3) * KQ + (bk_[q] >> 2);
+            Bs[buf][0][br_[q]][kb] = ok2 ? rb[q].re : (T)0;
+            Bs[buf][1][br_[q]][kb] = ok2 ? (OPB == OP_C ? -rb[q].im : rb[q].im) : (T)0;
         }
     };
-    V accr[2][2], acci[2][2];
+    V accr[IM][JN], acci[IM][JN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < IM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < JN; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) accr[i][j][r] = acci[i][j][r] = (T)0;
     const int nk = (d.K + BK - 1) / BK;
@@ -130,31 +141,36 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
     for (int kt = 0; kt < nk; ++kt) {
         const int buf = kt & 1;
         if (kt + 1 < nk) load((kt + 1) * BK);
-        T ar[2][4], ai[2][4], br[2][4], bi[2][4];
+        T ar[IM][KQ], ai[IM][KQ], br[JN][KQ], bi[JN][KQ];
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < IM; ++i)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                ar[i][q] = As[buf][0][wm + i * 16 + li][lk * 4 + q];
-                ai[i][q] = As[buf][1][wm + i * 16 + li][lk * 4 + q];
-                br[i][q] = Bs[buf][0][wn + i * 16 + li][lk * 4 + q];
-                bi[i][q] = Bs[buf][1][wn + i * 16 + li][lk * 4 + q];
+            for (int q = 0; q < KQ; ++q) {
+                ar[i][q] = As[buf][0][wm + i * 16 + li][lk * KQ + q];
+                ai[i][q] = As[buf][1][wm + i * 16 + li][lk * KQ + q];
             }
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int j = 0; j < JN; ++j)
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int q = 0; q < KQ; ++q) {
+                br[j][q] = Bs[buf][0][wn + j * 16 + li][lk * KQ + q];
+                bi[j][q] = Bs[buf][1][wn + j * 16 + li][lk * KQ + q];
+            }
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
+        for (int q = 0; q < KQ; ++q)
+#pragma unroll
+            for (int i = 0; i < IM; ++i)
+#pragma unroll
+                for (int j = 0; j < JN; ++j) {
                     accr[i][j] = Mfma<T>::mma(ar[i][q], br[j][q], accr[i][j]);
                     acci[i][j] = Mfma<T>::mma(ar[i][q], bi[j][q], acci[i][j]);
                 }
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < KQ; ++q)
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < IM; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
+                for (int j = 0; j < JN; ++j) {
                     accr[i][j] = Mfma<T>::mma(-ai[i][q], bi[j][q], accr[i][j]);
                     acci[i][j] = Mfma<T>::mma(ai[i][q], br[j][q], acci[i][j]);
                 }
@@ -162,9 +178,9 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < IM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < JN; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int gi = m0 + wm + i * 16 + Mfma<T>::row(lk, r);
@@ -182,6 +198,14 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
             }
 }
 
+// Tile shape (64 x 64 x 16 measured best on the contraction shapes of the
+// engine: larger tiles pad M = 320 and lower the occupancy).
+#ifndef YQ_GEMM_BM
+#define YQ_GEMM_BM 64
+#define YQ_GEMM_BN 64
+#define YQ_GEMM_BK 16
+#endif
+
 static int mfma_mode() {
     static int mode = -1;
     if (mode < 0) {
@@ -195,13 +219,14 @@ template <typename T>
 bool gemm_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s) {
     if (!mfma_mode()) return false;
     if (d.M <= 0 || d.N <= 0 || d.K <= 0 || d.outer <= 0 || d.inner <= 0) return false;
-    const int tiles = ((d.M + 63) / 64) * ((d.N + 63) / 64);
+    constexpr int BM = YQ_GEMM_BM, BN = YQ_GEMM_BN, BK = YQ_GEMM_BK;
+    const int tiles = ((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
     dim3 grid(tiles, d.inner, d.outer);
-#define YQ_GEMM_CASE(OA, OB)                                                                          \
-    if (d.opA == OA && d.opB == OB) {                                                                \
-        hipLaunchKernelGGL((gemm_mfma_kernel<T, OA, OB>), grid, dim3(256), 0, s, d, A, B, C);       \
-        YQ_HIP(hipGetLastError());                                                                   \
-        return true;                                                                                 \
+#define YQ_GEMM_CASE(OA, OB)                                                                            \
+    if (d.opA == OA && d.opB == OB) {                                                                  \
+        hipLaunchKernelGGL((gemm_mfma_kernel<T, OA, OB, BM, BN, BK>), grid, dim3(256), 0, s, d, A, B, C); \
+        YQ_HIP(hipGetLastError());                                                                     \
+        return true;                                                                                   \
     }
     YQ_GEMM_CASE(OP_N, OP_N)
     YQ_GEMM_CASE(OP_N, OP_T)

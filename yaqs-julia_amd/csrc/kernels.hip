@@ -6,6 +6,7 @@
 // eigen-decompositions, transfer matrices and jump probabilities are always
 // computed in double.
 #include "kernels.h"
+#include "reduce.h"
 
 #include <hip/hip_runtime.h>
 
@@ -31,10 +32,6 @@ __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_doub
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 // Block-wide sum of a double (blockDim.x multiple of 64, <= 1024).
 __device__ double block_sum(double v, double *sh) {
@@ -315,7 +312,7 @@ __device__ double2 tridiag_exp_lane(const double *alpha, const double *beta, int
     double bl = (lane >= 1 && lane < k) ? beta[lane] : 0.0;          // coupling to lane-1
     double br = (lane + 1 < k) ? beta[lane + 1] : 0.0;              // coupling to lane+1
     double rown = fabs(a) + fabs(bl) + fabs(br);
-    for (int o = 32; o > 0; o >>= 1) rown = fmax(rown, __shfl_xor(rown, o, 64));
+    rown = wave_max(rown);
     int s = (int)ceil(fabs(tau) * rown / 0.5);
     if (s < 1) s = 1;
     const double h = tau / s;
@@ -324,10 +321,9 @@ __device__ double2 tridiag_exp_lane(const double *alpha, const double *beta, int
         double2 term = v, acc = v;
         for (int m = 1; m <= 40; ++m) {
             // term <- (-i h T) term / m
-            const double tlx = __shfl(term.x, lane - 1 < 0 ? 0 : lane - 1, 64);
-            const double tly = __shfl(term.y, lane - 1 < 0 ? 0 : lane - 1, 64);
-            const double trx = __shfl(term.x, lane + 1 > 63 ? 63 : lane + 1, 64);
-            const double try_ = __shfl(term.y, lane + 1 > 63 ? 63 : lane + 1, 64);
+            // neighbours by DPP wavefront shifts (lane 0 / lane 63 read 0)
+            const double tlx = dpp_move<0x138>(term.x), tly = dpp_move<0x138>(term.y);   // lane - 1
+            const double trx = dpp_move<0x130>(term.x), try_ = dpp_move<0x130>(term.y);  // lane + 1
             double ux = a * term.x + bl * tlx + br * trx;
             double uy = a * term.y + bl * tly + br * try_;
             if (lane >= k) { ux = 0; uy = 0; }
@@ -336,7 +332,7 @@ __device__ double2 tridiag_exp_lane(const double *alpha, const double *beta, int
             term = make_double2(f * uy, -f * ux);
             acc = cadd(acc, term);
             double mag = fabs(term.x) + fabs(term.y);
-            for (int o = 32; o > 0; o >>= 1) mag = fmax(mag, __shfl_xor(mag, o, 64));
+            mag = wave_max(mag);
             if (mag < 1e-20) break;
         }
         v = acc;
@@ -363,7 +359,7 @@ __global__ __launch_bounds__(256) void lz_finalize_kernel(int j, int kmax, doubl
     const double aj = galpha[t];
     if (lane == 0) al[j] = aj;
     const double2 cv = tridiag_exp_lane(al, be, j + 1, dt, lane, aj);
-    const double cjx = __shfl(cv.x, j, 64), cjy = __shfl(cv.y, j, 64);
+    const double cjx = lane_read(cv.x, j), cjy = lane_read(cv.y, j);
     const double err = beta0[t] * b * sqrt(cjx * cjx + cjy * cjy);
     const bool done = (err <= tol) || (b == 0.0) || (j + 1 == kmax);
     if (done) {
