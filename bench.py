@@ -42,6 +42,18 @@ def _pkg():
     return mod
 
 
+def _pmc_traffic():
+    """HBM bytes per window launch of the GEMM kernel from the committed PMC
+    passes (FETCH_SIZE / WRITE_SIZE scaled by a copy-kernel calibration);
+    None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "c2_gemm_mfma_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def c2_job(Y, ntraj, nsteps, precision, L=30, chi=64):
     obs = [Y.Observable(f"z{i}", Y.gates.ZGate(), i) for i in range(1, L + 1)] + \
           [Y.Observable(f"x{i}", Y.gates.XGate(), i) for i in range(1, L + 1)]
@@ -221,7 +233,7 @@ def main():
         "roofline": {
             "bound": "mfma", "kernel": "gemm_mfma_kernel<float, opA, opB> (complex-FP32 real-split MFMA)",
             "achieved": achieved / 1e12, "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / FP32_MFMA_PEAK, "traffic": None,
+            "frac": achieved / FP32_MFMA_PEAK, "traffic": _pmc_traffic(),
             "launches": int(rep["gemm_launches"]),
             "avg_launch_us": 1e3 * rep["gemm_ms"] / max(1, rep["gemm_launches"]),
             "gemm_share_of_window": rep["gemm_ms"] * 1e-3 / rep["timed_s"],
