@@ -227,6 +227,7 @@ def main():
             "ntraj": args.ntraj, "L": L, "chi_max": 64, "precision": args.precision,
             "window_t": [args.warmup * 0.05, (args.warmup + args.steps) * 0.05],
             "max_bond_reached": int(rep["max_bond"]),
+            "concurrent_waves": int(rep["concurrent_waves"]),
             "flop_model_per_traj_site_step": rep["flop_model"] / (args.ntraj * L * args.steps),
             "matvecs": int(rep["matvecs"]),
         },

@@ -223,10 +223,13 @@ int64_t yaqs_job_footprint_bytes(const yaqs_analog_job *job);
 
 /* Measurement entry point (no counterpart in the reference, whose timing is
  * stdout-only, experiments/benchmark_julia.jl:66-69): runs an analog job
- * synchronously on the calling thread, treats its first `warmup_steps` time
- * steps as untimed warm-up and reports the stream-synchronised wall time of
- * the remaining steps, plus HIP-event timing of every matrix-core GEMM
- * launched inside that window. */
+ * synchronously, treats its first `warmup_steps` time steps as untimed
+ * warm-up and reports the stream-synchronised wall time of the remaining
+ * steps, plus HIP-event timing of every matrix-core GEMM launched inside
+ * that window.  Waves run one after the other on one stream (with the
+ * environment variable YAQS_BENCH_SPLIT=1, a job of at least 512
+ * trajectories that fits one wave runs as two concurrent half waves on two
+ * streams whose windows start together; the window is the longer one). */
 typedef struct {
     double timed_s;          /* wall time of the timed steps                      */
     int64_t timed_steps;     /* time steps inside the window, summed over waves   */
@@ -236,6 +239,7 @@ typedef struct {
     double flop_model;       /* reference-order H_eff FLOP in the window          */
     int64_t matvecs;         /* H_eff applies in the window                       */
     int64_t max_bond;
+    int64_t concurrent_waves; /* waves run side by side on separate streams       */
 } yaqs_bench_report;
 
 int yaqs_bench_analog(yaqs_ctx *ctx, const yaqs_analog_job *job, int64_t warmup_steps, double *traj_out,

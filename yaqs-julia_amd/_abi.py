@@ -80,7 +80,7 @@ class JobStats(ct.Structure):
 class BenchReport(ct.Structure):
     _fields_ = [("timed_s", ct.c_double), ("timed_steps", ct.c_int64), ("gemm_ms", ct.c_double),
                 ("gemm_launches", ct.c_int64), ("gemm_flop", ct.c_double), ("flop_model", ct.c_double),
-                ("matvecs", ct.c_int64), ("max_bond", ct.c_int64)]
+                ("matvecs", ct.c_int64), ("max_bond", ct.c_int64), ("concurrent_waves", ct.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

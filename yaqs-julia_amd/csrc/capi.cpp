@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -558,7 +559,6 @@ double yaqs_rng_uniform(uint64_t seed, uint64_t traj, uint64_t k) { return yq::r
 int yaqs_bench_analog(yaqs_ctx *ctx, const yaqs_analog_job *job, int64_t warmup_steps, double *traj_out,
                       double *mean_out, yaqs_bench_report *report) {
     if (!ctx) return fail(YAQS_EINVAL, "ctx is NULL");
-    hipStream_t stream = nullptr;
     try {
         yq::JobIn in = convert_analog(job);
         if (warmup_steps < 0 || warmup_steps + 2 > in.num_times)
@@ -566,37 +566,91 @@ int yaqs_bench_analog(yaqs_ctx *ctx, const yaqs_analog_job *job, int64_t warmup_
         in.bench_warmup = warmup_steps;
         yq::validate_job(in);
         int64_t bytes = 0;
-        const int64_t b = choose_batch(ctx, in, ctx->budget, &bytes);
+        int64_t b = choose_batch(ctx, in, ctx->budget, &bytes);
         if (b < 1) return fail(YAQS_ENOMEM_DEVICE, "job footprint exceeds the HBM budget");
         if (hipSetDevice(ctx->device) != hipSuccess) return fail(YAQS_EDEVICE, "hipSetDevice failed");
-        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
-            return fail(YAQS_EDEVICE, "hipStreamCreate failed");
-        yq::JobResult r;
-        yq::run_job(in, b, stream, r);
-        (void)hipStreamDestroy(stream);
-        stream = nullptr;
-        const int64_t nobs = (int64_t)in.obs.size();
-        if (traj_out && nobs) std::copy(r.traj.begin(), r.traj.begin() + nobs * r.ncol * in.num_traj, traj_out);
-        if (mean_out && nobs) std::copy(r.mean.begin(), r.mean.begin() + nobs * r.ncol, mean_out);
+        // With YAQS_BENCH_SPLIT=1 a large job whose trajectories fit in one wave
+        // runs as two concurrent half waves on two streams (the latency-bound
+        // phases of one overlap the matrix-core phases of the other: faster
+        // while bonds are small, slower at saturated chi=64, so off by
+        // default); otherwise the waves run one after the other on one stream.
+        const int64_t ntraj = in.num_traj;
+        static const bool split_env = [] {
+            const char *e = std::getenv("YAQS_BENCH_SPLIT");
+            return e && e[0] == '1';
+        }();
+        const bool split = split_env && ctx->max_batch == 0 && b >= ntraj && ntraj >= 512;
+        const int nconc = split ? 2 : 1;
+        if (split) b = (ntraj + 1) / 2;
+        in.wsync = split ? std::make_shared<yq::WindowSync>(nconc) : nullptr;
+        const int64_t nout = std::max<int64_t>(1, (int64_t)in.obs.size() * yq::num_columns(in) * ntraj);
+        double *d_out = nullptr;
+        YQ_HIP(hipMalloc(&d_out, nout * sizeof(double)));
+        YQ_HIP(hipMemset(d_out, 0, nout * sizeof(double)));
+        std::vector<yq::JobResult> res(nconc);
+        std::vector<std::string> errs(nconc);
+        auto lane = [&](int w) {
+            hipStream_t stream = nullptr;
+            try {
+                YQ_HIP(hipSetDevice(ctx->device));
+                YQ_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+                for (int64_t t0 = (int64_t)w * b; t0 < ntraj; t0 += (int64_t)nconc * b) {
+                    const int nb = (int)std::min<int64_t>(b, ntraj - t0);
+                    auto eng = yq::make_engine(in, (int)b, stream);
+                    eng->run(t0, nb, d_out, nullptr, res[w]);
+                }
+                YQ_HIP(hipStreamSynchronize(stream));
+            } catch (const std::exception &e) {
+                errs[w] = e.what();
+                if (in.wsync) in.wsync->release();
+            }
+            if (stream) (void)hipStreamDestroy(stream);
+        };
+        if (nconc == 1) {
+            lane(0);
+        } else {
+            std::vector<std::thread> th;
+            for (int w = 0; w < nconc; ++w) th.emplace_back(lane, w);
+            for (auto &t : th) t.join();
+        }
+        for (auto &e : errs)
+            if (!e.empty()) {
+                (void)hipFree(d_out);
+                return fail(YAQS_EDEVICE, e);
+            }
+        std::vector<double> traj((size_t)nout);
+        YQ_HIP(hipMemcpy(traj.data(), d_out, nout * sizeof(double), hipMemcpyDeviceToHost));
+        (void)hipFree(d_out);
+        const int64_t nobs = (int64_t)in.obs.size(), ncol = yq::num_columns(in);
+        if (traj_out && nobs) std::copy(traj.begin(), traj.begin() + nobs * ncol * ntraj, traj_out);
+        if (mean_out && nobs)
+            for (int64_t q = 0; q < nobs; ++q)
+                for (int64_t c = 0; c < ncol; ++c) {
+                    double s = 0;
+                    for (int64_t t = 0; t < ntraj; ++t) s += traj[((size_t)q * ncol + c) * ntraj + t];
+                    mean_out[q * ncol + c] = s / (double)ntraj;
+                }
         if (report) {
-            report->timed_s = r.timed_s;
-            report->timed_steps = r.timed_steps;
-            report->gemm_ms = r.gemm_ms;
-            report->gemm_launches = r.gemm_launches;
-            report->gemm_flop = r.gemm_flop;
-            report->flop_model = r.win_flop;
-            report->matvecs = r.win_matvecs;
-            report->max_bond = r.max_bond;
+            *report = yaqs_bench_report{};
+            for (const auto &r : res) {
+                // concurrent waves share the window start: the window is the longest of them
+                report->timed_s = split ? std::max(report->timed_s, r.timed_s) : report->timed_s + r.timed_s;
+                report->timed_steps += r.timed_steps;
+                report->gemm_ms += r.gemm_ms;
+                report->gemm_launches += r.gemm_launches;
+                report->gemm_flop += r.gemm_flop;
+                report->flop_model += r.win_flop;
+                report->matvecs += r.win_matvecs;
+                report->max_bond = std::max<int64_t>(report->max_bond, r.max_bond);
+            }
+            report->concurrent_waves = nconc;
         }
         return YAQS_OK;
     } catch (const yq::HipError &e) {
-        if (stream) (void)hipStreamDestroy(stream);
         return fail(YAQS_EDEVICE, e.what());
     } catch (const std::invalid_argument &e) {
-        if (stream) (void)hipStreamDestroy(stream);
         return fail(YAQS_EINVAL, e.what());
     } catch (const std::exception &e) {
-        if (stream) (void)hipStreamDestroy(stream);
         return fail(YAQS_EDEVICE, e.what());
     }
 }

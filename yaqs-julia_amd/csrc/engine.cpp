@@ -1172,7 +1172,7 @@ void Engine<T>::run_analog(int64_t t0, int nb_, double *d_out, JobResult &res) {
         if (J.bench_warmup < 0) return;
         if (j == J.bench_warmup + 1) {
             YQ_HIP(hipStreamSynchronize(s));
-            t_win = host_now();
+            t_win = J.wsync ? J.wsync->arrive() : host_now();
             flop0 = res.flop;
             mv0 = res.matvecs;
             g_prof = &prof;
@@ -1482,6 +1482,26 @@ class WaveEngineT : public WaveEngine {
     bool digital;
 };
 }  // namespace
+
+double WindowSync::arrive() {
+    std::unique_lock<std::mutex> lk(mu);
+    if (++arrived == n) {
+        t0 = host_now();
+        cv.notify_all();
+    } else {
+        cv.wait(lk, [&] { return arrived >= n; });
+    }
+    return t0;
+}
+
+void WindowSync::release() {
+    std::lock_guard<std::mutex> lk(mu);
+    if (arrived < n) {
+        arrived = n;
+        t0 = host_now();
+    }
+    cv.notify_all();
+}
 
 std::unique_ptr<WaveEngine> make_engine(const JobIn &job, int batch, hipStream_t stream) {
     validate_job(job);

@@ -3,8 +3,10 @@
 #pragma once
 
 #include <atomic>
+#include <condition_variable>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -52,6 +54,21 @@ struct JobIn {
     double ktol = 1e-10;
     uint64_t seed = 0, traj_offset = 0;
     int64_t bench_warmup = -1;  // >= 0: time the steps after this many (analog)
+    // Concurrent waves of one measurement share the start of the timed window.
+    std::shared_ptr<struct WindowSync> wsync;
+};
+
+// Start barrier of the timed window for waves running concurrently on
+// several streams: every wave syncs its stream, arrives, and all take the
+// time of the last arrival as the common start.
+struct WindowSync {
+    explicit WindowSync(int n) : n(n) {}
+    double arrive();
+    void release();  // a failed wave lets the others go on
+    std::mutex mu;
+    std::condition_variable cv;
+    int n, arrived = 0;
+    double t0 = 0.0;
 };
 
 struct JobResult {
