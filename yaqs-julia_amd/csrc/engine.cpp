@@ -168,7 +168,7 @@ struct Csr {
 };
 struct CsrDev {
     DevBuf rowptr, col, val;
-    int I = 0, J = 0;
+    int I = 0, J = 0, nnz = 0;
 };
 void upload_csr(CsrDev &d, const Csr &c, hipStream_t s) {
     upload(d.rowptr, c.rowptr, s);
@@ -176,6 +176,7 @@ void upload_csr(CsrDev &d, const Csr &c, hipStream_t s) {
     upload(d.val, c.val, s);
     d.I = c.I;
     d.J = c.J;
+    d.nnz = (int)c.col.size();
 }
 // Left form (matvec and left env): i = m d + s, j = s' Dr + n.
 Csr csr_left(const std::vector<c64> &W, int Dl, int d, int Dr) {
@@ -313,6 +314,28 @@ void gemmT(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hi
             p->used += 2;
         }
     }
+}
+
+// GEMM with the MPO step fused into its epilogue (same profiling as gemmT);
+// false when the fused kernel does not cover the shape.
+template <typename T>
+bool gemmW(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, int Dl, int dph, int Dr, int Q, cplx<T> *Y,
+           int64_t sY, const CsrDev &W, hipStream_t s) {
+    GemmProf *p = g_prof;
+    const bool prof = p && p->on;
+    if (prof) {
+        if (p->used + 2 > p->ev.size()) p->flush();
+        YQ_HIP(hipEventRecord(p->ev[p->used], s));
+    }
+    const bool ok = gemm_w_mfma<T>(d, A, B, P, Dl, dph, Dr, Q, Y, sY, W.rowptr.as<int>(), W.col.as<int>(),
+                                   W.val.as<c64>(), W.nnz, s);
+    if (prof && ok) {
+        YQ_HIP(hipEventRecord(p->ev[p->used + 1], s));
+        const int units = (d.active && g_nact >= 0) ? g_nact : d.outer;
+        p->fl[p->used / 2] = 8.0 * d.M * (double)d.N * d.K * d.inner * units;
+        p->used += 2;
+    }
+    return ok;
 }
 
 // --------------------------------------------------------------- engine ----
@@ -819,10 +842,12 @@ void Engine<T>::matvec(State &S, bool two, int i, const CT *x, int64_t sx, CT *y
     const int cl = S.wd[bl], cr = S.wd[br], Dl = Dm[bl], Dr = Dm[br];
     const CsrDev &W = two ? csr2[i] : csrL[i];
     CT *t1 = T1.as<CT>(), *t2 = T2.as<CT>();
-    gemmT<T>(gd(cl * Dl, d * cr, cl, OP_N, OP_N, cl, d * cr, d * cr, slotE[bl], sx, slotS, nb, act),
-             EL[bl].as<CT>(), x, t1, s);
-    wstep<T>(t1, t2, cl, Dl * d, d * Dr, cr, slotS, slotS, W.rowptr.as<int>(), W.col.as<int>(), W.val.as<c64>(), act,
-             nb, s);
+    const GemmDesc g1 = gd(cl * Dl, d * cr, cl, OP_N, OP_N, cl, d * cr, d * cr, slotE[bl], sx, slotS, nb, act);
+    if (!gemmW<T>(g1, EL[bl].as<CT>(), x, cl, Dl, d, Dr, cr, t2, slotS, W, s)) {
+        gemmT<T>(g1, EL[bl].as<CT>(), x, t1, s);
+        wstep<T>(t1, t2, cl, Dl * d, d * Dr, cr, slotS, slotS, W.rowptr.as<int>(), W.col.as<int>(), W.val.as<c64>(),
+                 act, nb, s);
+    }
     gemmT<T>(gd(cl * d, cr, Dr * cr, OP_N, OP_N, Dr * cr, cr, cr, slotS, slotE[br], sy, nb, act), t2,
              ER[br].as<CT>(), y, s);
 }
@@ -911,10 +936,12 @@ template <typename T>
 void Engine<T>::env_left(State &S, int i) {
     const int cl = S.wd[i], cr = S.wd[i + 1], Dl = Dm[i], Dr = Dm[i + 1];
     CT *t1 = T1.as<CT>(), *t2 = T2.as<CT>();
-    gemmT<T>(gd(cl * Dl, 2 * cr, cl, OP_N, OP_N, cl, 2 * cr, 2 * cr, slotE[i], slotA[i], slotS, nb), EL[i].as<CT>(),
-             A(S, i), t1, s);
-    wstep<T>(t1, t2, cl, Dl * 2, 2 * Dr, cr, slotS, slotS, csrL[i].rowptr.as<int>(), csrL[i].col.as<int>(),
-             csrL[i].val.as<c64>(), nullptr, nb, s);
+    const GemmDesc g1 = gd(cl * Dl, 2 * cr, cl, OP_N, OP_N, cl, 2 * cr, 2 * cr, slotE[i], slotA[i], slotS, nb);
+    if (!gemmW<T>(g1, EL[i].as<CT>(), A(S, i), cl, Dl, 2, Dr, cr, t2, slotS, csrL[i], s)) {
+        gemmT<T>(g1, EL[i].as<CT>(), A(S, i), t1, s);
+        wstep<T>(t1, t2, cl, Dl * 2, 2 * Dr, cr, slotS, slotS, csrL[i].rowptr.as<int>(), csrL[i].col.as<int>(),
+                 csrL[i].val.as<c64>(), nullptr, nb, s);
+    }
     gemmT<T>(gd(cr, Dr * cr, 2 * cl, OP_C, OP_N, cr, Dr * cr, Dr * cr, slotA[i], slotS, slotE[i + 1], nb), A(S, i), t2,
              EL[i + 1].as<CT>(), s);
 }

@@ -44,6 +44,17 @@ struct Mfma<double> {
     static __device__ __forceinline__ int row(int lk, int r) { return r * 4 + lk; }
 };
 
+// Arguments of the MPO step fused into a GEMM epilogue; W's non-zeros are
+// staged in LDS (at most W_NNZ_MAX, else the unfused kernels run).
+constexpr int W_NNZ_MAX = 256;
+struct WFuse {
+    int P = 0, Dl = 1, dph = 1, Dr = 1, Q = 0, pb = 0, qb = 0;
+    int64_t sY = 0;
+    const int *rowptr = nullptr, *col = nullptr;
+    const c64 *val = nullptr;
+    void *Y = nullptr;
+};
+
 // Element (m, k) of op(A) and (k, n) of op(B) in row-major storage.
 template <int OP>
 __device__ __forceinline__ int64_t a_index(int m, int k, int64_t ld) {
@@ -62,21 +73,44 @@ __device__ __forceinline__ int64_t b_index(int k, int n, int64_t ld) {
 // registers while the current one is multiplied (two LDS buffers, one
 // barrier per tile); loads outside the matrix read element 0 and are
 // zeroed at the LDS store, so there is no branch between the loads.
-template <typename T, int OPA, int OPB, int BM, int BN, int BK>
+//
+// FW: the MPO step is fused into the epilogue (WFuse).  The GEMM is then
+// T1[(p, m)][(s, q)] = A B with rows (p, m), m < Dl, and columns (s, q),
+// s < dph, q < Q; a tile covers all m of pb = 64 / Dl consecutive p and all
+// s of qb = 64 / dph consecutive q, stages T1 in LDS and writes
+// Y[p][(s', n)][q] = sum W[(s', n)][(m, s)] T1[(p, m)][(s, q)] (W in CSR,
+// rows s' Dr + n, columns m dph + s) straight to memory.
+template <typename T, int OPA, int OPB, int BM, int BN, int BK, bool FW>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T> *__restrict__ A,
-                                                        const cplx<T> *__restrict__ B, cplx<T> *__restrict__ C) {
+                                                        const cplx<T> *__restrict__ B, cplx<T> *__restrict__ C,
+                                                        WFuse w) {
     constexpr int LDK = BK + 4, KQ = BK / 4;  // k values per lane and tile
     constexpr int IM = BM / 32, JN = BN / 32;  // 16 x 16 blocks per wave
     typedef typename Mfma<T>::V V;
     const int t = blockIdx.z, jb = blockIdx.y;
     if (d.active && !d.active[t]) return;
-    const int tilesN = (d.N + BN - 1) / BN;
-    const int m0 = (blockIdx.x / tilesN) * BM, n0 = (blockIdx.x % tilesN) * BN;
+    int m0, n0, p0 = 0, q0 = 0, rows_ok = BM;
+    if (FW) {
+        const int tilesQ = (w.Q + w.qb - 1) / w.qb;
+        p0 = (blockIdx.x / tilesQ) * w.pb;
+        q0 = (blockIdx.x % tilesQ) * w.qb;
+        m0 = p0 * w.Dl;
+        n0 = 0;
+        rows_ok = w.pb * w.Dl;
+    } else {
+        const int tilesN = (d.N + BN - 1) / BN;
+        m0 = (blockIdx.x / tilesN) * BM;
+        n0 = (blockIdx.x % tilesN) * BN;
+    }
     const cplx<T> *Ab = A + t * d.sA_outer + jb * d.sA_inner;
     const cplx<T> *Bb = B + t * d.sB_outer + jb * d.sB_inner;
-    cplx<T> *Cb = C + t * d.sC_outer + jb * d.sC_inner;
-    __shared__ __align__(16) T As[2][2][BM][LDK];  // [buffer][re/im][m][k']
-    __shared__ __align__(16) T Bs[2][2][BN][LDK];  // [buffer][re/im][n][k']
+    constexpr int SM_AB = 2 * 2 * (BM + BN) * LDK;
+    // operand buffers; the fused epilogue reuses them for T1 and appends W
+    constexpr int SM_W = FW ? ((65 + W_NNZ_MAX) * (int)sizeof(int) + W_NNZ_MAX * 2 * (int)sizeof(T)) / (int)sizeof(T) + 2 : 0;
+    constexpr int SM_T1W = FW ? BM * (BN + 1) * 2 + SM_W : 0;
+    __shared__ __align__(16) T smem[SM_AB > SM_T1W ? SM_AB : SM_T1W];
+    T(*As)[2][BM][LDK] = reinterpret_cast<T(*)[2][BM][LDK]>(smem);            // [buffer][re/im][m][k']
+    T(*Bs)[2][BN][LDK] = reinterpret_cast<T(*)[2][BN][LDK]>(smem + 2 * 2 * BM * LDK);  // [buffer][re/im][n][k']
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
     const int li = lane & 15, lk = lane >> 4;
@@ -99,6 +133,22 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
         br_[q] = ROWB ? tid % BN : e / BK;
         bk_[q] = ROWB ? tid / BN + 4 * q : e % BK;
     }
+    // global column of each B element this thread loads (FW: tile column
+    // j -> (s = j / qb, q = q0 + j % qb) -> s Q + q)
+    int bgn[4];
+    bool bcol_ok[4], arow_ok[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (FW) {
+            const int sj = br_[q] / w.qb, qq = q0 + br_[q] % w.qb;
+            bgn[q] = sj * w.Q + qq;
+            bcol_ok[q] = sj < w.dph && qq < w.Q;
+        } else {
+            bgn[q] = n0 + br_[q];
+            bcol_ok[q] = bgn[q] < d.N;
+        }
+        arow_ok[q] = ar_[q] < rows_ok && m0 + ar_[q] < d.M;
+    }
     cplx<T> ra[4], rb[4];
     unsigned oka = 0, okb = 0;  // in-bounds flags of the pending loads, applied at the LDS store
     auto load = [&](int k0) {
@@ -106,11 +156,11 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int gm = m0 + ar_[q], gk = k0 + ak_[q];
-            const bool ok = gm < d.M && gk < d.K;
+            const bool ok = arow_ok[q] && gk < d.K;
             ra[q] = Ab[ok ? a_index<OPA>(gm, gk, d.lda) : 0];
             oka |= (ok ? 1u : 0u) << q;
-            const int gn = n0 + br_[q], gk2 = k0 + bk_[q];
-            const bool ok2 = gn < d.N && gk2 < d.K;
+            const int gn = bgn[q], gk2 = k0 + bk_[q];
+            const bool ok2 = bcol_ok[q] && gk2 < d.K;
             rb[q] = Bb[ok2 ? b_index<OPB>(gk2, gn, d.ldb) : 0];
             okb |= (ok2 ? 1u : 0u) << q;
         }
@@ -177,6 +227,53 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
         if (kt + 1 < nk) store(buf ^ 1);
         __syncthreads();
     }
+    if (FW) {
+        // stage T1 in LDS (the operand buffers are free after the last barrier)
+        static_assert(BM * (BN + 1) * 2 <= SM_AB, "T1 stage exceeds the operand buffers");
+        cplx<T>(*T1s)[BN + 1] = reinterpret_cast<cplx<T>(*)[BN + 1]>(smem);
+#pragma unroll
+        for (int i = 0; i < IM; ++i)
+#pragma unroll
+            for (int j = 0; j < JN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    T1s[wm + i * 16 + Mfma<T>::row(lk, r)][wn + j * 16 + li] = cmake<T>(accr[i][j][r], acci[i][j][r]);
+        __syncthreads();
+        // W in CSR staged next to T1 (at most 64 rows and W_NNZ_MAX non-zeros)
+        const int J = w.dph * w.Dr;
+        int *wrp = reinterpret_cast<int *>(smem + BM * (BN + 1) * 2);
+        int *wcol = wrp + 65;
+        cplx<T> *wval = reinterpret_cast<cplx<T> *>(wcol + W_NNZ_MAX);
+        const int nnz = w.rowptr[J];
+        for (int i = tid; i <= J; i += 256) wrp[i] = w.rowptr[i];
+        for (int i = tid; i < nnz; i += 256) {
+            const int c = w.col[i];
+            wcol[i] = (c / w.dph) * (BN + 1) + (c % w.dph) * w.qb;  // offset of (m, s) in the T1 stage
+            wval[i] = cmake<T>((T)w.val[i].re, (T)w.val[i].im);
+        }
+        __syncthreads();
+        // qb lanes per (p, row of W): q fastest for coalesced stores
+        const int qb = w.qb, qq = tid % qb, slot = tid / qb, nslot = 256 / qb;
+        const int q = q0 + qq;
+        cplx<T> *Yb = reinterpret_cast<cplx<T> *>(w.Y) + t * w.sY;
+        const cplx<T> *T1f = &T1s[0][0];
+        for (int pj = slot; pj < w.pb * J; pj += nslot) {
+            const int pp = pj / J, jj = pj - pp * J;
+            const int p = p0 + pp;
+            if (p >= w.P || q >= w.Q) continue;
+            const cplx<T> *base = T1f + pp * w.Dl * (BN + 1) + qq;
+            T ax = 0, ay = 0;
+            for (int z = wrp[jj]; z < wrp[jj + 1]; ++z) {
+                const cplx<T> wv = wval[z];
+                const cplx<T> x = base[wcol[z]];
+                ax += wv.re * x.re - wv.im * x.im;
+                ay += wv.re * x.im + wv.im * x.re;
+            }
+            Yb[((int64_t)p * J + jj) * w.Q + q] = cmake<T>(ax, ay);
+        }
+        return;
+    }
+    cplx<T> *Cb = C + t * d.sC_outer + jb * d.sC_inner;
 #pragma unroll
     for (int i = 0; i < IM; ++i)
 #pragma unroll
@@ -224,7 +321,8 @@ bool gemm_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C
     dim3 grid(tiles, d.inner, d.outer);
 #define YQ_GEMM_CASE(OA, OB)                                                                            \
     if (d.opA == OA && d.opB == OB) {                                                                  \
-        hipLaunchKernelGGL((gemm_mfma_kernel<T, OA, OB, BM, BN, BK>), grid, dim3(256), 0, s, d, A, B, C); \
+        hipLaunchKernelGGL((gemm_mfma_kernel<T, OA, OB, BM, BN, BK, false>), grid, dim3(256), 0, s, d, A, B, C, \
+                           WFuse{});                                                                   \
         YQ_HIP(hipGetLastError());                                                                     \
         return true;                                                                                   \
     }
@@ -240,6 +338,39 @@ bool gemm_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C
 #undef YQ_GEMM_CASE
     return false;
 }
+template <typename T>
+bool gemm_w_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, int Dl, int dph, int Dr, int Q,
+                 cplx<T> *Y, int64_t sY, const int *rowptr, const int *col, const c64 *val, int nnz, hipStream_t s) {
+    if (!mfma_mode()) return false;
+    if (nnz > W_NNZ_MAX) return false;  // W staged in LDS
+    if (d.opA != OP_N || d.opB != OP_N || d.inner != 1 || d.accumulate) return false;
+    if (Dl < 1 || Dl > 64 || (dph != 2 && dph != 4) || d.M != P * Dl || d.N != dph * Q) return false;
+    if (dph * Dr > 64) return false;  // W rows staged in LDS
+    if (P <= 0 || Q <= 0 || d.K <= 0 || d.outer <= 0) return false;
+    WFuse w;
+    w.P = P;
+    w.Dl = Dl;
+    w.dph = dph;
+    w.Dr = Dr;
+    w.Q = Q;
+    w.pb = 64 / Dl;
+    w.qb = 64 / dph;
+    w.sY = sY;
+    w.rowptr = rowptr;
+    w.col = col;
+    w.val = val;
+    w.Y = Y;
+    const int tiles = ((P + w.pb - 1) / w.pb) * ((Q + w.qb - 1) / w.qb);
+    dim3 grid(tiles, 1, d.outer);
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, OP_N, OP_N, 64, 64, 16, true>), grid, dim3(256), 0, s, d, A, B,
+                       (cplx<T> *)nullptr, w);
+    YQ_HIP(hipGetLastError());
+    return true;
+}
+template bool gemm_w_mfma<float>(const GemmDesc &, const c32 *, const c32 *, int, int, int, int, int, c32 *, int64_t,
+                                 const int *, const int *, const c64 *, int, hipStream_t);
+template bool gemm_w_mfma<double>(const GemmDesc &, const c64 *, const c64 *, int, int, int, int, int, c64 *, int64_t,
+                                  const int *, const int *, const c64 *, int, hipStream_t);
 template bool gemm_mfma<float>(const GemmDesc &, const c32 *, const c32 *, c32 *, hipStream_t);
 template bool gemm_mfma<double>(const GemmDesc &, const c64 *, const c64 *, c64 *, hipStream_t);
 

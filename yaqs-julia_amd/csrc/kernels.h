@@ -14,6 +14,13 @@ void gemm(const GemmDesc &d, const cplx<TA> *A, const cplx<TB> *B, cplx<TC> *C, 
 template <typename T>
 bool gemm_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s);
 
+// T1 = A B (both op N) with rows (p, m < Dl) and columns (s < dph, q < Q),
+// followed in the same kernel by the MPO step into Y (layout of wstep's
+// output).  Returns false when the shape is outside the fused kernel.
+template <typename T>
+bool gemm_w_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, int Dl, int dph, int Dr, int Q,
+                 cplx<T> *Y, int64_t sY, const int *rowptr, const int *col, const c64 *val, int nnz, hipStream_t s);
+
 template <typename T>
 void wstep(const cplx<T> *X, cplx<T> *Y, int P, int I, int J, int Q, int64_t sX, int64_t sY, const int *rowptr,
            const int *col, const c64 *val, const int *active, int B, hipStream_t s);
