@@ -332,7 +332,8 @@ bool gemmW(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, int Dl,
     if (prof && ok) {
         YQ_HIP(hipEventRecord(p->ev[p->used + 1], s));
         const int units = (d.active && g_nact >= 0) ? g_nact : d.outer;
-        p->fl[p->used / 2] = 8.0 * d.M * (double)d.N * d.K * d.inner * units;
+        // GEMM plus the sparse MPO step (one complex MAC per W non-zero per (p, q))
+        p->fl[p->used / 2] = (8.0 * d.M * (double)d.N * d.K * d.inner + 8.0 * W.nnz * (double)P * Q) * units;
         p->used += 2;
     }
     return ok;
