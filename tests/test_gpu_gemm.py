@@ -41,3 +41,77 @@ def test_gemm_forms(prec, tol, mfma):
         for M, N, K in [(64, 64, 16), (70, 33, 21), (5, 130, 67), (320, 256, 64), (1, 1, 1)]:
             err = _run(prec, opA, opB, M, N, K, mfma, rng)
             assert err < tol, (opA, opB, M, N, K, err)
+
+
+def _tile_run(lib, prec, opA, M, N, K, tile, a, b):
+    out = np.zeros(2 * M * N)
+    rc = lib.yaqs_debug_gemm_tile(prec, M, N, K, opA, 0, a.ctypes.data_as(c_dp), b.ctypes.data_as(c_dp),
+                                  out.ctypes.data_as(c_dp), tile)
+    assert rc == 0
+    return out
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 2e-5)])
+def test_tile_shapes_bit_identical(prec, tol):
+    """Every C tile shape the tuner may pick gives the same bits as the
+    default 64 x 64 tile (the same MFMA sequence per output element), and
+    matches numpy."""
+    print(f"case: gemm tile shapes prec={prec}", flush=True)
+    lib = _abi.load_library()
+    lib.yaqs_debug_gemm_tile.argtypes = [ct.c_int] * 6 + [c_dp, c_dp, c_dp, ct.c_int]
+    rng = np.random.default_rng(11)
+    op = {0: lambda X: X, 1: lambda X: X.T, 2: lambda X: X.conj().T}
+    for opA in range(3):
+        for M, N, K in [(80, 64, 16), (320, 256, 64), (33, 47, 70), (128, 20, 320), (1, 3, 5)]:
+            a_shape = (M, K) if opA == 0 else (K, M)
+            A = rng.standard_normal(a_shape) + 1j * rng.standard_normal(a_shape)
+            B = rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N))
+            a = np.ascontiguousarray(A).view(np.float64).ravel()
+            b = np.ascontiguousarray(B).view(np.float64).ravel()
+            base = _tile_run(lib, prec, opA, M, N, K, 0, a, b)
+            ref = op[opA](A) @ B
+            got = base.view(np.complex128).reshape(M, N)
+            assert np.abs(got - ref).max() / max(1.0, np.abs(ref).max()) < tol
+            for tile in range(1, 4):
+                other = _tile_run(lib, prec, opA, M, N, K, tile, a, b)
+                assert np.array_equal(other, base), (opA, M, N, K, tile)
+
+
+def _csr(Wd):
+    rows, cols = np.nonzero(np.abs(Wd) > 0)
+    rowptr = np.zeros(Wd.shape[0] + 1, dtype=np.int32)
+    np.add.at(rowptr, rows + 1, 1)
+    rowptr = np.cumsum(rowptr).astype(np.int32)
+    vals = np.ascontiguousarray(Wd[rows, cols]).view(np.float64).ravel()
+    return rowptr, cols.astype(np.int32), vals
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 2e-5)])
+def test_fused_mpo_step_matches_unfused(prec, tol):
+    """GEMM with the MPO step in its epilogue = GEMM, then wstep, bit for bit
+    (same FP64 accumulation order), and = numpy."""
+    print(f"case: fused MPO step prec={prec}", flush=True)
+    lib = _abi.load_library()
+    ip = ct.POINTER(ct.c_int)
+    lib.yaqs_debug_gemm_w.argtypes = [ct.c_int] * 7 + [c_dp, c_dp, ip, ip, c_dp, c_dp, c_dp]
+    rng = np.random.default_rng(5)
+    for P, Dl, dph, Dr, Q, K in [(64, 5, 2, 5, 64, 64), (13, 5, 4, 5, 37, 29), (64, 3, 2, 3, 128, 64),
+                                 (7, 1, 2, 5, 3, 9), (30, 5, 4, 5, 64, 30)]:
+        M, N, I, J = P * Dl, dph * Q, Dl * dph, dph * Dr
+        A = rng.standard_normal((M, K)) + 1j * rng.standard_normal((M, K))
+        B = rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N))
+        Wd = (rng.standard_normal((J, I)) + 1j * rng.standard_normal((J, I))) * (rng.random((J, I)) < 0.3)
+        rowptr, col, val = _csr(Wd)
+        a = np.ascontiguousarray(A).view(np.float64).ravel()
+        b = np.ascontiguousarray(B).view(np.float64).ravel()
+        yf = np.zeros(2 * P * J * Q)
+        yu = np.zeros(2 * P * J * Q)
+        rc = lib.yaqs_debug_gemm_w(prec, P, Dl, dph, Dr, Q, K, a.ctypes.data_as(c_dp), b.ctypes.data_as(c_dp),
+                                   rowptr.ctypes.data_as(ip), col.ctypes.data_as(ip), val.ctypes.data_as(c_dp),
+                                   yf.ctypes.data_as(c_dp), yu.ctypes.data_as(c_dp))
+        assert rc == 0, (P, Dl, dph, Dr, Q, K)
+        assert np.array_equal(yf, yu), (P, Dl, dph, Dr, Q, K)
+        X = (A @ B).reshape(P, Dl, dph, Q).reshape(P, I, Q)
+        ref = np.einsum("ji,piq->pjq", Wd, X)
+        got = yf.view(np.complex128).reshape(P, J, Q)
+        assert np.abs(got - ref).max() / max(1.0, np.abs(ref).max()) < tol, (P, Dl, dph, Dr, Q, K)

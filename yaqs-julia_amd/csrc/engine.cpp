@@ -13,6 +13,7 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <functional>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -23,6 +24,7 @@
 #include <stdexcept>
 
 #include "kernels.h"
+#include "tune.h"
 
 namespace yq {
 
@@ -296,15 +298,36 @@ thread_local GemmProf *g_prof = nullptr;
 // profiler counts algorithmic FLOP of active trajectories only.
 thread_local int g_nact = -1;
 
+// Tuner keys: dimensions in units of the 32-wide tile granule (K of the
+// 16-deep k tile), batch counts by power of two.
+inline int b32(int64_t x) { return (int)((x + 31) / 32); }
+inline int b16(int64_t x) { return (int)((x + 15) / 16); }
+inline int pow2b(int64_t x) {
+    int b = 0;
+    while ((int64_t)1 << b < x) ++b;
+    return b;
+}
+
 template <typename T>
 void gemmT(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s) {
+    // tile shape: every shape gives the same bits, so it is picked by speed
+    int tile = 0;
+    if (!d.accumulate && d.opB == OP_N && tune_enabled()) {
+        const TuneKey k = {0, (int)sizeof(T), d.opA, d.opB, b32(d.M), b32(d.N), b16(d.K), d.inner, pow2b(d.outer), 0};
+        tile = tuned_variant(
+            k, GEMM_TILES,
+            [&](int v) {
+                if (!gemm_mfma<T>(d, A, B, C, s, v)) gemm<T, T, T, T>(d, A, B, C, s);
+            },
+            s);
+    }
     GemmProf *p = g_prof;
-    const bool prof = p && p->on;
+    const bool prof = p && p->on && !tune_measuring();
     if (prof) {
         if (p->used + 2 > p->ev.size()) p->flush();
         YQ_HIP(hipEventRecord(p->ev[p->used], s));
     }
-    const bool mf = gemm_mfma<T>(d, A, B, C, s);
+    const bool mf = gemm_mfma<T>(d, A, B, C, s, tile);
     if (!mf) gemm<T, T, T, T>(d, A, B, C, s);
     if (prof) {
         if (mf) {
@@ -316,27 +339,45 @@ void gemmT(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hi
     }
 }
 
-// GEMM with the MPO step fused into its epilogue (same profiling as gemmT);
-// false when the fused kernel does not cover the shape.
+// T1 = A B followed by the MPO step into Y: the GEMM with the MPO step
+// fused into its epilogue, or `unfused` (the GEMM into T1, then wstep),
+// whichever the tuner measured faster for the shape (same bits either way).
 template <typename T>
-bool gemmW(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, int Dl, int dph, int Dr, int Q, cplx<T> *Y,
-           int64_t sY, const CsrDev &W, hipStream_t s) {
+void gemmW(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, int Dl, int dph, int Dr, int Q, cplx<T> *Y,
+           int64_t sY, const CsrDev &W, const std::function<void()> &unfused, hipStream_t s) {
+    auto fused = [&]() {
+        return gemm_w_mfma<T>(d, A, B, P, Dl, dph, Dr, Q, Y, sY, W.rowptr.as<int>(), W.col.as<int>(),
+                              W.val.as<c64>(), W.nnz, s);
+    };
+    bool use_fused = gemm_w_covers(d, P, Dl, dph, Dr, Q, W.nnz);
+    if (use_fused && tune_enabled()) {
+        const TuneKey k = {1, (int)sizeof(T), Dl, dph, Dr, b32(P), b32(Q), b16(d.K), W.nnz, pow2b(d.outer)};
+        use_fused = tuned_variant(
+                        k, 2,
+                        [&](int v) {
+                            if (v == 0) (void)fused();
+                            else unfused();
+                        },
+                        s) == 0;
+    }
+    if (!use_fused) {
+        unfused();
+        return;
+    }
     GemmProf *p = g_prof;
-    const bool prof = p && p->on;
+    const bool prof = p && p->on && !tune_measuring();
     if (prof) {
         if (p->used + 2 > p->ev.size()) p->flush();
         YQ_HIP(hipEventRecord(p->ev[p->used], s));
     }
-    const bool ok = gemm_w_mfma<T>(d, A, B, P, Dl, dph, Dr, Q, Y, sY, W.rowptr.as<int>(), W.col.as<int>(),
-                                   W.val.as<c64>(), W.nnz, s);
-    if (prof && ok) {
+    (void)fused();
+    if (prof) {
         YQ_HIP(hipEventRecord(p->ev[p->used + 1], s));
         const int units = (d.active && g_nact >= 0) ? g_nact : d.outer;
         // GEMM plus the sparse MPO step (one complex MAC per W non-zero per (p, q))
         p->fl[p->used / 2] = (8.0 * d.M * (double)d.N * d.K * d.inner + 8.0 * W.nnz * (double)P * Q) * units;
         p->used += 2;
     }
-    return ok;
 }
 
 // --------------------------------------------------------------- engine ----
@@ -844,11 +885,13 @@ void Engine<T>::matvec(State &S, bool two, int i, const CT *x, int64_t sx, CT *y
     const CsrDev &W = two ? csr2[i] : csrL[i];
     CT *t1 = T1.as<CT>(), *t2 = T2.as<CT>();
     const GemmDesc g1 = gd(cl * Dl, d * cr, cl, OP_N, OP_N, cl, d * cr, d * cr, slotE[bl], sx, slotS, nb, act);
-    if (!gemmW<T>(g1, EL[bl].as<CT>(), x, cl, Dl, d, Dr, cr, t2, slotS, W, s)) {
-        gemmT<T>(g1, EL[bl].as<CT>(), x, t1, s);
-        wstep<T>(t1, t2, cl, Dl * d, d * Dr, cr, slotS, slotS, W.rowptr.as<int>(), W.col.as<int>(), W.val.as<c64>(),
-                 act, nb, s);
-    }
+    gemmW<T>(g1, EL[bl].as<CT>(), x, cl, Dl, d, Dr, cr, t2, slotS, W,
+             [&]() {
+                 gemmT<T>(g1, EL[bl].as<CT>(), x, t1, s);
+                 wstep<T>(t1, t2, cl, Dl * d, d * Dr, cr, slotS, slotS, W.rowptr.as<int>(), W.col.as<int>(),
+                          W.val.as<c64>(), act, nb, s);
+             },
+             s);
     gemmT<T>(gd(cl * d, cr, Dr * cr, OP_N, OP_N, Dr * cr, cr, cr, slotS, slotE[br], sy, nb, act), t2,
              ER[br].as<CT>(), y, s);
 }
@@ -938,11 +981,13 @@ void Engine<T>::env_left(State &S, int i) {
     const int cl = S.wd[i], cr = S.wd[i + 1], Dl = Dm[i], Dr = Dm[i + 1];
     CT *t1 = T1.as<CT>(), *t2 = T2.as<CT>();
     const GemmDesc g1 = gd(cl * Dl, 2 * cr, cl, OP_N, OP_N, cl, 2 * cr, 2 * cr, slotE[i], slotA[i], slotS, nb);
-    if (!gemmW<T>(g1, EL[i].as<CT>(), A(S, i), cl, Dl, 2, Dr, cr, t2, slotS, csrL[i], s)) {
-        gemmT<T>(g1, EL[i].as<CT>(), A(S, i), t1, s);
-        wstep<T>(t1, t2, cl, Dl * 2, 2 * Dr, cr, slotS, slotS, csrL[i].rowptr.as<int>(), csrL[i].col.as<int>(),
-                 csrL[i].val.as<c64>(), nullptr, nb, s);
-    }
+    gemmW<T>(g1, EL[i].as<CT>(), A(S, i), cl, Dl, 2, Dr, cr, t2, slotS, csrL[i],
+             [&]() {
+                 gemmT<T>(g1, EL[i].as<CT>(), A(S, i), t1, s);
+                 wstep<T>(t1, t2, cl, Dl * 2, 2 * Dr, cr, slotS, slotS, csrL[i].rowptr.as<int>(),
+                          csrL[i].col.as<int>(), csrL[i].val.as<c64>(), nullptr, nb, s);
+             },
+             s);
     gemmT<T>(gd(cr, Dr * cr, 2 * cl, OP_C, OP_N, cr, Dr * cr, Dr * cr, slotA[i], slotS, slotE[i + 1], nb), A(S, i), t2,
              EL[i + 1].as<CT>(), s);
 }

@@ -2,8 +2,9 @@
 //
 // Complex products are split into real MFMAs on LDS-staged tiles:
 //   Re C += Re A Re B - Im A Im B,  Im C += Re A Im B + Im A Re B.
-// A 256-thread workgroup owns a 64x64 tile of C; each of its four
-// wavefronts owns a 32x32 quarter as 2x2 blocks of 16x16 MFMA outputs
+// A 256-thread workgroup owns a BM x BN tile of C (BM, BN in {32, 64}, the
+// shape chosen per GEMM shape by the engine's tuner); each of its four
+// wavefronts owns a quarter as (BM/32) x (BN/32) blocks of 16x16 MFMA outputs
 // (v_mfma_f32_16x16x4f32 for complex-FP32, v_mfma_f64_16x16x4f64 for
 // complex-FP64).  Tiles of A and B are staged through LDS split into real
 // and imaginary planes; the op (N, T, C) of each operand is a template
@@ -45,7 +46,9 @@ struct Mfma<double> {
 };
 
 // Arguments of the MPO step fused into a GEMM epilogue; W's non-zeros are
-// staged in LDS (at most W_NNZ_MAX, else the unfused kernels run).
+// staged in LDS (at most W_NNZ_MAX, else the unfused kernels run).  The
+// epilogue accumulates in FP64 with FP64 W like wstep_kernel, so fused and
+// unfused results are identical.
 constexpr int W_NNZ_MAX = 256;
 struct WFuse {
     int P = 0, Dl = 1, dph = 1, Dr = 1, Q = 0, pb = 0, qb = 0;
@@ -66,7 +69,9 @@ __device__ __forceinline__ int64_t b_index(int k, int n, int64_t ld) {
 }
 
 // Tile BM x BN of C per 256-thread workgroup (2 x 2 wavefronts, each
-// (BM/2) x (BN/2) as 16 x 16 MFMA blocks), K in steps of BK.  LDS rows hold
+// (BM/2) x (BN/2) as 16 x 16 MFMA blocks), K in steps of BK.  Every output
+// element sees the same sequence of MFMAs whatever the tile shape, so all
+// tile shapes give bit-identical results.  LDS rows hold
 // one row of op(A) (or one column of op(B)) with k permuted as
 // (k % 4) * (BK / 4) + k / 4, so the BK / 4 values a lane feeds to the MFMAs
 // of consecutive k steps are contiguous.  The next k tile is loaded into
@@ -106,7 +111,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
     const cplx<T> *Bb = B + t * d.sB_outer + jb * d.sB_inner;
     constexpr int SM_AB = 2 * 2 * (BM + BN) * LDK;
     // operand buffers; the fused epilogue reuses them for T1 and appends W
-    constexpr int SM_W = FW ? ((65 + W_NNZ_MAX) * (int)sizeof(int) + W_NNZ_MAX * 2 * (int)sizeof(T)) / (int)sizeof(T) + 2 : 0;
+    constexpr int SM_W = FW ? ((65 + W_NNZ_MAX) * (int)sizeof(int) + W_NNZ_MAX * (int)sizeof(c64)) / (int)sizeof(T) + 4 : 0;
     constexpr int SM_T1W = FW ? BM * (BN + 1) * 2 + SM_W : 0;
     __shared__ __align__(16) T smem[SM_AB > SM_T1W ? SM_AB : SM_T1W];
     T(*As)[2][BM][LDK] = reinterpret_cast<T(*)[2][BM][LDK]>(smem);            // [buffer][re/im][m][k']
@@ -116,29 +121,36 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
     const int li = lane & 15, lk = lane >> 4;
     // Two load mappings, chosen per operand by which index is contiguous in
     // memory.  Row-contiguous (op(A) with A stored K x M, op(B) = B): thread t
-    // owns row (t % BM) and k group g = t / BM, i.e. the four values
-    // k = g + 4 q that sit contiguously in its permuted LDS row, so loads stay
-    // coalesced along the rows and each plane is one conflict-free 16-byte
-    // store.  K-contiguous (op(A) = A, op(B) with B stored N x K): element
-    // e = t + 256 r is (row e / BK, k e % BK), loads coalesced along k and
-    // scalar LDS stores.
-    static_assert(BM == 64 && BN == 64 && BK == 16, "tile shape outside the load mappings");
+    // owns row (t % BM) and k group g = t / BM, i.e. the values
+    // k = g + (256 / BM) q (for BM = 64 the four that sit contiguously in its
+    // permuted LDS row, one conflict-free 16-byte store per plane), loads
+    // coalesced along the rows.  K-contiguous (op(A) = A, op(B) with B stored
+    // N x K): element e = t + 256 r is (row e / BK, k e % BK), loads
+    // coalesced along k and scalar LDS stores.
+    static_assert((BM == 32 || BM == 64) && (BN == 32 || BN == 64) && BK == 16, "tile shape outside the load mappings");
+    static_assert(!FW || (BM == 64 && BN == 64), "the fused epilogue assumes 64 x 64 tiles");
+    constexpr int NLA = BM * BK / 256, NLB = BN * BK / 256;  // loads per thread and k tile
+    constexpr int GA = 256 / BM, GB = 256 / BN;              // k groups of the row mapping
     constexpr bool ROWA = OPA != OP_N, ROWB = OPB == OP_N;
-    int ar_[4], ak_[4], br_[4], bk_[4];
+    int ar_[NLA], ak_[NLA], br_[NLB], bk_[NLB];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NLA; ++q) {
         const int e = tid + 256 * q;
         ar_[q] = ROWA ? tid % BM : e / BK;
-        ak_[q] = ROWA ? tid / BM + 4 * q : e % BK;
+        ak_[q] = ROWA ? tid / BM + GA * q : e % BK;
+    }
+#pragma unroll
+    for (int q = 0; q < NLB; ++q) {
+        const int e = tid + 256 * q;
         br_[q] = ROWB ? tid % BN : e / BK;
-        bk_[q] = ROWB ? tid / BN + 4 * q : e % BK;
+        bk_[q] = ROWB ? tid / BN + GB * q : e % BK;
     }
     // global column of each B element this thread loads (FW: tile column
     // j -> (s = j / qb, q = q0 + j % qb) -> s Q + q)
-    int bgn[4];
-    bool bcol_ok[4], arow_ok[4];
+    int bgn[NLB];
+    bool bcol_ok[NLB], arow_ok[NLA];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NLB; ++q) {
         if (FW) {
             const int sj = br_[q] / w.qb, qq = q0 + br_[q] % w.qb;
             bgn[q] = sj * w.Q + qq;
@@ -147,18 +159,22 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
             bgn[q] = n0 + br_[q];
             bcol_ok[q] = bgn[q] < d.N;
         }
-        arow_ok[q] = ar_[q] < rows_ok && m0 + ar_[q] < d.M;
     }
-    cplx<T> ra[4], rb[4];
+#pragma unroll
+    for (int q = 0; q < NLA; ++q) arow_ok[q] = ar_[q] < rows_ok && m0 + ar_[q] < d.M;
+    cplx<T> ra[NLA], rb[NLB];
     unsigned oka = 0, okb = 0;  // in-bounds flags of the pending loads, applied at the LDS store
     auto load = [&](int k0) {
         oka = okb = 0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NLA; ++q) {
             const int gm = m0 + ar_[q], gk = k0 + ak_[q];
             const bool ok = arow_ok[q] && gk < d.K;
             ra[q] = Ab[ok ? a_index<OPA>(gm, gk, d.lda) : 0];
             oka |= (ok ? 1u : 0u) << q;
+        }
+#pragma unroll
+        for (int q = 0; q < NLB; ++q) {
             const int gn = bgn[q], gk2 = k0 + bk_[q];
             const bool ok2 = bcol_ok[q] && gk2 < d.K;
             rb[q] = Bb[ok2 ? b_index<OPB>(gk2, gn, d.ldb) : 0];
@@ -167,11 +183,15 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const bool ok = (oka >> q) & 1u, ok2 = (okb >> q) & 1u;
+        for (int q = 0; q < NLA; ++q) {
+            const bool ok = (oka >> q) & 1u;
             const int ka = (ak_[q] & 3) * KQ + (ak_[q] >> 2);
             As[buf][0][ar_[q]][ka] = ok ? ra[q].re : (T)0;
             As[buf][1][ar_[q]][ka] = ok ? (OPA == OP_C ? -ra[q].im : ra[q].im) : (T)0;
+        }
+#pragma unroll
+        for (int q = 0; q < NLB; ++q) {
+            const bool ok2 = (okb >> q) & 1u;
             const int kb = (bk_[q] & 3) * KQ + (bk_[q] >> 2);
             Bs[buf][0][br_[q]][kb] = ok2 ? rb[q].re : (T)0;
             Bs[buf][1][br_[q]][kb] = ok2 ? (OPB == OP_C ? -rb[q].im : rb[q].im) : (T)0;
@@ -241,15 +261,16 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
         __syncthreads();
         // W in CSR staged next to T1 (at most 64 rows and W_NNZ_MAX non-zeros)
         const int J = w.dph * w.Dr;
-        int *wrp = reinterpret_cast<int *>(smem + BM * (BN + 1) * 2);
+        static_assert(BM * (BN + 1) * 2 * sizeof(T) % 16 == 0, "W stage alignment");
+        c64 *wval = reinterpret_cast<c64 *>(smem + BM * (BN + 1) * 2);
+        int *wrp = reinterpret_cast<int *>(wval + W_NNZ_MAX);
         int *wcol = wrp + 65;
-        cplx<T> *wval = reinterpret_cast<cplx<T> *>(wcol + W_NNZ_MAX);
         const int nnz = w.rowptr[J];
         for (int i = tid; i <= J; i += 256) wrp[i] = w.rowptr[i];
         for (int i = tid; i < nnz; i += 256) {
             const int c = w.col[i];
             wcol[i] = (c / w.dph) * (BN + 1) + (c % w.dph) * w.qb;  // offset of (m, s) in the T1 stage
-            wval[i] = cmake<T>((T)w.val[i].re, (T)w.val[i].im);
+            wval[i] = w.val[i];
         }
         __syncthreads();
         // qb lanes per (p, row of W): q fastest for coalesced stores
@@ -262,14 +283,17 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
             const int p = p0 + pp;
             if (p >= w.P || q >= w.Q) continue;
             const cplx<T> *base = T1f + pp * w.Dl * (BN + 1) + qq;
-            T ax = 0, ay = 0;
+            double ax = 0, ay = 0;
             for (int z = wrp[jj]; z < wrp[jj + 1]; ++z) {
-                const cplx<T> wv = wval[z];
+                const c64 wv = wval[z];
                 const cplx<T> x = base[wcol[z]];
-                ax += wv.re * x.re - wv.im * x.im;
-                ay += wv.re * x.im + wv.im * x.re;
+                // same operation order as wstep_kernel
+                ax = fma(wv.re, (double)x.re, ax);
+                ax = fma(-wv.im, (double)x.im, ax);
+                ay = fma(wv.re, (double)x.im, ay);
+                ay = fma(wv.im, (double)x.re, ay);
             }
-            Yb[((int64_t)p * J + jj) * w.Q + q] = cmake<T>(ax, ay);
+            Yb[((int64_t)p * J + jj) * w.Q + q] = cmake<T>((T)ax, (T)ay);
         }
         return;
     }
@@ -295,14 +319,6 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
             }
 }
 
-// Tile shape (64 x 64 x 16 measured best on the contraction shapes of the
-// engine: larger tiles pad M = 320 and lower the occupancy).
-#ifndef YQ_GEMM_BM
-#define YQ_GEMM_BM 64
-#define YQ_GEMM_BN 64
-#define YQ_GEMM_BK 16
-#endif
-
 static int mfma_mode() {
     static int mode = -1;
     if (mode < 0) {
@@ -312,19 +328,36 @@ static int mfma_mode() {
     return mode;
 }
 
-template <typename T>
-bool gemm_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s) {
-    if (!mfma_mode()) return false;
-    if (d.M <= 0 || d.N <= 0 || d.K <= 0 || d.outer <= 0 || d.inner <= 0) return false;
-    constexpr int BM = YQ_GEMM_BM, BN = YQ_GEMM_BN, BK = YQ_GEMM_BK;
+template <typename T, int OA, int OB, int BM, int BN>
+static void launch_tile(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s) {
     const int tiles = ((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
     dim3 grid(tiles, d.inner, d.outer);
-#define YQ_GEMM_CASE(OA, OB)                                                                            \
-    if (d.opA == OA && d.opB == OB) {                                                                  \
-        hipLaunchKernelGGL((gemm_mfma_kernel<T, OA, OB, BM, BN, BK, false>), grid, dim3(256), 0, s, d, A, B, C, \
-                           WFuse{});                                                                   \
-        YQ_HIP(hipGetLastError());                                                                     \
-        return true;                                                                                   \
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, OA, OB, BM, BN, 16, false>), grid, dim3(256), 0, s, d, A, B, C, WFuse{});
+    YQ_HIP(hipGetLastError());
+}
+
+template <typename T, int OA, int OB>
+static void launch_shape(int tile, const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C,
+                         hipStream_t s) {
+    if constexpr (OB == OP_N) {
+        switch (tile) {
+            case 1: return launch_tile<T, OA, OB, 32, 64>(d, A, B, C, s);
+            case 2: return launch_tile<T, OA, OB, 64, 32>(d, A, B, C, s);
+            case 3: return launch_tile<T, OA, OB, 32, 32>(d, A, B, C, s);
+            default: break;
+        }
+    }
+    launch_tile<T, OA, OB, 64, 64>(d, A, B, C, s);
+}
+
+template <typename T>
+bool gemm_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s, int tile) {
+    if (!mfma_mode()) return false;
+    if (d.M <= 0 || d.N <= 0 || d.K <= 0 || d.outer <= 0 || d.inner <= 0) return false;
+#define YQ_GEMM_CASE(OA, OB)                                    \
+    if (d.opA == OA && d.opB == OB) {                          \
+        launch_shape<T, OA, OB>(tile, d, A, B, C, s);          \
+        return true;                                           \
     }
     YQ_GEMM_CASE(OP_N, OP_N)
     YQ_GEMM_CASE(OP_N, OP_T)
@@ -338,15 +371,20 @@ bool gemm_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C
 #undef YQ_GEMM_CASE
     return false;
 }
-template <typename T>
-bool gemm_w_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, int Dl, int dph, int Dr, int Q,
-                 cplx<T> *Y, int64_t sY, const int *rowptr, const int *col, const c64 *val, int nnz, hipStream_t s) {
+bool gemm_w_covers(const GemmDesc &d, int P, int Dl, int dph, int Dr, int Q, int nnz) {
     if (!mfma_mode()) return false;
     if (nnz > W_NNZ_MAX) return false;  // W staged in LDS
     if (d.opA != OP_N || d.opB != OP_N || d.inner != 1 || d.accumulate) return false;
     if (Dl < 1 || Dl > 64 || (dph != 2 && dph != 4) || d.M != P * Dl || d.N != dph * Q) return false;
     if (dph * Dr > 64) return false;  // W rows staged in LDS
     if (P <= 0 || Q <= 0 || d.K <= 0 || d.outer <= 0) return false;
+    return true;
+}
+
+template <typename T>
+bool gemm_w_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, int Dl, int dph, int Dr, int Q,
+                 cplx<T> *Y, int64_t sY, const int *rowptr, const int *col, const c64 *val, int nnz, hipStream_t s) {
+    if (!gemm_w_covers(d, P, Dl, dph, Dr, Q, nnz)) return false;
     WFuse w;
     w.P = P;
     w.Dl = Dl;
@@ -371,19 +409,22 @@ template bool gemm_w_mfma<float>(const GemmDesc &, const c32 *, const c32 *, int
                                  const int *, const int *, const c64 *, int, hipStream_t);
 template bool gemm_w_mfma<double>(const GemmDesc &, const c64 *, const c64 *, int, int, int, int, int, c64 *, int64_t,
                                   const int *, const int *, const c64 *, int, hipStream_t);
-template bool gemm_mfma<float>(const GemmDesc &, const c32 *, const c32 *, c32 *, hipStream_t);
-template bool gemm_mfma<double>(const GemmDesc &, const c64 *, const c64 *, c64 *, hipStream_t);
+template bool gemm_mfma<float>(const GemmDesc &, const c32 *, const c32 *, c32 *, hipStream_t, int);
+template bool gemm_mfma<double>(const GemmDesc &, const c64 *, const c64 *, c64 *, hipStream_t, int);
 
 }  // namespace yq
 
-// Test hook (not part of the public header): one complex GEMM on the device
-// through either the matrix-core or the generic kernel, for the GEMM unit
-// tests.  A, B, C are host arrays of interleaved doubles, row-major, with
-// op(A) M x K and op(B) K x N; prec 0 = FP64, 1 = FP32.
-extern "C" int yaqs_debug_gemm(int prec, int M, int N, int K, int opA, int opB, const double *A, const double *B,
-                               double *Cout, int use_mfma) {
+// Test hooks (not part of the public header).
+//
+// yaqs_debug_gemm_tile: one complex GEMM on the device through the
+// matrix-core kernel with C tile `tile` (0 .. GEMM_TILES-1), or the generic
+// kernel for tile < 0.  A, B, C are host arrays of interleaved doubles,
+// row-major, with op(A) M x K and op(B) K x N; prec 0 = FP64, 1 = FP32.
+extern "C" int yaqs_debug_gemm_tile(int prec, int M, int N, int K, int opA, int opB, const double *A, const double *B,
+                                    double *Cout, int tile) {
     using namespace yq;
     try {
+        if (tile >= GEMM_TILES) return 1;
         const int64_t na = (int64_t)M * K, nbb = (int64_t)K * N, nc = (int64_t)M * N;
         GemmDesc d;
         d.M = M; d.N = N; d.K = K; d.opA = opA; d.opB = opB;
@@ -404,7 +445,7 @@ extern "C" int yaqs_debug_gemm(int prec, int M, int N, int K, int opA, int opB, 
             YQ_HIP(hipMalloc(&dc, nc * sizeof(cplx<T>)));
             YQ_HIP(hipMemcpy(da, ha.data(), na * sizeof(cplx<T>), hipMemcpyHostToDevice));
             YQ_HIP(hipMemcpy(db, hb.data(), nbb * sizeof(cplx<T>), hipMemcpyHostToDevice));
-            if (!(use_mfma && gemm_mfma<T>(d, da, db, dc, 0))) gemm<T, T, T, T>(d, da, db, dc, 0);
+            if (!(tile >= 0 && gemm_mfma<T>(d, da, db, dc, 0, tile))) gemm<T, T, T, T>(d, da, db, dc, 0);
             YQ_HIP(hipDeviceSynchronize());
             YQ_HIP(hipMemcpy(hc.data(), dc, nc * sizeof(cplx<T>), hipMemcpyDeviceToHost));
             (void)hipFree(da);
@@ -418,6 +459,79 @@ extern "C" int yaqs_debug_gemm(int prec, int M, int N, int K, int opA, int opB, 
         if (prec == 1) run((float *)nullptr);
         else run((double *)nullptr);
         return 0;
+    } catch (const std::exception &) {
+        return 3;
+    }
+}
+
+extern "C" int yaqs_debug_gemm(int prec, int M, int N, int K, int opA, int opB, const double *A, const double *B,
+                               double *Cout, int use_mfma) {
+    return yaqs_debug_gemm_tile(prec, M, N, K, opA, opB, A, B, Cout, use_mfma ? 0 : -1);
+}
+
+// yaqs_debug_gemm_w: T1 = A B (A: (P Dl) x K, B: K x (dph Q), row-major)
+// followed by the MPO step with W in CSR (rows J = dph Dr, columns
+// I = Dl dph, index m dph + s), through the fused kernel (Yf) and through
+// the unfused GEMM + wstep (Yu); Y is P x J x Q.  Returns 1 when the fused
+// kernel does not cover the shape.
+extern "C" int yaqs_debug_gemm_w(int prec, int P, int Dl, int dph, int Dr, int Q, int K, const double *A,
+                                 const double *B, const int *rowptr, const int *col, const double *val, double *Yf,
+                                 double *Yu) {
+    using namespace yq;
+    try {
+        const int M = P * Dl, N = dph * Q, J = dph * Dr, nnz = rowptr[J];
+        const int64_t na = (int64_t)M * K, nbb = (int64_t)K * N, nc = (int64_t)M * N, ny = (int64_t)P * J * Q;
+        GemmDesc d;
+        d.M = M; d.N = N; d.K = K; d.opA = OP_N; d.opB = OP_N;
+        d.lda = K; d.ldb = N; d.ldc = N;
+        d.sA_outer = d.sB_outer = d.sC_outer = 0;
+        d.sA_inner = d.sB_inner = d.sC_inner = 0;
+        d.inner = 1; d.outer = 1; d.accumulate = 0; d.active = nullptr;
+        int rc = 0;
+        auto run = [&](auto *tag) {
+            typedef std::remove_pointer_t<decltype(tag)> T;
+            std::vector<cplx<T>> ha(na), hb(nbb), hy(ny);
+            for (int64_t i = 0; i < na; ++i) ha[i] = cmake<T>((T)A[2 * i], (T)A[2 * i + 1]);
+            for (int64_t i = 0; i < nbb; ++i) hb[i] = cmake<T>((T)B[2 * i], (T)B[2 * i + 1]);
+            cplx<T> *da, *db, *dt, *dy;
+            int *drp, *dcol;
+            c64 *dval;
+            YQ_HIP(hipMalloc(&da, na * sizeof(cplx<T>)));
+            YQ_HIP(hipMalloc(&db, nbb * sizeof(cplx<T>)));
+            YQ_HIP(hipMalloc(&dt, nc * sizeof(cplx<T>)));
+            YQ_HIP(hipMalloc(&dy, ny * sizeof(cplx<T>)));
+            YQ_HIP(hipMalloc(&drp, (J + 1) * sizeof(int)));
+            YQ_HIP(hipMalloc(&dcol, (nnz + 1) * sizeof(int)));
+            YQ_HIP(hipMalloc(&dval, (nnz + 1) * sizeof(c64)));
+            YQ_HIP(hipMemcpy(da, ha.data(), na * sizeof(cplx<T>), hipMemcpyHostToDevice));
+            YQ_HIP(hipMemcpy(db, hb.data(), nbb * sizeof(cplx<T>), hipMemcpyHostToDevice));
+            YQ_HIP(hipMemcpy(drp, rowptr, (J + 1) * sizeof(int), hipMemcpyHostToDevice));
+            YQ_HIP(hipMemcpy(dcol, col, nnz * sizeof(int), hipMemcpyHostToDevice));
+            YQ_HIP(hipMemcpy(dval, val, nnz * sizeof(c64), hipMemcpyHostToDevice));
+            auto fetch = [&](double *out) {
+                YQ_HIP(hipDeviceSynchronize());
+                YQ_HIP(hipMemcpy(hy.data(), dy, ny * sizeof(cplx<T>), hipMemcpyDeviceToHost));
+                for (int64_t i = 0; i < ny; ++i) {
+                    out[2 * i] = (double)hy[i].re;
+                    out[2 * i + 1] = (double)hy[i].im;
+                }
+            };
+            if (gemm_w_mfma<T>(d, da, db, P, Dl, dph, Dr, Q, dy, 0, drp, dcol, dval, nnz, 0)) fetch(Yf);
+            else rc = 1;
+            if (!gemm_mfma<T>(d, da, db, dt, 0, 0)) gemm<T, T, T, T>(d, da, db, dt, 0);
+            wstep<T>(dt, dy, P, Dl * dph, J, Q, 0, 0, drp, dcol, dval, nullptr, 1, 0);
+            fetch(Yu);
+            (void)hipFree(da);
+            (void)hipFree(db);
+            (void)hipFree(dt);
+            (void)hipFree(dy);
+            (void)hipFree(drp);
+            (void)hipFree(dcol);
+            (void)hipFree(dval);
+        };
+        if (prec == 1) run((float *)nullptr);
+        else run((double *)nullptr);
+        return rc;
     } catch (const std::exception &) {
         return 3;
     }

@@ -183,8 +183,11 @@ __global__ __launch_bounds__(256) void wstep_kernel(const cplx<T> *__restrict__ 
         for (int z = rowptr[j]; z < rowptr[j + 1]; ++z) {
             const c64 w = val[z];
             const double2 xv = ld2(x[(int64_t)col[z] * Q]);
-            acc.x += w.re * xv.x - w.im * xv.y;
-            acc.y += w.re * xv.y + w.im * xv.x;
+            // explicit FMAs: the fused GEMM epilogue repeats this order exactly
+            acc.x = fma(w.re, xv.x, acc.x);
+            acc.x = fma(-w.im, xv.y, acc.x);
+            acc.y = fma(w.re, xv.y, acc.y);
+            acc.y = fma(w.im, xv.x, acc.y);
         }
         st2(y[(int64_t)j * Q], acc);
     }
