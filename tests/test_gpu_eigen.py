@@ -1,9 +1,10 @@
 """Batched Hermitian eigensolver of the bond splits (Householder + implicit
 QL, FP64) against numpy.linalg.eigh, on Gram matrices like those of the
-2-site splits: full rank, rank-deficient, degenerate and zero-padded.  Both
-the multi-kernel path (n <= 128 with at most 64 kept columns) and the
-single-workgroup solver (larger n or kept widths, QL budget overruns) are
-covered; the residual bound is ||G u - lambda u|| <= tol * ||G||."""
+2-site splits: full rank, rank-deficient, degenerate and zero-padded.  The
+multi-kernel path (n <= 128 with at most 64 kept columns), the wide path
+(n <= 512, at most 256 kept columns) and the single-workgroup solver
+(everything else, inverse-iteration breakdowns) are covered; the residual
+bound is ||G u - lambda u|| <= tol * ||G||."""
 import ctypes as ct
 
 import numpy as np
@@ -97,6 +98,31 @@ def test_batched_fast_path(n, max_bond):
         assert np.abs(Uk.conj().T @ Uk - np.eye(k)).max() < 1e-12 * n
         assert np.abs(G @ Uk - Uk * ev[b][:k]).max() < 1e-13 * n
         # same kept subspace as the single-workgroup solver where the cut is in a gap
+        if k < n and ev[b][k - 1] - ev[b][k] > 1e-6:
+            Pk = U[b][:, :k] @ U[b][:, :k].conj().T
+            P1 = U1[b][:, :k] @ U1[b][:, :k].conj().T
+            assert np.abs(Pk - P1).max() < 1e-9
+
+
+@pytest.mark.parametrize("n,max_bond", [(130, 64), (200, 100), (256, 128), (256, 256), (384, 128), (512, 256)])
+def test_batched_wide_path(n, max_bond):
+    """The global-memory path (128 < n <= 512, up to 256 kept columns) on a
+    batch of 10 matrices of every spectrum kind, against numpy and against
+    the single-workgroup solver."""
+    print(f"case: heev wide batch n={n} max_bond={max_bond}", flush=True)
+    Gs = _gram_batch(n, 10, n + 1)
+    ev, U, keep, fb = _heev_batch(Gs, max_bond=max_bond)
+    ev1, U1, keep1, _ = _heev_batch(Gs, max_bond=max_bond, path=1)
+    assert fb.sum() == 0
+    assert np.array_equal(keep, keep1)
+    for b in range(len(Gs)):
+        G = Gs[b]
+        ref = np.sort(np.linalg.eigvalsh(G))[::-1]
+        assert np.abs(ev[b] - ref).max() < 1e-13 * n
+        k = keep[b]
+        Uk = U[b][:, :k]
+        assert np.abs(Uk.conj().T @ Uk - np.eye(k)).max() < 1e-12 * n
+        assert np.abs(G @ Uk - Uk * ev[b][:k]).max() < 1e-13 * n
         if k < n and ev[b][k - 1] - ev[b][k] > 1e-6:
             Pk = U[b][:, :k] @ U[b][:, :k].conj().T
             P1 = U1[b][:, :k] @ U1[b][:, :k].conj().T

@@ -19,14 +19,22 @@
 //   4. ormtr_kernel: u = H_0 ... H_{n-2} x for the kept columns, with the
 //      n x 64 complex block held in registers (thread = (row residue, column))
 //      and one barrier per reflector.
+// Wide path (128 < n <= 512, at most 256 kept columns): the same four steps
+// with the matrix in global memory (hetrd_wide_kernel, full storage, one
+// 1024-thread workgroup per matrix), eigenvalues by the same bisection,
+// inverse iteration with the vectors in global memory (stein_wide_kernel),
+// the Gram-Schmidt pass as its own kernel (cgs_wide_kernel) and the
+// back-transform with 32 row residues per 32 columns.
 // Matrices whose inverse iteration breaks down (zero or non-finite vector),
-// and every matrix with n > 128 or more than 64 kept columns, go through
-// heev_lds_kernel: one workgroup per matrix doing Householder + implicit QL
-// + back-transform (slower, same conventions).
+// and every matrix outside both paths, go through heev_lds_kernel: one
+// workgroup per matrix doing Householder + implicit QL + back-transform
+// (slower, same conventions).
 // Output: U[:, r] for r < keep (eigenvalues sorted descending, row stride
 // n; the single-workgroup solver also fills the columns up to
 // min(n, max_bond)), evals (sorted descending), perm[r] = r, keep.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "kernels.h"
 #include "reduce.h"
@@ -123,6 +131,36 @@ __host__ __device__ inline HeevLay heev_lay(int n) {
     l.total = o;
     return l;
 }
+
+// Per-matrix work area of the wide path (128 < n <= 512, kw = kept columns
+// rounded up to 64, <= 256): A is the full-storage working copy of the
+// matrix (row k right of the diagonal receives v_k), LU / X / X2 are laid
+// out [row][kw].
+constexpr int WIDE_MAX_N = 512, WIDE_MAX_K = 256;
+struct HeevLayW {
+    int64_t A, tau, d, e, lu, X, X2, total;
+};
+__host__ __device__ inline HeevLayW heev_lay_wide(int n, int kw) {
+    HeevLayW l;
+    int64_t o = 0;
+    l.A = o;
+    o = align256(o + (int64_t)n * n * 16);
+    l.tau = o;
+    o = align256(o + (int64_t)n * 16);
+    l.d = o;
+    o = align256(o + (int64_t)n * 8);
+    l.e = o;
+    o = align256(o + (int64_t)n * 8);
+    l.lu = o;
+    o = align256(o + (int64_t)4 * n * kw * 8);
+    l.X = o;
+    o = align256(o + (int64_t)n * kw * 8);
+    l.X2 = o;
+    o = align256(o + (int64_t)n * kw * 8);
+    l.total = o;
+    return l;
+}
+inline int round64(int k) { return (k + 63) / 64 * 64; }
 
 }  // namespace
 
@@ -589,17 +627,19 @@ __device__ __forceinline__ double fast_rcp(double q) {
     return r * fma(-q, r, 2.0);
 }
 
-__global__ __launch_bounds__(LDS_MAX_N) void stebz_kernel(char *__restrict__ Wk, int64_t sW, int n, int ns,
-                                                          double *evals, int *perm, int64_t sE, int *keep, int rule,
-                                                          double thr, int max_bond, const int *__restrict__ mask) {
+// NMAX: largest n of the launch (blockDim >= n, a multiple of 64); the
+// tridiagonal (d, e) sits at byte offsets od / oe of each work area.
+template <int NMAX>
+__global__ __launch_bounds__(NMAX) void stebz_kernel(char *__restrict__ Wk, int64_t sW, int64_t od, int64_t oe, int n,
+                                                     int ns, double *evals, int *perm, int64_t sE, int *keep, int rule,
+                                                     double thr, int max_bond, const int *__restrict__ mask) {
     const int t = blockIdx.x;
     if (mask && !mask[t]) return;
     const int k = threadIdx.x;
-    __shared__ double dd[LDS_MAX_N], e2[LDS_MAX_N], red[3][2];
-    const HeevLay lay = heev_lay(n);
+    __shared__ double dd[NMAX], e2[NMAX], red[3][NMAX / 64];
     const char *base = Wk + t * sW;
-    const double *dg = (const double *)(base + lay.d);
-    const double *eg = (const double *)(base + lay.e);
+    const double *dg = (const double *)(base + od);
+    const double *eg = (const double *)(base + oe);
     double lo = 0, hi = 0, ee = 0;
     if (k < n) {
         const double di = dg[k], ei = k + 1 < n ? fabs(eg[k]) : 0.0, ep = k > 0 ? fabs(eg[k - 1]) : 0.0;
@@ -828,27 +868,34 @@ __global__ __launch_bounds__(64) void stein_kernel(char *__restrict__ Wk, int64_
     if (c == 0) fallback[t] = bad;
 }
 
-// 4. u = H_0 ... H_{n-2} x for the kept columns (n <= 128, kout <= 64).
-__global__ __launch_bounds__(256) void ormtr_kernel(char *__restrict__ Wk, int64_t sW, int n, int kout,
-                                                    c64 *__restrict__ U, int64_t sU, const int *__restrict__ mask,
-                                                    const int *__restrict__ fallback) {
+// 4. u = H_0 ... H_{n-2} x for the kept columns: CW columns per workgroup
+// (blockIdx.y selects them) and NR = NWV * 64 / CW row residues, each
+// thread holding rows rres + NR j of its column in registers (n <= NMAX).
+// v_k is row k of the matrix at byte offset ov (entries k+1 .. n-1), x is
+// laid out [row][ldx] at offset ox.
+template <int NMAX, int NWV, int CW>
+__global__ __launch_bounds__(NWV * 64) void ormtr_kernel(char *__restrict__ Wk, int64_t sW, int64_t ov, int64_t otau,
+                                                         int64_t ox, int ldx, int n, int kout, c64 *__restrict__ U,
+                                                         int64_t sU, const int *__restrict__ mask,
+                                                         const int *__restrict__ fallback) {
+    constexpr int NR = NWV * 64 / CW, NJ = NMAX / NR;
     const int t = blockIdx.x;
     if (mask && !mask[t]) return;
     if (fallback[t]) return;
-    const int tid = threadIdx.x, c = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    __shared__ double2 vb[3][LDS_MAX_N];
-    __shared__ double2 part[2][4][64];
-    const HeevLay lay = heev_lay(n);
+    const int tid = threadIdx.x, cl = tid % CW;
+    const int rres = CW == 64 ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid / CW;  // wave-uniform for CW = 64
+    const int c = blockIdx.y * CW + cl;
+    __shared__ double2 vb[3][NMAX];
+    __shared__ double2 part[2][NR][CW];
     const char *base = Wk + t * sW;
-    const double2 *__restrict__ Vg = (const double2 *)(base + lay.V);
-    const double2 *__restrict__ taug = (const double2 *)(base + lay.tau);
-    const double *__restrict__ Xg = (const double *)(base + lay.X);
-    constexpr int NJ = LDS_MAX_N / 4;
+    const double2 *__restrict__ Vg = (const double2 *)(base + ov);
+    const double2 *__restrict__ taug = (const double2 *)(base + otau);
+    const double *__restrict__ Xg = (const double *)(base + ox);
     double2 u[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
-        const int r = w + 4 * j;
-        u[j] = make_double2(r < n ? Xg[r * 64 + c] : 0.0, 0.0);
+        const int r = rres + NR * j;
+        u[j] = make_double2(r < n && c < ldx ? Xg[(int64_t)r * ldx + c] : 0.0, 0.0);
     }
     if (n >= 2 && tid < n) vb[(n - 2) % 3][tid] = tid >= n - 1 ? Vg[(int64_t)(n - 2) * n + tid] : make_double2(0, 0);
     __syncthreads();
@@ -860,21 +907,25 @@ __global__ __launch_bounds__(256) void ormtr_kernel(char *__restrict__ Wk, int64
         double2 acc = make_double2(0, 0);
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
-            const int r = w + 4 * j;
+            const int r = rres + NR * j;
             if (r > k && r < n) {
                 const double2 z = c_mulc(v[r], u[j]);
                 acc.x += z.x;
                 acc.y += z.y;
             }
         }
-        part[k & 1][w][c] = acc;
+        part[k & 1][rres][cl] = acc;
         __syncthreads();
-        const double2 s = make_double2(part[k & 1][0][c].x + part[k & 1][1][c].x + part[k & 1][2][c].x + part[k & 1][3][c].x,
-                                       part[k & 1][0][c].y + part[k & 1][1][c].y + part[k & 1][2][c].y + part[k & 1][3][c].y);
+        double2 s = make_double2(0, 0);
+#pragma unroll
+        for (int h = 0; h < NR; ++h) {
+            s.x += part[k & 1][h][cl].x;
+            s.y += part[k & 1][h][cl].y;
+        }
         const double2 f = c_mul(tk, s);
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
-            const int r = w + 4 * j;
+            const int r = rres + NR * j;
             if (r > k && r < n) {
                 const double2 z = c_mul(v[r], f);
                 u[j].x -= z.x;
@@ -886,13 +937,311 @@ __global__ __launch_bounds__(256) void ormtr_kernel(char *__restrict__ Wk, int64
         c64 *Ut = U + t * sU;
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
-            const int r = w + 4 * j;
+            const int r = rres + NR * j;
             if (r < n) Ut[(int64_t)r * n + c] = cmake<double>(u[j].x, u[j].y);
         }
     }
 }
 
-int64_t heev_work_bytes(int n) { return n <= LDS_MAX_N ? heev_lay(n).total : 0; }
+// ------------------------------------------------------------- wide path
+// 1w. Householder tridiagonalisation for 128 < n <= 512: one 1024-thread
+// workgroup per matrix on a full-storage copy A in global memory (the whole
+// Hermitian matrix is kept so that every access is a contiguous row: the
+// column below the diagonal is read as the conjugate of row k, and one
+// wavefront handles one row of the Hermitian product and of the rank-2
+// update).  v_k goes to row k right of the diagonal, which the reduction no
+// longer reads.  Same conventions as hetrd_kernel.
+constexpr int WT_WAVES = 16;
+__global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_kernel(const c64 *__restrict__ G, int64_t sG,
+                                                                   char *__restrict__ Wk, int64_t sW, int n, int kw,
+                                                                   const int *__restrict__ mask) {
+    constexpr int NT = WT_WAVES * 64;
+    const int t = blockIdx.x;
+    if (mask && !mask[t]) return;
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    __shared__ double2 vs[WIDE_MAX_N], ps[WIDE_MAX_N];
+    __shared__ double rs[WT_WAVES];
+    __shared__ double2 rs2[WT_WAVES];
+    const HeevLayW lay = heev_lay_wide(n, kw);
+    char *base = Wk + t * sW;
+    double2 *Am = (double2 *)(base + lay.A);
+    double2 *taug = (double2 *)(base + lay.tau);
+    double *dg = (double *)(base + lay.d);
+    double *eg = (double *)(base + lay.e);
+    const c64 *Gt = G + t * sG;
+    for (int64_t e = tid; e < (int64_t)n * n; e += NT) Am[e] = make_double2(Gt[e].re, Gt[e].im);
+    __syncthreads();
+    for (int k = 0; k + 1 < n; ++k) {
+        const int m = n - k - 1;
+        double2 *rowk = Am + (int64_t)k * n + k + 1;  // conj of the column below the diagonal
+        double xs = 0;
+        for (int r = 1 + tid; r < m; r += NT) {
+            const double2 a = rowk[r];
+            xs += a.x * a.x + a.y * a.y;
+        }
+        xs = wsum(xs);
+        if (lane == 0) rs[w] = xs;
+        __syncthreads();
+        xs = 0;
+#pragma unroll
+        for (int i = 0; i < WT_WAVES; ++i) xs += rs[i];
+        const double2 alpha = c_conj(rowk[0]);
+        double2 tk = make_double2(0, 0), inv = make_double2(0, 0);
+        double beta = alpha.x;
+        if (!(xs == 0.0 && alpha.y == 0.0)) {
+            const double nr = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xs);
+            beta = alpha.x >= 0 ? -nr : nr;
+            tk = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+            const double2 den = make_double2(alpha.x - beta, alpha.y);
+            const double dn = den.x * den.x + den.y * den.y;
+            inv = make_double2(den.x / dn, -den.y / dn);
+        }
+        for (int r = tid; r < m; r += NT) vs[r] = r == 0 ? make_double2(1.0, 0.0) : c_mul(inv, c_conj(rowk[r]));
+        if (tid == 0) {
+            dg[k] = Am[(int64_t)k * n + k].x;
+            eg[k] = beta;
+            taug[k] = tk;
+        }
+        __syncthreads();
+        for (int r = tid; r < m; r += NT) rowk[r] = vs[r];
+        // p = tau A22 v, one wavefront per row
+        for (int rr = w; rr < m; rr += WT_WAVES) {
+            const double2 *arow = Am + (int64_t)(k + 1 + rr) * n + k + 1;
+            double2 acc = make_double2(0, 0);
+            for (int c = lane; c < m; c += 64) {
+                const double2 z = c_mul(arow[c], vs[c]);
+                acc.x += z.x;
+                acc.y += z.y;
+            }
+            acc.x = wsum(acc.x);
+            acc.y = wsum(acc.y);
+            if (lane == 0) ps[rr] = c_mul(tk, acc);
+        }
+        __syncthreads();
+        double2 pvp = make_double2(0, 0);
+        for (int r = tid; r < m; r += NT) {
+            const double2 z = c_mulc(ps[r], vs[r]);
+            pvp.x += z.x;
+            pvp.y += z.y;
+        }
+        pvp.x = wsum(pvp.x);
+        pvp.y = wsum(pvp.y);
+        if (lane == 0) rs2[w] = pvp;
+        __syncthreads();
+        double2 pv = make_double2(0, 0);
+#pragma unroll
+        for (int i = 0; i < WT_WAVES; ++i) {
+            pv.x += rs2[i].x;
+            pv.y += rs2[i].y;
+        }
+        const double2 al = c_mul(make_double2(-0.5 * tk.x, -0.5 * tk.y), pv);
+        // A22 -= v w^H + w v^H with w = p + al v
+        for (int rr = w; rr < m; rr += WT_WAVES) {
+            double2 *arow = Am + (int64_t)(k + 1 + rr) * n + k + 1;
+            const double2 vr = vs[rr], pr = ps[rr];
+            const double2 wr = make_double2(pr.x + al.x * vr.x - al.y * vr.y, pr.y + al.x * vr.y + al.y * vr.x);
+            for (int c = lane; c < m; c += 64) {
+                const double2 vc = vs[c], pc = ps[c];
+                const double2 wc = make_double2(pc.x + al.x * vc.x - al.y * vc.y, pc.y + al.x * vc.y + al.y * vc.x);
+                double2 a = arow[c];
+                a.x -= vr.x * wc.x + vr.y * wc.y + wr.x * vc.x + wr.y * vc.y;
+                a.y -= vr.y * wc.x - vr.x * wc.y + wr.y * vc.x - wr.x * vc.y;
+                arow[c] = a;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        dg[n - 1] = Am[(int64_t)(n - 1) * n + n - 1].x;
+        eg[n - 1] = 0.0;
+    }
+}
+
+// 3w. Kept eigenvectors of T by inverse iteration for the wide path: the
+// algorithm of stein_kernel with kw lanes (kw / 64 wavefronts, lane c = the
+// c-th largest eigenvalue) and X in global memory [row][kw]; the
+// orthogonalisation pass is cgs_wide_kernel.
+__global__ __launch_bounds__(WIDE_MAX_K) void stein_wide_kernel(char *__restrict__ Wk, int64_t sW, int n, int kw,
+                                                                const double *__restrict__ evals, int64_t sE,
+                                                                const int *__restrict__ keep,
+                                                                const int *__restrict__ mask, int *__restrict__ fallback) {
+    const int t = blockIdx.x;
+    if (mask && !mask[t]) return;
+    const int c = threadIdx.x;
+    const int kc = min(keep[t], kw);
+    const HeevLayW lay = heev_lay_wide(n, kw);
+    char *base = Wk + t * sW;
+    const double *dg = (const double *)(base + lay.d);
+    const double *eg = (const double *)(base + lay.e);
+    double *ui = (double *)(base + lay.lu);  // [i][kw]: 1 / U diagonal
+    double *ub = ui + (int64_t)n * kw;       // U first superdiagonal
+    double *uc = ub + (int64_t)n * kw;       // U second superdiagonal
+    double *ml = uc + (int64_t)n * kw;       // L multipliers
+    double *X = (double *)(base + lay.X);    // X[row * kw + col]
+    __shared__ double dd[WIDE_MAX_N], es[WIDE_MAX_N];
+    __shared__ int bad;
+    for (int i = c; i < n; i += blockDim.x) {
+        dd[i] = dg[i];
+        es[i] = i + 1 < n ? eg[i] : 0.0;
+    }
+    if (c == 0) bad = 0;
+    __syncthreads();
+    const double *ev = evals + t * sE;
+    double tn = 0;
+    for (int i = 0; i < n; ++i) tn = fmax(tn, fabs(dd[i]) + fabs(es[i]) + (i ? fabs(es[i - 1]) : 0.0));
+    const double tiny = EPS * fmax(tn, 1e-300);
+    const double ctol = 1e-9 * tn;
+    const bool lead = c < kc && (c == 0 || ev[c - 1] - ev[c] > ctol);
+    int c1 = c + 1;
+    if (lead)
+        while (c1 < kc && ev[c1 - 1] - ev[c1] <= ctol) ++c1;
+    if (lead) {
+        for (int j = c; j < c1; ++j) {
+            const double lam = ev[j];
+            uint64_t piv[WIDE_MAX_N / 64];
+#pragma unroll
+            for (int q = 0; q < WIDE_MAX_N / 64; ++q) piv[q] = 0;
+            double a_cur = dd[0] - lam, b_cur = n > 1 ? es[0] : 0.0;
+            auto put_diag = [&](int i, double u) {
+                if (fabs(u) < tiny) u = u < 0.0 ? -tiny : tiny;
+                ui[(int64_t)i * kw + c] = 1.0 / u;
+            };
+            for (int i = 0; i + 1 < n; ++i) {
+                const double ci = es[i];
+                const double a1 = dd[i + 1] - lam;
+                const double b1 = i + 2 < n ? es[i + 1] : 0.0;
+                const int64_t o = (int64_t)i * kw + c;
+                if (fabs(a_cur) >= fabs(ci)) {
+                    const double l = a_cur != 0.0 ? ci / a_cur : 0.0;
+                    put_diag(i, a_cur);
+                    ub[o] = b_cur;
+                    uc[o] = 0.0;
+                    ml[o] = l;
+                    a_cur = a1 - l * b_cur;
+                    b_cur = b1;
+                } else {
+                    const double l = a_cur / ci;
+                    put_diag(i, ci);
+                    ub[o] = a1;
+                    uc[o] = b1;
+                    ml[o] = l;
+                    piv[i >> 6] |= 1ull << (i & 63);
+                    a_cur = b_cur - l * a1;
+                    b_cur = -l * b1;
+                }
+            }
+            put_diag(n - 1, a_cur);
+            double *x = X + j;  // column j, stride kw
+            for (int i = 0; i < n; ++i) {
+                const uint32_t h = (uint32_t)(i * 2654435761u) ^ (uint32_t)((j + 1) * 2246822519u);
+                x[(int64_t)i * kw] = 0.5 + (double)((h >> 8) & 0xffff) * (1.0 / 65536.0);
+            }
+            for (int it = 0; it < 3; ++it) {
+                double r0 = x[0];
+                for (int i = 0; i + 1 < n; ++i) {
+                    const double l = ml[(int64_t)i * kw + c];
+                    double r1 = x[(int64_t)(i + 1) * kw];
+                    if ((piv[i >> 6] >> (i & 63)) & 1ull) {
+                        const double tmp = r0;
+                        r0 = r1;
+                        r1 = tmp;
+                    }
+                    x[(int64_t)i * kw] = r0;
+                    r0 = r1 - l * r0;
+                }
+                x[(int64_t)(n - 1) * kw] = r0;
+                double y2 = 0.0, y1 = 0.0;
+                for (int i = n - 1; i >= 0; --i) {
+                    const int64_t o = (int64_t)i * kw + c;
+                    const double y = (x[(int64_t)i * kw] - ub[o] * y1 - uc[o] * y2) * ui[o];
+                    x[(int64_t)i * kw] = y;
+                    y2 = y1;
+                    y1 = y;
+                }
+                for (int p = c; p < j; ++p) {
+                    double sp = 0;
+                    for (int i = 0; i < n; ++i) sp += X[(int64_t)i * kw + p] * x[(int64_t)i * kw];
+                    for (int i = 0; i < n; ++i) x[(int64_t)i * kw] -= sp * X[(int64_t)i * kw + p];
+                }
+                double nr = 0;
+                for (int i = 0; i < n; ++i) nr += x[(int64_t)i * kw] * x[(int64_t)i * kw];
+                if (!(nr > 0.0) || !isfinite(nr)) {
+                    bad = 1;
+                    break;
+                }
+                const double inv = 1.0 / sqrt(nr);
+                for (int i = 0; i < n; ++i) x[(int64_t)i * kw] *= inv;
+            }
+        }
+    }
+    __syncthreads();
+    if (c >= kc)
+        for (int i = 0; i < n; ++i) X[(int64_t)i * kw + c] = 0.0;
+    if (c == 0) fallback[t] = bad;
+}
+
+// 3w'. One classical Gram-Schmidt pass over the kept columns (as at the end
+// of stein_kernel): S[p][c] = x_p . x_c for p < c (into the LU area), then
+// X2[:, c] = x_c - sum_{p<c} S[p][c] x_p from the unmodified inputs, then
+// unit columns written back to X.  Thread c owns column c.
+__global__ __launch_bounds__(WIDE_MAX_K) void cgs_wide_kernel(char *__restrict__ Wk, int64_t sW, int n, int kw,
+                                                              const int *__restrict__ keep, const int *__restrict__ mask,
+                                                              const int *__restrict__ fallback) {
+    const int t = blockIdx.x;
+    if (mask && !mask[t]) return;
+    if (fallback[t]) return;
+    const int c = threadIdx.x;
+    const int kc = min(keep[t], kw);
+    const HeevLayW lay = heev_lay_wide(n, kw);
+    char *base = Wk + t * sW;
+    double *S = (double *)(base + lay.lu);  // [p][kw]
+    double *X = (double *)(base + lay.X);
+    double *X2 = (double *)(base + lay.X2);
+    constexpr int CH = 16;
+    // wave-uniform bound on the columns any lane of this wave needs
+    const int cmax = min(kw, (c | 63) + 1);
+    for (int p0 = 0; p0 < cmax; p0 += CH) {
+        double acc[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) acc[q] = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double xc = X[(int64_t)i * kw + c];
+#pragma unroll
+            for (int q = 0; q < CH; ++q) acc[q] += X[(int64_t)i * kw + p0 + q] * xc;
+        }
+#pragma unroll
+        for (int q = 0; q < CH; ++q) S[(int64_t)(p0 + q) * kw + c] = p0 + q < c ? acc[q] : 0.0;
+    }
+    for (int i = 0; i < n; ++i) X2[(int64_t)i * kw + c] = X[(int64_t)i * kw + c];
+    for (int p0 = 0; p0 < cmax; p0 += CH) {
+        double sp[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) sp[q] = S[(int64_t)(p0 + q) * kw + c];
+        for (int i = 0; i < n; ++i) {
+            double y = X2[(int64_t)i * kw + c];
+#pragma unroll
+            for (int q = 0; q < CH; ++q) y -= sp[q] * X[(int64_t)i * kw + p0 + q];
+            X2[(int64_t)i * kw + c] = y;
+        }
+    }
+    double nr = 0;
+    for (int i = 0; i < n; ++i) {
+        const double y = X2[(int64_t)i * kw + c];
+        nr += y * y;
+    }
+    const double inv = (c < kc && nr > 0.0) ? 1.0 / sqrt(nr) : 0.0;
+    __syncthreads();  // every lane has read the original X
+    for (int i = 0; i < n; ++i) X[(int64_t)i * kw + c] = X2[(int64_t)i * kw + c] * inv;
+}
+
+int64_t heev_work_bytes(int n, int kmax) {
+    // calls with any n' <= n use the same slots
+    const int64_t narrow = heev_lay(std::min(n, LDS_MAX_N)).total;
+    if (n <= LDS_MAX_N) return narrow;
+    if (n > WIDE_MAX_N) return narrow;
+    const int kw = round64(std::min(std::min(n, kmax), WIDE_MAX_K));
+    return std::max(narrow, heev_lay_wide(n, kw).total);
+}
 
 void heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, double2 *Pg, int64_t sP, void *work,
           int64_t sW, int *fallback, int n, int ns, double *evals, int *perm, int64_t sE, int *keep, int rule,
@@ -900,30 +1249,52 @@ void heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, d
     if (n > 1024) throw HipError("heev: n > 1024");
     int kout = ns < max_bond ? ns : max_bond;
     if (kout > n) kout = n;
-    const bool fast = n <= LDS_MAX_N && kout <= 64 && work && fallback;
+    const bool fast = n <= LDS_MAX_N && kout <= 64 && work && fallback && heev_lay(n).total <= sW;
+    const int kw = round64(kout);
+    const bool wide = n > LDS_MAX_N && n <= WIDE_MAX_N && kw <= WIDE_MAX_K && work && fallback &&
+                      heev_lay_wide(n, kw).total <= sW;
     const int np = n * (n + 1) / 2;
     size_t shmem = (size_t)n * (4 * sizeof(double) + sizeof(int)) + 16;
     if (n <= LDS_MAX_N) shmem += (size_t)(np + n) * sizeof(double2);
     else if (!Pg) throw HipError("heev: no global scratch for n > 128");
-    if (!fast) {
+    char *W = (char *)work;
+    if (fast) {
+        const HeevLay lay = heev_lay(n);
+        const size_t sh1 = (size_t)(np + (4 + TRD_PARTS) * n) * sizeof(double2);
+        hipLaunchKernelGGL(hetrd_kernel, dim3(B), dim3(TRD_WAVES * 64), sh1, s, G, sG, W, sW, n, mask);
+        YQ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(stebz_kernel<LDS_MAX_N>, dim3(B), dim3(LDS_MAX_N), 0, s, W, sW, lay.d, lay.e, n, ns, evals,
+                           perm, sE, keep, rule, thr, max_bond, mask);
+        YQ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(stein_kernel, dim3(B), dim3(64), (size_t)(n * 64 + 2 * n) * sizeof(double), s, W, sW, n,
+                           kout, (const double *)evals, sE, (const int *)keep, mask, fallback);
+        YQ_HIP(hipGetLastError());
+        hipLaunchKernelGGL((ormtr_kernel<LDS_MAX_N, 4, 64>), dim3(B, 1), dim3(256), 0, s, W, sW, lay.V, lay.tau, lay.X, 64,
+                           n, kout, U, sU, mask, (const int *)fallback);
+        YQ_HIP(hipGetLastError());
+    } else if (wide) {
+        const HeevLayW lay = heev_lay_wide(n, kw);
+        hipLaunchKernelGGL(hetrd_wide_kernel, dim3(B), dim3(WT_WAVES * 64), 0, s, G, sG, W, sW, n, kw, mask);
+        YQ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(stebz_kernel<WIDE_MAX_N>, dim3(B), dim3(round64(n)), 0, s, W, sW, lay.d, lay.e, n, ns,
+                           evals, perm, sE, keep, rule, thr, max_bond, mask);
+        YQ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(stein_wide_kernel, dim3(B), dim3(kw), 0, s, W, sW, n, kw, (const double *)evals, sE,
+                           (const int *)keep, mask, fallback);
+        YQ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(cgs_wide_kernel, dim3(B), dim3(kw), 0, s, W, sW, n, kw, (const int *)keep, mask,
+                           (const int *)fallback);
+        YQ_HIP(hipGetLastError());
+        hipLaunchKernelGGL((ormtr_kernel<WIDE_MAX_N, 16, 32>), dim3(B, kw / 32), dim3(1024), 0, s, W, sW, lay.A, lay.tau,
+                           lay.X, kw, n, kout, U, sU, mask, (const int *)fallback);
+        YQ_HIP(hipGetLastError());
+    } else {
         hipLaunchKernelGGL(heev_lds_kernel, dim3(B), dim3(256), shmem, s, G, sG, U, sU, Z, sZ, Pg, sP, n, ns, evals,
                            perm, sE, keep, rule, thr, max_bond, mask, (const int *)nullptr);
         YQ_HIP(hipGetLastError());
         return;
     }
-    char *W = (char *)work;
-    const size_t sh1 = (size_t)(np + (4 + TRD_PARTS) * n) * sizeof(double2);
-    hipLaunchKernelGGL(hetrd_kernel, dim3(B), dim3(TRD_WAVES * 64), sh1, s, G, sG, W, sW, n, mask);
-    YQ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(stebz_kernel, dim3(B), dim3(LDS_MAX_N), 0, s, W, sW, n, ns, evals, perm, sE, keep, rule, thr,
-                       max_bond, mask);
-    YQ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(stein_kernel, dim3(B), dim3(64), (size_t)(n * 64 + 2 * n) * sizeof(double), s, W, sW, n, kout,
-                       (const double *)evals, sE, (const int *)keep, mask, fallback);
-    YQ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(ormtr_kernel, dim3(B), dim3(256), 0, s, W, sW, n, kout, U, sU, mask, (const int *)fallback);
-    YQ_HIP(hipGetLastError());
-    // matrices whose QL exceeded the rotation budget: full single-workgroup solver
+    // matrices whose inverse iteration broke down: full single-workgroup solver
     hipLaunchKernelGGL(heev_lds_kernel, dim3(B), dim3(256), shmem, s, G, sG, U, sU, Z, sZ, Pg, sP, n, ns, evals, perm,
                        sE, keep, rule, thr, max_bond, mask, (const int *)fallback);
     YQ_HIP(hipGetLastError());
@@ -946,7 +1317,7 @@ extern "C" int yaqs_debug_heev_batch(int n, int count, const double *G, double *
         double2 *dP = nullptr;
         int *dperm, *dkeep, *dfb;
         char *dW = nullptr;
-        const int64_t sW = path == 1 ? 0 : heev_work_bytes(n);
+        const int64_t sW = path == 1 ? 0 : heev_work_bytes(n, max_bond);
         YQ_HIP(hipMalloc(&dG, nn * count * sizeof(c64)));
         YQ_HIP(hipMalloc(&dU, nn * count * sizeof(c64)));
         YQ_HIP(hipMalloc(&dZ, sZ * count * sizeof(double)));

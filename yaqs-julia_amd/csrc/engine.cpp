@@ -502,7 +502,7 @@ int64_t Engine<T>::footprint(const JobIn &job, int B) {
     bytes += 2 * gd * gd * (int64_t)sizeof(c64);                       // Gram + eigenvectors
     bytes += (gd * gd + 2 * gd + 4) * (int64_t)sizeof(double);         // eigensolver scratch
     if (gd > 128) bytes += gd * (gd + 1) / 2 * (int64_t)sizeof(c64);
-    bytes += heev_work_bytes((int)gd) + (int64_t)sizeof(int);           // fast eigensolver work area
+    bytes += heev_work_bytes((int)gd, (int)std::min<int64_t>(gd, job.max_bond)) + (int64_t)sizeof(int);  // eigensolver work
     bytes += (2 * cm * cm + std::max<int64_t>(nmax, 2 * cm * cm)) * (int64_t)sizeof(c64);  // transfer
     bytes += (int64_t)L * 20 * sizeof(c64) * 2 + (job.kdim + 2) * 64 * 16 + 256;
     return bytes * B;
@@ -572,7 +572,7 @@ void Engine<T>::setup() {
         slotP = gdim * (gdim + 1) / 2;
         Pbuf.alloc((size_t)B * slotP * sizeof(double2));
     }
-    slotH = heev_work_bytes((int)gdim);
+    slotH = heev_work_bytes((int)gdim, (int)std::min<int64_t>(gdim, J.max_bond));
     if (slotH) {
         Hwork.alloc((size_t)B * slotH);
         Hfb.alloc((size_t)B * sizeof(int));

@@ -55,10 +55,11 @@ constexpr int LZ_CHUNK_ELEMS = 2048;
 // Hermitian eigen-decomposition of a batch of FP64 Gram matrices (eigen.hip):
 // sorted eigenvalues, kept rank under rule (0: T-A, 1: T-C, 2: T-D) and the
 // eigenvectors of the kept rank as the first columns of U (row stride n).
-// work / sW: per-matrix work area of heev_work_bytes(n) bytes (null when that
-// is 0), fallback: B ints; Z / sZ: n*n + 2n + 4 doubles per matrix; Pg / sP:
-// n(n+1)/2 double2 per matrix when n > 128.
-int64_t heev_work_bytes(int n);
+// work / sW: per-matrix work area of heev_work_bytes(n, kmax) bytes (kmax:
+// most kept columns of any call; null when that is 0), fallback: B ints;
+// Z / sZ: n*n + 2n + 4 doubles per matrix; Pg / sP: n(n+1)/2 double2 per
+// matrix when n > 128.
+int64_t heev_work_bytes(int n, int kmax);
 void heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, double2 *Pg, int64_t sP, void *work,
           int64_t sW, int *fallback, int n, int ns, double *evals, int *perm, int64_t sE, int *keep, int rule,
           double thr, int max_bond, const int *mask, int B, hipStream_t s);

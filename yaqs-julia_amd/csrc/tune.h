@@ -3,7 +3,8 @@
 // matrix-core GEMM; the MPO step fused into the GEMM epilogue or run as its
 // own kernel).  They give bit-identical results, so the choice only
 // affects speed: the fastest is measured with HIP events on first use of a
-// shape and cached for the process, per device.
+// shape and cached for the process, per device (and in the file named by
+// YAQS_TUNE_CACHE, when set, for later processes).
 #pragma once
 
 #include <hip/hip_runtime.h>
