@@ -92,6 +92,54 @@ def cpu_window(Y, ntraj, warmup, steps, nthreads):
     return O.run_analog_window(h, warmup, nthreads=nthreads)
 
 
+def c3_job(Y, ntraj, layers=40, L=40, chi=128, precision="mixed", p=0.01):
+    """Noisy brickwork circuit: L=40 qubits, `layers` layers of Haar-random
+    SU(4) gates (odd layers on bonds (1,2),(3,4)..., even on (2,3)...),
+    depolarizing p as pauli_x/y/z with strength p/3 on every qubit (applied
+    after each gate on its two qubits, dt=1), TEBD threshold 1e-9, chi_max=128,
+    <Z_i> at the end."""
+    circ = Y.haar_brickwork(L, layers, seed=0)
+    obs = [Y.Observable(f"z{i}", Y.gates.ZGate(), i) for i in range(1, L + 1)]
+    cfg = Y.StrongMeasurementConfig(obs, num_traj=ntraj, max_bond_dim=chi, truncation_threshold=1e-9)
+    procs = [{"name": n, "sites": [i], "strength": p / 3} for i in range(1, L + 1)
+             for n in ("pauli_x", "pauli_y", "pauli_z")]
+    noise = Y.NoiseModel(procs, L)
+    prec = Y._abi.PREC_MIXED if precision == "mixed" else Y._abi.PREC_FP64
+    return Y._abi.pack_digital(Y.MPS(L), circ, cfg, noise, precision=prec, seed=0), cfg
+
+
+def c3_main(Y, args):
+    """Digital workload line: trajectory-site-layers per second of the whole
+    job (all trajectories in one call, from |0...0>)."""
+    L, layers = 40, args.layers
+    ctx = Y.Context(0)
+    h, cfg = c3_job(Y, args.ntraj, layers=layers, precision=args.precision)
+    ncol = int(ctx.lib.yaqs_digital_num_samples(h.job))
+    print(f"[bench] C3: L={L} layers={layers} ntraj={args.ntraj} {args.precision}", flush=True)
+    t0 = time.perf_counter()
+    jid = ctx.submit_digital(h)
+    _, mean, bonds, st = ctx.wait(jid, len(cfg.observables), ncol, args.ntraj, want_bonds=True)
+    wall = time.perf_counter() - t0
+    ctx.close()
+    value = args.ntraj * L * layers / st["wall_s"]
+    print(f"[bench] C3 GPU {st['wall_s']:.2f}s -> {value:.4g} traj-site-layers/s, max bond {int(bonds.max())}",
+          flush=True)
+    out = {
+        "metric": "traj-site-layers/s (digital TJM, L=40 brickwork)", "value": value,
+        "unit": "traj-site-layers/s", "n_gpus": 1, "steps": layers, "warmup": 0,
+        "ms_per_step": 1e3 * st["wall_s"] / layers, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 contractions + f64 Gram/eigensolver" if args.precision == "mixed" else "c128",
+        "data": "synthetic (Haar SU(4) brickwork from seed 0, seeded jumps)",
+        "config": {"workload": "C3: L=40, Haar SU(4) brickwork, depolarizing p=0.01 as pauli_x/y/z p/3 per gate "
+                               "qubit, TEBD thr 1e-9, chi_max=128, <Z_i> at the end",
+                   "ntraj": args.ntraj, "L": L, "layers": layers, "chi_max": 128, "precision": args.precision,
+                   "max_bond_reached": int(bonds.max()), "host_wall_s": wall},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def serve_stream(Y, njobs, seed=0, mean_gap_s=0.25):
     """Serving stream shaped like the mixed-request workload, scaled to finish
     in about a minute: Poisson arrivals of analog jobs drawn from the seed
@@ -187,6 +235,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=12, help="jobs of the serving stream (0 = skip)")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3"],
+                    help="c2: the analog headline line; c3: the digital brickwork line")
+    ap.add_argument("--layers", type=int, default=40, help="c3: circuit layers")
     ap.add_argument("--trace-summary", metavar="CSV", help="summarise a rocprofv3 kernel trace and exit")
     ap.add_argument("--trace-last", type=int, default=0, help="dispatches of the timed window (from the bench line)")
     args = ap.parse_args()
@@ -198,6 +249,11 @@ def main():
               "with a single reduction at the end, which this single-GPU build does not launch.", file=sys.stderr)
         sys.exit(2)
     Y = _pkg()
+    if args.workload == "c3":
+        if args.ntraj == 1000:
+            args.ntraj = 512
+        c3_main(Y, args)
+        return
     lib = Y._abi.load_library()
     ctx = Y.Context(0)
     print(f"[bench] C2 window: warmup={args.warmup} steps={args.steps} ntraj={args.ntraj} {args.precision}",

@@ -441,6 +441,10 @@ class Engine {
     std::vector<int64_t> gate_off;
     DevBuf dissd_ops;  // digital dissipation (dt = 1) per gate pair and site: [(2 a + r) * 4]
     std::vector<int> dgt_active;
+    // gate pairs whose two dissipators are multiples of the identity (Pauli
+    // processes only): dscal_ops[a * 4] = their product, applied to the centre
+    DevBuf dscal_ops;
+    std::vector<int> dgt_scalar;
 
     void setup();
     void alloc_state(State &S);
@@ -790,6 +794,22 @@ void Engine<T>::setup() {
                     }
                 }
             upload(dissd_ops, dops, s);
+            std::vector<c64> sops((size_t)std::max(1, L) * 4, mk(0, 0));
+            dgt_scalar.assign((size_t)L, 0);
+            for (int a = 0; a + 1 < L; ++a) {
+                bool sc = true;
+                double f = 1.0;
+                for (int r = 0; r < 2; ++r) {
+                    const c64 *M = &dops[(size_t)(2 * a + r) * 4];
+                    sc = sc && M[1].re == 0 && M[1].im == 0 && M[2].re == 0 && M[2].im == 0 && M[0].im == 0 &&
+                         M[3].im == 0 && M[0].re == M[3].re;
+                    f *= M[0].re;
+                }
+                dgt_scalar[a] = sc ? 1 : 0;
+                sops[(size_t)a * 4] = mk(f, 0);
+                sops[(size_t)a * 4 + 3] = mk(f, 0);
+            }
+            upload(dscal_ops, sops, s);
             std::vector<c64> gops;
             gate_off.assign(J.gates.size(), -1);
             for (size_t g = 0; g < J.gates.size(); ++g) {
@@ -1321,6 +1341,8 @@ void Engine<T>::run_digital(int64_t t0, int nb_, double *d_out, int64_t *h_bonds
         }
         if (!cur.empty()) layers.push_back(cur);
     }
+    static const bool progress = std::getenv("YAQS_PROGRESS") != nullptr;
+    const double t_run = host_now();
     bool at_start = false;
     std::vector<int> after;
     for (int b : barrier_layers) {
@@ -1371,13 +1393,20 @@ void Engine<T>::run_digital(int64_t t0, int nb_, double *d_out, int64_t *h_bonds
             const bool noisy = J.has_noise && !J.noise.empty() && pc.n > 0;
             if (noisy) {
                 // dissipation on the gate's sites (dt = 1), then restore the centre at a+1
-                for (int r = 0; r < 2; ++r) {
-                    const int q = a + r;
-                    if (!dgt_active[2 * a + r]) continue;
-                    local_op<T>(A(PHI, q), slotA[q], PHI.wd[q], PHI.wd[q + 1], dissd_ops.as<c64>() + (2 * a + r) * 4,
-                                nullptr, nullptr, nb, s);
+                if (dgt_scalar[a]) {
+                    // f_a I on the isometry A_a and f_{a+1} I on the centre: the QR of
+                    // f_a A_a would return A_a and move f_a into the centre
+                    local_op<T>(A(PHI, a + 1), slotA[a + 1], PHI.wd[a + 1], PHI.wd[a + 2],
+                                dscal_ops.as<c64>() + (int64_t)a * 4, nullptr, nullptr, nb, s);
+                } else {
+                    for (int r = 0; r < 2; ++r) {
+                        const int q = a + r;
+                        if (!dgt_active[2 * a + r]) continue;
+                        local_op<T>(A(PHI, q), slotA[q], PHI.wd[q], PHI.wd[q + 1],
+                                    dissd_ops.as<c64>() + (2 * a + r) * 4, nullptr, nullptr, nb, s);
+                    }
+                    qr_step(PHI, a, nullptr);  // A_a isometric again, norm in A_{a+1}
                 }
-                qr_step(PHI, a, nullptr);  // A_a isometric again, norm in A_{a+1}
                 const int64_t nc = (int64_t)PHI.wd[a + 1] * 2 * PHI.wd[a + 2];
                 norm2<T>(A(PHI, a + 1), slotA[a + 1], nc, norm2v.as<double>(), nullptr, nb, s);
                 jump_decide(norm2v.as<double>(), J.seed, traj0, rngk.as<uint64_t>(), 1, jstate.as<int>(),
@@ -1444,6 +1473,14 @@ void Engine<T>::run_digital(int64_t t0, int nb_, double *d_out, int64_t *h_bonds
         }
         const int lnum = (int)li + 1;
         if (std::find(after.begin(), after.end(), lnum) != after.end()) sample(idx++);
+        if (progress) {
+            YQ_HIP(hipStreamSynchronize(s));
+            int mb = 0;
+            for (int b = 0; b <= L; ++b) mb = std::max(mb, PHI.wd[b]);
+            fprintf(stderr, "[yaqs] traj %lld+%d layer %d/%d  %.2fs  max bond %d\n", (long long)t0, nb, lnum,
+                    (int)layers.size(), host_now() - t_run, mb);
+            fflush(stderr);
+        }
     }
     if (!at_start && after.empty() && !J.obs.empty()) sample(0);
 }
