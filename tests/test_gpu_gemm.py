@@ -96,7 +96,8 @@ def test_fused_mpo_step_matches_unfused(prec, tol):
     lib.yaqs_debug_gemm_w.argtypes = [ct.c_int] * 7 + [c_dp, c_dp, ip, ip, c_dp, c_dp, c_dp]
     rng = np.random.default_rng(5)
     for P, Dl, dph, Dr, Q, K in [(64, 5, 2, 5, 64, 64), (13, 5, 4, 5, 37, 29), (64, 3, 2, 3, 128, 64),
-                                 (7, 1, 2, 5, 3, 9), (30, 5, 4, 5, 64, 30)]:
+                                 (7, 1, 2, 5, 3, 9), (30, 5, 4, 5, 64, 30), (20, 2, 2, 3, 50, 17),
+                                 (33, 4, 4, 4, 40, 64)]:
         M, N, I, J = P * Dl, dph * Q, Dl * dph, dph * Dr
         A = rng.standard_normal((M, K)) + 1j * rng.standard_normal((M, K))
         B = rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N))

@@ -298,9 +298,10 @@ thread_local GemmProf *g_prof = nullptr;
 // profiler counts algorithmic FLOP of active trajectories only.
 thread_local int g_nact = -1;
 
-// Tuner keys: dimensions in units of the 32-wide tile granule (K of the
-// 16-deep k tile), batch counts by power of two.
-inline int b32(int64_t x) { return (int)((x + 31) / 32); }
+// Tuner keys: GEMM dimensions in units of 16 (the MFMA block and the k
+// tile), MPO-step dimensions P and Q in units of 8, batch counts by power
+// of two.
+inline int b8(int64_t x) { return (int)((x + 7) / 8); }
 inline int b16(int64_t x) { return (int)((x + 15) / 16); }
 inline int pow2b(int64_t x) {
     int b = 0;
@@ -313,7 +314,7 @@ void gemmT(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hi
     // tile shape: every shape gives the same bits, so it is picked by speed
     int tile = 0;
     if (!d.accumulate && d.opB == OP_N && tune_enabled()) {
-        const TuneKey k = {0, (int)sizeof(T), d.opA, d.opB, b32(d.M), b32(d.N), b16(d.K), d.inner, pow2b(d.outer), 0};
+        const TuneKey k = {0, (int)sizeof(T), d.opA, d.opB, b16(d.M), b16(d.N), b16(d.K), d.inner, pow2b(d.outer), 0};
         tile = tuned_variant(
             k, GEMM_TILES,
             [&](int v) {
@@ -351,7 +352,7 @@ void gemmW(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, int Dl,
     };
     bool use_fused = gemm_w_covers(d, P, Dl, dph, Dr, Q, W.nnz);
     if (use_fused && tune_enabled()) {
-        const TuneKey k = {1, (int)sizeof(T), Dl, dph, Dr, b32(P), b32(Q), b16(d.K), W.nnz, pow2b(d.outer)};
+        const TuneKey k = {1, (int)sizeof(T), Dl, dph, Dr, b8(P), b8(Q), b16(d.K), W.nnz, pow2b(d.outer)};
         use_fused = tuned_variant(
                         k, 2,
                         [&](int v) {

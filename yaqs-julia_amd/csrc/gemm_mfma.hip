@@ -81,8 +81,9 @@ __device__ __forceinline__ int64_t b_index(int k, int n, int64_t ld) {
 //
 // FW: the MPO step is fused into the epilogue (WFuse).  The GEMM is then
 // T1[(p, m)][(s, q)] = A B with rows (p, m), m < Dl, and columns (s, q),
-// s < dph, q < Q; a tile covers all m of pb = 64 / Dl consecutive p and all
-// s of qb = 64 / dph consecutive q, stages T1 in LDS and writes
+// s < dph, q < Q; a tile of BM = 16 Dl rows covers all m of pb = 16
+// consecutive p and all s of qb = 64 / dph consecutive q (no padded rows),
+// stages T1 in LDS and writes
 // Y[p][(s', n)][q] = sum W[(s', n)][(m, s)] T1[(p, m)][(s, q)] (W in CSR,
 // rows s' Dr + n, columns m dph + s) straight to memory.
 template <typename T, int OPA, int OPB, int BM, int BN, int BK, bool FW>
@@ -90,7 +91,9 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
                                                         const cplx<T> *__restrict__ B, cplx<T> *__restrict__ C,
                                                         WFuse w) {
     constexpr int LDK = BK + 4, KQ = BK / 4;  // k values per lane and tile
-    constexpr int IM = BM / 32, JN = BN / 32;  // 16 x 16 blocks per wave
+    // 16 x 16 blocks per wave: 2 x 2 wavefronts of (BM/2) x (BN/2), or for
+    // the fused kernel 1 x 4 wavefronts of BM x 16 (BM = 16 Dl rows)
+    constexpr int IM = FW ? BM / 16 : BM / 32, JN = FW ? 1 : BN / 32;
     typedef typename Mfma<T>::V V;
     const int t = blockIdx.z, jb = blockIdx.y;
     if (d.active && !d.active[t]) return;
@@ -117,7 +120,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
     T(*As)[2][BM][LDK] = reinterpret_cast<T(*)[2][BM][LDK]>(smem);            // [buffer][re/im][m][k']
     T(*Bs)[2][BN][LDK] = reinterpret_cast<T(*)[2][BN][LDK]>(smem + 2 * 2 * BM * LDK);  // [buffer][re/im][n][k']
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
+    const int wm = FW ? 0 : (wave >> 1) * (BM / 2), wn = FW ? wave * 16 : (wave & 1) * (BN / 2);
     const int li = lane & 15, lk = lane >> 4;
     // Two load mappings, chosen per operand by which index is contiguous in
     // memory.  Row-contiguous (op(A) with A stored K x M, op(B) = B): thread t
@@ -127,8 +130,10 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
     // coalesced along the rows.  K-contiguous (op(A) = A, op(B) with B stored
     // N x K): element e = t + 256 r is (row e / BK, k e % BK), loads
     // coalesced along k and scalar LDS stores.
-    static_assert((BM == 32 || BM == 64) && (BN == 32 || BN == 64) && BK == 16, "tile shape outside the load mappings");
-    static_assert(!FW || (BM == 64 && BN == 64), "the fused epilogue assumes 64 x 64 tiles");
+    static_assert(FW || ((BM == 32 || BM == 64) && (BN == 32 || BN == 64)), "tile shape outside the load mappings");
+    static_assert(!FW || (BM % 16 == 0 && BM <= 80 && BN == 64 && OPA == OP_N && OPB == OP_N),
+                  "the fused kernel takes op N operands, 64 columns and 16 Dl rows");
+    static_assert(BK == 16, "k tile");
     constexpr int NLA = BM * BK / 256, NLB = BN * BK / 256;  // loads per thread and k tile
     constexpr int GA = 256 / BM, GB = 256 / BN;              // k groups of the row mapping
     constexpr bool ROWA = OPA != OP_N, ROWB = OPB == OP_N;
@@ -375,7 +380,7 @@ bool gemm_w_covers(const GemmDesc &d, int P, int Dl, int dph, int Dr, int Q, int
     if (!mfma_mode()) return false;
     if (nnz > W_NNZ_MAX) return false;  // W staged in LDS
     if (d.opA != OP_N || d.opB != OP_N || d.inner != 1 || d.accumulate) return false;
-    if (Dl < 1 || Dl > 64 || (dph != 2 && dph != 4) || d.M != P * Dl || d.N != dph * Q) return false;
+    if (Dl < 1 || Dl > 5 || (dph != 2 && dph != 4) || d.M != P * Dl || d.N != dph * Q) return false;
     if (dph * Dr > 64) return false;  // W rows staged in LDS
     if (P <= 0 || Q <= 0 || d.K <= 0 || d.outer <= 0) return false;
     return true;
@@ -391,7 +396,7 @@ bool gemm_w_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, i
     w.dph = dph;
     w.Dr = Dr;
     w.Q = Q;
-    w.pb = 64 / Dl;
+    w.pb = 16;
     w.qb = 64 / dph;
     w.sY = sY;
     w.rowptr = rowptr;
@@ -400,8 +405,20 @@ bool gemm_w_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, i
     w.Y = Y;
     const int tiles = ((P + w.pb - 1) / w.pb) * ((Q + w.qb - 1) / w.qb);
     dim3 grid(tiles, 1, d.outer);
-    hipLaunchKernelGGL((gemm_mfma_kernel<T, OP_N, OP_N, 64, 64, 16, true>), grid, dim3(256), 0, s, d, A, B,
-                       (cplx<T> *)nullptr, w);
+#define YQ_GEMM_W(DL)                                                                                         \
+    case DL:                                                                                                  \
+        hipLaunchKernelGGL((gemm_mfma_kernel<T, OP_N, OP_N, 16 * DL, 64, 16, true>), grid, dim3(256), 0, s, d, A, \
+                           B, (cplx<T> *)nullptr, w);                                                        \
+        break;
+    switch (Dl) {
+        YQ_GEMM_W(1)
+        YQ_GEMM_W(2)
+        YQ_GEMM_W(3)
+        YQ_GEMM_W(4)
+        YQ_GEMM_W(5)
+        default: return false;
+    }
+#undef YQ_GEMM_W
     YQ_HIP(hipGetLastError());
     return true;
 }
