@@ -253,12 +253,15 @@ GemmDesc gd(int M, int N, int K, int opA, int opB, int64_t lda, int64_t ldb, int
 }
 
 // HIP-event timing of matrix-core GEMM launches inside a measurement window.
+// A launch's FLOP = per-unit FLOP x units (active trajectories); launches
+// queued before their active count is known carry units -1 until resolve().
 struct GemmProf {
     bool on = false;
     std::vector<hipEvent_t> ev;
     std::vector<double> fl;
+    std::vector<int> un;
     size_t used = 0;
-    double ms = 0, flop = 0;
+    double ms = 0, flop = 0, pend = 0;
     int64_t n = 0;
     hipStream_t s = nullptr;
     void start(hipStream_t st) {
@@ -267,11 +270,23 @@ struct GemmProf {
             ev.resize(4096);
             for (auto &e : ev) YQ_HIP(hipEventCreate(&e));
             fl.resize(2048);
+            un.resize(2048);
         }
         used = 0;
-        ms = flop = 0;
+        ms = flop = pend = 0;
         n = 0;
         on = true;
+    }
+    void record(double per_unit, int units) {
+        fl[used / 2] = per_unit;
+        un[used / 2] = units;
+        used += 2;
+    }
+    void resolve(int units) {
+        for (size_t i = 0; i < used; i += 2)
+            if (un[i / 2] < 0) un[i / 2] = units;
+        flop += pend * units;
+        pend = 0;
     }
     void flush() {
         if (!used) return;
@@ -280,7 +295,8 @@ struct GemmProf {
             float x = 0;
             YQ_HIP(hipEventElapsedTime(&x, ev[i], ev[i + 1]));
             ms += x;
-            flop += fl[i / 2];
+            if (un[i / 2] < 0) pend += fl[i / 2];
+            else flop += fl[i / 2] * un[i / 2];
             ++n;
         }
         used = 0;
@@ -294,9 +310,14 @@ struct GemmProf {
     }
 };
 thread_local GemmProf *g_prof = nullptr;
-// Trajectories active in the current masked launch (Lanczos), -1 = all: the
+// Trajectories active in the current masked launch (Lanczos), -1 = all, -2 =
+// not known yet (a Lanczos iteration queued ahead of its count): the
 // profiler counts algorithmic FLOP of active trajectories only.
 thread_local int g_nact = -1;
+inline int launch_units(const GemmDesc &d) {
+    if (!d.active || g_nact == -1) return d.outer;
+    return g_nact >= 0 ? g_nact : -1;
+}
 
 // Tuner keys: GEMM dimensions in units of 16 (the MFMA block and the k
 // tile), MPO-step dimensions P and Q in units of 8, batch counts by power
@@ -313,7 +334,7 @@ template <typename T>
 void gemmT(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s) {
     // tile shape: every shape gives the same bits, so it is picked by speed
     int tile = 0;
-    if (!d.accumulate && d.opB == OP_N && tune_enabled()) {
+    if (!d.accumulate && d.opB == OP_N && g_nact != -2 && tune_enabled()) {
         const TuneKey k = {0, (int)sizeof(T), d.opA, d.opB, b16(d.M), b16(d.N), b16(d.K), d.inner, pow2b(d.outer), 0};
         tile = tuned_variant(
             k, GEMM_TILES,
@@ -333,9 +354,7 @@ void gemmT(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hi
     if (prof) {
         if (mf) {
             YQ_HIP(hipEventRecord(p->ev[p->used + 1], s));
-            const int units = (d.active && g_nact >= 0) ? g_nact : d.outer;
-            p->fl[p->used / 2] = 8.0 * d.M * (double)d.N * d.K * d.inner * units;
-            p->used += 2;
+            p->record(8.0 * d.M * (double)d.N * d.K * d.inner, launch_units(d));
         }
     }
 }
@@ -351,7 +370,7 @@ void gemmW(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, int Dl,
                               W.val.as<c64>(), W.nnz, s);
     };
     bool use_fused = gemm_w_covers(d, P, Dl, dph, Dr, Q, W.nnz);
-    if (use_fused && tune_enabled()) {
+    if (use_fused && g_nact != -2 && tune_enabled()) {
         const TuneKey k = {1, (int)sizeof(T), Dl, dph, Dr, b8(P), b8(Q), b16(d.K), W.nnz, pow2b(d.outer)};
         use_fused = tuned_variant(
                         k, 2,
@@ -374,10 +393,8 @@ void gemmW(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, int Dl,
     (void)fused();
     if (prof) {
         YQ_HIP(hipEventRecord(p->ev[p->used + 1], s));
-        const int units = (d.active && g_nact >= 0) ? g_nact : d.outer;
         // GEMM plus the sparse MPO step (one complex MAC per W non-zero per (p, q))
-        p->fl[p->used / 2] = (8.0 * d.M * (double)d.N * d.K * d.inner + 8.0 * W.nnz * (double)P * Q) * units;
-        p->used += 2;
+        p->record(8.0 * d.M * (double)d.N * d.K * d.inner + 8.0 * W.nnz * (double)P * Q, launch_units(d));
     }
 }
 
@@ -399,6 +416,13 @@ class Engine {
     typedef cplx<T> CT;
 
     Engine(const JobIn &job, int B, hipStream_t s) : J(job), B(B), s(s) { setup(); }
+    ~Engine() {
+        for (auto &e : lz_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (h_pinned) (void)hipHostFree(h_pinned);
+    }
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
 
     void run_analog(int64_t t0, int nb, double *d_out, JobResult &res);
     void run_digital(int64_t t0, int nb, double *d_out, int64_t *h_bonds, JobResult &res);
@@ -435,6 +459,7 @@ class Engine {
     bool any_obs2 = false;
     std::vector<char> obs2_at;
     int *h_pinned = nullptr;
+    hipEvent_t lz_ev[2] = {nullptr, nullptr};  // Lanczos active counts copied to h_pinned[1 + (j & 1)]
     uint64_t traj0 = 0;
     int64_t nchunk_max = 0;
     std::map<int, PairCand> pair_cand;  // digital candidate tables per gate pair
@@ -615,6 +640,7 @@ void Engine<T>::setup() {
     rngk.alloc(B * sizeof(uint64_t));
     count.alloc(sizeof(int));
     YQ_HIP(hipHostMalloc((void **)&h_pinned, 16 * sizeof(int), hipHostMallocDefault));
+    for (auto &e : lz_ev) YQ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
     // ---- noise: dissipation operators per site and jump candidates
     auto site_ops = [&](double dt, std::vector<c64> &ops, std::vector<int> *act) {
@@ -926,13 +952,13 @@ void Engine<T>::expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, Jo
     const double fl = two ? 8.0 * (4.0 * Dm[i] * cl * cr * (double)(cl + cr) + 16.0 * Dm[i] * Dm[i] * (double)cl * cr)
                           : 8.0 * (2.0 * Dm[i] * cl * cr * (double)(cl + cr) + 4.0 * Dm[i] * Dm[i] * (double)cl * cr);
     lz_init<T>(x, sx, V.as<CT>(), sV(), n, beta0.as<double>(), active.as<int>(), nit.as<int>(), nb, s);
-    int nact = nb;
-    for (int j = 0; j < kdim; ++j) {
-        g_nact = nact;
+    // One Lanczos iteration for the still-active trajectories (every kernel is
+    // masked by the device-side active flags); its active count is copied to
+    // the host slot of its parity.
+    auto iteration = [&](int j, int units) {
+        g_nact = units;
         matvec(S, two, i, Vq(j), sV(), Vq(j + 1), sV(), active.as<int>());
         g_nact = -1;
-        res.matvecs += nact;
-        res.flop += fl * nact;
         lz_dots<T>(V.as<CT>(), sV(), nmax, n, j, part.as<c64>(), kdim + 1, active.as<int>(), nb, s);
         lz_update<T>(V.as<CT>(), sV(), nmax, n, j, part.as<c64>(), kdim + 1, partn.as<double>(), galpha.as<double>(),
                      active.as<int>(), nb, s);
@@ -941,7 +967,25 @@ void Engine<T>::expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, Jo
                     beta.as<double>(), beta0.as<double>(), coef.as<c64>(), nit.as<int>(), active.as<int>(),
                     lscale.as<double>(), nactive.as<int>(), nb, s);
         lz_scale<T>(V.as<CT>(), sV(), nmax, n, j + 1, lscale.as<double>(), active.as<int>(), nb, s);
-        nact = sync_count(nactive.as<int>());
+        YQ_HIP(hipMemcpyAsync(h_pinned + 1 + (j & 1), nactive.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        YQ_HIP(hipEventRecord(lz_ev[j & 1], s));
+    };
+    // Iteration j+1 is queued before the host waits for the count of
+    // iteration j, so the device never idles on that round trip; when
+    // iteration j left no trajectory active, j+1 ran fully masked.
+    int nact = nb;
+    iteration(0, nact);
+    res.matvecs += nact;
+    res.flop += fl * nact;
+    for (int j = 0; j < kdim; ++j) {
+        const bool more = j + 1 < kdim;
+        if (more) iteration(j + 1, -2);  // active count resolved below
+        YQ_HIP(hipEventSynchronize(lz_ev[j & 1]));
+        nact = h_pinned[1 + (j & 1)];
+        if (!more) break;
+        if (g_prof) g_prof->resolve(nact);
+        res.matvecs += nact;
+        res.flop += fl * nact;
         if (nact == 0) break;
     }
     lz_combine<T>(x, sx, V.as<CT>(), sV(), nmax, n, beta0.as<double>(), coef.as<c64>(), nit.as<int>(), kdim, nullptr,

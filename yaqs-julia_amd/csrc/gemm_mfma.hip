@@ -86,29 +86,42 @@ __device__ __forceinline__ int64_t b_index(int k, int n, int64_t ld) {
 // stages T1 in LDS and writes
 // Y[p][(s', n)][q] = sum W[(s', n)][(m, s)] T1[(p, m)][(s, q)] (W in CSR,
 // rows s' Dr + n, columns m dph + s) straight to memory.
+constexpr int GEMM_XCDS = 8;  // accelerator complex dies of an MI355X
+inline unsigned gemm_blocks(int ntile, int inner, int outer) {
+    return (unsigned)ntile * inner * ((outer + GEMM_XCDS - 1) / GEMM_XCDS * GEMM_XCDS);
+}
+
 template <typename T, int OPA, int OPB, int BM, int BN, int BK, bool FW>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T> *__restrict__ A,
                                                         const cplx<T> *__restrict__ B, cplx<T> *__restrict__ C,
-                                                        WFuse w) {
+                                                        WFuse w, int ntile) {
     constexpr int LDK = BK + 4, KQ = BK / 4;  // k values per lane and tile
     // 16 x 16 blocks per wave: 2 x 2 wavefronts of (BM/2) x (BN/2), or for
     // the fused kernel 1 x 4 wavefronts of BM x 16 (BM = 16 Dl rows)
     constexpr int IM = FW ? BM / 16 : BM / 32, JN = FW ? 1 : BN / 32;
     typedef typename Mfma<T>::V V;
-    const int t = blockIdx.z, jb = blockIdx.y;
+    // XCD-aware block order: workgroups are dealt round-robin to the XCDs,
+    // so block b runs on XCD b % GEMM_XCDS; all tiles of trajectory t are
+    // given to blocks on XCD t % GEMM_XCDS, where they share that XCD's L2
+    // (the A and B operands of a trajectory are fetched from HBM once).
+    const int units = ntile * d.inner;
+    const int xcd = blockIdx.x % GEMM_XCDS, kx = blockIdx.x / GEMM_XCDS;
+    const int t = (kx / units) * GEMM_XCDS + xcd, u = kx % units;
+    const int jb = u / ntile, tile = u % ntile;
+    if (t >= d.outer) return;
     if (d.active && !d.active[t]) return;
     int m0, n0, p0 = 0, q0 = 0, rows_ok = BM;
     if (FW) {
         const int tilesQ = (w.Q + w.qb - 1) / w.qb;
-        p0 = (blockIdx.x / tilesQ) * w.pb;
-        q0 = (blockIdx.x % tilesQ) * w.qb;
+        p0 = (tile / tilesQ) * w.pb;
+        q0 = (tile % tilesQ) * w.qb;
         m0 = p0 * w.Dl;
         n0 = 0;
         rows_ok = w.pb * w.Dl;
     } else {
         const int tilesN = (d.N + BN - 1) / BN;
-        m0 = (blockIdx.x / tilesN) * BM;
-        n0 = (blockIdx.x % tilesN) * BN;
+        m0 = (tile / tilesN) * BM;
+        n0 = (tile % tilesN) * BN;
     }
     const cplx<T> *Ab = A + t * d.sA_outer + jb * d.sA_inner;
     const cplx<T> *Bb = B + t * d.sB_outer + jb * d.sB_inner;
@@ -336,8 +349,8 @@ static int mfma_mode() {
 template <typename T, int OA, int OB, int BM, int BN>
 static void launch_tile(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s) {
     const int tiles = ((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
-    dim3 grid(tiles, d.inner, d.outer);
-    hipLaunchKernelGGL((gemm_mfma_kernel<T, OA, OB, BM, BN, 16, false>), grid, dim3(256), 0, s, d, A, B, C, WFuse{});
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, OA, OB, BM, BN, 16, false>), dim3(gemm_blocks(tiles, d.inner, d.outer)),
+                       dim3(256), 0, s, d, A, B, C, WFuse{}, tiles);
     YQ_HIP(hipGetLastError());
 }
 
@@ -404,11 +417,11 @@ bool gemm_w_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, i
     w.val = val;
     w.Y = Y;
     const int tiles = ((P + w.pb - 1) / w.pb) * ((Q + w.qb - 1) / w.qb);
-    dim3 grid(tiles, 1, d.outer);
+    const dim3 grid(gemm_blocks(tiles, 1, d.outer));
 #define YQ_GEMM_W(DL)                                                                                         \
     case DL:                                                                                                  \
         hipLaunchKernelGGL((gemm_mfma_kernel<T, OP_N, OP_N, 16 * DL, 64, 16, true>), grid, dim3(256), 0, s, d, A, \
-                           B, (cplx<T> *)nullptr, w);                                                        \
+                           B, (cplx<T> *)nullptr, w, tiles);                                                 \
         break;
     switch (Dl) {
         YQ_GEMM_W(1)
