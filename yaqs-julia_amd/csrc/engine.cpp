@@ -443,7 +443,8 @@ class Engine {
     std::vector<DevBuf> EL, ER;
     std::vector<CsrDev> csrL, csrR, csr2;
     DevBuf theta, V, T1, T2, G, U, evals, perm, keep, Cf, absorb, Zbuf, Pbuf, Hwork, Hfb;
-    int64_t slotZ = 0, slotP = 0, slotH = 0;
+    DevBuf cgsS, cgsF;  // CGS2 projection scratch and Frobenius norms
+    int64_t slotZ = 0, slotP = 0, slotH = 0, slotCgs = 0;
     DevBuf Ebuf0, Ebuf1, Xbuf, rho1, rho2;
     DevBuf beta0, galpha, lscale, alpha, beta, coef, nit, active, part, partn, nactive;
     DevBuf norm2v, fac, jstate, chosen, opidx, mask, rngk, wbuf, count;
@@ -528,13 +529,14 @@ int64_t Engine<T>::footprint(const JobIn &job, int B) {
     int64_t elems = sites * (job.digital ? 1 : (noise ? 3 : 2));  // PHI (+ PSI, + scratch)
     if (!job.digital) elems += 2 * envs + (int64_t)(job.kdim + 1) * nmax + 2 * smax;
     elems += nmax + cm * cm + sites;  // theta, Cf, absorb (bounded by the state size)
+    elems += cm * 32;                 // CGS2 projection scratch
     int64_t bytes = elems * (int64_t)sizeof(cplx<T>);
     bytes += 2 * gd * gd * (int64_t)sizeof(c64);                       // Gram + eigenvectors
     bytes += (gd * gd + 2 * gd + 4) * (int64_t)sizeof(double);         // eigensolver scratch
     if (gd > 128) bytes += gd * (gd + 1) / 2 * (int64_t)sizeof(c64);
     bytes += heev_work_bytes((int)gd, (int)std::min<int64_t>(gd, job.max_bond)) + (int64_t)sizeof(int);  // eigensolver work
     bytes += (2 * cm * cm + std::max<int64_t>(nmax, 2 * cm * cm)) * (int64_t)sizeof(c64);  // transfer
-    bytes += (int64_t)L * 20 * sizeof(c64) * 2 + (job.kdim + 2) * 64 * 16 + 256;
+    bytes += (int64_t)L * 20 * sizeof(c64) * 2 + (job.kdim + 2) * 64 * 16 + 256 + 8;
     return bytes * B;
 }
 
@@ -611,6 +613,9 @@ void Engine<T>::setup() {
     perm.alloc((size_t)B * (gdim + 2) * sizeof(int));
     keep.alloc((size_t)B * sizeof(int));
     Cf.alloc((size_t)B * capmax * capmax * sizeof(CT));
+    slotCgs = (int64_t)capmax * 32;  // nvec x panel width (<= 32)
+    cgsS.alloc((size_t)B * slotCgs * sizeof(CT));
+    cgsF.alloc((size_t)B * sizeof(double));
     int64_t amax = 0;
     for (int i = 0; i < L; ++i) amax = std::max(amax, slotA[i]);
     absorb.alloc((size_t)B * amax * sizeof(CT));
@@ -1122,7 +1127,8 @@ template <typename T>
 void Engine<T>::lq_sweep(State &S, int from, const int *msk) {
     for (int i = from; i >= 1; --i) {
         const int c0 = S.wd[i], c1 = S.wd[i + 1];
-        cgs2<T>(A(S, i), slotA[i], c0, 2 * c1, 2 * c1, 1, Cf.as<CT>(), (int64_t)capmax * capmax, msk, nb, s);
+        cgs2<T>(A(S, i), slotA[i], c0, 2 * c1, 2 * c1, 1, Cf.as<CT>(), (int64_t)capmax * capmax, cgsS.as<CT>(), slotCgs,
+                cgsF.as<double>(), msk, nb, s);
         absorb_left(S, i, msk);
     }
 }
@@ -1131,7 +1137,8 @@ void Engine<T>::lq_sweep(State &S, int from, const int *msk) {
 template <typename T>
 void Engine<T>::qr_step(State &S, int i, const int *msk) {
     const int c0 = S.wd[i], c1 = S.wd[i + 1], c2 = S.wd[i + 2];
-    cgs2<T>(A(S, i), slotA[i], c1, 2 * c0, 1, c1, Cf.as<CT>(), (int64_t)capmax * capmax, msk, nb, s);
+    cgs2<T>(A(S, i), slotA[i], c1, 2 * c0, 1, c1, Cf.as<CT>(), (int64_t)capmax * capmax, cgsS.as<CT>(), slotCgs,
+            cgsF.as<double>(), msk, nb, s);
     // A_{i+1} <- R A_{i+1} with R[j][k] = Cf[k][j]
     gemmT<T>(gd(c1, 2 * c2, c1, OP_T, OP_N, c1, 2 * c2, 2 * c2, (int64_t)capmax * capmax, slotA[i + 1], slotA[i + 1],
                 nb, msk),

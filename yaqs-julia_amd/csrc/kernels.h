@@ -67,9 +67,13 @@ template <typename T>
 void write_iso(const c64 *U, int64_t sU, const int *perm, int64_t sE, const int *keep, int n, int kw, int mode,
                cplx<T> *Out, int64_t sO, const int *mask, int B, hipStream_t s);
 
+// CGS2 orthonormalisation of nvec strided vectors (kernels.hip); S / sS:
+// scratch of cgs2_scratch_elems(nvec, len) elements per trajectory, fro: B
+// doubles.
+int64_t cgs2_scratch_elems(int nvec, int len);
 template <typename T>
-void cgs2(cplx<T> *A, int64_t sA, int nvec, int len, int64_t vs, int64_t es, cplx<T> *Cf, int64_t sC, const int *mask,
-          int B, hipStream_t s);
+void cgs2(cplx<T> *A, int64_t sA, int nvec, int len, int64_t vs, int64_t es, cplx<T> *Cf, int64_t sC, cplx<T> *S,
+          int64_t sS, double *fro, const int *mask, int B, hipStream_t s);
 
 template <typename T>
 void local_op(cplx<T> *A, int64_t sA, int dl, int dr, const c64 *ops, const int *op_idx, const int *mask, int B,

@@ -5,6 +5,10 @@
 // is float (mixed mode) or double; norms, Lanczos scalars, Gram matrices,
 // eigen-decompositions, transfer matrices and jump probabilities are always
 // computed in double.
+#include <algorithm>
+#include <type_traits>
+#include <vector>
+
 #include "kernels.h"
 #include "reduce.h"
 
@@ -517,76 +521,214 @@ template void write_iso<double>(const c64 *, int64_t, const int *, int64_t, cons
 
 // ------------------------------------------------- CGS2 orthonormalisation --
 // Orthonormalises nvec vectors v_k[e] = A[k*vs + e*es] (e < len) in order by
-// classical Gram-Schmidt applied twice (FP64 arithmetic), writing the
-// coefficients Cf[k][j] = <q_j, a_k> (j <= k, row-major nvec x nvec).  A
-// vector whose remaining norm is <= 1e-14 ‖A‖_F becomes exactly zero (a dead
-// direction, e.g. a zero-padded bond index).  Rows give LQ, columns QR.
+// classical Gram-Schmidt applied twice, writing the coefficients
+// Cf[k][j] = <q_j, a_k> (j <= k, row-major nvec x nvec, zero above the
+// diagonal).  A vector whose remaining norm is <= 1e-14 ‖A‖_F becomes
+// exactly zero (a dead direction, e.g. a zero-padded bond index).  Rows give
+// LQ (vs = row stride, es = 1), columns QR (vs = 1, es = row stride).
+//
+// Blocked (panels of pw vectors): each panel is first projected twice
+// against all earlier vectors with two batched matrix-core GEMMs per pass
+// (H = Q^H P, then P -= Q H), then orthonormalised within itself by
+// cgs2_panel_kernel with the panel held in LDS in FP64.
+
+// ‖A‖_F per trajectory (FP64).
 template <typename T>
-__global__ __launch_bounds__(256) void cgs2_kernel(cplx<T> *A, int64_t sA, int nvec, int len, int64_t vs, int64_t es,
-                                                   cplx<T> *Cf, int64_t sC, const int *mask) {
+__global__ __launch_bounds__(256) void cgs_fro_kernel(const cplx<T> *A, int64_t sA, int nvec, int len, int64_t vs,
+                                                      int64_t es, double *fro, const int *mask) {
     __shared__ double sh[16];
-    __shared__ double2 h[1024];
+    const int t = blockIdx.x;
+    if (mask && !mask[t]) return;
+    const cplx<T> *M = A + t * sA;
+    double f = 0;
+    for (int64_t e = threadIdx.x; e < (int64_t)nvec * len; e += blockDim.x) {
+        const int k = (int)(e / len), i = (int)(e % len);
+        const double2 z = ld2(M[k * vs + i * es]);
+        f += z.x * z.x + z.y * z.y;
+    }
+    f = block_sum(f, sh);
+    if (threadIdx.x == 0) fro[t] = sqrt(f);
+}
+
+// Coefficients of one projection pass of panel [p0, p0+pw): H[j][q] (j < p0)
+// from the GEMM scratch S (QR: S = H, p0 x pw; LQ: S = H^T, pw x p0), added
+// to Cf[p0 + q][j] (stored on pass 0), and S replaced by the operand of the
+// update GEMM: -H (QR) or -H^T (LQ).
+template <typename T>
+__global__ __launch_bounds__(256) void cgs_coef_kernel(cplx<T> *S, int64_t sS, cplx<T> *Cf, int64_t sC, int nvec,
+                                                       int p0, int pw, int lq, int pass, const int *mask) {
+    const int t = blockIdx.y;
+    if (mask && !mask[t]) return;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p0 * pw) return;
+    cplx<T> *St = S + t * sS;
+    cplx<T> *C = Cf + t * sC;
+    int j, q;
+    double2 h;
+    if (lq) {
+        q = e / p0;
+        j = e % p0;
+        h = ld2(St[e]);
+    } else {
+        j = e / pw;
+        q = e % pw;
+        h = ld2(St[e]);
+    }
+    st2(St[e], make_double2(-h.x, -h.y));
+    cplx<T> &c = C[(int64_t)(p0 + q) * nvec + j];
+    st2(c, pass ? cadd(ld2(c), h) : h);
+}
+
+// CGS2 of the vectors [p0, p0+pw) among themselves (they are already
+// orthogonal to the earlier ones), panel in LDS as FP64 [pw][len]; writes
+// Cf[k][j] for p0 <= j <= k and zeroes Cf[k][j] for j > k.
+template <typename T>
+__global__ __launch_bounds__(256) void cgs2_panel_kernel(cplx<T> *A, int64_t sA, int nvec, int len, int64_t vs,
+                                                         int64_t es, int p0, int pw, cplx<T> *Cf, int64_t sC,
+                                                         const double *fro, const int *mask) {
+    extern __shared__ double2 pan[];  // [pw][len]
+    __shared__ double sh[16];
+    __shared__ double2 h[64];
     const int t = blockIdx.x;
     if (mask && !mask[t]) return;
     cplx<T> *M = A + t * sA;
     cplx<T> *C = Cf + t * sC;
-    for (int64_t e = threadIdx.x; e < (int64_t)nvec * nvec; e += blockDim.x) C[e].re = C[e].im = (T)0;
-    double fro = 0;
-    for (int64_t e = threadIdx.x; e < (int64_t)nvec * len; e += blockDim.x) {
-        const int k = (int)(e / len), i = (int)(e % len);
-        double2 z = ld2(M[k * vs + i * es]);
-        fro += z.x * z.x + z.y * z.y;
-    }
-    fro = sqrt(block_sum(fro, sh));
-    const double dead = 1e-14 * fro;
+    const double dead = 1e-14 * fro[t];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int k = 0; k < nvec; ++k) {
+    // load, coalesced along whichever index is contiguous
+    if (vs == 1) {
+        for (int e = threadIdx.x; e < pw * len; e += blockDim.x) {
+            const int i = e / pw, q = e % pw;
+            pan[q * len + i] = ld2(M[(int64_t)(p0 + q) + (int64_t)i * es]);
+        }
+    } else {
+        for (int e = threadIdx.x; e < pw * len; e += blockDim.x) {
+            const int q = e / len, i = e % len;
+            pan[q * len + i] = ld2(M[(int64_t)(p0 + q) * vs + (int64_t)i * es]);
+        }
+    }
+    for (int e = threadIdx.x; e < pw * nvec; e += blockDim.x) {
+        const int q = e / nvec, j = e % nvec;
+        if (j >= p0 + q) C[(int64_t)(p0 + q) * nvec + j] = cmake<T>((T)0, (T)0);  // diagonal set below
+    }
+    __syncthreads();
+    for (int q = 0; q < pw; ++q) {
+        double2 *v = pan + q * len;
         for (int pass = 0; pass < 2; ++pass) {
-            if (k > 0) {
-                for (int j = wid; j < k; j += nw) {
+            if (q > 0) {
+                for (int j = wid; j < q; j += nw) {
+                    const double2 *u = pan + j * len;
                     double2 acc = make_double2(0, 0);
-                    for (int i = lane; i < len; i += 64)
-                        acc = cadd(acc, cmulc(ld2(M[j * vs + i * es]), ld2(M[k * vs + i * es])));
+                    for (int i = lane; i < len; i += 64) acc = cadd(acc, cmulc(u[i], v[i]));
                     acc.x = wave_sum(acc.x);
                     acc.y = wave_sum(acc.y);
                     if (lane == 0) h[j] = acc;
                 }
                 __syncthreads();
                 for (int i = threadIdx.x; i < len; i += blockDim.x) {
-                    double2 v = ld2(M[k * vs + i * es]);
-                    for (int j = 0; j < k; ++j) v = csub(v, cmul(h[j], ld2(M[j * vs + i * es])));
-                    st2(M[k * vs + i * es], v);
+                    double2 x = v[i];
+                    for (int j = 0; j < q; ++j) x = csub(x, cmul(h[j], pan[j * len + i]));
+                    v[i] = x;
                 }
-                for (int j = threadIdx.x; j < k; j += blockDim.x) {
-                    double2 c = ld2(C[(int64_t)k * nvec + j]);
-                    st2(C[(int64_t)k * nvec + j], cadd(c, h[j]));
+                for (int j = threadIdx.x; j < q; j += blockDim.x) {
+                    cplx<T> &c = C[(int64_t)(p0 + q) * nvec + p0 + j];
+                    st2(c, pass ? cadd(ld2(c), h[j]) : h[j]);
                 }
                 __syncthreads();
             }
         }
         double nr = 0;
-        for (int i = threadIdx.x; i < len; i += blockDim.x) {
-            double2 z = ld2(M[k * vs + i * es]);
-            nr += z.x * z.x + z.y * z.y;
-        }
+        for (int i = threadIdx.x; i < len; i += blockDim.x) nr += v[i].x * v[i].x + v[i].y * v[i].y;
         nr = sqrt(block_sum(nr, sh));
         const bool live = nr > dead && nr > 0.0;
         const double inv = live ? 1.0 / nr : 0.0;
-        for (int i = threadIdx.x; i < len; i += blockDim.x) st2(M[k * vs + i * es], cscale(ld2(M[k * vs + i * es]), inv));
-        if (threadIdx.x == 0) st2(C[(int64_t)k * nvec + k], make_double2(live ? nr : 0.0, 0.0));
+        for (int i = threadIdx.x; i < len; i += blockDim.x) v[i] = cscale(v[i], inv);
+        if (threadIdx.x == 0) C[(int64_t)(p0 + q) * nvec + p0 + q] = cmake<T>((T)(live ? nr : 0.0), (T)0);
         __syncthreads();
+    }
+    if (vs == 1) {
+        for (int e = threadIdx.x; e < pw * len; e += blockDim.x) {
+            const int i = e / pw, q = e % pw;
+            st2(M[(int64_t)(p0 + q) + (int64_t)i * es], pan[q * len + i]);
+        }
+    } else {
+        for (int e = threadIdx.x; e < pw * len; e += blockDim.x) {
+            const int q = e / len, i = e % len;
+            st2(M[(int64_t)(p0 + q) * vs + (int64_t)i * es], pan[q * len + i]);
+        }
     }
 }
 
-template <typename T>
-void cgs2(cplx<T> *A, int64_t sA, int nvec, int len, int64_t vs, int64_t es, cplx<T> *Cf, int64_t sC, const int *mask,
-          int B, hipStream_t s) {
-    if (nvec > 1024) throw HipError("cgs2: more than 1024 vectors");
-    hipLaunchKernelGGL(cgs2_kernel<T>, dim3(B), dim3(256), 0, s, A, sA, nvec, len, vs, es, Cf, sC, mask);
-    YQ_HIP(hipGetLastError());
+// Panel width: the panel (pw x len FP64 complex) fits in 128 KB of LDS.
+inline int cgs_panel_width(int len) {
+    int pw = 32;
+    while (pw > 1 && (int64_t)pw * len * 16 > 128 * 1024) pw /= 2;
+    return pw;
 }
-template void cgs2<float>(c32 *, int64_t, int, int, int64_t, int64_t, c32 *, int64_t, const int *, int, hipStream_t);
-template void cgs2<double>(c64 *, int64_t, int, int, int64_t, int64_t, c64 *, int64_t, const int *, int, hipStream_t);
+
+int64_t cgs2_scratch_elems(int nvec, int len) { return (int64_t)nvec * cgs_panel_width(len); }
+
+template <typename T>
+void cgs2(cplx<T> *A, int64_t sA, int nvec, int len, int64_t vs, int64_t es, cplx<T> *Cf, int64_t sC, cplx<T> *S,
+          int64_t sS, double *fro, const int *mask, int B, hipStream_t s) {
+    if (len > 8192) throw HipError("cgs2: vectors longer than 8192");
+    if (vs != 1 && es != 1) throw HipError("cgs2: one of the strides must be 1");
+    const bool lq = vs != 1;
+    const int PB = cgs_panel_width(len);
+    hipLaunchKernelGGL(cgs_fro_kernel<T>, dim3(B), dim3(256), 0, s, A, sA, nvec, len, vs, es, fro, mask);
+    YQ_HIP(hipGetLastError());
+    for (int p0 = 0; p0 < nvec; p0 += PB) {
+        const int pw = std::min(PB, nvec - p0);
+        if (p0 > 0) {
+            // H = Q^H P over the earlier vectors Q = [0, p0), then P -= Q H, twice
+            GemmDesc g1{}, g2{};
+            g1.inner = g2.inner = 1;
+            g1.outer = g2.outer = B;
+            g1.active = g2.active = mask;
+            g1.sA_outer = g1.sB_outer = sA;
+            g1.sC_outer = sS;
+            g2.sA_outer = lq ? sS : sA;
+            g2.sB_outer = lq ? sA : sS;
+            g2.sC_outer = sA;
+            g2.accumulate = 1;
+            const cplx<T> *g1a, *g1b, *g2a, *g2b;
+            cplx<T> *P = lq ? A + (int64_t)p0 * vs : A + p0;
+            if (!lq) {
+                // A is len x nvec row-major (ld es): H (p0 x pw) = Q^H P
+                g1.M = p0; g1.N = pw; g1.K = len; g1.opA = OP_C; g1.opB = OP_N;
+                g1.lda = es; g1.ldb = es; g1.ldc = pw;
+                g1a = A; g1b = P;
+                // P (len x pw) += Q (len x p0) (-H) (p0 x pw)
+                g2.M = len; g2.N = pw; g2.K = p0; g2.opA = OP_N; g2.opB = OP_N;
+                g2.lda = es; g2.ldb = pw; g2.ldc = es;
+                g2a = A; g2b = S;
+            } else {
+                // A is nvec x len row-major (ld vs): S (pw x p0) = P Q^H = H^T
+                g1.M = pw; g1.N = p0; g1.K = len; g1.opA = OP_N; g1.opB = OP_C;
+                g1.lda = vs; g1.ldb = vs; g1.ldc = p0;
+                g1a = P; g1b = A;
+                // P (pw x len) += (-H^T) (pw x p0) Q (p0 x len)
+                g2.M = pw; g2.N = len; g2.K = p0; g2.opA = OP_N; g2.opB = OP_N;
+                g2.lda = p0; g2.ldb = vs; g2.ldc = vs;
+                g2a = S; g2b = A;
+            }
+            for (int pass = 0; pass < 2; ++pass) {
+                if (!gemm_mfma<T>(g1, g1a, g1b, S, s, 0)) gemm<T, T, T, T>(g1, g1a, g1b, S, s);
+                hipLaunchKernelGGL(cgs_coef_kernel<T>, dim3((p0 * pw + 255) / 256, B), dim3(256), 0, s, S, sS, Cf, sC,
+                                   nvec, p0, pw, lq ? 1 : 0, pass, mask);
+                YQ_HIP(hipGetLastError());
+                if (!gemm_mfma<T>(g2, g2a, g2b, P, s, 0)) gemm<T, T, T, T>(g2, g2a, g2b, P, s);
+            }
+        }
+        hipLaunchKernelGGL(cgs2_panel_kernel<T>, dim3(B), dim3(256), (size_t)pw * len * sizeof(double2), s, A, sA, nvec,
+                           len, vs, es, p0, pw, Cf, sC, (const double *)fro, mask);
+        YQ_HIP(hipGetLastError());
+    }
+}
+template void cgs2<float>(c32 *, int64_t, int, int, int64_t, int64_t, c32 *, int64_t, c32 *, int64_t, double *,
+                          const int *, int, hipStream_t);
+template void cgs2<double>(c64 *, int64_t, int, int, int64_t, int64_t, c64 *, int64_t, c64 *, int64_t, double *,
+                           const int *, int, hipStream_t);
 
 // ------------------------------------------------------- local operators --
 // A[a][s][b] <- Σ_s' M[s][s'] A[a][s'][b]; M given per trajectory through
@@ -974,3 +1116,50 @@ void fill_int(int *p, int v, int n, hipStream_t s) {
 }
 
 }  // namespace yq
+
+// Test hook (not part of the public header): CGS2 of `count` matrices of
+// nvec vectors of length len (host arrays of interleaved doubles, one matrix
+// after the other), as rows (lq = 1: nvec x len row-major) or as columns
+// (lq = 0: len x nvec row-major); A is overwritten with Q, Cf receives the
+// nvec x nvec coefficients.  prec 0 = FP64, 1 = FP32.
+extern "C" int yaqs_debug_cgs2(int prec, int nvec, int len, int lq, int count, double *A, double *Cf) {
+    using namespace yq;
+    try {
+        const int64_t na = (int64_t)nvec * len, nc = (int64_t)nvec * nvec;
+        const int64_t ns = cgs2_scratch_elems(nvec, len);
+        auto run = [&](auto *tag) {
+            typedef std::remove_pointer_t<decltype(tag)> T;
+            std::vector<cplx<T>> ha(na * count), hc(nc * count);
+            for (int64_t i = 0; i < na * count; ++i) ha[i] = cmake<T>((T)A[2 * i], (T)A[2 * i + 1]);
+            cplx<T> *da, *dc, *dsc;
+            double *df;
+            YQ_HIP(hipMalloc(&da, na * count * sizeof(cplx<T>)));
+            YQ_HIP(hipMalloc(&dc, nc * count * sizeof(cplx<T>)));
+            YQ_HIP(hipMalloc(&dsc, ns * count * sizeof(cplx<T>)));
+            YQ_HIP(hipMalloc(&df, count * sizeof(double)));
+            YQ_HIP(hipMemcpy(da, ha.data(), na * count * sizeof(cplx<T>), hipMemcpyHostToDevice));
+            if (lq) cgs2<T>(da, na, nvec, len, len, 1, dc, nc, dsc, ns, df, nullptr, count, 0);
+            else cgs2<T>(da, na, nvec, len, 1, nvec, dc, nc, dsc, ns, df, nullptr, count, 0);
+            YQ_HIP(hipDeviceSynchronize());
+            YQ_HIP(hipMemcpy(ha.data(), da, na * count * sizeof(cplx<T>), hipMemcpyDeviceToHost));
+            YQ_HIP(hipMemcpy(hc.data(), dc, nc * count * sizeof(cplx<T>), hipMemcpyDeviceToHost));
+            (void)hipFree(da);
+            (void)hipFree(dc);
+            (void)hipFree(dsc);
+            (void)hipFree(df);
+            for (int64_t i = 0; i < na * count; ++i) {
+                A[2 * i] = (double)ha[i].re;
+                A[2 * i + 1] = (double)ha[i].im;
+            }
+            for (int64_t i = 0; i < nc * count; ++i) {
+                Cf[2 * i] = (double)hc[i].re;
+                Cf[2 * i + 1] = (double)hc[i].im;
+            }
+        };
+        if (prec == 1) run((float *)nullptr);
+        else run((double *)nullptr);
+        return 0;
+    } catch (const std::exception &) {
+        return 3;
+    }
+}
