@@ -948,9 +948,13 @@ __global__ __launch_bounds__(NWV * 64) void ormtr_kernel(char *__restrict__ Wk, 
 // workgroup per matrix on a full-storage copy A in global memory (the whole
 // Hermitian matrix is kept so that every access is a contiguous row: the
 // column below the diagonal is read as the conjugate of row k, and one
-// wavefront handles one row of the Hermitian product and of the rank-2
-// update).  v_k goes to row k right of the diagonal, which the reduction no
-// longer reads.  Same conventions as hetrd_kernel.
+// wavefront handles one row).  v_k goes to row k right of the diagonal,
+// which the reduction no longer reads.  Same conventions as hetrd_kernel.
+// The pass is HBM-bound, so the rank-2 update of step k and the Hermitian
+// product of step k+1 share one sweep over the trailing matrix: row k+1 is
+// updated first, v_{k+1} is formed from it, then every further row is
+// updated and immediately multiplied by v_{k+1} (one read and one write of
+// the trailing matrix per column).
 constexpr int WT_WAVES = 16;
 __global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_kernel(const c64 *__restrict__ G, int64_t sG,
                                                                    char *__restrict__ Wk, int64_t sW, int n, int kw,
@@ -959,7 +963,7 @@ __global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_kernel(const c64 *__
     const int t = blockIdx.x;
     if (mask && !mask[t]) return;
     const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    __shared__ double2 vs[WIDE_MAX_N], ps[WIDE_MAX_N];
+    __shared__ double2 vbuf[2][WIDE_MAX_N], pbuf[2][WIDE_MAX_N];
     __shared__ double rs[WT_WAVES];
     __shared__ double2 rs2[WT_WAVES];
     const HeevLayW lay = heev_lay_wide(n, kw);
@@ -971,9 +975,12 @@ __global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_kernel(const c64 *__
     const c64 *Gt = G + t * sG;
     for (int64_t e = tid; e < (int64_t)n * n; e += NT) Am[e] = make_double2(Gt[e].re, Gt[e].im);
     __syncthreads();
-    for (int k = 0; k + 1 < n; ++k) {
+    // Householder vector of column k from row k of the current matrix (its
+    // conjugate is the column below the diagonal): v into vs, v_k into row k,
+    // d_k, e_k, tau_k; returns tau_k.  Two barriers.
+    auto reflector = [&](int k, double2 *vs) -> double2 {
         const int m = n - k - 1;
-        double2 *rowk = Am + (int64_t)k * n + k + 1;  // conj of the column below the diagonal
+        double2 *rowk = Am + (int64_t)k * n + k + 1;
         double xs = 0;
         for (int r = 1 + tid; r < m; r += NT) {
             const double2 a = rowk[r];
@@ -1004,20 +1011,10 @@ __global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_kernel(const c64 *__
         }
         __syncthreads();
         for (int r = tid; r < m; r += NT) rowk[r] = vs[r];
-        // p = tau A22 v, one wavefront per row
-        for (int rr = w; rr < m; rr += WT_WAVES) {
-            const double2 *arow = Am + (int64_t)(k + 1 + rr) * n + k + 1;
-            double2 acc = make_double2(0, 0);
-            for (int c = lane; c < m; c += 64) {
-                const double2 z = c_mul(arow[c], vs[c]);
-                acc.x += z.x;
-                acc.y += z.y;
-            }
-            acc.x = wsum(acc.x);
-            acc.y = wsum(acc.y);
-            if (lane == 0) ps[rr] = c_mul(tk, acc);
-        }
-        __syncthreads();
+        return tk;
+    };
+    // al = -tau (p^H v) / 2 over m entries.  One barrier.
+    auto pv_alpha = [&](int m, const double2 *vs, const double2 *ps, double2 tk) -> double2 {
         double2 pvp = make_double2(0, 0);
         for (int r = tid; r < m; r += NT) {
             const double2 z = c_mulc(ps[r], vs[r]);
@@ -1034,10 +1031,34 @@ __global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_kernel(const c64 *__
             pv.x += rs2[i].x;
             pv.y += rs2[i].y;
         }
-        const double2 al = c_mul(make_double2(-0.5 * tk.x, -0.5 * tk.y), pv);
-        // A22 -= v w^H + w v^H with w = p + al v
+        return c_mul(make_double2(-0.5 * tk.x, -0.5 * tk.y), pv);
+    };
+    // step 0: reflector and the Hermitian product p = tau A22 v from scratch
+    int cur = 0;
+    double2 tk = reflector(0, vbuf[0]);
+    {
+        const int m = n - 1;
+        const double2 *vs = vbuf[0];
         for (int rr = w; rr < m; rr += WT_WAVES) {
-            double2 *arow = Am + (int64_t)(k + 1 + rr) * n + k + 1;
+            const double2 *arow = Am + (int64_t)(1 + rr) * n + 1;
+            double2 acc = make_double2(0, 0);
+            for (int c = lane; c < m; c += 64) {
+                const double2 z = c_mul(arow[c], vs[c]);
+                acc.x += z.x;
+                acc.y += z.y;
+            }
+            acc.x = wsum(acc.x);
+            acc.y = wsum(acc.y);
+            if (lane == 0) pbuf[0][rr] = c_mul(tk, acc);
+        }
+        __syncthreads();
+    }
+    double2 al = pv_alpha(n - 1, vbuf[0], pbuf[0], tk);
+    for (int k = 0; k + 1 < n; ++k) {
+        const int m = n - k - 1;
+        const double2 *vs = vbuf[cur], *ps = pbuf[cur];
+        // A22 -= v w^H + w v^H with w = p + al v; row k+1 first
+        auto update_row = [&](int rr, double2 *arow, double2 *vnext, double2 *acc) {
             const double2 vr = vs[rr], pr = ps[rr];
             const double2 wr = make_double2(pr.x + al.x * vr.x - al.y * vr.y, pr.y + al.x * vr.y + al.y * vr.x);
             for (int c = lane; c < m; c += 64) {
@@ -1047,9 +1068,30 @@ __global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_kernel(const c64 *__
                 a.x -= vr.x * wc.x + vr.y * wc.y + wr.x * vc.x + wr.y * vc.y;
                 a.y -= vr.y * wc.x - vr.x * wc.y + wr.y * vc.x - wr.x * vc.y;
                 arow[c] = a;
+                if (vnext && c >= 1) {
+                    const double2 z = c_mul(a, vnext[c - 1]);
+                    acc->x += z.x;
+                    acc->y += z.y;
+                }
             }
+        };
+        if (w == 0) update_row(0, Am + (int64_t)(k + 1) * n + k + 1, nullptr, nullptr);
+        __syncthreads();
+        if (k + 2 >= n) break;  // the last column needs no reflector
+        const int nxt = cur ^ 1;
+        const double2 tk1 = reflector(k + 1, vbuf[nxt]);
+        // remaining rows: update, then their product with v_{k+1}
+        for (int rr = 1 + w; rr < m; rr += WT_WAVES) {
+            double2 acc = make_double2(0, 0);
+            update_row(rr, Am + (int64_t)(k + 1 + rr) * n + k + 1, vbuf[nxt], &acc);
+            acc.x = wsum(acc.x);
+            acc.y = wsum(acc.y);
+            if (lane == 0) pbuf[nxt][rr - 1] = c_mul(tk1, acc);
         }
         __syncthreads();
+        al = pv_alpha(m - 1, vbuf[nxt], pbuf[nxt], tk1);
+        tk = tk1;
+        cur = nxt;
     }
     if (tid == 0) {
         dg[n - 1] = Am[(int64_t)(n - 1) * n + n - 1].x;
@@ -1243,9 +1285,24 @@ int64_t heev_work_bytes(int n, int kmax) {
     return std::max(narrow, heev_lay_wide(n, kw).total);
 }
 
-void heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, double2 *Pg, int64_t sP, void *work,
+static size_t heev_lds_shmem(int n) {
+    size_t shmem = (size_t)n * (4 * sizeof(double) + sizeof(int)) + 16;
+    if (n <= LDS_MAX_N) shmem += (size_t)(n * (n + 1) / 2 + n) * sizeof(double2);
+    return shmem;
+}
+
+void heev_fixup(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, double2 *Pg, int64_t sP,
+                const int *fallback, int n, int ns, double *evals, int *perm, int64_t sE, int *keep, int rule,
+                double thr, int max_bond, const int *mask, int B, hipStream_t s) {
+    if (n > LDS_MAX_N && !Pg) throw HipError("heev: no global scratch for n > 128");
+    hipLaunchKernelGGL(heev_lds_kernel, dim3(B), dim3(256), heev_lds_shmem(n), s, G, sG, U, sU, Z, sZ, Pg, sP, n, ns,
+                       evals, perm, sE, keep, rule, thr, max_bond, mask, fallback);
+    YQ_HIP(hipGetLastError());
+}
+
+bool heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, double2 *Pg, int64_t sP, void *work,
           int64_t sW, int *fallback, int n, int ns, double *evals, int *perm, int64_t sE, int *keep, int rule,
-          double thr, int max_bond, const int *mask, int B, hipStream_t s) {
+          double thr, int max_bond, const int *mask, int B, hipStream_t s, bool fixup) {
     if (n > 1024) throw HipError("heev: n > 1024");
     int kout = ns < max_bond ? ns : max_bond;
     if (kout > n) kout = n;
@@ -1254,9 +1311,7 @@ void heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, d
     const bool wide = n > LDS_MAX_N && n <= WIDE_MAX_N && kw <= WIDE_MAX_K && work && fallback &&
                       heev_lay_wide(n, kw).total <= sW;
     const int np = n * (n + 1) / 2;
-    size_t shmem = (size_t)n * (4 * sizeof(double) + sizeof(int)) + 16;
-    if (n <= LDS_MAX_N) shmem += (size_t)(np + n) * sizeof(double2);
-    else if (!Pg) throw HipError("heev: no global scratch for n > 128");
+    if (n > LDS_MAX_N && !Pg) throw HipError("heev: no global scratch for n > 128");
     char *W = (char *)work;
     if (fast) {
         const HeevLay lay = heev_lay(n);
@@ -1289,15 +1344,14 @@ void heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, d
                            lay.X, kw, n, kout, U, sU, mask, (const int *)fallback);
         YQ_HIP(hipGetLastError());
     } else {
-        hipLaunchKernelGGL(heev_lds_kernel, dim3(B), dim3(256), shmem, s, G, sG, U, sU, Z, sZ, Pg, sP, n, ns, evals,
-                           perm, sE, keep, rule, thr, max_bond, mask, (const int *)nullptr);
-        YQ_HIP(hipGetLastError());
-        return;
+        heev_fixup(G, sG, U, sU, Z, sZ, Pg, sP, nullptr, n, ns, evals, perm, sE, keep, rule, thr, max_bond, mask, B, s);
+        return false;
     }
     // matrices whose inverse iteration broke down: full single-workgroup solver
-    hipLaunchKernelGGL(heev_lds_kernel, dim3(B), dim3(256), shmem, s, G, sG, U, sU, Z, sZ, Pg, sP, n, ns, evals, perm,
-                       sE, keep, rule, thr, max_bond, mask, (const int *)fallback);
-    YQ_HIP(hipGetLastError());
+    if (fixup)
+        heev_fixup(G, sG, U, sU, Z, sZ, Pg, sP, fallback, n, ns, evals, perm, sE, keep, rule, thr, max_bond, mask, B,
+                   s);
+    return true;
 }
 
 }  // namespace yq

@@ -1020,13 +1020,25 @@ void Engine<T>::split(State &S, int i, bool right, int rule, double thr) {
     else
         gemm<T, T, double, double>(gd(n, n, m, OP_C, OP_N, n, n, n, nmax, nmax, slotG, nb), theta.as<CT>(),
                                    theta.as<CT>(), G.as<c64>(), s);
-    heev(G.as<c64>(), slotG, U.as<c64>(), slotG, Zbuf.as<double>(), slotZ, Pbuf.p ? Pbuf.as<double2>() : nullptr,
-         slotP, Hwork.p, slotH, Hfb.p ? Hfb.as<int>() : nullptr, gn, ns, evals.as<double>(), perm.as<int>(), gdim + 2, keep.as<int>(), rule, thr,
-         (int)std::min<int64_t>(J.max_bond, 1 << 30), nullptr, nb, s);
-    // new working dimension = max kept rank over the wave
-    std::vector<int> hk(nb);
+    const int mb = (int)std::min<int64_t>(J.max_bond, 1 << 30);
+    double2 *Pg = Pbuf.p ? Pbuf.as<double2>() : nullptr;
+    int *fb = Hfb.p ? Hfb.as<int>() : nullptr;
+    const bool multi = heev(G.as<c64>(), slotG, U.as<c64>(), slotG, Zbuf.as<double>(), slotZ, Pg, slotP, Hwork.p, slotH,
+                            fb, gn, ns, evals.as<double>(), perm.as<int>(), gdim + 2, keep.as<int>(), rule, thr, mb,
+                            nullptr, nb, s, false);
+    // new working dimension = max kept rank over the wave; the kept ranks and
+    // the inverse-iteration breakdown flags come back in one readback, and the
+    // single-workgroup solver runs only when a matrix needs it
+    std::vector<int> hk(nb), hf(multi ? nb : 0);
     YQ_HIP(hipMemcpyAsync(hk.data(), keep.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (multi) YQ_HIP(hipMemcpyAsync(hf.data(), fb, nb * sizeof(int), hipMemcpyDeviceToHost, s));
     YQ_HIP(hipStreamSynchronize(s));
+    if (multi && std::any_of(hf.begin(), hf.end(), [](int f) { return f != 0; })) {
+        heev_fixup(G.as<c64>(), slotG, U.as<c64>(), slotG, Zbuf.as<double>(), slotZ, Pg, slotP, fb, gn, ns,
+                   evals.as<double>(), perm.as<int>(), gdim + 2, keep.as<int>(), rule, thr, mb, nullptr, nb, s);
+        YQ_HIP(hipMemcpyAsync(hk.data(), keep.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        YQ_HIP(hipStreamSynchronize(s));
+    }
     int kw = 1;
     for (int v : hk) kw = std::max(kw, v);
     if (kw > cap[i + 1]) throw std::runtime_error("bond dimension exceeds capacity");
@@ -1475,26 +1487,8 @@ void Engine<T>::run_digital(int64_t t0, int nb_, double *d_out, int64_t *h_bonds
                                       mask.as<int>(), nb, s);
                     rho<T>(Xbuf.as<c64>(), slotX, theta.as<CT>(), nmax, cl, 4, c2, rho2.as<c64>(), (int64_t)L * 16,
                            mask.as<int>(), nb, s);
-                    // partial traces of ρ2 into the 1-site matrices of sites a and a+1:
-                    // site a: ρ_a[s][s'] = Σ_{s2} ρ2[(2s+s2)][(2s'+s2)]
-                    // site a+1: ρ_{a+1}[s][s'] = Σ_{s1} ρ2[(2s1+s)][(2s1+s')]
-                    std::vector<c64> h2((size_t)nb * 16);
-                    YQ_HIP(hipMemcpy2DAsync(h2.data(), 16 * sizeof(c64), rho2.p, (size_t)L * 16 * sizeof(c64),
-                                            16 * sizeof(c64), nb, hipMemcpyDeviceToHost, s));
-                    YQ_HIP(hipStreamSynchronize(s));
-                    std::vector<c64> h1((size_t)nb * L * 4, mk(0, 0));
-                    for (int t = 0; t < nb; ++t)
-                        for (int x = 0; x < 2; ++x)
-                            for (int y = 0; y < 2; ++y) {
-                                cd ra = 0, rb = 0;
-                                for (int z = 0; z < 2; ++z) {
-                                    ra += toc(h2[(size_t)t * 16 + (2 * x + z) * 4 + (2 * y + z)]);
-                                    rb += toc(h2[(size_t)t * 16 + (2 * z + x) * 4 + (2 * z + y)]);
-                                }
-                                h1[((size_t)t * L + a) * 4 + x * 2 + y] = fromc(ra);
-                                h1[((size_t)t * L + a + 1) * 4 + x * 2 + y] = fromc(rb);
-                            }
-                    YQ_HIP(hipMemcpyAsync(rho1.p, h1.data(), h1.size() * sizeof(c64), hipMemcpyHostToDevice, s));
+                    // 1-site matrices of sites a and a+1 (partial traces of ρ2)
+                    ptrace_pair(rho2.as<c64>(), (int64_t)L * 16, a, rho1.as<c64>(), (int64_t)L * 4, L, nb, s);
                     jump_choose(rho1.as<c64>(), (int64_t)L * 4, L, pc.n, pc.site.as<int>(), pc.LdL.as<c64>(),
                                 pc.gamma.as<double>(), 1.0, J.seed, traj0, rngk.as<uint64_t>(), jstate.as<int>(),
                                 chosen.as<int>(), wbuf.as<double>(), nb, s);

@@ -60,9 +60,17 @@ constexpr int LZ_CHUNK_ELEMS = 2048;
 // Z / sZ: n*n + 2n + 4 doubles per matrix; Pg / sP: n(n+1)/2 double2 per
 // matrix when n > 128.
 int64_t heev_work_bytes(int n, int kmax);
-void heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, double2 *Pg, int64_t sP, void *work,
+// Returns true when the multi-kernel path ran; with fixup = false the caller
+// then runs heev_fixup for the matrices flagged in `fallback` itself (for
+// instance only when its readback shows one).
+bool heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, double2 *Pg, int64_t sP, void *work,
           int64_t sW, int *fallback, int n, int ns, double *evals, int *perm, int64_t sE, int *keep, int rule,
-          double thr, int max_bond, const int *mask, int B, hipStream_t s);
+          double thr, int max_bond, const int *mask, int B, hipStream_t s, bool fixup = true);
+// The single-workgroup solver for the matrices with fallback[t] != 0 (all
+// matrices when fallback is null).
+void heev_fixup(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, double2 *Pg, int64_t sP,
+                const int *fallback, int n, int ns, double *evals, int *perm, int64_t sE, int *keep, int rule,
+                double thr, int max_bond, const int *mask, int B, hipStream_t s);
 template <typename T>
 void write_iso(const c64 *U, int64_t sU, const int *perm, int64_t sE, const int *keep, int n, int kw, int mode,
                cplx<T> *Out, int64_t sO, const int *mask, int B, hipStream_t s);
@@ -75,6 +83,9 @@ template <typename T>
 void cgs2(cplx<T> *A, int64_t sA, int nvec, int len, int64_t vs, int64_t es, cplx<T> *Cf, int64_t sC, cplx<T> *S,
           int64_t sS, double *fro, const int *mask, int B, hipStream_t s);
 
+// ρ1 of sites a and a+1 (L x 2 x 2 per trajectory, zero elsewhere) as partial
+// traces of the pair's ρ2 (4 x 4 per trajectory).
+void ptrace_pair(const c64 *rho2, int64_t s2, int a, c64 *rho1, int64_t s1, int L, int B, hipStream_t s);
 template <typename T>
 void local_op(cplx<T> *A, int64_t sA, int dl, int dr, const c64 *ops, const int *op_idx, const int *mask, int B,
               hipStream_t s);

@@ -730,6 +730,32 @@ template void cgs2<float>(c32 *, int64_t, int, int, int64_t, int64_t, c32 *, int
 template void cgs2<double>(c64 *, int64_t, int, int, int64_t, int64_t, c64 *, int64_t, c64 *, int64_t, double *,
                            const int *, int, hipStream_t);
 
+// ----------------------------------------------------- pair partial traces --
+// One-site density matrices of sites a and a+1 from the 2-site ρ2 of the pair
+// (row index 2 s_a + s_{a+1}): ρ_a[x][y] = Σ_z ρ2[2x+z][2y+z],
+// ρ_{a+1}[x][y] = Σ_z ρ2[2z+x][2z+y]; every other site of ρ1 is zeroed.
+__global__ void ptrace_pair_kernel(const c64 *rho2, int64_t s2, int a, c64 *rho1, int64_t s1, int L) {
+    const int t = blockIdx.x;
+    const c64 *r2 = rho2 + t * s2;
+    c64 *r1 = rho1 + t * s1;
+    for (int e = threadIdx.x; e < 4 * L; e += blockDim.x) {
+        const int site = e / 4, x = (e % 4) / 2, y = e % 2;
+        double2 acc = make_double2(0, 0);
+        if (site == a || site == a + 1)
+            for (int z = 0; z < 2; ++z) {
+                const c64 v = site == a ? r2[(2 * x + z) * 4 + (2 * y + z)] : r2[(2 * z + x) * 4 + (2 * z + y)];
+                acc.x += v.re;
+                acc.y += v.im;
+            }
+        r1[e] = cmake<double>(acc.x, acc.y);
+    }
+}
+
+void ptrace_pair(const c64 *rho2, int64_t s2, int a, c64 *rho1, int64_t s1, int L, int B, hipStream_t s) {
+    hipLaunchKernelGGL(ptrace_pair_kernel, dim3(B), dim3(64), 0, s, rho2, s2, a, rho1, s1, L);
+    YQ_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------- local operators --
 // A[a][s][b] <- Σ_s' M[s][s'] A[a][s'][b]; M given per trajectory through
 // op_idx (index into ops, -1 = skip) or shared (op_idx == nullptr -> ops[0]).
