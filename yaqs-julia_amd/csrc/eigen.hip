@@ -571,41 +571,54 @@ __global__ __launch_bounds__(TRD_WAVES * 64) void hetrd_kernel(const c64 *__rest
         const double2 al = c_mul(make_double2(-0.5 * tk.x, -0.5 * tk.y), pv);
         double2 *xn = xb + (cur ^ 1) * n;
         double xsn = 0;
-        // two rows per pass (r and r + TRD_WAVES), sharing the column loads
-        for (int r = w; r < m; r += 2 * TRD_WAVES) {
-            const int r2 = r + TRD_WAVES;
-            const bool two = r2 < m;
-            const double2 vr = vf[r], pr = pf[r];
-            const double2 wr = make_double2(pr.x + al.x * vr.x - al.y * vr.y, pr.y + al.x * vr.y + al.y * vr.x);
-            const double2 vr2 = two ? vf[r2] : make_double2(0, 0), pr2 = two ? pf[r2] : make_double2(0, 0);
-            const double2 wr2 =
-                make_double2(pr2.x + al.x * vr2.x - al.y * vr2.y, pr2.y + al.x * vr2.y + al.y * vr2.x);
-            const int rowb = pidx(k + 1 + r, k + 1), rowb2 = pidx(k + 1 + r2, k + 1);
-            const int rmax = two ? r2 : r;
-            for (int c = lane; c <= rmax; c += 64) {
-                const double2 vc = vf[c];
-                const double2 pc = pf[c];
-                const double2 wc = make_double2(pc.x + al.x * vc.x - al.y * vc.y, pc.y + al.x * vc.y + al.y * vc.x);
-                if (c <= r) {
-                    double2 a = P[rowb + c];
-                    // a -= vr conj(wc) + wr conj(vc)
-                    a.x -= vr.x * wc.x + vr.y * wc.y + wr.x * vc.x + wr.y * vc.y;
-                    a.y -= vr.y * wc.x - vr.x * wc.y + wr.y * vc.x - wr.x * vc.y;
-                    P[rowb + c] = a;
-                    if (c == 0 && r >= 1) {
-                        xn[r - 1] = a;
-                        if (r >= 2) xsn += a.x * a.x + a.y * a.y;
+        // lane columns c0 = lane, c1 = lane + 64 (their v and w in registers);
+        // four rows per wave and pass, all loads issued before the stores
+        const int c0 = lane, c1 = lane + 64;
+        double2 vc0 = make_double2(0, 0), wc0 = make_double2(0, 0), vc1 = make_double2(0, 0), wc1 = make_double2(0, 0);
+        if (c0 < m) {
+            vc0 = vf[c0];
+            const double2 pc = pf[c0];
+            wc0 = make_double2(pc.x + al.x * vc0.x - al.y * vc0.y, pc.y + al.x * vc0.y + al.y * vc0.x);
+        }
+        if (c1 < m) {
+            vc1 = vf[c1];
+            const double2 pc = pf[c1];
+            wc1 = make_double2(pc.x + al.x * vc1.x - al.y * vc1.y, pc.y + al.x * vc1.y + al.y * vc1.x);
+        }
+        for (int r0 = w; r0 < m; r0 += 4 * TRD_WAVES) {
+            double2 a[4][2], vr[4], wr[4];
+            int rb[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int r = r0 + u * TRD_WAVES;
+                const bool ok = r < m;
+                vr[u] = ok ? vf[r] : make_double2(0, 0);
+                const double2 pr = ok ? pf[r] : make_double2(0, 0);
+                wr[u] = make_double2(pr.x + al.x * vr[u].x - al.y * vr[u].y, pr.y + al.x * vr[u].y + al.y * vr[u].x);
+                rb[u] = pidx(k + 1 + r, k + 1);
+                a[u][0] = ok && c0 <= r ? P[rb[u] + c0] : make_double2(0, 0);
+                a[u][1] = ok && c1 <= r ? P[rb[u] + c1] : make_double2(0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int r = r0 + u * TRD_WAVES;
+                if (r >= m) continue;
+                // a -= vr conj(wc) + wr conj(vc)
+                if (c0 <= r) {
+                    double2 x = a[u][0];
+                    x.x -= vr[u].x * wc0.x + vr[u].y * wc0.y + wr[u].x * vc0.x + wr[u].y * vc0.y;
+                    x.y -= vr[u].y * wc0.x - vr[u].x * wc0.y + wr[u].y * vc0.x - wr[u].x * vc0.y;
+                    P[rb[u] + c0] = x;
+                    if (c0 == 0 && r >= 1) {
+                        xn[r - 1] = x;
+                        if (r >= 2) xsn += x.x * x.x + x.y * x.y;
                     }
                 }
-                if (two) {
-                    double2 a = P[rowb2 + c];
-                    a.x -= vr2.x * wc.x + vr2.y * wc.y + wr2.x * vc.x + wr2.y * vc.y;
-                    a.y -= vr2.y * wc.x - vr2.x * wc.y + wr2.y * vc.x - wr2.x * vc.y;
-                    P[rowb2 + c] = a;
-                    if (c == 0) {
-                        xn[r2 - 1] = a;
-                        if (r2 >= 2) xsn += a.x * a.x + a.y * a.y;
-                    }
+                if (c1 <= r) {
+                    double2 x = a[u][1];
+                    x.x -= vr[u].x * wc1.x + vr[u].y * wc1.y + wr[u].x * vc1.x + wr[u].y * vc1.y;
+                    x.y -= vr[u].y * wc1.x - vr[u].x * wc1.y + wr[u].y * vc1.x - wr[u].x * vc1.y;
+                    P[rb[u] + c1] = x;
                 }
             }
         }
