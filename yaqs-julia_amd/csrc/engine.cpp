@@ -975,20 +975,33 @@ void Engine<T>::expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, Jo
         YQ_HIP(hipMemcpyAsync(h_pinned + 1 + (j & 1), nactive.p, sizeof(int), hipMemcpyDeviceToHost, s));
         YQ_HIP(hipEventRecord(lz_ev[j & 1], s));
     };
-    // Iteration j+1 is queued before the host waits for the count of
-    // iteration j, so the device never idles on that round trip; when
-    // iteration j left no trajectory active, j+1 ran fully masked.
+    // With `ahead`, iteration j+1 is queued before the host waits for the
+    // count of iteration j, so the device does not idle on that round trip;
+    // when iteration j left no trajectory active, j+1 ran fully masked.
+    // YAQS_LZ_AHEAD=1 queues iteration j+1 before waiting for the count of
+    // iteration j (measured neutral at C2 and slightly slower on the serving
+    // stream, where the masked extra iterations compete for the device, so
+    // off by default)
+    static const bool ahead = [] {
+        const char *e = std::getenv("YAQS_LZ_AHEAD");
+        return e && e[0] == '1';
+    }();
     int nact = nb;
     iteration(0, nact);
     res.matvecs += nact;
     res.flop += fl * nact;
     for (int j = 0; j < kdim; ++j) {
         const bool more = j + 1 < kdim;
-        if (more) iteration(j + 1, -2);  // active count resolved below
+        if (more && ahead) iteration(j + 1, -2);  // active count resolved below
         YQ_HIP(hipEventSynchronize(lz_ev[j & 1]));
         nact = h_pinned[1 + (j & 1)];
         if (!more) break;
-        if (g_prof) g_prof->resolve(nact);
+        if (!ahead) {
+            if (nact == 0) break;
+            iteration(j + 1, nact);
+        } else if (g_prof) {
+            g_prof->resolve(nact);
+        }
         res.matvecs += nact;
         res.flop += fl * nact;
         if (nact == 0) break;
