@@ -800,50 +800,29 @@ __global__ __launch_bounds__(64) void stein_kernel(char *__restrict__ Wk, int64_
                 x[i * 64] = 0.5 + (double)((h >> 8) & 0xffff) * (1.0 / 65536.0);
             }
             for (int it = 0; it < 3; ++it) {
-                // forward: apply P and L (r0 carries row i); the factors of
-                // SB rows are loaded together ahead of the dependent chain
-                constexpr int SB = 8;
+                // forward: apply P and L (r0 carries row i)
                 double r0 = x[0];
-                for (int i0 = 0; i0 + 1 < n; i0 += SB) {
-                    double lb[SB];
-#pragma unroll
-                    for (int u = 0; u < SB; ++u) lb[u] = i0 + u + 1 < n ? ml[(i0 + u) * 64 + c] : 0.0;
-#pragma unroll
-                    for (int u = 0; u < SB; ++u) {
-                        const int i = i0 + u;
-                        if (i + 1 >= n) break;
-                        double r1 = x[(i + 1) * 64];
-                        if ((piv[i >> 6] >> (i & 63)) & 1ull) {
-                            const double tmp = r0;
-                            r0 = r1;
-                            r1 = tmp;
-                        }
-                        x[i * 64] = r0;
-                        r0 = r1 - lb[u] * r0;
+#pragma unroll 8
+                for (int i = 0; i + 1 < n; ++i) {
+                    const double l = ml[i * 64 + c];
+                    double r1 = x[(i + 1) * 64];
+                    if ((piv[i >> 6] >> (i & 63)) & 1ull) {
+                        const double tmp = r0;
+                        r0 = r1;
+                        r1 = tmp;
                     }
+                    x[i * 64] = r0;
+                    r0 = r1 - l * r0;
                 }
                 x[(n - 1) * 64] = r0;
                 // back substitution with U
                 double y2 = 0.0, y1 = 0.0;
-                for (int i0 = n - 1; i0 >= 0; i0 -= SB) {
-                    double bb[SB], cb[SB], ib[SB];
-#pragma unroll
-                    for (int u = 0; u < SB; ++u) {
-                        const int i = i0 - u;
-                        const bool ok = i >= 0;
-                        bb[u] = ok ? ub[i * 64 + c] : 0.0;
-                        cb[u] = ok ? uc[i * 64 + c] : 0.0;
-                        ib[u] = ok ? ui[i * 64 + c] : 0.0;
-                    }
-#pragma unroll
-                    for (int u = 0; u < SB; ++u) {
-                        const int i = i0 - u;
-                        if (i < 0) break;
-                        const double y = (x[i * 64] - bb[u] * y1 - cb[u] * y2) * ib[u];
-                        x[i * 64] = y;
-                        y2 = y1;
-                        y1 = y;
-                    }
+#pragma unroll 8
+                for (int i = n - 1; i >= 0; --i) {
+                    const double y = (x[i * 64] - ub[i * 64 + c] * y1 - uc[i * 64 + c] * y2) * ui[i * 64 + c];
+                    x[i * 64] = y;
+                    y2 = y1;
+                    y1 = y;
                 }
                 // against the earlier vectors of the cluster
                 for (int p = c; p < j; ++p) {
@@ -1213,54 +1192,26 @@ __global__ __launch_bounds__(WIDE_MAX_K) void stein_wide_kernel(char *__restrict
                 x[(int64_t)i * kw] = 0.5 + (double)((h >> 8) & 0xffff) * (1.0 / 65536.0);
             }
             for (int it = 0; it < 3; ++it) {
-                // factors (and x) of SB rows loaded together ahead of the chain
-                constexpr int SB = 8;
                 double r0 = x[0];
-                for (int i0 = 0; i0 + 1 < n; i0 += SB) {
-                    double lb[SB], xb[SB];
-#pragma unroll
-                    for (int u = 0; u < SB; ++u) {
-                        const bool ok = i0 + u + 1 < n;
-                        lb[u] = ok ? ml[(int64_t)(i0 + u) * kw + c] : 0.0;
-                        xb[u] = ok ? x[(int64_t)(i0 + u + 1) * kw] : 0.0;
+                for (int i = 0; i + 1 < n; ++i) {
+                    const double l = ml[(int64_t)i * kw + c];
+                    double r1 = x[(int64_t)(i + 1) * kw];
+                    if ((piv[i >> 6] >> (i & 63)) & 1ull) {
+                        const double tmp = r0;
+                        r0 = r1;
+                        r1 = tmp;
                     }
-#pragma unroll
-                    for (int u = 0; u < SB; ++u) {
-                        const int i = i0 + u;
-                        if (i + 1 >= n) break;
-                        double r1 = xb[u];
-                        if ((piv[i >> 6] >> (i & 63)) & 1ull) {
-                            const double tmp = r0;
-                            r0 = r1;
-                            r1 = tmp;
-                        }
-                        x[(int64_t)i * kw] = r0;
-                        r0 = r1 - lb[u] * r0;
-                    }
+                    x[(int64_t)i * kw] = r0;
+                    r0 = r1 - l * r0;
                 }
                 x[(int64_t)(n - 1) * kw] = r0;
                 double y2 = 0.0, y1 = 0.0;
-                for (int i0 = n - 1; i0 >= 0; i0 -= SB) {
-                    double bb[SB], cb[SB], ib[SB], xb[SB];
-#pragma unroll
-                    for (int u = 0; u < SB; ++u) {
-                        const int i = i0 - u;
-                        const bool ok = i >= 0;
-                        const int64_t o = (int64_t)(ok ? i : 0) * kw + c;
-                        bb[u] = ok ? ub[o] : 0.0;
-                        cb[u] = ok ? uc[o] : 0.0;
-                        ib[u] = ok ? ui[o] : 0.0;
-                        xb[u] = ok ? x[(int64_t)(ok ? i : 0) * kw] : 0.0;
-                    }
-#pragma unroll
-                    for (int u = 0; u < SB; ++u) {
-                        const int i = i0 - u;
-                        if (i < 0) break;
-                        const double y = (xb[u] - bb[u] * y1 - cb[u] * y2) * ib[u];
-                        x[(int64_t)i * kw] = y;
-                        y2 = y1;
-                        y1 = y;
-                    }
+                for (int i = n - 1; i >= 0; --i) {
+                    const int64_t o = (int64_t)i * kw + c;
+                    const double y = (x[(int64_t)i * kw] - ub[o] * y1 - uc[o] * y2) * ui[o];
+                    x[(int64_t)i * kw] = y;
+                    y2 = y1;
+                    y1 = y;
                 }
                 for (int p = c; p < j; ++p) {
                     double sp = 0;
