@@ -137,6 +137,26 @@ def c3_main(Y, args):
                    "max_bond_reached": int(bonds.max()), "host_wall_s": wall},
         "roofline": None, "cpu_baseline": None,
     }
+    if not args.no_cpu:
+        # bounded sample: the oracle follows the reference's full-chain noise
+        # sweeps in FP64 and costs about a minute per saturated layer and
+        # trajectory, so it runs the first cpu_layers layers of the same
+        # circuit (bond dimension still growing: an upper bound of its rate)
+        if ROOT not in sys.path:
+            sys.path.insert(0, ROOT)
+        from oracle import oracle as O
+        nthreads = args.cpu_traj or min(16, os.cpu_count() or 1)
+        lay = args.cpu_layers
+        hc, _ = c3_job(Y, nthreads, layers=lay, precision="fp64")
+        print(f"[bench] CPU oracle, {nthreads} trajectories on {nthreads} threads, layers 1..{lay}", flush=True)
+        t0 = time.perf_counter()
+        O.run_digital(hc, nthreads=nthreads)
+        t = time.perf_counter() - t0
+        cv = nthreads * L * lay / t
+        out["cpu_baseline"] = {"value": cv, "unit": "traj-site-layers/s", "cores": nthreads, "kind": "port",
+                               "sample": f"FP64 oracle, {nthreads} trajectories (one per thread), the first {lay} "
+                                         "layers of the same circuit"}
+        print(f"[bench] CPU oracle: {cv:.4g} traj-site-layers/s on {nthreads} threads", flush=True)
     print(json.dumps(out), flush=True)
 
 
@@ -238,6 +258,7 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c2", "c3"],
                     help="c2: the analog headline line; c3: the digital brickwork line")
     ap.add_argument("--layers", type=int, default=40, help="c3: circuit layers")
+    ap.add_argument("--cpu-layers", type=int, default=7, help="c3: layers of the bounded oracle sample")
     ap.add_argument("--trace-summary", metavar="CSV", help="summarise a rocprofv3 kernel trace and exit")
     ap.add_argument("--trace-last", type=int, default=0, help="dispatches of the timed window (from the bench line)")
     args = ap.parse_args()

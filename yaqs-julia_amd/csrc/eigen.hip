@@ -24,7 +24,8 @@
 // 1024-thread workgroup per matrix), eigenvalues by the same bisection,
 // inverse iteration with the vectors in global memory (stein_wide_kernel),
 // the Gram-Schmidt pass as its own kernel (cgs_wide_kernel) and the
-// back-transform with 32 row residues per 32 columns.
+// back-transform with 16 row residues per 64 columns (n <= 256) or 32 per
+// 32 columns.
 // Matrices whose inverse iteration breaks down (zero or non-finite vector),
 // and every matrix outside both paths, go through heev_lds_kernel: one
 // workgroup per matrix doing Householder + implicit QL + back-transform
@@ -1337,7 +1338,7 @@ bool heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, d
         hipLaunchKernelGGL(stein_kernel, dim3(B), dim3(64), (size_t)(n * 64 + 2 * n) * sizeof(double), s, W, sW, n,
                            kout, (const double *)evals, sE, (const int *)keep, mask, fallback);
         YQ_HIP(hipGetLastError());
-        hipLaunchKernelGGL((ormtr_kernel<LDS_MAX_N, 4, 64>), dim3(B, 1), dim3(256), 0, s, W, sW, lay.V, lay.tau, lay.X, 64,
+        hipLaunchKernelGGL((ormtr_kernel<LDS_MAX_N, 8, 64>), dim3(B, 1), dim3(512), 0, s, W, sW, lay.V, lay.tau, lay.X, 64,
                            n, kout, U, sU, mask, (const int *)fallback);
         YQ_HIP(hipGetLastError());
     } else if (wide) {
@@ -1353,8 +1354,12 @@ bool heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, d
         hipLaunchKernelGGL(cgs_wide_kernel, dim3(B), dim3(kw), 0, s, W, sW, n, kw, (const int *)keep, mask,
                            (const int *)fallback);
         YQ_HIP(hipGetLastError());
-        hipLaunchKernelGGL((ormtr_kernel<WIDE_MAX_N, 16, 32>), dim3(B, kw / 32), dim3(1024), 0, s, W, sW, lay.A, lay.tau,
-                           lay.X, kw, n, kout, U, sU, mask, (const int *)fallback);
+        if (n <= 256)  // 16 row residues per 64 columns
+            hipLaunchKernelGGL((ormtr_kernel<256, 16, 64>), dim3(B, kw / 64), dim3(1024), 0, s, W, sW, lay.A,
+                               lay.tau, lay.X, kw, n, kout, U, sU, mask, (const int *)fallback);
+        else
+            hipLaunchKernelGGL((ormtr_kernel<WIDE_MAX_N, 16, 32>), dim3(B, kw / 32), dim3(1024), 0, s, W, sW, lay.A,
+                               lay.tau, lay.X, kw, n, kout, U, sU, mask, (const int *)fallback);
         YQ_HIP(hipGetLastError());
     } else {
         heev_fixup(G, sG, U, sU, Z, sZ, Pg, sP, nullptr, n, ns, evals, perm, sE, keep, rule, thr, max_bond, mask, B, s);
