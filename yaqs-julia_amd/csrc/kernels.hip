@@ -241,12 +241,15 @@ __global__ __launch_bounds__(256) void lz_init_kernel(const cplx<T> *x, int64_t 
 // w = V_{j+1} (the raw matvec output).
 constexpr int LZ_CHUNK = 2048;  // elements per block (256 threads x 8)
 
+// Each wavefront reduces its partial of every q with DPP and parks it in
+// LDS; one barrier, then lanes q <= j add the four wavefront partials.
 template <typename T>
 __global__ __launch_bounds__(256) void lz_dots_kernel(const cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j,
                                                       c64 *part, int nchunk, int kslots, const int *active) {
-    __shared__ double sh[16];
+    __shared__ double2 sh[4][64];
     const int t = blockIdx.y, c = blockIdx.x;
     if (!active[t]) return;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const cplx<T> *Vt = V + t * sV;
     const cplx<T> *w = Vt + (int64_t)(j + 1) * nmax;
     double2 wv[8];
@@ -264,13 +267,17 @@ __global__ __launch_bounds__(256) void lz_dots_kernel(const cplx<T> *V, int64_t 
             const int64_t e = base + threadIdx.x + 256 * r;
             if (e < n) acc = cadd(acc, cmulc(ld2(vq[e]), wv[r]));
         }
-        acc = block_sum2(acc, sh);
-        if (threadIdx.x == 0) {
-            c64 o;
-            o.re = acc.x;
-            o.im = acc.y;
-            part[((int64_t)t * kslots + q) * nchunk + c] = o;
-        }
+        acc.x = wave_sum(acc.x);
+        acc.y = wave_sum(acc.y);
+        if (lane == 0) sh[wid][q] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x <= j) {
+        const int q = threadIdx.x;
+        c64 o;
+        o.re = sh[0][q].x + sh[1][q].x + sh[2][q].x + sh[3][q].x;
+        o.im = sh[0][q].y + sh[1][q].y + sh[2][q].y + sh[3][q].y;
+        part[((int64_t)t * kslots + q) * nchunk + c] = o;
     }
 }
 
