@@ -20,8 +20,9 @@
  *
  * Ownership: inputs are caller-owned and deep-copied at submit; outputs are
  * caller-owned and written only inside yaqs_wait / yaqs_run_*.
- * Threading: a context owns two device worker threads (one HIP stream
- * each) that run admitted trajectory waves concurrently; every call is
+ * Threading: a context owns four device worker threads (one HIP stream
+ * each; YAQS_WORKERS=1..8 overrides) that run admitted trajectory waves
+ * concurrently; every call is
  * thread-safe.  Errors: every call returns a yaqs_status; the message of the
  * last failure on the calling thread is yaqs_last_error().
  */
