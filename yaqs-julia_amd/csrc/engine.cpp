@@ -446,7 +446,7 @@ class Engine {
     DevBuf cgsS, cgsF;  // CGS2 projection scratch and Frobenius norms
     int64_t slotZ = 0, slotP = 0, slotH = 0, slotCgs = 0;
     DevBuf Ebuf0, Ebuf1, Xbuf, rho1, rho2;
-    DevBuf beta0, galpha, lscale, alpha, beta, coef, nit, active, part, partn, nactive;
+    DevBuf beta0, galpha, alpha, beta, coef, nit, active, part, partn, nactive;
     DevBuf norm2v, fac, jstate, chosen, opidx, mask, rngk, wbuf, count;
     DevBuf diss_half, diss_full, diss_one;  // per-site 2x2 dissipation operators
     DevBuf invL_half, invL_full;            // inverse of the last site's operator
@@ -627,7 +627,6 @@ void Engine<T>::setup() {
     nchunk_max = (nmax + LZ_CHUNK_ELEMS - 1) / LZ_CHUNK_ELEMS;
     beta0.alloc(B * sizeof(double));
     galpha.alloc(B * sizeof(double));
-    lscale.alloc(B * sizeof(double));
     alpha.alloc((size_t)B * kdim * sizeof(double));
     beta.alloc((size_t)B * (kdim + 1) * sizeof(double));
     coef.alloc((size_t)B * kdim * sizeof(c64));
@@ -635,7 +634,7 @@ void Engine<T>::setup() {
     active.alloc(B * sizeof(int));
     part.alloc((size_t)B * (kdim + 1) * nchunk_max * sizeof(c64));
     partn.alloc((size_t)B * nchunk_max * sizeof(double));
-    nactive.alloc(sizeof(int));
+    nactive.alloc(kdim * sizeof(int));  // one counter per Lanczos iteration
     norm2v.alloc(B * sizeof(double));
     fac.alloc(B * sizeof(double));
     jstate.alloc(B * sizeof(int));
@@ -956,7 +955,8 @@ void Engine<T>::expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, Jo
     const int64_t n = (int64_t)cl * d * cr;
     const double fl = two ? 8.0 * (4.0 * Dm[i] * cl * cr * (double)(cl + cr) + 16.0 * Dm[i] * Dm[i] * (double)cl * cr)
                           : 8.0 * (2.0 * Dm[i] * cl * cr * (double)(cl + cr) + 4.0 * Dm[i] * Dm[i] * (double)cl * cr);
-    lz_init<T>(x, sx, V.as<CT>(), sV(), n, beta0.as<double>(), active.as<int>(), nit.as<int>(), nb, s);
+    lz_init<T>(x, sx, V.as<CT>(), sV(), n, beta0.as<double>(), active.as<int>(), nit.as<int>(), nactive.as<int>(),
+               kdim, nb, s);
     // One Lanczos iteration for the still-active trajectories (every kernel is
     // masked by the device-side active flags); its active count is copied to
     // the host slot of its parity.
@@ -967,12 +967,10 @@ void Engine<T>::expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, Jo
         lz_dots<T>(V.as<CT>(), sV(), nmax, n, j, part.as<c64>(), kdim + 1, active.as<int>(), nb, s);
         lz_update<T>(V.as<CT>(), sV(), nmax, n, j, part.as<c64>(), kdim + 1, partn.as<double>(), galpha.as<double>(),
                      active.as<int>(), nb, s);
-        fill_int(nactive.as<int>(), 0, 1, s);
-        lz_finalize(j, kdim, dt, J.ktol, partn.as<double>(), n, galpha.as<double>(), alpha.as<double>(),
-                    beta.as<double>(), beta0.as<double>(), coef.as<c64>(), nit.as<int>(), active.as<int>(),
-                    lscale.as<double>(), nactive.as<int>(), nb, s);
-        lz_scale<T>(V.as<CT>(), sV(), nmax, n, j + 1, lscale.as<double>(), active.as<int>(), nb, s);
-        YQ_HIP(hipMemcpyAsync(h_pinned + 1 + (j & 1), nactive.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        lz_finalize<T>(V.as<CT>(), sV(), nmax, n, j, kdim, dt, J.ktol, partn.as<double>(), galpha.as<double>(),
+                       alpha.as<double>(), beta.as<double>(), beta0.as<double>(), coef.as<c64>(), nit.as<int>(),
+                       active.as<int>(), nactive.as<int>(), nb, s);
+        YQ_HIP(hipMemcpyAsync(h_pinned + 1 + (j & 1), nactive.as<int>() + j, sizeof(int), hipMemcpyDeviceToHost, s));
         YQ_HIP(hipEventRecord(lz_ev[j & 1], s));
     };
     // With `ahead`, iteration j+1 is queued before the host waits for the

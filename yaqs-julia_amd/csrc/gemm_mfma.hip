@@ -291,27 +291,41 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
             wval[i] = w.val[i];
         }
         __syncthreads();
-        // qb lanes per (p, row of W): q fastest for coalesced stores
+        // qb lanes per (group of PG rows p, row of W): q fastest for coalesced
+        // stores; each W entry read from LDS serves PG independent rows
+        constexpr int PG = 4;
         const int qb = w.qb, qq = tid % qb, slot = tid / qb, nslot = 256 / qb;
         const int q = q0 + qq;
         cplx<T> *Yb = reinterpret_cast<cplx<T> *>(w.Y) + t * w.sY;
         const cplx<T> *T1f = &T1s[0][0];
-        for (int pj = slot; pj < w.pb * J; pj += nslot) {
-            const int pp = pj / J, jj = pj - pp * J;
-            const int p = p0 + pp;
-            if (p >= w.P || q >= w.Q) continue;
-            const cplx<T> *base = T1f + pp * w.Dl * (BN + 1) + qq;
-            double ax = 0, ay = 0;
-            for (int z = wrp[jj]; z < wrp[jj + 1]; ++z) {
-                const c64 wv = wval[z];
-                const cplx<T> x = base[wcol[z]];
-                // same operation order as wstep_kernel
-                ax = fma(wv.re, (double)x.re, ax);
-                ax = fma(-wv.im, (double)x.im, ax);
-                ay = fma(wv.re, (double)x.im, ay);
-                ay = fma(wv.im, (double)x.re, ay);
+        const int ngp = (w.pb + PG - 1) / PG;  // pb = 16: no partial group
+        const int rs = w.Dl * (BN + 1);        // T1 stage stride of p
+        if (q < w.Q) {
+            for (int u = slot; u < ngp * J; u += nslot) {
+                const int pg = u / J, jj = u - pg * J;
+                const cplx<T> *base = T1f + pg * PG * rs + qq;
+                double ax[PG], ay[PG];
+#pragma unroll
+                for (int r = 0; r < PG; ++r) ax[r] = ay[r] = 0;
+                for (int z = wrp[jj]; z < wrp[jj + 1]; ++z) {
+                    const c64 wv = wval[z];
+                    const int off = wcol[z];
+#pragma unroll
+                    for (int r = 0; r < PG; ++r) {
+                        const cplx<T> x = base[r * rs + off];
+                        // same operation order as wstep_kernel
+                        ax[r] = fma(wv.re, (double)x.re, ax[r]);
+                        ax[r] = fma(-wv.im, (double)x.im, ax[r]);
+                        ay[r] = fma(wv.re, (double)x.im, ay[r]);
+                        ay[r] = fma(wv.im, (double)x.re, ay[r]);
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < PG; ++r) {
+                    const int p = p0 + pg * PG + r;
+                    if (p < w.P) Yb[((int64_t)p * J + jj) * w.Q + q] = cmake<T>((T)ax[r], (T)ay[r]);
+                }
             }
-            Yb[((int64_t)p * J + jj) * w.Q + q] = cmake<T>((T)ax, (T)ay);
         }
         return;
     }

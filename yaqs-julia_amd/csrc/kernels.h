@@ -34,19 +34,19 @@ void wstep(const cplx<T> *X, cplx<T> *Y, int P, int I, int J, int Q, int64_t sX,
 
 template <typename T>
 void lz_init(const cplx<T> *x, int64_t sx, cplx<T> *V, int64_t sV, int64_t n, double *beta0, int *active, int *nit,
-             int B, hipStream_t s);
+             int *nactive, int nslots, int B, hipStream_t s);
 template <typename T>
 void lz_dots(const cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, c64 *part, int kslots, const int *active,
              int B, hipStream_t s);
 template <typename T>
 void lz_update(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, const c64 *part, int kslots, double *partn,
                double *galpha, const int *active, int B, hipStream_t s);
-void lz_finalize(int j, int kmax, double dt, double tol, const double *partn, int64_t n, const double *galpha,
-                 double *alpha, double *beta, const double *beta0, c64 *coef, int *nit, int *active, double *scale,
-                 int *nactive, int B, hipStream_t s);
+// β = ‖w‖, V_{j+1} = w/β and the stopping rule; trajectories that go on are
+// counted in nactive[j] (cleared by lz_init for every slot).
 template <typename T>
-void lz_scale(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int q, const double *scale, const int *active, int B,
-              hipStream_t s);
+void lz_finalize(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, int kmax, double dt, double tol,
+                 const double *partn, const double *galpha, double *alpha, double *beta, const double *beta0, c64 *coef,
+                 int *nit, int *active, int *nactive, int B, hipStream_t s);
 template <typename T>
 void lz_combine(cplx<T> *x, int64_t sx, const cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, const double *beta0,
                 const c64 *coef, const int *nit, int kmax, const int *mask, int B, hipStream_t s);

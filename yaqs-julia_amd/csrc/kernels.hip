@@ -211,12 +211,15 @@ template void wstep<double>(const c64 *, c64 *, int, int, int, int, int64_t, int
                             const c64 *, const int *, int, hipStream_t);
 
 // ------------------------------------------------------------- Lanczos -----
-// lz_init: β0 = ‖x‖, V0 = x/β0, active = β0 > 0.
+// lz_init: β0 = ‖x‖, V0 = x/β0, active = β0 > 0; block 0 clears the
+// per-iteration active counters nactive[0 .. nslots).
 template <typename T>
 __global__ __launch_bounds__(256) void lz_init_kernel(const cplx<T> *x, int64_t sx, cplx<T> *V, int64_t sV, int64_t n,
-                                                      double *beta0, int *active, int *nit) {
+                                                      double *beta0, int *active, int *nit, int *nactive, int nslots) {
     __shared__ double sh[16];
     const int t = blockIdx.x;
+    if (t == 0)
+        for (int i = threadIdx.x; i < nslots; i += blockDim.x) nactive[i] = 0;
     const cplx<T> *xt = x + t * sx;
     cplx<T> *v = V + t * sV;
     double s = 0;
@@ -345,29 +348,46 @@ __device__ double2 tridiag_exp_lane(const double *alpha, const double *beta, int
             const double f = h / m;
             term = make_double2(f * uy, -f * ux);
             acc = cadd(acc, term);
-            double mag = fabs(term.x) + fabs(term.y);
-            mag = wave_max(mag);
-            if (mag < 1e-20) break;
+            // stop once no lane's term reaches 1e-20 (one compare and a ballot)
+            if (__ballot(fabs(term.x) + fabs(term.y) >= 1e-20) == 0) break;
         }
         v = acc;
     }
     return lane < k ? v : make_double2(0, 0);
 }
 
-// lz_finalize: one wavefront per trajectory.  β = ‖w‖, c = exp(-i dt T) e0,
-// stopping rule β0·β·|c_j| <= tol (shared with the oracle), else prepare
-// V_{j+1} = w/β.
-__global__ __launch_bounds__(256) void lz_finalize_kernel(int j, int kmax, double dt, double tol, const double *partn,
+// lz_finalize: one block per chunk and trajectory.  β = ‖w‖ from the chunk
+// partials and V_{j+1} = w/β on the block's chunk; the block of chunk 0
+// also computes, on its first wavefront, c = exp(-i dt T) e0 and the
+// stopping rule β0·β·|c_j| <= tol (shared with the oracle).  A trajectory
+// that stops keeps nit = j + 1 and leaves the active set; V_{j+1} is then
+// never read, so it does not matter whether the other blocks of that
+// trajectory saw it active and scaled their chunk.  Trajectories that go on
+// are counted in nactive[j].
+template <typename T>
+__global__ __launch_bounds__(256) void lz_finalize_kernel(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j,
+                                                          int kmax, double dt, double tol, const double *partn,
                                                           int nchunk, const double *galpha, double *alpha, double *beta,
                                                           const double *beta0, c64 *coef, int *nit, int *active,
-                                                          double *scale, int *nactive, int B) {
+                                                          int *nactive) {
+    const int t = blockIdx.y, c = blockIdx.x;
+    if (!active[t]) return;
     const int lane = threadIdx.x & 63;
-    const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (t >= B || !active[t]) return;
     double s = 0;
     for (int i = lane; i < nchunk; i += 64) s += partn[(int64_t)t * nchunk + i];
     s = wave_sum(s);
     const double b = sqrt(s);
+    if (b > 0.0) {
+        const double f = 1.0 / b;
+        cplx<T> *v = V + t * sV + (int64_t)(j + 1) * nmax;
+        const int64_t base = (int64_t)c * LZ_CHUNK;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int64_t e = base + threadIdx.x + 256 * r;
+            if (e < n) st2(v[e], cscale(ld2(v[e]), f));
+        }
+    }
+    if (c != 0 || threadIdx.x >= 64) return;
     double *al = alpha + (int64_t)t * kmax;
     double *be = beta + (int64_t)t * (kmax + 1);
     const double aj = galpha[t];
@@ -389,23 +409,7 @@ __global__ __launch_bounds__(256) void lz_finalize_kernel(int j, int kmax, doubl
         }
     } else if (lane == 0) {
         be[j + 1] = b;
-        scale[t] = 1.0 / b;
-        atomicAdd(nactive, 1);
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void lz_scale_kernel(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int q,
-                                                       const double *scale, const int *active) {
-    const int t = blockIdx.y;
-    if (!active[t]) return;
-    cplx<T> *v = V + t * sV + (int64_t)q * nmax;
-    const double f = scale[t];
-    const int64_t base = (int64_t)blockIdx.x * LZ_CHUNK;
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        const int64_t e = base + threadIdx.x + 256 * r;
-        if (e < n) st2(v[e], cscale(ld2(v[e]), f));
+        atomicAdd(nactive + j, 1);
     }
 }
 
@@ -434,8 +438,9 @@ __global__ __launch_bounds__(256) void lz_combine_kernel(cplx<T> *x, int64_t sx,
 
 template <typename T>
 void lz_init(const cplx<T> *x, int64_t sx, cplx<T> *V, int64_t sV, int64_t n, double *beta0, int *active, int *nit,
-             int B, hipStream_t s) {
-    hipLaunchKernelGGL(lz_init_kernel<T>, dim3(B), dim3(256), 0, s, x, sx, V, sV, n, beta0, active, nit);
+             int *nactive, int nslots, int B, hipStream_t s) {
+    hipLaunchKernelGGL(lz_init_kernel<T>, dim3(B), dim3(256), 0, s, x, sx, V, sV, n, beta0, active, nit, nactive,
+                       nslots);
     YQ_HIP(hipGetLastError());
 }
 template <typename T>
@@ -454,19 +459,13 @@ void lz_update(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, const c64
                        partn, galpha, active);
     YQ_HIP(hipGetLastError());
 }
-void lz_finalize(int j, int kmax, double dt, double tol, const double *partn, int64_t n, const double *galpha,
-                 double *alpha, double *beta, const double *beta0, c64 *coef, int *nit, int *active, double *scale,
-                 int *nactive, int B, hipStream_t s) {
-    const int nchunk = (int)((n + LZ_CHUNK - 1) / LZ_CHUNK);
-    hipLaunchKernelGGL(lz_finalize_kernel, dim3((B + 3) / 4), dim3(256), 0, s, j, kmax, dt, tol, partn, nchunk, galpha,
-                       alpha, beta, beta0, coef, nit, active, scale, nactive, B);
-    YQ_HIP(hipGetLastError());
-}
 template <typename T>
-void lz_scale(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int q, const double *scale, const int *active, int B,
-              hipStream_t s) {
+void lz_finalize(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, int kmax, double dt, double tol,
+                 const double *partn, const double *galpha, double *alpha, double *beta, const double *beta0, c64 *coef,
+                 int *nit, int *active, int *nactive, int B, hipStream_t s) {
     const int nchunk = (int)((n + LZ_CHUNK - 1) / LZ_CHUNK);
-    hipLaunchKernelGGL(lz_scale_kernel<T>, dim3(nchunk, B), dim3(256), 0, s, V, sV, nmax, n, q, scale, active);
+    hipLaunchKernelGGL(lz_finalize_kernel<T>, dim3(nchunk, B), dim3(256), 0, s, V, sV, nmax, n, j, kmax, dt, tol, partn,
+                       nchunk, galpha, alpha, beta, beta0, coef, nit, active, nactive);
     YQ_HIP(hipGetLastError());
 }
 template <typename T>
@@ -479,14 +478,15 @@ void lz_combine(cplx<T> *x, int64_t sx, const cplx<T> *V, int64_t sV, int64_t nm
 }
 
 #define YQ_LZ_INST(T)                                                                                              \
-    template void lz_init<T>(const cplx<T> *, int64_t, cplx<T> *, int64_t, int64_t, double *, int *, int *, int,    \
-                             hipStream_t);                                                                         \
+    template void lz_init<T>(const cplx<T> *, int64_t, cplx<T> *, int64_t, int64_t, double *, int *, int *, int *,  \
+                             int, int, hipStream_t);                                                               \
     template void lz_dots<T>(const cplx<T> *, int64_t, int64_t, int64_t, int, c64 *, int, const int *, int,         \
                              hipStream_t);                                                                         \
     template void lz_update<T>(cplx<T> *, int64_t, int64_t, int64_t, int, const c64 *, int, double *, double *,     \
                                const int *, int, hipStream_t);                                                     \
-    template void lz_scale<T>(cplx<T> *, int64_t, int64_t, int64_t, int, const double *, const int *, int,          \
-                              hipStream_t);                                                                        \
+    template void lz_finalize<T>(cplx<T> *, int64_t, int64_t, int64_t, int, int, double, double, const double *,     \
+                                 const double *, double *, double *, const double *, c64 *, int *, int *, int *, int, \
+                                 hipStream_t);                                                                     \
     template void lz_combine<T>(cplx<T> *, int64_t, const cplx<T> *, int64_t, int64_t, int64_t, const double *,     \
                                 const c64 *, const int *, int, const int *, int, hipStream_t);
 YQ_LZ_INST(float)
