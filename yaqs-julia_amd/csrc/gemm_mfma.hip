@@ -1,6 +1,10 @@
 // Batched complex GEMM on the CDNA4 matrix cores (gfx950).
 //
-// Complex products are split into real MFMAs on LDS-staged tiles:
+// Complex products are split into real MFMAs on LDS-staged tiles.  FP32
+// takes three real products per complex one (the 3M method):
+//   P1 = Re A Re B, P2 = Im A Im B, P3 = (Re A + Im A)(Re B + Im B),
+//   Re C = P1 - P2, Im C = P3 - P1 - P2,
+// FP64 the four of the definition:
 //   Re C += Re A Re B - Im A Im B,  Im C += Re A Im B + Im A Re B.
 // A 256-thread workgroup owns a BM x BN tile of C (BM, BN in {32, 64}, the
 // shape chosen per GEMM shape by the engine's tuner); each of its four
@@ -215,13 +219,17 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
             Bs[buf][1][br_[q]][kb] = ok2 ? (OPB == OP_C ? -rb[q].im : rb[q].im) : (T)0;
         }
     };
-    V accr[IM][JN], acci[IM][JN];
+    // complex-FP32: three real products per complex one (P1 = Re A Re B,
+    // P2 = Im A Im B, P3 = (Re A + Im A)(Re B + Im B); Re C = P1 - P2,
+    // Im C = P3 - P1 - P2); complex-FP64: the four real products
+    constexpr bool G3M = std::is_same<T, float>::value;
+    V accr[IM][JN], acci[IM][JN], accs[IM][JN];
 #pragma unroll
     for (int i = 0; i < IM; ++i)
 #pragma unroll
         for (int j = 0; j < JN; ++j)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) accr[i][j][r] = acci[i][j][r] = (T)0;
+            for (int r = 0; r < 4; ++r) accr[i][j][r] = acci[i][j][r] = accs[i][j][r] = (T)0;
     const int nk = (d.K + BK - 1) / BK;
     load(0);
     store(0);
@@ -244,26 +252,60 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
                 br[j][q] = Bs[buf][0][wn + j * 16 + li][lk * KQ + q];
                 bi[j][q] = Bs[buf][1][wn + j * 16 + li][lk * KQ + q];
             }
-#pragma unroll
-        for (int q = 0; q < KQ; ++q)
-#pragma unroll
-            for (int i = 0; i < IM; ++i)
-#pragma unroll
-                for (int j = 0; j < JN; ++j) {
-                    accr[i][j] = Mfma<T>::mma(ar[i][q], br[j][q], accr[i][j]);
-                    acci[i][j] = Mfma<T>::mma(ar[i][q], bi[j][q], acci[i][j]);
-                }
-#pragma unroll
-        for (int q = 0; q < KQ; ++q)
+        if constexpr (G3M) {
+            T as[IM][KQ], bs[JN][KQ];
 #pragma unroll
             for (int i = 0; i < IM; ++i)
 #pragma unroll
-                for (int j = 0; j < JN; ++j) {
-                    accr[i][j] = Mfma<T>::mma(-ai[i][q], bi[j][q], accr[i][j]);
-                    acci[i][j] = Mfma<T>::mma(ai[i][q], br[j][q], acci[i][j]);
-                }
+                for (int q = 0; q < KQ; ++q) as[i][q] = ar[i][q] + ai[i][q];
+#pragma unroll
+            for (int j = 0; j < JN; ++j)
+#pragma unroll
+                for (int q = 0; q < KQ; ++q) bs[j][q] = br[j][q] + bi[j][q];
+#pragma unroll
+            for (int q = 0; q < KQ; ++q)
+#pragma unroll
+                for (int i = 0; i < IM; ++i)
+#pragma unroll
+                    for (int j = 0; j < JN; ++j) {
+                        accr[i][j] = Mfma<T>::mma(ar[i][q], br[j][q], accr[i][j]);
+                        acci[i][j] = Mfma<T>::mma(ai[i][q], bi[j][q], acci[i][j]);
+                        accs[i][j] = Mfma<T>::mma(as[i][q], bs[j][q], accs[i][j]);
+                    }
+        } else {
+#pragma unroll
+            for (int q = 0; q < KQ; ++q)
+#pragma unroll
+                for (int i = 0; i < IM; ++i)
+#pragma unroll
+                    for (int j = 0; j < JN; ++j) {
+                        accr[i][j] = Mfma<T>::mma(ar[i][q], br[j][q], accr[i][j]);
+                        acci[i][j] = Mfma<T>::mma(ar[i][q], bi[j][q], acci[i][j]);
+                    }
+#pragma unroll
+            for (int q = 0; q < KQ; ++q)
+#pragma unroll
+                for (int i = 0; i < IM; ++i)
+#pragma unroll
+                    for (int j = 0; j < JN; ++j) {
+                        accr[i][j] = Mfma<T>::mma(-ai[i][q], bi[j][q], accr[i][j]);
+                        acci[i][j] = Mfma<T>::mma(ai[i][q], br[j][q], acci[i][j]);
+                    }
+        }
         if (kt + 1 < nk) store(buf ^ 1);
         __syncthreads();
+    }
+    if constexpr (G3M) {
+#pragma unroll
+        for (int i = 0; i < IM; ++i)
+#pragma unroll
+            for (int j = 0; j < JN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const T p1 = accr[i][j][r], p2 = acci[i][j][r];
+                    accr[i][j][r] = p1 - p2;
+                    acci[i][j][r] = (accs[i][j][r] - p1) - p2;
+                }
     }
     if (FW) {
         // stage T1 in LDS (the operand buffers are free after the last barrier)
