@@ -321,13 +321,22 @@ __global__ __launch_bounds__(256) void lz_update_kernel(cplx<T> *V, int64_t sV, 
     }
 }
 
-// exp(-i tau T) e_0 for the symmetric tridiagonal T (k <= 64) by Taylor
-// sub-steps with |h|·‖T‖∞ <= 1/2, one lane per component.
-__device__ double2 tridiag_exp_lane(const double *alpha, const double *beta, int k, double tau, int lane,
-                                    double a_last) {
-    double a = lane < k ? (lane == k - 1 ? a_last : alpha[lane]) : 0.0;
-    double bl = (lane >= 1 && lane < k) ? beta[lane] : 0.0;          // coupling to lane-1
-    double br = (lane + 1 < k) ? beta[lane + 1] : 0.0;              // coupling to lane+1
+// exp(-i tau T) e_0 for the symmetric tridiagonal T (k <= 64), one lane per
+// component (diagonal a, couplings bl to lane - 1 and br to lane + 1, zero
+// on lanes >= k).  Neighbours are read by DPP wavefront shifts (lane 0 /
+// lane 63 read 0).
+//
+// Chebyshev expansion (Jacobi-Anger): with the Gershgorin interval
+// [c - r, c + r] of T, T' = (T - c) / r and x = tau r,
+//   exp(-i tau T) e0 = e^{-i tau c} sum_k eps_k (-i)^k J_k(x) T_k(T') e0,
+// eps_0 = 1, eps_k = 2 (k >= 1).  T_k(T') e0 comes from the three-term
+// recurrence on real vectors (|T_k(T')| <= 1 on the spectrum), the Bessel
+// values J_k(x) from Miller's backward recurrence normalised by
+// J_0 + 2 sum J_2k = 1, started where the bound |J_k(x)| <= (|x|/2)^k / k!
+// is below 1e-30; the series stops where the bound is below 1e-17.  When
+// that start would pass lane 60 (|x| above about 25) the Taylor sub-steps
+// below run instead.
+__device__ double2 tridiag_exp_taylor(double a, double bl, double br, int k, double tau, int lane) {
     double rown = fabs(a) + fabs(bl) + fabs(br);
     rown = wave_max(rown);
     int s = (int)ceil(fabs(tau) * rown / 0.5);
@@ -338,7 +347,6 @@ __device__ double2 tridiag_exp_lane(const double *alpha, const double *beta, int
         double2 term = v, acc = v;
         for (int m = 1; m <= 40; ++m) {
             // term <- (-i h T) term / m
-            // neighbours by DPP wavefront shifts (lane 0 / lane 63 read 0)
             const double tlx = dpp_move<0x138>(term.x), tly = dpp_move<0x138>(term.y);   // lane - 1
             const double trx = dpp_move<0x130>(term.x), try_ = dpp_move<0x130>(term.y);  // lane + 1
             double ux = a * term.x + bl * tlx + br * trx;
@@ -356,6 +364,70 @@ __device__ double2 tridiag_exp_lane(const double *alpha, const double *beta, int
     return lane < k ? v : make_double2(0, 0);
 }
 
+__device__ double2 tridiag_exp_lane(const double *alpha, const double *beta, int k, double tau, int lane,
+                                    double a_last) {
+    const bool in = lane < k;
+    const double a = in ? (lane == k - 1 ? a_last : alpha[lane]) : 0.0;
+    const double bl = (lane >= 1 && in) ? beta[lane] : 0.0;  // coupling to lane-1
+    const double br = (lane + 1 < k) ? beta[lane + 1] : 0.0;  // coupling to lane+1
+    const double rad = fabs(bl) + fabs(br);
+    const double lo = wave_min(in ? a - rad : 1e300), hi = wave_max(in ? a + rad : -1e300);
+    const double c = 0.5 * (lo + hi), r = 0.5 * (hi - lo);
+    const double x = tau * r, hx = 0.5 * fabs(x);
+    // number of terms (bound below 1e-17) and Miller start (bound below 1e-30)
+    int K = 0, N = 0;
+    if (x != 0.0) {
+        double bnd = 1.0;
+        for (N = 1; N <= 61; ++N) {
+            bnd *= hx / N;
+            if (K == 0 && bnd < 1e-17) K = N;
+            if (bnd < 1e-30) break;
+        }
+        if (N > 60) return tridiag_exp_taylor(a, bl, br, k, tau, lane);
+    }
+    // lane q keeps J_q(x)
+    double jq = lane == 0 ? 1.0 : 0.0;
+    if (K > 0) {
+        double jp = 0.0, jc = 1e-250, nsum = (N & 1) ? 0.0 : 2.0 * jc;
+        jq = lane == N ? jc : 0.0;
+        const double tx = 2.0 / x;
+        for (int q = N; q >= 1; --q) {
+            const double jm = (tx * q) * jc - jp;  // J_{q-1}
+            jp = jc;
+            jc = jm;
+            if (lane == q - 1) jq = jm;
+            if (((q - 1) & 1) == 0) nsum += q == 1 ? jm : 2.0 * jm;
+        }
+        jq /= nsum;
+    }
+    // T_q(T') e0 on real vectors, accumulated with the coefficients
+    const double ap = in && K > 0 ? (a - c) / r : 0.0;
+    const double blp = K > 0 ? bl / r : 0.0, brp = K > 0 ? br / r : 0.0;
+    double w0 = lane == 0 ? 1.0 : 0.0;
+    double2 acc = make_double2(lane_read(jq, 0) * w0, 0.0);
+    if (K > 0) {
+        double w1 = ap * w0 + blp * dpp_move<0x138>(w0) + brp * dpp_move<0x130>(w0);
+        acc.y -= 2.0 * lane_read(jq, 1) * w1;
+        for (int q = 2; q <= K; ++q) {
+            const double tw = ap * w1 + blp * dpp_move<0x138>(w1) + brp * dpp_move<0x130>(w1);
+            const double w2 = 2.0 * tw - w0;
+            w0 = w1;
+            w1 = w2;
+            const double cq = 2.0 * lane_read(jq, q);
+            switch (q & 3) {
+                case 0: acc.x += cq * w2; break;
+                case 1: acc.y -= cq * w2; break;
+                case 2: acc.x -= cq * w2; break;
+                default: acc.y += cq * w2; break;
+            }
+        }
+    }
+    double sn, cs;
+    sincos(tau * c, &sn, &cs);
+    const double2 v = make_double2(cs * acc.x + sn * acc.y, cs * acc.y - sn * acc.x);
+    return in ? v : make_double2(0, 0);
+}
+
 // lz_finalize: one block per chunk and trajectory.  β = ‖w‖ from the chunk
 // partials and V_{j+1} = w/β on the block's chunk; the block of chunk 0
 // also computes, on its first wavefront, c = exp(-i dt T) e0 and the
@@ -363,14 +435,24 @@ __device__ double2 tridiag_exp_lane(const double *alpha, const double *beta, int
 // that stops keeps nit = j + 1 and leaves the active set; V_{j+1} is then
 // never read, so it does not matter whether the other blocks of that
 // trajectory saw it active and scaled their chunk.  Trajectories that go on
-// are counted in nactive[j].
+// are counted in nactive[j].  The B chunk-0 blocks come first in the 1-D
+// grid, so their stopping rules start with the first dispatch wave and
+// overlap the scaling of the other chunks.
 template <typename T>
 __global__ __launch_bounds__(256) void lz_finalize_kernel(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j,
                                                           int kmax, double dt, double tol, const double *partn,
                                                           int nchunk, const double *galpha, double *alpha, double *beta,
                                                           const double *beta0, c64 *coef, int *nit, int *active,
-                                                          int *nactive) {
-    const int t = blockIdx.y, c = blockIdx.x;
+                                                          int *nactive, int B) {
+    const int64_t L = blockIdx.x;
+    int t, c;
+    if (L < B) {
+        t = (int)L;
+        c = 0;
+    } else {
+        t = (int)((L - B) / (nchunk - 1));
+        c = 1 + (int)((L - B) % (nchunk - 1));
+    }
     if (!active[t]) return;
     const int lane = threadIdx.x & 63;
     double s = 0;
@@ -464,8 +546,8 @@ void lz_finalize(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, int kma
                  const double *partn, const double *galpha, double *alpha, double *beta, const double *beta0, c64 *coef,
                  int *nit, int *active, int *nactive, int B, hipStream_t s) {
     const int nchunk = (int)((n + LZ_CHUNK - 1) / LZ_CHUNK);
-    hipLaunchKernelGGL(lz_finalize_kernel<T>, dim3(nchunk, B), dim3(256), 0, s, V, sV, nmax, n, j, kmax, dt, tol, partn,
-                       nchunk, galpha, alpha, beta, beta0, coef, nit, active, nactive);
+    hipLaunchKernelGGL(lz_finalize_kernel<T>, dim3((unsigned)((int64_t)nchunk * B)), dim3(256), 0, s, V, sV, nmax, n, j,
+                       kmax, dt, tol, partn, nchunk, galpha, alpha, beta, beta0, coef, nit, active, nactive, B);
     YQ_HIP(hipGetLastError());
 }
 template <typename T>
@@ -1148,7 +1230,45 @@ void fill_int(int *p, int v, int n, hipStream_t s) {
     YQ_HIP(hipGetLastError());
 }
 
+// one wavefront: c = exp(-i tau T) e0 of the Lanczos stopping rule, for the
+// test hook below (alpha: k values, beta: k values, beta[0] unused)
+__global__ __launch_bounds__(64) void tridiag_exp_probe_kernel(const double *alpha, const double *beta, int k,
+                                                                double tau, double2 *out) {
+    const int lane = threadIdx.x;
+    const double2 v = tridiag_exp_lane(alpha, beta, k, tau, lane, alpha[k - 1]);
+    if (lane < k) out[lane] = v;
+}
+
 }  // namespace yq
+
+// Test hook (not part of the public header): the device tridiagonal
+// exponential of the Lanczos stopping rule, c = exp(-i tau T) e0 for T with
+// diagonal alpha[0..k) and couplings beta[1..k) (host arrays, 1 <= k <= 64);
+// out: k complex values as interleaved doubles.
+extern "C" int yaqs_debug_tridiag_exp(int k, const double *alpha, const double *beta, double tau, double *out) {
+    using namespace yq;
+    if (k < 1 || k > 64) return 1;
+    try {
+        double *da, *db;
+        double2 *dout;
+        YQ_HIP(hipMalloc(&da, k * sizeof(double)));
+        YQ_HIP(hipMalloc(&db, (k + 1) * sizeof(double)));
+        YQ_HIP(hipMalloc(&dout, k * sizeof(double2)));
+        YQ_HIP(hipMemcpy(da, alpha, k * sizeof(double), hipMemcpyHostToDevice));
+        YQ_HIP(hipMemset(db, 0, (k + 1) * sizeof(double)));
+        YQ_HIP(hipMemcpy(db, beta, k * sizeof(double), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(tridiag_exp_probe_kernel, dim3(1), dim3(64), 0, 0, da, db, k, tau, dout);
+        YQ_HIP(hipGetLastError());
+        YQ_HIP(hipDeviceSynchronize());
+        YQ_HIP(hipMemcpy(out, dout, k * sizeof(double2), hipMemcpyDeviceToHost));
+        (void)hipFree(da);
+        (void)hipFree(db);
+        (void)hipFree(dout);
+        return 0;
+    } catch (const std::exception &) {
+        return 3;
+    }
+}
 
 // Test hook (not part of the public header): CGS2 of `count` matrices of
 // nvec vectors of length len (host arrays of interleaved doubles, one matrix
