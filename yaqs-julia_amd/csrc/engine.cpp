@@ -333,15 +333,18 @@ inline int pow2b(int64_t x) {
 template <typename T>
 void gemmT(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s) {
     // tile shape: every shape gives the same bits, so it is picked by speed
+    // (a launch queued ahead of its active count takes the cached choice
+    // only: a measurement would wait on the stream)
     int tile = 0;
-    if (!d.accumulate && d.opB == OP_N && g_nact != -2 && tune_enabled()) {
+    if (!d.accumulate && d.opB == OP_N && tune_enabled()) {
         const TuneKey k = {0, (int)sizeof(T), d.opA, d.opB, b16(d.M), b16(d.N), b16(d.K), d.inner, pow2b(d.outer), 0};
-        tile = tuned_variant(
-            k, GEMM_TILES,
-            [&](int v) {
-                if (!gemm_mfma<T>(d, A, B, C, s, v)) gemm<T, T, T, T>(d, A, B, C, s);
-            },
-            s);
+        tile = g_nact == -2 ? cached_variant(k)
+                            : tuned_variant(
+                                  k, GEMM_TILES,
+                                  [&](int v) {
+                                      if (!gemm_mfma<T>(d, A, B, C, s, v)) gemm<T, T, T, T>(d, A, B, C, s);
+                                  },
+                                  s);
     }
     GemmProf *p = g_prof;
     const bool prof = p && p->on && !tune_measuring();
@@ -370,15 +373,16 @@ void gemmW(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, int Dl,
                               W.val.as<c64>(), W.nnz, s);
     };
     bool use_fused = gemm_w_covers(d, P, Dl, dph, Dr, Q, W.nnz);
-    if (use_fused && g_nact != -2 && tune_enabled()) {
+    if (use_fused && tune_enabled()) {
         const TuneKey k = {1, (int)sizeof(T), Dl, dph, Dr, b8(P), b8(Q), b16(d.K), W.nnz, pow2b(d.outer)};
-        use_fused = tuned_variant(
-                        k, 2,
-                        [&](int v) {
-                            if (v == 0) (void)fused();
-                            else unfused();
-                        },
-                        s) == 0;
+        use_fused = (g_nact == -2 ? cached_variant(k)
+                                  : tuned_variant(
+                                        k, 2,
+                                        [&](int v) {
+                                            if (v == 0) (void)fused();
+                                            else unfused();
+                                        },
+                                        s)) == 0;
     }
     if (!use_fused) {
         unfused();

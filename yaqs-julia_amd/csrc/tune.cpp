@@ -70,6 +70,16 @@ bool tune_enabled() {
 
 bool tune_measuring() { return g_measuring; }
 
+int cached_variant(const TuneKey &key) {
+    int dev = 0;
+    YQ_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!enabled_locked()) return 0;
+    load_locked();
+    auto it = g_cache.find(std::make_pair(dev, key));
+    return it == g_cache.end() ? 0 : it->second;
+}
+
 int tuned_variant(const TuneKey &key, int n, const std::function<void(int)> &launch, hipStream_t s) {
     if (n <= 1) return 0;
     int dev = 0;

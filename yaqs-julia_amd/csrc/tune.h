@@ -24,6 +24,10 @@ using TuneKey = std::array<int, 10>;
 // measuring when tuning is off (YAQS_TUNE=0 or tune_set_enabled(0)).
 int tuned_variant(const TuneKey &key, int n, const std::function<void(int)> &launch, hipStream_t s);
 
+// The cached choice for key, or 0 when none is cached or tuning is off;
+// never measures (for launches queued ahead of a host decision).
+int cached_variant(const TuneKey &key);
+
 bool tune_enabled();
 // True on the calling thread while a measurement runs (profilers skip it).
 bool tune_measuring();
