@@ -6,6 +6,7 @@
 // eigen-decompositions, transfer matrices and jump probabilities are always
 // computed in double.
 #include <algorithm>
+#include <stdexcept>
 #include <type_traits>
 #include <vector>
 
@@ -242,7 +243,38 @@ __global__ __launch_bounds__(256) void lz_init_kernel(const cplx<T> *x, int64_t 
 
 // lz_dots: part[t][q][c] = Σ_{e in chunk c} conj(V_q[e]) w[e] for q <= j,
 // w = V_{j+1} (the raw matvec output).
-constexpr int LZ_CHUNK = 2048;  // elements per block (256 threads x 8)
+constexpr int LZ_CHUNK = 2048;  // elements per block (256 threads x 4 pairs)
+
+// Krylov vectors are read and written as pairs of consecutive elements
+// (one 16-byte access per lane for complex-FP32): every vector length n is
+// even (2 or 4 physical indices) and every vector starts at an even
+// element (even nmax and slot strides).
+template <typename T>
+struct LzPair;
+template <>
+struct LzPair<float> {
+    static __device__ __forceinline__ void ld(const c32 *p, int64_t e2, double2 &a, double2 &b) {
+        const float4 v = reinterpret_cast<const float4 *>(p)[e2];
+        a = make_double2(v.x, v.y);
+        b = make_double2(v.z, v.w);
+    }
+    static __device__ __forceinline__ void st(c32 *p, int64_t e2, double2 a, double2 b) {
+        reinterpret_cast<float4 *>(p)[e2] = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
+    }
+};
+template <>
+struct LzPair<double> {
+    static __device__ __forceinline__ void ld(const c64 *p, int64_t e2, double2 &a, double2 &b) {
+        const double2 *q = reinterpret_cast<const double2 *>(p) + 2 * e2;
+        a = q[0];
+        b = q[1];
+    }
+    static __device__ __forceinline__ void st(c64 *p, int64_t e2, double2 a, double2 b) {
+        double2 *q = reinterpret_cast<double2 *>(p) + 2 * e2;
+        q[0] = a;
+        q[1] = b;
+    }
+};
 
 // Each wavefront reduces its partial of every q with DPP and parks it in
 // LDS; one barrier, then lanes q <= j add the four wavefront partials.
@@ -255,20 +287,26 @@ __global__ __launch_bounds__(256) void lz_dots_kernel(const cplx<T> *V, int64_t 
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const cplx<T> *Vt = V + t * sV;
     const cplx<T> *w = Vt + (int64_t)(j + 1) * nmax;
-    double2 wv[8];
-    const int64_t base = (int64_t)c * LZ_CHUNK;
+    const int64_t n2 = n >> 1, base = (int64_t)c * (LZ_CHUNK / 2);
+    double2 wa[4], wb[4];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        const int64_t e = base + threadIdx.x + 256 * r;
-        wv[r] = e < n ? ld2(w[e]) : make_double2(0, 0);
+    for (int r = 0; r < 4; ++r) {
+        const int64_t e2 = base + threadIdx.x + 256 * r;
+        wa[r] = wb[r] = make_double2(0, 0);
+        if (e2 < n2) LzPair<T>::ld(w, e2, wa[r], wb[r]);
     }
     for (int q = 0; q <= j; ++q) {
         const cplx<T> *vq = Vt + (int64_t)q * nmax;
         double2 acc = make_double2(0, 0);
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const int64_t e = base + threadIdx.x + 256 * r;
-            if (e < n) acc = cadd(acc, cmulc(ld2(vq[e]), wv[r]));
+        for (int r = 0; r < 4; ++r) {
+            const int64_t e2 = base + threadIdx.x + 256 * r;
+            if (e2 < n2) {
+                double2 va, vb;
+                LzPair<T>::ld(vq, e2, va, vb);
+                acc = cadd(acc, cmulc(va, wa[r]));
+                acc = cadd(acc, cmulc(vb, wb[r]));
+            }
         }
         acc.x = wave_sum(acc.x);
         acc.y = wave_sum(acc.y);
@@ -285,6 +323,9 @@ __global__ __launch_bounds__(256) void lz_dots_kernel(const cplx<T> *V, int64_t 
 }
 
 // lz_update: g_q = Σ_c part; w -= Σ_q g_q V_q; partn[t][c] = ‖w_chunk‖².
+// q runs outside the pairs, so each basis vector is read by four
+// independent loads per lane (the subtraction order per element is q
+// ascending either way).
 template <typename T>
 __global__ __launch_bounds__(256) void lz_update_kernel(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j,
                                                         const c64 *part, int nchunk, int kslots, double *partn,
@@ -302,17 +343,39 @@ __global__ __launch_bounds__(256) void lz_update_kernel(cplx<T> *V, int64_t sV, 
     __syncthreads();
     cplx<T> *Vt = V + t * sV;
     cplx<T> *w = Vt + (int64_t)(j + 1) * nmax;
-    double nrm = 0;
-    const int64_t base = (int64_t)c * LZ_CHUNK;
+    const int64_t n2 = n >> 1, base = (int64_t)c * (LZ_CHUNK / 2);
+    double2 xa[4], xb[4];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        const int64_t e = base + threadIdx.x + 256 * r;
-        if (e >= n) continue;
-        double2 acc = ld2(w[e]);
-        for (int q = 0; q <= j; ++q) acc = csub(acc, cmul(g[q], ld2(Vt[(int64_t)q * nmax + e])));
-        st2(w[e], acc);
-        const double2 back = ld2(w[e]);
-        nrm += back.x * back.x + back.y * back.y;
+    for (int r = 0; r < 4; ++r) {
+        const int64_t e2 = base + threadIdx.x + 256 * r;
+        xa[r] = xb[r] = make_double2(0, 0);
+        if (e2 < n2) LzPair<T>::ld(w, e2, xa[r], xb[r]);
+    }
+    for (int q = 0; q <= j; ++q) {
+        const double2 gq = g[q];
+        const cplx<T> *vq = Vt + (int64_t)q * nmax;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t e2 = base + threadIdx.x + 256 * r;
+            if (e2 < n2) {
+                double2 va, vb;
+                LzPair<T>::ld(vq, e2, va, vb);
+                xa[r] = csub(xa[r], cmul(gq, va));
+                xb[r] = csub(xb[r], cmul(gq, vb));
+            }
+        }
+    }
+    double nrm = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int64_t e2 = base + threadIdx.x + 256 * r;
+        if (e2 < n2) {
+            LzPair<T>::st(w, e2, xa[r], xb[r]);
+            // the norm of the stored (storage-precision) values
+            double2 ba, bb;
+            LzPair<T>::ld(w, e2, ba, bb);
+            nrm += ba.x * ba.x + ba.y * ba.y + bb.x * bb.x + bb.y * bb.y;
+        }
     }
     nrm = block_sum(nrm, sh);
     if (threadIdx.x == 0) {
@@ -528,6 +591,7 @@ void lz_init(const cplx<T> *x, int64_t sx, cplx<T> *V, int64_t sV, int64_t n, do
 template <typename T>
 void lz_dots(const cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, c64 *part, int kslots, const int *active,
              int B, hipStream_t s) {
+    if ((n | nmax | sV) & 1) throw std::invalid_argument("Lanczos vectors must have even length and offsets");
     const int nchunk = (int)((n + LZ_CHUNK - 1) / LZ_CHUNK);
     hipLaunchKernelGGL(lz_dots_kernel<T>, dim3(nchunk, B), dim3(256), 0, s, V, sV, nmax, n, j, part, nchunk, kslots,
                        active);
@@ -536,6 +600,7 @@ void lz_dots(const cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, c64 *
 template <typename T>
 void lz_update(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, const c64 *part, int kslots, double *partn,
                double *galpha, const int *active, int B, hipStream_t s) {
+    if ((n | nmax | sV) & 1) throw std::invalid_argument("Lanczos vectors must have even length and offsets");
     const int nchunk = (int)((n + LZ_CHUNK - 1) / LZ_CHUNK);
     hipLaunchKernelGGL(lz_update_kernel<T>, dim3(nchunk, B), dim3(256), 0, s, V, sV, nmax, n, j, part, nchunk, kslots,
                        partn, galpha, active);
