@@ -61,8 +61,12 @@ def test_tile_shapes_bit_identical(prec, tol):
     lib.yaqs_debug_gemm_tile.argtypes = [ct.c_int] * 6 + [c_dp, c_dp, c_dp, ct.c_int]
     rng = np.random.default_rng(11)
     op = {0: lambda X: X, 1: lambda X: X.T, 2: lambda X: X.conj().T}
+    # tile 4 (k tiles loaded straight into LDS) applies to complex-FP32,
+    # op(A) = N and whole 64 x 64 tiles: (320, 256, 64), (64, 128, 48),
+    # (128, 64, 320); elsewhere it runs as tile 0
     for opA in range(3):
-        for M, N, K in [(80, 64, 16), (320, 256, 64), (33, 47, 70), (128, 20, 320), (1, 3, 5)]:
+        for M, N, K in [(80, 64, 16), (320, 256, 64), (33, 47, 70), (128, 20, 320), (1, 3, 5), (64, 128, 48),
+                        (128, 64, 320)]:
             a_shape = (M, K) if opA == 0 else (K, M)
             A = rng.standard_normal(a_shape) + 1j * rng.standard_normal(a_shape)
             B = rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N))
@@ -72,7 +76,7 @@ def test_tile_shapes_bit_identical(prec, tol):
             ref = op[opA](A) @ B
             got = base.view(np.complex128).reshape(M, N)
             assert np.abs(got - ref).max() / max(1.0, np.abs(ref).max()) < tol
-            for tile in range(1, 4):
+            for tile in range(1, 5):
                 other = _tile_run(lib, prec, opA, M, N, K, tile, a, b)
                 assert np.array_equal(other, base), (opA, M, N, K, tile)
 
@@ -89,15 +93,18 @@ def _csr(Wd):
 @pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 2e-5)])
 def test_fused_mpo_step_matches_unfused(prec, tol):
     """GEMM with the MPO step in its epilogue = GEMM, then wstep, bit for bit
-    (same FP64 accumulation order), and = numpy."""
+    (same FP64 accumulation order), with register-staged and with
+    direct-to-LDS k tiles, and = numpy."""
     print(f"case: fused MPO step prec={prec}", flush=True)
     lib = _abi.load_library()
     ip = ct.POINTER(ct.c_int)
-    lib.yaqs_debug_gemm_w.argtypes = [ct.c_int] * 7 + [c_dp, c_dp, ip, ip, c_dp, c_dp, c_dp]
+    lib.yaqs_debug_gemm_w.argtypes = [ct.c_int] * 7 + [c_dp, c_dp, ip, ip, c_dp, c_dp, c_dp, c_dp]
     rng = np.random.default_rng(5)
+    n_dl = 0
     for P, Dl, dph, Dr, Q, K in [(64, 5, 2, 5, 64, 64), (13, 5, 4, 5, 37, 29), (64, 3, 2, 3, 128, 64),
                                  (7, 1, 2, 5, 3, 9), (30, 5, 4, 5, 64, 30), (20, 2, 2, 3, 50, 17),
-                                 (33, 4, 4, 4, 40, 64)]:
+                                 (33, 4, 4, 4, 40, 64), (32, 5, 4, 5, 32, 48), (16, 1, 4, 2, 16, 16),
+                                 (64, 5, 4, 5, 64, 64)]:
         M, N, I, J = P * Dl, dph * Q, Dl * dph, dph * Dr
         A = rng.standard_normal((M, K)) + 1j * rng.standard_normal((M, K))
         B = rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N))
@@ -107,12 +114,20 @@ def test_fused_mpo_step_matches_unfused(prec, tol):
         b = np.ascontiguousarray(B).view(np.float64).ravel()
         yf = np.zeros(2 * P * J * Q)
         yu = np.zeros(2 * P * J * Q)
+        yd = np.zeros(2 * P * J * Q)
         rc = lib.yaqs_debug_gemm_w(prec, P, Dl, dph, Dr, Q, K, a.ctypes.data_as(c_dp), b.ctypes.data_as(c_dp),
                                    rowptr.ctypes.data_as(ip), col.ctypes.data_as(ip), val.ctypes.data_as(c_dp),
-                                   yf.ctypes.data_as(c_dp), yu.ctypes.data_as(c_dp))
-        assert rc == 0, (P, Dl, dph, Dr, Q, K)
+                                   yf.ctypes.data_as(c_dp), yu.ctypes.data_as(c_dp), yd.ctypes.data_as(c_dp))
+        assert rc & 1 == 0, (P, Dl, dph, Dr, Q, K)
         assert np.array_equal(yf, yu), (P, Dl, dph, Dr, Q, K)
+        # k tiles straight into LDS: complex-FP32 with P % 16, Q % (64 / dph), K % 16 == 0
+        fits = prec == 1 and P % 16 == 0 and Q % (64 // dph) == 0 and K % 16 == 0
+        assert (rc & 4 == 0) == fits, (P, Dl, dph, Dr, Q, K, rc)
+        if fits:
+            n_dl += 1
+            assert np.array_equal(yd, yf), (P, Dl, dph, Dr, Q, K)
         X = (A @ B).reshape(P, Dl, dph, Q).reshape(P, I, Q)
         ref = np.einsum("ji,piq->pjq", Wd, X)
         got = yf.view(np.complex128).reshape(P, J, Q)
         assert np.abs(got - ref).max() / max(1.0, np.abs(ref).max()) < tol, (P, Dl, dph, Dr, Q, K)
+    assert n_dl == (5 if prec == 1 else 0)

@@ -321,7 +321,7 @@ inline int launch_units(const GemmDesc &d) {
 
 // Tuner keys: GEMM dimensions in units of 16 (the MFMA block and the k
 // tile), MPO-step dimensions P and Q in units of 8, batch counts by power
-// of two.
+// of two, and whether the direct-to-LDS variant applies.
 inline int b8(int64_t x) { return (int)((x + 7) / 8); }
 inline int b16(int64_t x) { return (int)((x + 15) / 16); }
 inline int pow2b(int64_t x) {
@@ -337,7 +337,8 @@ void gemmT(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hi
     // only: a measurement would wait on the stream)
     int tile = 0;
     if (!d.accumulate && d.opB == OP_N && tune_enabled()) {
-        const TuneKey k = {0, (int)sizeof(T), d.opA, d.opB, b16(d.M), b16(d.N), b16(d.K), d.inner, pow2b(d.outer), 0};
+        const TuneKey k = {0,        (int)sizeof(T),    d.opA, d.opB, b16(d.M), b16(d.N), b16(d.K), d.inner,
+                           pow2b(d.outer), gemm_dl_fits<T>(d, A, B) ? 1 : 0};
         tile = g_nact == -2 ? cached_variant(k)
                             : tuned_variant(
                                   k, GEMM_TILES,
@@ -368,23 +369,27 @@ void gemmT(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hi
 template <typename T>
 void gemmW(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, int Dl, int dph, int Dr, int Q, cplx<T> *Y,
            int64_t sY, const CsrDev &W, const std::function<void()> &unfused, hipStream_t s) {
-    auto fused = [&]() {
+    // variants: 0 fused, 1 unfused, 2 fused with k tiles loaded straight
+    // into LDS (where that kernel applies)
+    auto fused = [&](bool dl) {
         return gemm_w_mfma<T>(d, A, B, P, Dl, dph, Dr, Q, Y, sY, W.rowptr.as<int>(), W.col.as<int>(),
-                              W.val.as<c64>(), W.nnz, s);
+                              W.val.as<c64>(), W.nnz, s, dl);
     };
-    bool use_fused = gemm_w_covers(d, P, Dl, dph, Dr, Q, W.nnz);
-    if (use_fused && tune_enabled()) {
-        const TuneKey k = {1, (int)sizeof(T), Dl, dph, Dr, b8(P), b8(Q), b16(d.K), W.nnz, pow2b(d.outer)};
-        use_fused = (g_nact == -2 ? cached_variant(k)
-                                  : tuned_variant(
-                                        k, 2,
-                                        [&](int v) {
-                                            if (v == 0) (void)fused();
-                                            else unfused();
-                                        },
-                                        s)) == 0;
+    int var = gemm_w_covers(d, P, Dl, dph, Dr, Q, W.nnz) ? 0 : 1;
+    if (var == 0 && tune_enabled()) {
+        const bool dl = gemm_w_dl_fits<T>(d, A, B, Dl);
+        const int nv = dl ? 3 : 2;
+        const TuneKey k = {dl ? 3 : 1, (int)sizeof(T), Dl, dph, Dr, b8(P), b8(Q), b16(d.K), W.nnz, pow2b(d.outer)};
+        var = g_nact == -2 ? cached_variant(k)
+                           : tuned_variant(
+                                 k, nv,
+                                 [&](int v) {
+                                     if (v == 1) unfused();
+                                     else (void)fused(v == 2);
+                                 },
+                                 s);
     }
-    if (!use_fused) {
+    if (var == 1) {
         unfused();
         return;
     }
@@ -394,7 +399,7 @@ void gemmW(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, int Dl,
         if (p->used + 2 > p->ev.size()) p->flush();
         YQ_HIP(hipEventRecord(p->ev[p->used], s));
     }
-    (void)fused();
+    (void)fused(var == 2);
     if (prof) {
         YQ_HIP(hipEventRecord(p->ev[p->used + 1], s));
         // GEMM plus the sparse MPO step (one complex MAC per W non-zero per (p, q))

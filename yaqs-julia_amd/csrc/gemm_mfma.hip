@@ -17,6 +17,7 @@
 
 #include "kernels.h"
 
+#include <cstdint>
 #include <cstdlib>
 #include <type_traits>
 #include <vector>
@@ -95,7 +96,7 @@ inline unsigned gemm_blocks(int ntile, int inner, int outer) {
     return (unsigned)ntile * inner * ((outer + GEMM_XCDS - 1) / GEMM_XCDS * GEMM_XCDS);
 }
 
-template <typename T, int OPA, int OPB, int BM, int BN, int BK, bool FW>
+template <typename T, int OPA, int OPB, int BM, int BN, int BK, bool FW, bool DL = false>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T> *__restrict__ A,
                                                         const cplx<T> *__restrict__ B, cplx<T> *__restrict__ C,
                                                         WFuse w, int ntile) {
@@ -133,7 +134,11 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
     // operand buffers; the fused epilogue reuses them for T1 and appends W
     constexpr int SM_W = FW ? ((65 + W_NNZ_MAX) * (int)sizeof(int) + W_NNZ_MAX * (int)sizeof(c64)) / (int)sizeof(T) + 4 : 0;
     constexpr int SM_T1W = FW ? BM * (BN + 1) * 2 + SM_W : 0;
-    __shared__ __align__(16) T smem[SM_AB > SM_T1W ? SM_AB : SM_T1W];
+    // DL: three k-tile stages of the operands as stored (complex interleaved)
+    constexpr int DL_SA = BM * 128, DL_SB = BK * 512, DL_ST = DL_SA + DL_SB;
+    constexpr int SM_DL = DL ? 3 * DL_ST / (int)sizeof(T) : 0;
+    constexpr int SM_1 = SM_AB > SM_T1W ? SM_AB : SM_T1W;
+    __shared__ __align__(1024) T smem[SM_1 > SM_DL ? SM_1 : SM_DL];
     T(*As)[2][BM][LDK] = reinterpret_cast<T(*)[2][BM][LDK]>(smem);            // [buffer][re/im][m][k']
     T(*Bs)[2][BN][LDK] = reinterpret_cast<T(*)[2][BN][LDK]>(smem + 2 * 2 * BM * LDK);  // [buffer][re/im][n][k']
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -230,38 +235,87 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
         for (int j = 0; j < JN; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) accr[i][j][r] = acci[i][j][r] = accs[i][j][r] = (T)0;
-    const int nk = (d.K + BK - 1) / BK;
-    load(0);
-    store(0);
-    __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-        const int buf = kt & 1;
-        if (kt + 1 < nk) load((kt + 1) * BK);
-        T ar[IM][KQ], ai[IM][KQ], br[JN][KQ], bi[JN][KQ];
+    if constexpr (DL) {
+        // Direct-to-LDS k tiles (complex-FP32, op N operands, whole tiles):
+        // each wavefront load instruction (global_load_lds_dwordx4) moves
+        // 64 x 16 bytes, two consecutive complex elements per lane, into a
+        // 1 KB block of the stage; no registers hold the tiles, so two k
+        // tiles are in flight while a third is multiplied.  A stage holds
+        // A as BM rows of 8 16-byte chunks (chunk kc of row m at position
+        // kc ^ (m % 8)) and B as 16 k rows of 32 chunks (chunk nc of row k
+        // at position nc ^ 8 (k % 4)), so the lanes of an MFMA operand read
+        // hit distinct banks.  The MFMA sequence per output element is the
+        // one of the register-staged path (same bits).
+        static_assert(std::is_same<T, float>::value && OPA == OP_N && OPB == OP_N && BN == 64 && BK == 16,
+                      "direct-to-LDS tiles: complex-FP32, op N, 64 columns");
+        constexpr int NBA = BM / 8, NBB = BK / 2;          // 1 KB blocks of A and B per stage
+        constexpr int NA = (NBA + 3) / 4, NB = NBB / 4;    // load instructions per wave (at most)
+        constexpr int NMIN = NBA / 4 + NB;                 // per tile, fewest of any wave
+        static_assert(NBB % 4 == 0, "B blocks per wave");
+        char *sm = reinterpret_cast<char *>(smem);
+        int64_t aoff[NA], boff[NB];
 #pragma unroll
-        for (int i = 0; i < IM; ++i)
+        for (int v = 0; v < NA; ++v) {
+            const int m = 8 * (wave + 4 * v) + (lane >> 3), kc = (lane & 7) ^ (m & 7);
+            aoff[v] = (int64_t)(m0 + m) * d.lda + 2 * kc;
+        }
 #pragma unroll
-            for (int q = 0; q < KQ; ++q) {
-                ar[i][q] = As[buf][0][wm + i * 16 + li][lk * KQ + q];
-                ai[i][q] = As[buf][1][wm + i * 16 + li][lk * KQ + q];
+        for (int v = 0; v < NB; ++v) {
+            const int k = 2 * (wave + 4 * v) + (lane >> 5), nc = (lane & 31) ^ (8 * (k & 3));
+            int col = n0 + 2 * nc;
+            if (FW) col = (2 * nc / w.qb) * w.Q + q0 + (2 * nc) % w.qb;
+            boff[v] = (int64_t)k * d.ldb + col;
+        }
+        // The loads are issued by inline asm: the compiler would otherwise
+        // treat them as LDS writes in flight and wait for all of them before
+        // every LDS read.  The explicit vmcnt wait and the barrier of each
+        // step order them before the reads of their stage.  M0 carries the
+        // LDS address (the compiler keeps nothing in M0 in this kernel: LDS
+        // instructions on gfx950 do not read it); the instruction writes no
+        // register.
+        auto dma = [&](const cplx<T> *g, const char *l) {
+            const unsigned la = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)l);
+            asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(la), "v"(g)
+                         : "memory");
+        };
+        auto issue = [&](int kt, int st) {
+            const int k0 = kt * BK;
+            char *sa = sm + st * DL_ST, *sb = sa + DL_SA;
+#pragma unroll
+            for (int v = 0; v < NA; ++v)
+                if (wave + 4 * v < NBA) dma(Ab + aoff[v] + k0, sa + (wave + 4 * v) * 1024);
+#pragma unroll
+            for (int v = 0; v < NB; ++v) dma(Bb + boff[v] + (int64_t)k0 * d.ldb, sb + (wave + 4 * v) * 1024);
+        };
+        auto compute = [&](int st) {
+            const char *sa = sm + st * DL_ST, *sb = sa + DL_SA;
+            T ar[IM][KQ], ai[IM][KQ], br[JN][KQ], bi[JN][KQ], as[IM][KQ], bs[JN][KQ];
+#pragma unroll
+            for (int i = 0; i < IM; ++i) {
+                const int m = wm + 16 * i + li;
+#pragma unroll
+                for (int q = 0; q < KQ; ++q) {
+                    const int kc = 2 * q + (lk >> 1);
+                    const float2 v = *reinterpret_cast<const float2 *>(sa + m * 128 + ((kc ^ (m & 7)) << 4) +
+                                                                       ((lk & 1) << 3));
+                    ar[i][q] = v.x;
+                    ai[i][q] = v.y;
+                    as[i][q] = v.x + v.y;
+                }
             }
 #pragma unroll
-        for (int j = 0; j < JN; ++j)
+            for (int j = 0; j < JN; ++j) {
+                const int n = wn + 16 * j + li;
 #pragma unroll
-            for (int q = 0; q < KQ; ++q) {
-                br[j][q] = Bs[buf][0][wn + j * 16 + li][lk * KQ + q];
-                bi[j][q] = Bs[buf][1][wn + j * 16 + li][lk * KQ + q];
+                for (int q = 0; q < KQ; ++q) {
+                    const int k = 4 * q + lk;
+                    const float2 v = *reinterpret_cast<const float2 *>(sb + k * 512 + (((n >> 1) ^ (8 * (k & 3))) << 4) +
+                                                                       ((n & 1) << 3));
+                    br[j][q] = v.x;
+                    bi[j][q] = v.y;
+                    bs[j][q] = v.x + v.y;
+                }
             }
-        if constexpr (G3M) {
-            T as[IM][KQ], bs[JN][KQ];
-#pragma unroll
-            for (int i = 0; i < IM; ++i)
-#pragma unroll
-                for (int q = 0; q < KQ; ++q) as[i][q] = ar[i][q] + ai[i][q];
-#pragma unroll
-            for (int j = 0; j < JN; ++j)
-#pragma unroll
-                for (int q = 0; q < KQ; ++q) bs[j][q] = br[j][q] + bi[j][q];
 #pragma unroll
             for (int q = 0; q < KQ; ++q)
 #pragma unroll
@@ -272,28 +326,88 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
                         acci[i][j] = Mfma<T>::mma(ai[i][q], bi[j][q], acci[i][j]);
                         accs[i][j] = Mfma<T>::mma(as[i][q], bs[j][q], accs[i][j]);
                     }
-        } else {
-#pragma unroll
-            for (int q = 0; q < KQ; ++q)
-#pragma unroll
-                for (int i = 0; i < IM; ++i)
-#pragma unroll
-                    for (int j = 0; j < JN; ++j) {
-                        accr[i][j] = Mfma<T>::mma(ar[i][q], br[j][q], accr[i][j]);
-                        acci[i][j] = Mfma<T>::mma(ar[i][q], bi[j][q], acci[i][j]);
-                    }
-#pragma unroll
-            for (int q = 0; q < KQ; ++q)
-#pragma unroll
-                for (int i = 0; i < IM; ++i)
-#pragma unroll
-                    for (int j = 0; j < JN; ++j) {
-                        accr[i][j] = Mfma<T>::mma(-ai[i][q], bi[j][q], accr[i][j]);
-                        acci[i][j] = Mfma<T>::mma(ai[i][q], br[j][q], acci[i][j]);
-                    }
+        };
+        const int nk = d.K / BK;
+        issue(0, 0);
+        if (nk > 1) issue(1, 1);
+        int st = 0;
+        for (int kt = 0; kt < nk; ++kt) {
+            // tile kt has landed (the NMIN or more loads of tile kt + 1 may be in
+            // flight) and this wave's LDS reads of the previous step are done
+            if (kt + 1 < nk) __builtin_amdgcn_s_waitcnt(NMIN | (0x7 << 4));
+            else __builtin_amdgcn_s_waitcnt(0x7 << 4);
+            __builtin_amdgcn_s_barrier();
+            if (kt + 2 < nk) issue(kt + 2, st == 0 ? 2 : st - 1);
+            compute(st);
+            st = st == 2 ? 0 : st + 1;
         }
-        if (kt + 1 < nk) store(buf ^ 1);
+        if (FW) __syncthreads();  // the epilogue reuses the stages
+    } else {
+        const int nk = (d.K + BK - 1) / BK;
+        load(0);
+        store(0);
         __syncthreads();
+        for (int kt = 0; kt < nk; ++kt) {
+            const int buf = kt & 1;
+            if (kt + 1 < nk) load((kt + 1) * BK);
+            T ar[IM][KQ], ai[IM][KQ], br[JN][KQ], bi[JN][KQ];
+    #pragma unroll
+            for (int i = 0; i < IM; ++i)
+    #pragma unroll
+                for (int q = 0; q < KQ; ++q) {
+                    ar[i][q] = As[buf][0][wm + i * 16 + li][lk * KQ + q];
+                    ai[i][q] = As[buf][1][wm + i * 16 + li][lk * KQ + q];
+                }
+    #pragma unroll
+            for (int j = 0; j < JN; ++j)
+    #pragma unroll
+                for (int q = 0; q < KQ; ++q) {
+                    br[j][q] = Bs[buf][0][wn + j * 16 + li][lk * KQ + q];
+                    bi[j][q] = Bs[buf][1][wn + j * 16 + li][lk * KQ + q];
+                }
+            if constexpr (G3M) {
+                T as[IM][KQ], bs[JN][KQ];
+    #pragma unroll
+                for (int i = 0; i < IM; ++i)
+    #pragma unroll
+                    for (int q = 0; q < KQ; ++q) as[i][q] = ar[i][q] + ai[i][q];
+    #pragma unroll
+                for (int j = 0; j < JN; ++j)
+    #pragma unroll
+                    for (int q = 0; q < KQ; ++q) bs[j][q] = br[j][q] + bi[j][q];
+    #pragma unroll
+                for (int q = 0; q < KQ; ++q)
+    #pragma unroll
+                    for (int i = 0; i < IM; ++i)
+    #pragma unroll
+                        for (int j = 0; j < JN; ++j) {
+                            accr[i][j] = Mfma<T>::mma(ar[i][q], br[j][q], accr[i][j]);
+                            acci[i][j] = Mfma<T>::mma(ai[i][q], bi[j][q], acci[i][j]);
+                            accs[i][j] = Mfma<T>::mma(as[i][q], bs[j][q], accs[i][j]);
+                        }
+            } else {
+    #pragma unroll
+                for (int q = 0; q < KQ; ++q)
+    #pragma unroll
+                    for (int i = 0; i < IM; ++i)
+    #pragma unroll
+                        for (int j = 0; j < JN; ++j) {
+                            accr[i][j] = Mfma<T>::mma(ar[i][q], br[j][q], accr[i][j]);
+                            acci[i][j] = Mfma<T>::mma(ar[i][q], bi[j][q], acci[i][j]);
+                        }
+    #pragma unroll
+                for (int q = 0; q < KQ; ++q)
+    #pragma unroll
+                    for (int i = 0; i < IM; ++i)
+    #pragma unroll
+                        for (int j = 0; j < JN; ++j) {
+                            accr[i][j] = Mfma<T>::mma(-ai[i][q], bi[j][q], accr[i][j]);
+                            acci[i][j] = Mfma<T>::mma(ai[i][q], br[j][q], acci[i][j]);
+                        }
+            }
+            if (kt + 1 < nk) store(buf ^ 1);
+            __syncthreads();
+        }
     }
     if constexpr (G3M) {
 #pragma unroll
@@ -402,12 +516,23 @@ static int mfma_mode() {
     return mode;
 }
 
-template <typename T, int OA, int OB, int BM, int BN>
+template <typename T, int OA, int OB, int BM, int BN, bool DL = false>
 static void launch_tile(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s) {
     const int tiles = ((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
-    hipLaunchKernelGGL((gemm_mfma_kernel<T, OA, OB, BM, BN, 16, false>), dim3(gemm_blocks(tiles, d.inner, d.outer)),
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, OA, OB, BM, BN, 16, false, DL>), dim3(gemm_blocks(tiles, d.inner, d.outer)),
                        dim3(256), 0, s, d, A, B, C, WFuse{}, tiles);
     YQ_HIP(hipGetLastError());
+}
+
+// The direct-to-LDS k tiles need complex-FP32 op N operands, whole BM x 64
+// C tiles and k tiles of 16, and 16-byte aligned element pairs (even
+// leading dimensions and batch strides, 16-byte aligned base pointers).
+template <typename T>
+static bool dl_fits(const GemmDesc &d, const void *A, const void *B, int BM) {
+    if (!std::is_same<T, float>::value || d.opA != OP_N || d.opB != OP_N) return false;
+    if (d.M % BM || d.N % 64 || d.K % 16) return false;
+    if ((d.lda | d.ldb | d.sA_outer | d.sB_outer | d.sA_inner | d.sB_inner) & 1) return false;
+    return (((uintptr_t)A | (uintptr_t)B) & 15) == 0;
 }
 
 template <typename T, int OA, int OB>
@@ -418,6 +543,11 @@ static void launch_shape(int tile, const GemmDesc &d, const cplx<T> *A, const cp
             case 1: return launch_tile<T, OA, OB, 32, 64>(d, A, B, C, s);
             case 2: return launch_tile<T, OA, OB, 64, 32>(d, A, B, C, s);
             case 3: return launch_tile<T, OA, OB, 32, 32>(d, A, B, C, s);
+            case 4:
+                if constexpr (std::is_same<T, float>::value && OA == OP_N) {
+                    if (dl_fits<T>(d, A, B, 64)) return launch_tile<T, OA, OB, 64, 64, true>(d, A, B, C, s);
+                }
+                break;
             default: break;
         }
     }
@@ -456,9 +586,21 @@ bool gemm_w_covers(const GemmDesc &d, int P, int Dl, int dph, int Dr, int Q, int
 }
 
 template <typename T>
+bool gemm_dl_fits(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B) {
+    return mfma_mode() && d.M > 0 && d.N > 0 && d.K > 0 && dl_fits<T>(d, A, B, 64);
+}
+
+template <typename T>
+bool gemm_w_dl_fits(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int Dl) {
+    return Dl >= 1 && Dl <= 5 && dl_fits<T>(d, A, B, 16 * Dl);
+}
+
+template <typename T>
 bool gemm_w_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, int Dl, int dph, int Dr, int Q,
-                 cplx<T> *Y, int64_t sY, const int *rowptr, const int *col, const c64 *val, int nnz, hipStream_t s) {
+                 cplx<T> *Y, int64_t sY, const int *rowptr, const int *col, const c64 *val, int nnz, hipStream_t s,
+                 bool dl) {
     if (!gemm_w_covers(d, P, Dl, dph, Dr, Q, nnz)) return false;
+    dl = dl && gemm_w_dl_fits<T>(d, A, B, Dl);
     WFuse w;
     w.P = P;
     w.Dl = Dl;
@@ -474,10 +616,17 @@ bool gemm_w_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, i
     w.Y = Y;
     const int tiles = ((P + w.pb - 1) / w.pb) * ((Q + w.qb - 1) / w.qb);
     const dim3 grid(gemm_blocks(tiles, 1, d.outer));
-#define YQ_GEMM_W(DL)                                                                                         \
-    case DL:                                                                                                  \
-        hipLaunchKernelGGL((gemm_mfma_kernel<T, OP_N, OP_N, 16 * DL, 64, 16, true>), grid, dim3(256), 0, s, d, A, \
-                           B, (cplx<T> *)nullptr, w, tiles);                                                 \
+#define YQ_GEMM_W(DL)                                                                                          \
+    case DL:                                                                                                   \
+        if constexpr (std::is_same<T, float>::value) {                                                         \
+            if (dl) {                                                                                          \
+                hipLaunchKernelGGL((gemm_mfma_kernel<T, OP_N, OP_N, 16 * DL, 64, 16, true, true>), grid, dim3(256), \
+                                   0, s, d, A, B, (cplx<T> *)nullptr, w, tiles);                               \
+                break;                                                                                         \
+            }                                                                                                  \
+        }                                                                                                      \
+        hipLaunchKernelGGL((gemm_mfma_kernel<T, OP_N, OP_N, 16 * DL, 64, 16, true>), grid, dim3(256), 0, s, d, A,  \
+                           B, (cplx<T> *)nullptr, w, tiles);                                                  \
         break;
     switch (Dl) {
         YQ_GEMM_W(1)
@@ -492,9 +641,13 @@ bool gemm_w_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, i
     return true;
 }
 template bool gemm_w_mfma<float>(const GemmDesc &, const c32 *, const c32 *, int, int, int, int, int, c32 *, int64_t,
-                                 const int *, const int *, const c64 *, int, hipStream_t);
+                                 const int *, const int *, const c64 *, int, hipStream_t, bool);
 template bool gemm_w_mfma<double>(const GemmDesc &, const c64 *, const c64 *, int, int, int, int, int, c64 *, int64_t,
-                                  const int *, const int *, const c64 *, int, hipStream_t);
+                                  const int *, const int *, const c64 *, int, hipStream_t, bool);
+template bool gemm_dl_fits<float>(const GemmDesc &, const c32 *, const c32 *);
+template bool gemm_dl_fits<double>(const GemmDesc &, const c64 *, const c64 *);
+template bool gemm_w_dl_fits<float>(const GemmDesc &, const c32 *, const c32 *, int);
+template bool gemm_w_dl_fits<double>(const GemmDesc &, const c64 *, const c64 *, int);
 template bool gemm_mfma<float>(const GemmDesc &, const c32 *, const c32 *, c32 *, hipStream_t, int);
 template bool gemm_mfma<double>(const GemmDesc &, const c64 *, const c64 *, c64 *, hipStream_t, int);
 
@@ -558,11 +711,13 @@ extern "C" int yaqs_debug_gemm(int prec, int M, int N, int K, int opA, int opB, 
 // yaqs_debug_gemm_w: T1 = A B (A: (P Dl) x K, B: K x (dph Q), row-major)
 // followed by the MPO step with W in CSR (rows J = dph Dr, columns
 // I = Dl dph, index m dph + s), through the fused kernel (Yf) and through
-// the unfused GEMM + wstep (Yu); Y is P x J x Q.  Returns 1 when the fused
-// kernel does not cover the shape.
+// the unfused GEMM + wstep (Yu), and, when Yd is not null, through the
+// fused kernel with k tiles loaded straight into LDS (Yd); Y is P x J x Q.
+// Returns 0, or 1 when the fused kernel does not cover the shape, plus 4
+// when Yd was asked for and the direct-to-LDS kernel does not apply.
 extern "C" int yaqs_debug_gemm_w(int prec, int P, int Dl, int dph, int Dr, int Q, int K, const double *A,
                                  const double *B, const int *rowptr, const int *col, const double *val, double *Yf,
-                                 double *Yu) {
+                                 double *Yu, double *Yd) {
     using namespace yq;
     try {
         const int M = P * Dl, N = dph * Q, J = dph * Dr, nnz = rowptr[J];
@@ -604,6 +759,12 @@ extern "C" int yaqs_debug_gemm_w(int prec, int P, int Dl, int dph, int Dr, int Q
             };
             if (gemm_w_mfma<T>(d, da, db, P, Dl, dph, Dr, Q, dy, 0, drp, dcol, dval, nnz, 0)) fetch(Yf);
             else rc = 1;
+            if (Yd) {
+                if (rc == 0 && gemm_w_dl_fits<T>(d, da, db, Dl) &&
+                    gemm_w_mfma<T>(d, da, db, P, Dl, dph, Dr, Q, dy, 0, drp, dcol, dval, nnz, 0, true))
+                    fetch(Yd);
+                else rc |= 4;
+            }
             if (!gemm_mfma<T>(d, da, db, dt, 0, 0)) gemm<T, T, T, T>(d, da, db, dt, 0);
             wstep<T>(dt, dy, P, Dl * dph, J, Q, 0, 0, drp, dcol, dval, nullptr, 1, 0);
             fetch(Yu);

@@ -11,22 +11,31 @@ void gemm(const GemmDesc &d, const cplx<TA> *A, const cplx<TB> *B, cplx<TC> *C, 
 // MFMA GEMM for op(A) in {N, C}, op(B) = N, same precision in and out
 // (float: v_mfma_f32_16x16x4f32, double: v_mfma_f64_16x16x4f64).  Returns
 // false when the descriptor is outside what the kernel covers.  tile picks
-// the C tile per workgroup (GEMM_TILES shapes, GEMM_TILE_BM/BN; all give
-// bit-identical results); shapes other than tile 0 exist for op(A) in
-// {N, T, C} with op(B) = N and fall back to tile 0 otherwise.
-constexpr int GEMM_TILES = 4;
-constexpr int GEMM_TILE_BM[GEMM_TILES] = {64, 32, 64, 32};
-constexpr int GEMM_TILE_BN[GEMM_TILES] = {64, 64, 32, 32};
+// the kernel variant (all give bit-identical results): 0-3 the C tile per
+// workgroup (GEMM_TILE_BM/BN) with register-staged k tiles, for op(A) in
+// {N, T, C} with op(B) = N (other ops take tile 0); 4 the 64 x 64 tile with
+// k tiles loaded straight into LDS (complex-FP32, op N operands, whole
+// tiles, 16-byte aligned operands; tile 0 otherwise).
+constexpr int GEMM_TILES = 5;
+constexpr int GEMM_TILE_BM[GEMM_TILES] = {64, 32, 64, 32, 64};
+constexpr int GEMM_TILE_BN[GEMM_TILES] = {64, 64, 32, 32, 64};
 template <typename T>
 bool gemm_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s, int tile = 0);
+// True when tile 4 (k tiles straight into LDS) applies to the descriptor.
+template <typename T>
+bool gemm_dl_fits(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B);
 
 // T1 = A B (both op N) with rows (p, m < Dl) and columns (s < dph, q < Q),
 // followed in the same kernel by the MPO step into Y (layout of wstep's
-// output).  Returns false when the shape is outside the fused kernel.
+// output).  Returns false when the shape is outside the fused kernel.  dl:
+// k tiles loaded straight into LDS where gemm_w_dl_fits (same bits).
 bool gemm_w_covers(const GemmDesc &d, int P, int Dl, int dph, int Dr, int Q, int nnz);
 template <typename T>
+bool gemm_w_dl_fits(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int Dl);
+template <typename T>
 bool gemm_w_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, int Dl, int dph, int Dr, int Q,
-                 cplx<T> *Y, int64_t sY, const int *rowptr, const int *col, const c64 *val, int nnz, hipStream_t s);
+                 cplx<T> *Y, int64_t sY, const int *rowptr, const int *col, const c64 *val, int nnz, hipStream_t s,
+                 bool dl = false);
 
 template <typename T>
 void wstep(const cplx<T> *X, cplx<T> *Y, int P, int I, int J, int Q, int64_t sX, int64_t sY, const int *rowptr,
