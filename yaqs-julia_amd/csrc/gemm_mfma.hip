@@ -134,9 +134,12 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
     // operand buffers; the fused epilogue reuses them for T1 and appends W
     constexpr int SM_W = FW ? ((65 + W_NNZ_MAX) * (int)sizeof(int) + W_NNZ_MAX * (int)sizeof(c64)) / (int)sizeof(T) + 4 : 0;
     constexpr int SM_T1W = FW ? BM * (BN + 1) * 2 + SM_W : 0;
-    // DL: three k-tile stages of the operands as stored (complex interleaved)
-    constexpr int DL_SA = BM * 128, DL_SB = BK * 512, DL_ST = DL_SA + DL_SB;
-    constexpr int SM_DL = DL ? 3 * DL_ST / (int)sizeof(T) : 0;
+    // DL: k-tile stages of the operands as stored (complex interleaved);
+    // three stages of 16 k, or four of 8 k (the fused kernel: 5 wavefronts'
+    // worth of LDS less, so three workgroups fit a CU)
+    constexpr int DL_NS = BK == 16 ? 3 : 4;
+    constexpr int DL_SA = BM * BK * 8, DL_SB = BK * 512, DL_ST = DL_SA + DL_SB;
+    constexpr int SM_DL = DL ? DL_NS * DL_ST / (int)sizeof(T) : 0;
     constexpr int SM_1 = SM_AB > SM_T1W ? SM_AB : SM_T1W;
     __shared__ __align__(1024) T smem[SM_1 > SM_DL ? SM_1 : SM_DL];
     T(*As)[2][BM][LDK] = reinterpret_cast<T(*)[2][BM][LDK]>(smem);            // [buffer][re/im][m][k']
@@ -155,8 +158,9 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
     static_assert(FW || ((BM == 32 || BM == 64) && (BN == 32 || BN == 64)), "tile shape outside the load mappings");
     static_assert(!FW || (BM % 16 == 0 && BM <= 80 && BN == 64 && OPA == OP_N && OPB == OP_N),
                   "the fused kernel takes op N operands, 64 columns and 16 Dl rows");
-    static_assert(BK == 16, "k tile");
-    constexpr int NLA = BM * BK / 256, NLB = BN * BK / 256;  // loads per thread and k tile
+    static_assert(BK == 16 || (DL && BK == 8), "k tile");
+    // loads per thread and k tile of the register-staged path (DL: unused)
+    constexpr int NLA = DL ? 1 : BM * BK / 256, NLB = DL ? 1 : BN * BK / 256;
     constexpr int GA = 256 / BM, GB = 256 / BN;              // k groups of the row mapping
     constexpr bool ROWA = OPA != OP_N, ROWB = OPB == OP_N;
     int ar_[NLA], ak_[NLA], br_[NLB], bk_[NLB];
@@ -239,24 +243,27 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
         // Direct-to-LDS k tiles (complex-FP32, op N operands, whole tiles):
         // each wavefront load instruction (global_load_lds_dwordx4) moves
         // 64 x 16 bytes, two consecutive complex elements per lane, into a
-        // 1 KB block of the stage; no registers hold the tiles, so two k
-        // tiles are in flight while a third is multiplied.  A stage holds
-        // A as BM rows of 8 16-byte chunks (chunk kc of row m at position
-        // kc ^ (m % 8)) and B as 16 k rows of 32 chunks (chunk nc of row k
-        // at position nc ^ 8 (k % 4)), so the lanes of an MFMA operand read
-        // hit distinct banks.  The MFMA sequence per output element is the
-        // one of the register-staged path (same bits).
-        static_assert(std::is_same<T, float>::value && OPA == OP_N && OPB == OP_N && BN == 64 && BK == 16,
+        // 1 KB block of the stage; no registers hold the tiles, so DL_NS - 1
+        // k tiles are in flight while one is multiplied.  A stage holds A as
+        // BM rows of CPR = BK/2 16-byte chunks (chunk kc of row m at position
+        // kc ^ fa(m), fa(m) = m % 8 for 8 chunks a row, (m / 4) % 4 for 4)
+        // and B as BK k rows of 32 chunks (chunk nc of row k at position
+        // nc ^ 8 (k % 4)), so the lanes of an MFMA operand read hit distinct
+        // banks.  The MFMA sequence per output element is the one of the
+        // register-staged path (same bits).
+        static_assert(std::is_same<T, float>::value && OPA == OP_N && OPB == OP_N && BN == 64,
                       "direct-to-LDS tiles: complex-FP32, op N, 64 columns");
-        constexpr int NBA = BM / 8, NBB = BK / 2;          // 1 KB blocks of A and B per stage
+        constexpr int NS = DL_NS, CPR = BK / 2, RPB = 1024 / (BK * 8);  // chunks a row, rows a block
+        constexpr int NBA = BM / RPB, NBB = BK / 2;        // 1 KB blocks of A and B per stage
         constexpr int NA = (NBA + 3) / 4, NB = NBB / 4;    // load instructions per wave (at most)
         constexpr int NMIN = NBA / 4 + NB;                 // per tile, fewest of any wave
-        static_assert(NBB % 4 == 0, "B blocks per wave");
+        static_assert(BM % RPB == 0 && NBB % 4 == 0, "whole 1 KB blocks, B blocks per wave");
+        auto fa = [](int m) { return CPR == 8 ? (m & 7) : ((m >> 2) & 3); };
         char *sm = reinterpret_cast<char *>(smem);
         int64_t aoff[NA], boff[NB];
 #pragma unroll
         for (int v = 0; v < NA; ++v) {
-            const int m = 8 * (wave + 4 * v) + (lane >> 3), kc = (lane & 7) ^ (m & 7);
+            const int m = RPB * (wave + 4 * v) + lane / CPR, kc = (lane % CPR) ^ fa(m);
             aoff[v] = (int64_t)(m0 + m) * d.lda + 2 * kc;
         }
 #pragma unroll
@@ -296,7 +303,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
 #pragma unroll
                 for (int q = 0; q < KQ; ++q) {
                     const int kc = 2 * q + (lk >> 1);
-                    const float2 v = *reinterpret_cast<const float2 *>(sa + m * 128 + ((kc ^ (m & 7)) << 4) +
+                    const float2 v = *reinterpret_cast<const float2 *>(sa + m * (BK * 8) + ((kc ^ fa(m)) << 4) +
                                                                        ((lk & 1) << 3));
                     ar[i][q] = v.x;
                     ai[i][q] = v.y;
@@ -328,18 +335,20 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
                     }
         };
         const int nk = d.K / BK;
-        issue(0, 0);
-        if (nk > 1) issue(1, 1);
+        for (int kt = 0; kt < NS - 1 && kt < nk; ++kt) issue(kt, kt);
         int st = 0;
         for (int kt = 0; kt < nk; ++kt) {
-            // tile kt has landed (the NMIN or more loads of tile kt + 1 may be in
-            // flight) and this wave's LDS reads of the previous step are done
-            if (kt + 1 < nk) __builtin_amdgcn_s_waitcnt(NMIN | (0x7 << 4));
+            // tile kt has landed (the loads of the later tiles already issued,
+            // NMIN or more per tile, may be in flight) and this wave's LDS
+            // reads of the previous step are done
+            const int later = min(NS - 2, nk - 1 - kt);
+            if (later >= 2) __builtin_amdgcn_s_waitcnt((2 * NMIN) | (0x7 << 4));
+            else if (later == 1) __builtin_amdgcn_s_waitcnt(NMIN | (0x7 << 4));
             else __builtin_amdgcn_s_waitcnt(0x7 << 4);
             __builtin_amdgcn_s_barrier();
-            if (kt + 2 < nk) issue(kt + 2, st == 0 ? 2 : st - 1);
+            if (kt + NS - 1 < nk) issue(kt + NS - 1, st == 0 ? NS - 1 : st - 1);
             compute(st);
-            st = st == 2 ? 0 : st + 1;
+            st = st == NS - 1 ? 0 : st + 1;
         }
         if (FW) __syncthreads();  // the epilogue reuses the stages
     } else {
@@ -423,7 +432,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
     }
     if (FW) {
         // stage T1 in LDS (the operand buffers are free after the last barrier)
-        static_assert(BM * (BN + 1) * 2 <= SM_AB, "T1 stage exceeds the operand buffers");
+        static_assert(SM_T1W <= (int)(sizeof(smem) / sizeof(T)), "T1 and W stage exceed the shared buffer");
         cplx<T>(*T1s)[BN + 1] = reinterpret_cast<cplx<T>(*)[BN + 1]>(smem);
 #pragma unroll
         for (int i = 0; i < IM; ++i)
@@ -620,7 +629,7 @@ bool gemm_w_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, int P, i
     case DL:                                                                                                   \
         if constexpr (std::is_same<T, float>::value) {                                                         \
             if (dl) {                                                                                          \
-                hipLaunchKernelGGL((gemm_mfma_kernel<T, OP_N, OP_N, 16 * DL, 64, 16, true, true>), grid, dim3(256), \
+                hipLaunchKernelGGL((gemm_mfma_kernel<T, OP_N, OP_N, 16 * DL, 64, 8, true, true>), grid, dim3(256),  \
                                    0, s, d, A, B, (cplx<T> *)nullptr, w, tiles);                               \
                 break;                                                                                         \
             }                                                                                                  \
