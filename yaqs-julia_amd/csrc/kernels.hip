@@ -6,6 +6,7 @@
 // eigen-decompositions, transfer matrices and jump probabilities are always
 // computed in double.
 #include <algorithm>
+#include <cstdint>
 #include <stdexcept>
 #include <type_traits>
 #include <vector>
@@ -558,11 +559,13 @@ __global__ __launch_bounds__(256) void lz_finalize_kernel(cplx<T> *V, int64_t sV
     }
 }
 
-// x = β0 Σ_{q<nit} c_q V_q  (trajectories with β0 > 0)
+// x = β0 Σ_{q<nit} c_q V_q  (trajectories with β0 > 0); the basis is read
+// in element pairs with q outside the pairs (per element the sum runs over
+// q ascending); x is written in pairs when its slots allow (xpair).
 template <typename T>
 __global__ __launch_bounds__(256) void lz_combine_kernel(cplx<T> *x, int64_t sx, const cplx<T> *V, int64_t sV,
                                                          int64_t nmax, int64_t n, const double *beta0, const c64 *coef,
-                                                         const int *nit, int kmax, const int *mask) {
+                                                         const int *nit, int kmax, const int *mask, int xpair) {
     const int t = blockIdx.y;
     if (mask && !mask[t]) return;
     const double b0 = beta0[t];
@@ -570,14 +573,35 @@ __global__ __launch_bounds__(256) void lz_combine_kernel(cplx<T> *x, int64_t sx,
     const int k = nit[t];
     const cplx<T> *Vt = V + t * sV;
     const c64 *ct = coef + (int64_t)t * kmax;
-    const int64_t base = (int64_t)blockIdx.x * LZ_CHUNK;
+    const int64_t n2 = n >> 1, base = (int64_t)blockIdx.x * (LZ_CHUNK / 2);
+    double2 xa[4], xb[4];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        const int64_t e = base + threadIdx.x + 256 * r;
-        if (e >= n) continue;
-        double2 acc = make_double2(0, 0);
-        for (int q = 0; q < k; ++q) acc = cadd(acc, cmul(make_double2(ct[q].re, ct[q].im), ld2(Vt[(int64_t)q * nmax + e])));
-        st2(x[t * sx + e], cscale(acc, b0));
+    for (int r = 0; r < 4; ++r) xa[r] = xb[r] = make_double2(0, 0);
+    for (int q = 0; q < k; ++q) {
+        const double2 c = make_double2(ct[q].re, ct[q].im);
+        const cplx<T> *vq = Vt + (int64_t)q * nmax;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t e2 = base + threadIdx.x + 256 * r;
+            if (e2 < n2) {
+                double2 va, vb;
+                LzPair<T>::ld(vq, e2, va, vb);
+                xa[r] = cadd(xa[r], cmul(c, va));
+                xb[r] = cadd(xb[r], cmul(c, vb));
+            }
+        }
+    }
+    cplx<T> *xt = x + t * sx;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int64_t e2 = base + threadIdx.x + 256 * r;
+        if (e2 >= n2) continue;
+        if (xpair) {
+            LzPair<T>::st(xt, e2, cscale(xa[r], b0), cscale(xb[r], b0));
+        } else {
+            st2(xt[2 * e2], cscale(xa[r], b0));
+            st2(xt[2 * e2 + 1], cscale(xb[r], b0));
+        }
     }
 }
 
@@ -618,9 +642,11 @@ void lz_finalize(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, int kma
 template <typename T>
 void lz_combine(cplx<T> *x, int64_t sx, const cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, const double *beta0,
                 const c64 *coef, const int *nit, int kmax, const int *mask, int B, hipStream_t s) {
+    if ((n | nmax | sV) & 1) throw std::invalid_argument("Lanczos vectors must have even length and offsets");
+    const int xpair = ((sx & 1) == 0 && ((uintptr_t)x & (2 * sizeof(cplx<T>) - 1)) == 0) ? 1 : 0;
     const int nchunk = (int)((n + LZ_CHUNK - 1) / LZ_CHUNK);
     hipLaunchKernelGGL(lz_combine_kernel<T>, dim3(nchunk, B), dim3(256), 0, s, x, sx, V, sV, nmax, n, beta0, coef, nit,
-                       kmax, mask);
+                       kmax, mask, xpair);
     YQ_HIP(hipGetLastError());
 }
 
