@@ -470,6 +470,19 @@ class Engine {
     std::vector<char> obs2_at;
     int *h_pinned = nullptr;
     hipEvent_t lz_ev[2] = {nullptr, nullptr};  // Lanczos active counts copied to h_pinned[1 + (j & 1)]
+    // The count a device-to-host copy writes over the -1 left in
+    // h_pinned[slot]: the host spins on the pinned word (the next launches
+    // go out as soon as the count lands, without the event's wake-up), then
+    // falls back to the event after some tens of milliseconds.
+    int wait_pinned(int slot, hipEvent_t ev) {
+        for (int i = 0; i < 200000; ++i) {
+            const int v = __atomic_load_n(h_pinned + slot, __ATOMIC_ACQUIRE);
+            if (v >= 0) return v;
+            __builtin_ia32_pause();
+        }
+        YQ_HIP(hipEventSynchronize(ev));
+        return __atomic_load_n(h_pinned + slot, __ATOMIC_ACQUIRE);
+    }
     uint64_t traj0 = 0;
     int64_t nchunk_max = 0;
     std::map<int, PairCand> pair_cand;  // digital candidate tables per gate pair
@@ -979,6 +992,8 @@ void Engine<T>::expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, Jo
         lz_finalize<T>(V.as<CT>(), sV(), nmax, n, j, kdim, dt, J.ktol, partn.as<double>(), galpha.as<double>(),
                        alpha.as<double>(), beta.as<double>(), beta0.as<double>(), coef.as<c64>(), nit.as<int>(),
                        active.as<int>(), nactive.as<int>(), nb, s);
+        // the copy overwrites the -1 left in the host slot (see wait_pinned)
+        __atomic_store_n(h_pinned + 1 + (j & 1), -1, __ATOMIC_RELEASE);
         YQ_HIP(hipMemcpyAsync(h_pinned + 1 + (j & 1), nactive.as<int>() + j, sizeof(int), hipMemcpyDeviceToHost, s));
         YQ_HIP(hipEventRecord(lz_ev[j & 1], s));
     };
@@ -1000,8 +1015,7 @@ void Engine<T>::expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, Jo
     for (int j = 0; j < kdim; ++j) {
         const bool more = j + 1 < kdim;
         if (more && ahead) iteration(j + 1, -2);  // active count resolved below
-        YQ_HIP(hipEventSynchronize(lz_ev[j & 1]));
-        nact = h_pinned[1 + (j & 1)];
+        nact = wait_pinned(1 + (j & 1), lz_ev[j & 1]);
         if (!more) break;
         if (!ahead) {
             if (nact == 0) break;
