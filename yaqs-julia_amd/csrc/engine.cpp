@@ -1025,7 +1025,13 @@ void Engine<T>::expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, Jo
         }
         res.matvecs += nact;
         res.flop += fl * nact;
-        if (nact == 0) break;
+        if (nact == 0) {
+            // the masked iteration j+1 queued ahead still copies its count
+            // into the other host slot: let it land before the slots are
+            // reused by the next exponential
+            if (ahead) YQ_HIP(hipEventSynchronize(lz_ev[(j + 1) & 1]));
+            break;
+        }
     }
     lz_combine<T>(x, sx, V.as<CT>(), sV(), nmax, n, beta0.as<double>(), coef.as<c64>(), nit.as<int>(), kdim, nullptr,
                   nb, s);
