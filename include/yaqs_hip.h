@@ -227,10 +227,12 @@ int64_t yaqs_job_footprint_bytes(const yaqs_analog_job *job);
  * synchronously, treats its first `warmup_steps` time steps as untimed
  * warm-up and reports the stream-synchronised wall time of the remaining
  * steps, plus HIP-event timing of every matrix-core GEMM launched inside
- * that window.  Waves run one after the other on one stream (with the
- * environment variable YAQS_BENCH_SPLIT=1, a job of at least 512
- * trajectories that fits one wave runs as two concurrent half waves on two
- * streams whose windows start together; the window is the longer one). */
+ * that window.  The waves are the ones the context would run: a job that
+ * fits one wave is split into up to three concurrent waves of at least 256
+ * trajectories (environment variable YAQS_WAVES, 1..8; 1 = one wave), run
+ * side by side on their own streams with windows that start together (the
+ * window is the longest); waves limited by the HBM budget run one after the
+ * other on one stream. */
 typedef struct {
     double timed_s;          /* wall time of the timed steps                      */
     int64_t timed_steps;     /* time steps inside the window, summed over waves   */

@@ -206,16 +206,37 @@ struct yaqs_ctx {
 
 namespace {
 
+// Concurrent waves for a job whose trajectories fit one wave: at least 256
+// trajectories a wave, YAQS_WAVES (1..8) of them, three by default.  The
+// latency-bound phases of one wave (Lanczos tails, host round trips, the
+// eigensolver) overlap the matrix-core phases of the others (C2, 1000
+// trajectories: 14.7k traj-site-steps/s as three waves, 14.5k as two, 14.3k
+// as one or four).
+int concurrent_waves(int64_t ntraj) {
+    static const int k = [] {
+        const char *e = std::getenv("YAQS_WAVES");
+        return e ? std::max(1, std::min(8, atoi(e))) : 3;
+    }();
+    return (int)std::max<int64_t>(1, std::min<int64_t>(k, ntraj / 256));
+}
+
 // Batch size and bytes for a job under the remaining budget; 0 if it cannot
-// start now.
-int64_t choose_batch(yaqs_ctx *c, const yq::JobIn &in, int64_t avail, int64_t *bytes) {
+// start now.  conc: the waves meant to run side by side (1 unless the job
+// fits one wave and is split by concurrent_waves).
+int64_t choose_batch(yaqs_ctx *c, const yq::JobIn &in, int64_t avail, int64_t *bytes, int *conc = nullptr) {
     const int64_t per = std::max<int64_t>(1, yq::footprint_bytes(in, 1));
     int64_t b = std::min<int64_t>(in.num_traj, c->max_batch > 0 ? c->max_batch : 4096);
     b = std::min<int64_t>(b, avail / per);
     if (b < 1) return 0;
     // balance waves: same number of waves, fewer padded slots
-    const int64_t waves = (in.num_traj + b - 1) / b;
+    int64_t waves = (in.num_traj + b - 1) / b;
+    int k = 1;
+    if (waves == 1 && c->max_batch == 0) {
+        k = std::min<int>(concurrent_waves(in.num_traj), std::max<int>(1, (int)c->workers.size()));
+        waves = k;
+    }
     b = (in.num_traj + waves - 1) / waves;
+    if (conc) *conc = k;
     *bytes = yq::footprint_bytes(in, b);
     return b;
 }
@@ -575,22 +596,16 @@ int yaqs_bench_analog(yaqs_ctx *ctx, const yaqs_analog_job *job, int64_t warmup_
         in.bench_warmup = warmup_steps;
         yq::validate_job(in);
         int64_t bytes = 0;
-        int64_t b = choose_batch(ctx, in, ctx->budget, &bytes);
+        int conc = 1;
+        int64_t b = choose_batch(ctx, in, ctx->budget, &bytes, &conc);
         if (b < 1) return fail(YAQS_ENOMEM_DEVICE, "job footprint exceeds the HBM budget");
         if (hipSetDevice(ctx->device) != hipSuccess) return fail(YAQS_EDEVICE, "hipSetDevice failed");
-        // With YAQS_BENCH_SPLIT=1 a large job whose trajectories fit in one wave
-        // runs as two concurrent half waves on two streams (the latency-bound
-        // phases of one overlap the matrix-core phases of the other: faster
-        // while bonds are small, slower at saturated chi=64, so off by
-        // default); otherwise the waves run one after the other on one stream.
+        // The waves run as the context would run them: side by side on
+        // their own streams when choose_batch split the job for concurrency
+        // (concurrent_waves), else one after the other on one stream.
         const int64_t ntraj = in.num_traj;
-        static const bool split_env = [] {
-            const char *e = std::getenv("YAQS_BENCH_SPLIT");
-            return e && e[0] == '1';
-        }();
-        const bool split = split_env && ctx->max_batch == 0 && b >= ntraj && ntraj >= 512;
-        const int nconc = split ? 2 : 1;
-        if (split) b = (ntraj + 1) / 2;
+        const bool split = conc > 1;
+        const int nconc = conc;
         in.wsync = split ? std::make_shared<yq::WindowSync>(nconc) : nullptr;
         const int64_t nout = std::max<int64_t>(1, (int64_t)in.obs.size() * yq::num_columns(in) * ntraj);
         double *d_out = nullptr;
