@@ -105,3 +105,27 @@ def test_late_small_job_overtakes_long_job():
     t_small, t_big = rs[3]["wall_s"], rb[3]["wall_s"]
     assert t_small < t_big, (t_small, t_big)
     assert np.isfinite(rb[1]).all() and np.isfinite(rs[1]).all()
+
+
+def test_concurrent_waves_match_one_wave():
+    """A job that fits one wave runs as concurrent waves of at least 256
+    trajectories (three at most) on the context's workers; every
+    trajectory's result equals the one-wave run (max_batch = num_traj) to
+    the padding-order tolerance (FP64 1e-10), and the job statistics count
+    the same trajectories."""
+    print("case: 768 trajectories as concurrent waves vs one wave", flush=True)
+    L = 6
+    ntraj = 768
+    cfg = Y.TimeEvolutionConfig(_zx(L), 0.3, dt=0.05, num_traj=ntraj, max_bond_dim=8, truncation_threshold=1e-10)
+    H = Y.init_ising(L, 1.0, 0.5)
+    out = []
+    for mb in (0, ntraj):
+        ctx = Y.Context(0, max_batch=mb)
+        h = _abi.pack_analog(Y.MPS(L), H, cfg, Y.relaxation_dephasing(L, 0.1), precision=_abi.PREC_FP64, seed=21)
+        out.append(ctx.wait(ctx.submit_analog(h), len(cfg.observables), len(cfg.times), ntraj))
+        ctx.close()
+    split, one = out
+    assert np.isfinite(split[0]).all()
+    assert np.abs(split[0] - one[0]).max() < 1e-10, np.abs(split[0] - one[0]).max()
+    assert np.abs(split[1] - one[1]).max() < 1e-10
+    assert split[3]["jumps"] == one[3]["jumps"]
