@@ -988,10 +988,10 @@ void Engine<T>::expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, Jo
         g_nact = -1;
         lz_dots<T>(V.as<CT>(), sV(), nmax, n, j, part.as<c64>(), kdim + 1, active.as<int>(), nb, s);
         lz_update<T>(V.as<CT>(), sV(), nmax, n, j, part.as<c64>(), kdim + 1, partn.as<double>(), galpha.as<double>(),
-                     active.as<int>(), nb, s);
-        lz_finalize<T>(V.as<CT>(), sV(), nmax, n, j, kdim, dt, J.ktol, partn.as<double>(), galpha.as<double>(),
-                       alpha.as<double>(), beta.as<double>(), beta0.as<double>(), coef.as<c64>(), nit.as<int>(),
-                       active.as<int>(), nactive.as<int>(), nb, s);
+                     active.as<int>(), beta.as<double>(), kdim, nb, s);
+        lz_finalize(j, kdim, dt, J.ktol, partn.as<double>(), n, galpha.as<double>(), alpha.as<double>(),
+                    beta.as<double>(), beta0.as<double>(), coef.as<c64>(), nit.as<int>(), active.as<int>(),
+                    nactive.as<int>(), nb, s);
         // the copy overwrites the -1 left in the host slot (see wait_pinned)
         __atomic_store_n(h_pinned + 1 + (j & 1), -1, __ATOMIC_RELEASE);
         YQ_HIP(hipMemcpyAsync(h_pinned + 1 + (j & 1), nactive.as<int>() + j, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1033,7 +1033,8 @@ void Engine<T>::expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, Jo
             break;
         }
     }
-    lz_combine<T>(x, sx, V.as<CT>(), sV(), nmax, n, beta0.as<double>(), coef.as<c64>(), nit.as<int>(), kdim, nullptr,
+    lz_combine<T>(x, sx, V.as<CT>(), sV(), nmax, n, beta0.as<double>(), coef.as<c64>(), nit.as<int>(), beta.as<double>(),
+                  kdim, nullptr,
                   nb, s);
 }
 

@@ -47,18 +47,21 @@ void lz_init(const cplx<T> *x, int64_t sx, cplx<T> *V, int64_t sV, int64_t n, do
 template <typename T>
 void lz_dots(const cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, c64 *part, int kslots, const int *active,
              int B, hipStream_t s);
+// Deferred normalisation: the Krylov basis is stored unnormalised, V_q =
+// v_q / s_q with s_0 = 1 and s_q = 1 / beta[q]; lz_update scales the matvec
+// output by s_j and takes the coefficients of the stored vectors.
 template <typename T>
 void lz_update(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, const c64 *part, int kslots, double *partn,
-               double *galpha, const int *active, int B, hipStream_t s);
-// β = ‖w‖, V_{j+1} = w/β and the stopping rule; trajectories that go on are
-// counted in nactive[j] (cleared by lz_init for every slot).
-template <typename T>
-void lz_finalize(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, int kmax, double dt, double tol,
-                 const double *partn, const double *galpha, double *alpha, double *beta, const double *beta0, c64 *coef,
-                 int *nit, int *active, int *nactive, int B, hipStream_t s);
+               double *galpha, const int *active, const double *beta, int kmax, int B, hipStream_t s);
+// β = ‖w‖ (stored in beta[j+1] when the trajectory goes on) and the
+// stopping rule; trajectories that go on are counted in nactive[j] (cleared
+// by lz_init for every slot).
+void lz_finalize(int j, int kmax, double dt, double tol, const double *partn, int64_t n, const double *galpha,
+                 double *alpha, double *beta, const double *beta0, c64 *coef, int *nit, int *active, int *nactive, int B,
+                 hipStream_t s);
 template <typename T>
 void lz_combine(cplx<T> *x, int64_t sx, const cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, const double *beta0,
-                const c64 *coef, const int *nit, int kmax, const int *mask, int B, hipStream_t s);
+                const c64 *coef, const int *nit, const double *beta, int kmax, const int *mask, int B, hipStream_t s);
 constexpr int LZ_CHUNK_ELEMS = 2048;
 
 // Hermitian eigen-decomposition of a batch of FP64 Gram matrices (eigen.hip):

@@ -277,6 +277,11 @@ struct LzPair<double> {
     }
 };
 
+// Deferred normalisation: the basis is stored unnormalised, V_q = v_q / s_q
+// with s_0 = 1 (lz_init stores v_0) and s_q = 1 / β_q (β_q = beta[q], the
+// norm of the q-th orthogonalised matvec output, stored as it is).
+__device__ __forceinline__ double lz_scale_of(const double *be, int q) { return q == 0 ? 1.0 : 1.0 / be[q]; }
+
 // Each wavefront reduces its partial of every q with DPP and parks it in
 // LDS; one barrier, then lanes q <= j add the four wavefront partials.
 template <typename T>
@@ -330,16 +335,25 @@ __global__ __launch_bounds__(256) void lz_dots_kernel(const cplx<T> *V, int64_t 
 template <typename T>
 __global__ __launch_bounds__(256) void lz_update_kernel(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j,
                                                         const c64 *part, int nchunk, int kslots, double *partn,
-                                                        double *galpha, const int *active) {
+                                                        double *galpha, const int *active, const double *beta,
+                                                        int kmax) {
     __shared__ double sh[16];
     __shared__ double2 g[64];
     const int t = blockIdx.y, c = blockIdx.x;
     if (!active[t]) return;
+    // stored V_q = v_q / s_q, w = H V_j: <v_q, H v_j> = s_q s_j <V_q, w>, and
+    // H v_j - Σ <v_q, H v_j> v_q = s_j w - Σ h_q V_q with h_q = s_q <v_q, H v_j>
+    const double *be = beta + (int64_t)t * (kmax + 1);
+    const double sj = lz_scale_of(be, j);
     if (threadIdx.x <= j) {
+        const int q = threadIdx.x;
         double2 s = make_double2(0, 0);
-        const c64 *p = part + ((int64_t)t * kslots + threadIdx.x) * nchunk;
+        const c64 *p = part + ((int64_t)t * kslots + q) * nchunk;
         for (int i = 0; i < nchunk; ++i) s = cadd(s, make_double2(p[i].re, p[i].im));
-        g[threadIdx.x] = s;
+        const double sq = lz_scale_of(be, q);
+        s = cscale(s, sq * sj);
+        if (q == j && c == 0) galpha[t] = s.x;
+        g[q] = cscale(s, sq);
     }
     __syncthreads();
     cplx<T> *Vt = V + t * sV;
@@ -350,7 +364,11 @@ __global__ __launch_bounds__(256) void lz_update_kernel(cplx<T> *V, int64_t sV, 
     for (int r = 0; r < 4; ++r) {
         const int64_t e2 = base + threadIdx.x + 256 * r;
         xa[r] = xb[r] = make_double2(0, 0);
-        if (e2 < n2) LzPair<T>::ld(w, e2, xa[r], xb[r]);
+        if (e2 < n2) {
+            LzPair<T>::ld(w, e2, xa[r], xb[r]);
+            xa[r] = cscale(xa[r], sj);
+            xb[r] = cscale(xb[r], sj);
+        }
     }
     for (int q = 0; q <= j; ++q) {
         const double2 gq = g[q];
@@ -379,10 +397,7 @@ __global__ __launch_bounds__(256) void lz_update_kernel(cplx<T> *V, int64_t sV, 
         }
     }
     nrm = block_sum(nrm, sh);
-    if (threadIdx.x == 0) {
-        partn[(int64_t)t * nchunk + c] = nrm;
-        if (c == 0) galpha[t] = g[j].x;
-    }
+    if (threadIdx.x == 0) partn[(int64_t)t * nchunk + c] = nrm;
 }
 
 // exp(-i tau T) e_0 for the symmetric tridiagonal T (k <= 64), one lane per
@@ -492,48 +507,22 @@ __device__ double2 tridiag_exp_lane(const double *alpha, const double *beta, int
     return in ? v : make_double2(0, 0);
 }
 
-// lz_finalize: one block per chunk and trajectory.  β = ‖w‖ from the chunk
-// partials and V_{j+1} = w/β on the block's chunk; the block of chunk 0
-// also computes, on its first wavefront, c = exp(-i dt T) e0 and the
-// stopping rule β0·β·|c_j| <= tol (shared with the oracle).  A trajectory
-// that stops keeps nit = j + 1 and leaves the active set; V_{j+1} is then
-// never read, so it does not matter whether the other blocks of that
-// trajectory saw it active and scaled their chunk.  Trajectories that go on
-// are counted in nactive[j].  The B chunk-0 blocks come first in the 1-D
-// grid, so their stopping rules start with the first dispatch wave and
-// overlap the scaling of the other chunks.
-template <typename T>
-__global__ __launch_bounds__(256) void lz_finalize_kernel(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j,
-                                                          int kmax, double dt, double tol, const double *partn,
+// lz_finalize: one wavefront per trajectory.  β = ‖w‖ from the chunk
+// partials (w, stored unnormalised, becomes V_{j+1} with s_{j+1} = 1/β),
+// c = exp(-i dt T) e0 and the stopping rule β0·β·|c_j| <= tol (shared with
+// the oracle).  A trajectory that stops keeps nit = j + 1 and leaves the
+// active set; trajectories that go on are counted in nactive[j].
+__global__ __launch_bounds__(256) void lz_finalize_kernel(int j, int kmax, double dt, double tol, const double *partn,
                                                           int nchunk, const double *galpha, double *alpha, double *beta,
                                                           const double *beta0, c64 *coef, int *nit, int *active,
                                                           int *nactive, int B) {
-    const int64_t L = blockIdx.x;
-    int t, c;
-    if (L < B) {
-        t = (int)L;
-        c = 0;
-    } else {
-        t = (int)((L - B) / (nchunk - 1));
-        c = 1 + (int)((L - B) % (nchunk - 1));
-    }
-    if (!active[t]) return;
     const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (t >= B || !active[t]) return;
     double s = 0;
     for (int i = lane; i < nchunk; i += 64) s += partn[(int64_t)t * nchunk + i];
     s = wave_sum(s);
     const double b = sqrt(s);
-    if (b > 0.0) {
-        const double f = 1.0 / b;
-        cplx<T> *v = V + t * sV + (int64_t)(j + 1) * nmax;
-        const int64_t base = (int64_t)c * LZ_CHUNK;
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const int64_t e = base + threadIdx.x + 256 * r;
-            if (e < n) st2(v[e], cscale(ld2(v[e]), f));
-        }
-    }
-    if (c != 0 || threadIdx.x >= 64) return;
     double *al = alpha + (int64_t)t * kmax;
     double *be = beta + (int64_t)t * (kmax + 1);
     const double aj = galpha[t];
@@ -559,13 +548,14 @@ __global__ __launch_bounds__(256) void lz_finalize_kernel(cplx<T> *V, int64_t sV
     }
 }
 
-// x = β0 Σ_{q<nit} c_q V_q  (trajectories with β0 > 0); the basis is read
+// x = β0 Σ_{q<nit} c_q v_q = β0 Σ c_q s_q V_q  (trajectories with β0 > 0); the basis is read
 // in element pairs with q outside the pairs (per element the sum runs over
 // q ascending); x is written in pairs when its slots allow (xpair).
 template <typename T>
 __global__ __launch_bounds__(256) void lz_combine_kernel(cplx<T> *x, int64_t sx, const cplx<T> *V, int64_t sV,
                                                          int64_t nmax, int64_t n, const double *beta0, const c64 *coef,
-                                                         const int *nit, int kmax, const int *mask, int xpair) {
+                                                         const int *nit, int kmax, const int *mask, int xpair,
+                                                         const double *beta) {
     const int t = blockIdx.y;
     if (mask && !mask[t]) return;
     const double b0 = beta0[t];
@@ -577,8 +567,9 @@ __global__ __launch_bounds__(256) void lz_combine_kernel(cplx<T> *x, int64_t sx,
     double2 xa[4], xb[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) xa[r] = xb[r] = make_double2(0, 0);
+    const double *be = beta + (int64_t)t * (kmax + 1);
     for (int q = 0; q < k; ++q) {
-        const double2 c = make_double2(ct[q].re, ct[q].im);
+        const double2 c = cscale(make_double2(ct[q].re, ct[q].im), lz_scale_of(be, q));
         const cplx<T> *vq = Vt + (int64_t)q * nmax;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -623,30 +614,29 @@ void lz_dots(const cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, c64 *
 }
 template <typename T>
 void lz_update(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, const c64 *part, int kslots, double *partn,
-               double *galpha, const int *active, int B, hipStream_t s) {
+               double *galpha, const int *active, const double *beta, int kmax, int B, hipStream_t s) {
     if ((n | nmax | sV) & 1) throw std::invalid_argument("Lanczos vectors must have even length and offsets");
     const int nchunk = (int)((n + LZ_CHUNK - 1) / LZ_CHUNK);
     hipLaunchKernelGGL(lz_update_kernel<T>, dim3(nchunk, B), dim3(256), 0, s, V, sV, nmax, n, j, part, nchunk, kslots,
-                       partn, galpha, active);
+                       partn, galpha, active, beta, kmax);
     YQ_HIP(hipGetLastError());
 }
-template <typename T>
-void lz_finalize(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, int kmax, double dt, double tol,
-                 const double *partn, const double *galpha, double *alpha, double *beta, const double *beta0, c64 *coef,
-                 int *nit, int *active, int *nactive, int B, hipStream_t s) {
+void lz_finalize(int j, int kmax, double dt, double tol, const double *partn, int64_t n, const double *galpha,
+                 double *alpha, double *beta, const double *beta0, c64 *coef, int *nit, int *active, int *nactive, int B,
+                 hipStream_t s) {
     const int nchunk = (int)((n + LZ_CHUNK - 1) / LZ_CHUNK);
-    hipLaunchKernelGGL(lz_finalize_kernel<T>, dim3((unsigned)((int64_t)nchunk * B)), dim3(256), 0, s, V, sV, nmax, n, j,
-                       kmax, dt, tol, partn, nchunk, galpha, alpha, beta, beta0, coef, nit, active, nactive, B);
+    hipLaunchKernelGGL(lz_finalize_kernel, dim3((B + 3) / 4), dim3(256), 0, s, j, kmax, dt, tol, partn, nchunk, galpha,
+                       alpha, beta, beta0, coef, nit, active, nactive, B);
     YQ_HIP(hipGetLastError());
 }
 template <typename T>
 void lz_combine(cplx<T> *x, int64_t sx, const cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, const double *beta0,
-                const c64 *coef, const int *nit, int kmax, const int *mask, int B, hipStream_t s) {
+                const c64 *coef, const int *nit, const double *beta, int kmax, const int *mask, int B, hipStream_t s) {
     if ((n | nmax | sV) & 1) throw std::invalid_argument("Lanczos vectors must have even length and offsets");
     const int xpair = ((sx & 1) == 0 && ((uintptr_t)x & (2 * sizeof(cplx<T>) - 1)) == 0) ? 1 : 0;
     const int nchunk = (int)((n + LZ_CHUNK - 1) / LZ_CHUNK);
     hipLaunchKernelGGL(lz_combine_kernel<T>, dim3(nchunk, B), dim3(256), 0, s, x, sx, V, sV, nmax, n, beta0, coef, nit,
-                       kmax, mask, xpair);
+                       kmax, mask, xpair, beta);
     YQ_HIP(hipGetLastError());
 }
 
@@ -656,12 +646,9 @@ void lz_combine(cplx<T> *x, int64_t sx, const cplx<T> *V, int64_t sV, int64_t nm
     template void lz_dots<T>(const cplx<T> *, int64_t, int64_t, int64_t, int, c64 *, int, const int *, int,         \
                              hipStream_t);                                                                         \
     template void lz_update<T>(cplx<T> *, int64_t, int64_t, int64_t, int, const c64 *, int, double *, double *,     \
-                               const int *, int, hipStream_t);                                                     \
-    template void lz_finalize<T>(cplx<T> *, int64_t, int64_t, int64_t, int, int, double, double, const double *,     \
-                                 const double *, double *, double *, const double *, c64 *, int *, int *, int *, int, \
-                                 hipStream_t);                                                                     \
+                               const int *, const double *, int, int, hipStream_t);                                \
     template void lz_combine<T>(cplx<T> *, int64_t, const cplx<T> *, int64_t, int64_t, int64_t, const double *,     \
-                                const c64 *, const int *, int, const int *, int, hipStream_t);
+                                const c64 *, const int *, const double *, int, const int *, int, hipStream_t);
 YQ_LZ_INST(float)
 YQ_LZ_INST(double)
 
