@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <map>
+#include <set>
 #include <mutex>
 #include <utility>
 #include <vector>
@@ -15,6 +16,7 @@ namespace yq {
 namespace {
 std::mutex g_mu;
 std::map<std::pair<int, TuneKey>, int> g_cache;  // (device, key) -> variant
+std::set<std::pair<int, TuneKey>> g_busy;         // keys being measured
 int g_enabled = -1;
 bool g_loaded = false;
 thread_local bool g_measuring = false;
@@ -91,10 +93,12 @@ int tuned_variant(const TuneKey &key, int n, const std::function<void(int)> &lau
         load_locked();
         auto it = g_cache.find(ck);
         if (it != g_cache.end()) return it->second;
+        // another thread measures this key: take the default variant for
+        // this launch instead of a second, concurrent measurement
+        if (!g_busy.insert(ck).second) return 0;
     }
-    // Measure outside the lock (another thread may measure the same key;
-    // both arrive at a valid choice).  One untimed run per variant first:
-    // it loads the code object and lets nested sites tune themselves.
+    // Measure outside the lock.  One untimed run per variant first: it
+    // loads the code object and lets nested sites tune themselves.
     hipEvent_t e0, e1;
     YQ_HIP(hipEventCreate(&e0));
     YQ_HIP(hipEventCreate(&e1));
@@ -104,8 +108,10 @@ int tuned_variant(const TuneKey &key, int n, const std::function<void(int)> &lau
     std::vector<float> t_min(n, 0.0f);
     try {
         for (int v = 0; v < n; ++v) launch(v);
-        // two interleaved rounds of two launches per variant, minimum kept
-        for (int round = 0; round < 2; ++round)
+        // three interleaved rounds of two launches per variant, minimum
+        // kept (launches of other streams sharing the device inflate single
+        // samples)
+        for (int round = 0; round < 3; ++round)
             for (int v = 0; v < n; ++v) {
                 YQ_HIP(hipEventRecord(e0, s));
                 for (int r = 0; r < 2; ++r) launch(v);
@@ -123,12 +129,15 @@ int tuned_variant(const TuneKey &key, int n, const std::function<void(int)> &lau
         g_measuring = outer;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_busy.erase(ck);
         throw;
     }
     g_measuring = outer;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     std::lock_guard<std::mutex> lk(g_mu);
+    g_busy.erase(ck);
     if (g_cache.emplace(ck, best).second) append_locked(dev, key, best);
     return best;
 }
