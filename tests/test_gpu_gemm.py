@@ -61,7 +61,7 @@ def test_tile_shapes_bit_identical(prec, tol):
     lib.yaqs_debug_gemm_tile.argtypes = [ct.c_int] * 6 + [c_dp, c_dp, c_dp, ct.c_int]
     rng = np.random.default_rng(11)
     op = {0: lambda X: X, 1: lambda X: X.T, 2: lambda X: X.conj().T}
-    # tile 4 (k tiles loaded straight into LDS) applies to complex-FP32,
+    # tiles 4 and 5 (k tiles loaded straight into LDS) apply to complex-FP32,
     # op(A) = N and whole 64 x 64 tiles: (320, 256, 64), (64, 128, 48),
     # (128, 64, 320); elsewhere it runs as tile 0
     for opA in range(3):
@@ -76,7 +76,7 @@ def test_tile_shapes_bit_identical(prec, tol):
             ref = op[opA](A) @ B
             got = base.view(np.complex128).reshape(M, N)
             assert np.abs(got - ref).max() / max(1.0, np.abs(ref).max()) < tol
-            for tile in range(1, 5):
+            for tile in range(1, 6):
                 other = _tile_run(lib, prec, opA, M, N, K, tile, a, b)
                 assert np.array_equal(other, base), (opA, M, N, K, tile)
 

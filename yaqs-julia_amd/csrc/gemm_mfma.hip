@@ -525,10 +525,10 @@ static int mfma_mode() {
     return mode;
 }
 
-template <typename T, int OA, int OB, int BM, int BN, bool DL = false>
+template <typename T, int OA, int OB, int BM, int BN, bool DL = false, int BK = 16>
 static void launch_tile(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s) {
     const int tiles = ((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
-    hipLaunchKernelGGL((gemm_mfma_kernel<T, OA, OB, BM, BN, 16, false, DL>), dim3(gemm_blocks(tiles, d.inner, d.outer)),
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, OA, OB, BM, BN, BK, false, DL>), dim3(gemm_blocks(tiles, d.inner, d.outer)),
                        dim3(256), 0, s, d, A, B, C, WFuse{}, tiles);
     YQ_HIP(hipGetLastError());
 }
@@ -553,8 +553,12 @@ static void launch_shape(int tile, const GemmDesc &d, const cplx<T> *A, const cp
             case 2: return launch_tile<T, OA, OB, 64, 32>(d, A, B, C, s);
             case 3: return launch_tile<T, OA, OB, 32, 32>(d, A, B, C, s);
             case 4:
+            case 5:
                 if constexpr (std::is_same<T, float>::value && OA == OP_N) {
-                    if (dl_fits<T>(d, A, B, 64)) return launch_tile<T, OA, OB, 64, 64, true>(d, A, B, C, s);
+                    if (dl_fits<T>(d, A, B, 64)) {
+                        if (tile == 4) return launch_tile<T, OA, OB, 64, 64, true>(d, A, B, C, s);
+                        return launch_tile<T, OA, OB, 64, 64, true, 8>(d, A, B, C, s);
+                    }
                 }
                 break;
             default: break;

@@ -13,12 +13,13 @@ void gemm(const GemmDesc &d, const cplx<TA> *A, const cplx<TB> *B, cplx<TC> *C, 
 // false when the descriptor is outside what the kernel covers.  tile picks
 // the kernel variant (all give bit-identical results): 0-3 the C tile per
 // workgroup (GEMM_TILE_BM/BN) with register-staged k tiles, for op(A) in
-// {N, T, C} with op(B) = N (other ops take tile 0); 4 the 64 x 64 tile with
-// k tiles loaded straight into LDS (complex-FP32, op N operands, whole
-// tiles, 16-byte aligned operands; tile 0 otherwise).
-constexpr int GEMM_TILES = 5;
-constexpr int GEMM_TILE_BM[GEMM_TILES] = {64, 32, 64, 32, 64};
-constexpr int GEMM_TILE_BN[GEMM_TILES] = {64, 64, 32, 32, 64};
+// {N, T, C} with op(B) = N (other ops take tile 0); 4 and 5 the 64 x 64
+// tile with k tiles loaded straight into LDS, three stages of 16 k or four
+// of 8 k (complex-FP32, op N operands, whole tiles, 16-byte aligned
+// operands; tile 0 otherwise).
+constexpr int GEMM_TILES = 6;
+constexpr int GEMM_TILE_BM[GEMM_TILES] = {64, 32, 64, 32, 64, 64};
+constexpr int GEMM_TILE_BN[GEMM_TILES] = {64, 64, 32, 32, 64, 64};
 template <typename T>
 bool gemm_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s, int tile = 0);
 // True when tile 4 (k tiles straight into LDS) applies to the descriptor.
