@@ -129,3 +129,28 @@ def test_concurrent_waves_match_one_wave():
     assert np.abs(split[0] - one[0]).max() < 1e-10, np.abs(split[0] - one[0]).max()
     assert np.abs(split[1] - one[1]).max() < 1e-10
     assert split[3]["jumps"] == one[3]["jumps"]
+
+
+def test_digital_concurrent_waves_match_one_wave():
+    """The same for a digital job (512 trajectories: two concurrent waves):
+    results, bond dimensions and jumps equal the one-wave run."""
+    print("case: digital 512 trajectories as concurrent waves vs one wave", flush=True)
+    L = 6
+    ntraj = 512
+    circ = Y.haar_brickwork(L, 4, seed=5)
+    obs = [Y.Observable(f"z{i}", Y.gates.ZGate(), i) for i in range(1, L + 1)]
+    cfg = Y.StrongMeasurementConfig(obs, num_traj=ntraj, max_bond_dim=8, truncation_threshold=1e-9)
+    noise = Y.NoiseModel([{"name": n, "sites": [i], "strength": 0.02} for i in range(1, L + 1)
+                          for n in ("pauli_x", "pauli_z")], L)
+    out = []
+    for mb in (0, ntraj):
+        ctx = Y.Context(0, max_batch=mb)
+        h = _abi.pack_digital(Y.MPS(L), circ, cfg, noise, precision=_abi.PREC_FP64, seed=13)
+        ncol = int(ctx.lib.yaqs_digital_num_samples(h.job))
+        out.append(ctx.wait(ctx.submit_digital(h), len(obs), ncol, ntraj, want_bonds=True))
+        ctx.close()
+    split, one = out
+    assert np.abs(split[0] - one[0]).max() < 1e-10, np.abs(split[0] - one[0]).max()
+    assert np.abs(split[1] - one[1]).max() < 1e-10
+    assert np.array_equal(split[2], one[2])
+    assert split[3]["jumps"] == one[3]["jumps"]
