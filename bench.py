@@ -79,7 +79,9 @@ def gpu_window(Y, lib, ctx, ntraj, warmup, steps, precision):
                                mean.ctypes.data_as(Y._abi.c_dp), ct.byref(rep))
     if rc != 0:
         raise RuntimeError(f"yaqs_bench_analog failed ({rc}): {lib.yaqs_last_error().decode()}")
-    return rep.as_dict(), mean.reshape(nobs, ncol)
+    out = rep.as_dict()
+    out["traj"] = traj.reshape(nobs, ncol, ntraj)
+    return out, mean.reshape(nobs, ncol)
 
 
 def cpu_window(Y, ntraj, warmup, steps, nthreads):
@@ -343,6 +345,14 @@ def main():
         out["plain_fp64_gpu"] = {"value": pv, "unit": "traj-site-steps/s", "ntraj": args.plain_traj,
                                  "batch": 1, "speedup_of_batched": value / pv}
         print(f"[bench] plain FP64 per-trajectory GPU: {pv:.4g} traj-site-steps/s", flush=True)
+        # the metric's <Z> error: the same trajectories (same seeds) of the
+        # timed job against the FP64 path, every <Z_i> at every time step
+        k = min(args.plain_traj, args.ntraj)
+        dz = np.abs(rep["traj"][:L, :, :k] - prep["traj"][:L, :, :k])
+        out["accuracy"] = {"max_abs_err_Z": float(dz.max()), "trajectories": k, "tolerance": 1e-4,
+                           "vs": "plain FP64 GPU path (FP64 oracle parity 1e-10 in tests), same seeds, "
+                                 "every <Z_i> at every time step of trajectories 0..%d" % (k - 1)}
+        print(f"[bench] mixed vs FP64, <Z_i> of {k} trajectories: max |err| {dz.max():.2e}", flush=True)
     if args.serve_jobs > 0:
         print(f"[bench] serving stream, {args.serve_jobs} jobs", flush=True)
         out["serving"] = serve_stream(Y, args.serve_jobs)
