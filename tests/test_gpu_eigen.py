@@ -140,8 +140,10 @@ def _check(G, tol=1e-12):
     assert np.abs(G @ U - U * ev[:k]).max() < tol * scale * n
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 8, 33, 64, 100, 128, 160, 256])
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 33, 64, 100, 128, 160, 256, 512, 600, 1024])
 def test_random_gram(n):
+    # 512: the widest of the wide path (chi = 256 splits); 600 and 1024 (the
+    # chi = 512 splits of the large Ising chain): the single-workgroup solver
     print(f"case: heev n={n}", flush=True)
     rng = np.random.default_rng(n)
     X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
