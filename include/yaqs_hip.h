@@ -228,7 +228,7 @@ int64_t yaqs_job_footprint_bytes(const yaqs_analog_job *job);
  * warm-up and reports the stream-synchronised wall time of the remaining
  * steps, plus HIP-event timing of every matrix-core GEMM launched inside
  * that window.  The waves are the ones the context would run: a job that
- * fits one wave is split into up to three concurrent waves of at least 256
+ * fits one wave is split into up to three concurrent waves of at least 128
  * trajectories (environment variable YAQS_WAVES, 1..8; 1 = one wave), run
  * side by side on their own streams with windows that start together (the
  * window is the longest); waves limited by the HBM budget run one after the

@@ -206,18 +206,19 @@ struct yaqs_ctx {
 
 namespace {
 
-// Concurrent waves for a job whose trajectories fit one wave: at least 256
+// Concurrent waves for a job whose trajectories fit one wave: at least 128
 // trajectories a wave, YAQS_WAVES (1..8) of them, three by default.  The
 // latency-bound phases of one wave (Lanczos tails, host round trips, the
 // eigensolver) overlap the matrix-core phases of the others (C2, 1000
 // trajectories: 14.7k traj-site-steps/s as three waves, 14.5k as two, 14.3k
-// as one or four).
+// as one or four; the digital brickwork, 512 trajectories: 49.8k
+// traj-site-layers/s as three waves, 47.3k as two, 42.6k as one).
 int concurrent_waves(int64_t ntraj) {
     static const int k = [] {
         const char *e = std::getenv("YAQS_WAVES");
         return e ? std::max(1, std::min(8, atoi(e))) : 3;
     }();
-    return (int)std::max<int64_t>(1, std::min<int64_t>(k, ntraj / 256));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(k, ntraj / 128));
 }
 
 // Batch size and bytes for a job under the remaining budget; 0 if it cannot
