@@ -151,3 +151,42 @@ def test_batch_invariance(ctx, oracle_lib):
     a, _, _ = _gpu(ctx, Y.MPS(L), H, cfg10, noise, seed=8)
     b, _, _ = _gpu(ctx, Y.MPS(L), H, cfg20, noise, seed=8)
     assert np.abs(a - b[:, :, :10]).max() < 1e-10
+
+
+@pytest.mark.parametrize("unravel,kw", [("projector", {}), ("unitary_2pt", {"theta0": 0.6}),
+                                        ("unitary_gauss", {"sigma": 0.5, "gauss_M": 7})])
+def test_unraveled_dephasing_parity(ctx, oracle_lib, unravel, kw):
+    """Projector, two-point and Gaussian unitary unravelings of pauli_z
+    dephasing (src/Noise.jl:134-252): 1-site processes with explicit
+    matrices, non-Pauli names and (for the unitary ones) norm-preserving
+    jumps.  Per-trajectory FP64 parity with the oracle, Ising L=6 from |++..>,
+    lowering gamma=0.05 standard, dephasing gamma=0.2 unraveled."""
+    print(f"case: unraveled dephasing {unravel} L=6 16 traj fp64", flush=True)
+    L = 6
+    procs = [{"name": "lowering", "sites": [i], "strength": 0.05} for i in range(1, L + 1)]
+    procs += [{"name": "pauli_z", "sites": [i], "strength": 0.2, "unraveling": unravel} for i in range(1, L + 1)]
+    noise = Y.NoiseModel(procs, L, **kw)
+    cfg = Y.TimeEvolutionConfig(_zx(L), 1.0, dt=0.05, num_traj=16, max_bond_dim=16, truncation_threshold=1e-10)
+    H = Y.init_ising(L, 1.0, 0.5)
+    g, gm, st = _gpu(ctx, Y.MPS(L, state="x+"), H, cfg, noise, seed=4)
+    r, rm, rst = _oracle(oracle_lib, Y.MPS(L, state="x+"), H, cfg, noise, seed=4)
+    assert st["jumps"] == rst["jumps"] and st["jumps"] > 0
+    if unravel == "projector":
+        # (I -+ Z)/2 projections make bonds exactly rank-deficient
+        assert np.abs(g - r).max() < TOL_RANK_DEFICIENT
+        assert np.median(np.abs(g - r)) < TOL_FP64
+    else:
+        assert np.abs(g - r).max() < TOL_FP64
+
+
+def test_unraveled_crosstalk_is_rejected(ctx):
+    """Unravelings of adjacent 2-site processes give non-product jump
+    operators (I + X(x)X, exp(i theta X(x)X)); the engine applies 2-site jumps
+    as two 1-site factors, so job submission fails with a clear error."""
+    print("case: unraveled crosstalk rejected", flush=True)
+    L = 4
+    cfg = Y.TimeEvolutionConfig(_zx(L), 0.2, dt=0.05, num_traj=2, max_bond_dim=8)
+    for unravel in ("projector", "unitary_2pt"):
+        noise = Y.NoiseModel([{"name": "crosstalk_xx", "sites": [2, 3], "strength": 0.1, "unraveling": unravel}], L)
+        with pytest.raises(Y.YaqsError, match="2-site non-Pauli"):
+            _gpu(ctx, Y.MPS(L), Y.init_ising(L, 1.0, 0.5), cfg, noise)

@@ -74,3 +74,21 @@ def test_digital_mixed_precision(ctx, oracle_lib):
                                          precision=_abi.PREC_MIXED, seed=1)
     if st["jumps"] == rst["jumps"]:
         assert np.abs(gm - rm).max() < 1e-4
+
+
+def test_brickwork_unraveled_noise_parity(ctx, oracle_lib):
+    """Two-point unitary unraveling of dephasing and the projector
+    unraveling of bit flips (src/Noise.jl:134-193) in the digital path."""
+    print("case: digital unraveled noise L=8 depth 8, 16 traj", flush=True)
+    L = 8
+    circ = Y.haar_brickwork(L, 8, seed=6)
+    obs = [Y.Observable(f"z{i}", Y.gates.ZGate(), i) for i in range(1, L + 1)]
+    cfg = Y.StrongMeasurementConfig(obs, num_traj=16, max_bond_dim=16, truncation_threshold=1e-9)
+    procs = [{"name": "pauli_z", "sites": [i], "strength": 0.03, "unraveling": "unitary_2pt", "theta0": 0.7}
+             for i in range(1, L + 1)]
+    procs += [{"name": "pauli_x", "sites": [i], "strength": 0.02, "unraveling": "projector"} for i in range(1, L + 1)]
+    noise = Y.NoiseModel(procs, L)
+    g, gm, gb, st, r, rm, rb, rst = _run(ctx, oracle_lib, L, circ, cfg, noise, seed=8)
+    assert st["jumps"] == rst["jumps"] and st["jumps"] > 0
+    assert np.abs(g - r).max() < 1e-9
+    assert np.array_equal(gb, rb)
