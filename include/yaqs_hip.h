@@ -132,15 +132,38 @@ typedef enum {
     YAQS_GATE_SAMPLE = 3
 } yaqs_gate_kind;
 
+/* How a 2-qubit gate is applied (apply_window!, src/DigitalTJM.jl:307-358).
+ *
+ * YAQS_GATE_EXACT: TJMOptions local_method = :TEBD.  `matrix` is the gate;
+ *   apply_local_gate_exact! (src/DigitalTJM.jl:251-305): Θ' = G Θ, SVD,
+ *   drop singular values from the tail while the discarded weight stays
+ *   <= threshold, at least 1 and at most max_bond_dim kept.
+ * YAQS_GATE_WINDOW_TDVP: local_method = :TDVP (the reference default,
+ *   src/DigitalTJM.jl:76).  `matrix` is the window Hamiltonian
+ *   h = coeff * g_a (x) g_b of construct_window_mpo (src/DigitalTJM.jl:192-236,
+ *   g_a on the lower site, in the same index order as gate matrices); the
+ *   window of an adjacent pair is the pair itself, so the circuit sweep is
+ *   its edge step alone (src/Algorithms.jl:606-627, 762-800): Θ' =
+ *   exp(-i h) Θ (the Krylov space of h on Θ closes after at most four
+ *   vectors, so the engine applies exp(-i h) directly), then
+ *   split_mps_tensor_svd (src/Algorithms.jl:409-445): keep max(k, 2) where
+ *   k is the first tail index at which the discarded weight reaches the
+ *   threshold, at most max_bond_dim. */
+typedef enum {
+    YAQS_GATE_EXACT = 0,
+    YAQS_GATE_WINDOW_TDVP = 1
+} yaqs_gate_method;
+
 typedef struct {
     int32_t kind;
-    int32_t _pad;
-    int64_t site0, site1;    /* 1-based; for 2Q gates sites must be adjacent (TEBD) */
+    int32_t method;          /* yaqs_gate_method; 2Q gates only, 0 otherwise */
+    int64_t site0, site1;    /* 1-based; for 2Q gates sites must be adjacent */
     double matrix[32];
 } yaqs_gate;
 
-/* Digital TJM job: run_digital_tjm with TJMOptions(local_method=:TEBD)
- * (src/DigitalTJM.jl:363-450). */
+/* Digital TJM job: run_digital_tjm (src/DigitalTJM.jl:363-450) with the
+ * local method chosen per 2-qubit gate (yaqs_gate_method); long-range
+ * gates (SWAP networks or long windows) are not supported. */
 typedef struct {
     int64_t L;
     const yaqs_mps_tensor *mps;

@@ -14,7 +14,8 @@
 //   Dissipation        src/Dissipation.jl:14-159
 //   Jumps              src/StochasticProcess.jl:35-330
 //   Simulator loop     src/Simulator.jl:22-125
-//   DigitalTJM (TEBD)  src/DigitalTJM.jl:86-305, 363-450
+//   DigitalTJM         src/DigitalTJM.jl:86-305, 335-358, 363-450 (TEBD and
+//                      TDVP windows of adjacent pairs)
 // The dense kernels the reference takes from LAPACK/KrylovKit are written
 // here from their textbook definitions: one-sided (Hestenes) Jacobi SVD,
 // Householder QR/LQ, Lanczos with full re-orthogonalisation, and a cyclic
@@ -1219,6 +1220,7 @@ void run_trajectory(const yaqs_analog_job *job, const MPSState &psi0, const std:
 // ----------------------------------------------------------- digital ------
 struct Gate {
     int kind;
+    int method = 0;  // yaqs_gate_method: the matrix is the gate (0) or the window Hamiltonian (1)
     int s0, s1;
     cd m[16];
 };
@@ -1291,6 +1293,38 @@ void tebd_gate(MPSState &s, const cd *op, int s1, int s2, double thr, int max_bo
     s.oc = s2;
 }
 
+// apply_window! with local_method :TDVP on an adjacent pair (a, a+1)
+// (src/DigitalTJM.jl:335-358).  With padding 0 the window is the pair: the
+// centre moves to a, the window MPO is coeff g_a (x) g_b with 1x1 bonds
+// (construct_window_mpo, :192-236; `h` here is that 4x4 operator in the
+// gate-matrix index order) and the window state keeps the real outer bonds,
+// whose environments are identities (_init_envs, src/Algorithms.jl:801-818).
+// _tdvp_sweep_circuit_2site! on two sites is its edge step alone
+// (src/Algorithms.jl:606-627, 762-800): Krylov exp(-i * 1.0 * H_eff) Θ with
+// krylovdim 25, then split_mps_tensor_svd; A_a = U, A_b = S V†, centre b.
+void window_tdvp_gate(MPSState &s, const cd *h, int a, int b, double thr, int max_bond) {
+    shift_oc(s, a);
+    const T3 &A = s.t[a - 1], &B = s.t[b - 1];
+    const int l = A.dl, r = B.dr;
+    cvec th = merge2(A, B);
+    auto heff = [&](const cvec &x) { return apply_op2(h, x, l, r); };
+    cvec thn = expm_krylov(heff, th, 1.0, 25, 1e-10);
+    Cfg cfg;
+    cfg.max_bond = max_bond;
+    cfg.thr = thr;
+    cvec U, Vt;
+    std::vector<double> S;
+    int keep = 0;
+    split_svd(thn, l, 2, 2, r, cfg, U, S, Vt, keep);
+    T3 nA(l, 2, keep), nB(keep, 2, r);
+    for (size_t q = 0; q < (size_t)l * 2 * keep; ++q) nA.v[q] = U[q];
+    for (int j = 0; j < 2 * r; ++j)
+        for (int q = 0; q < keep; ++q) nB.v[q + (size_t)keep * j] = S[q] * Vt[q + (size_t)keep * j];
+    s.t[a - 1] = nA;
+    s.t[b - 1] = nB;
+    s.oc = b;
+}
+
 void run_digital_trajectory(const yaqs_digital_job *job, const MPSState &psi0, const std::vector<Gate> &gates,
                             const std::vector<std::vector<int>> &layers, const std::vector<int> &sample_after,
                             bool sample_at_start, int nsamp, const std::vector<Proc> *noise,
@@ -1318,7 +1352,8 @@ void run_digital_trajectory(const yaqs_digital_job *job, const MPSState &psi0, c
             // apply_window! sorts the sites and applies the matrix unchanged
             // on (min, max) (src/DigitalTJM.jl:307-334)
             int a = std::min(G.s0, G.s1), b = std::max(G.s0, G.s1);
-            tebd_gate(st, G.m, a, b, thr, mb);
+            if (G.method == YAQS_GATE_WINDOW_TDVP) window_tdvp_gate(st, G.m, a, b, thr, mb);
+            else tebd_gate(st, G.m, a, b, thr, mb);
             if (noise && !noise->empty()) {
                 std::vector<Proc> local;  // create_local_noise_model (:172-188)
                 for (const Proc &p : *noise) {
@@ -1454,6 +1489,11 @@ int oracle_run_digital(const yaqs_digital_job *job, double *traj_out, double *me
     for (int64_t g = 0; g < job->n_gates; ++g) {
         const yaqs_gate &G = job->gates[g];
         gates[g].kind = G.kind;
+        gates[g].method = G.method;
+        if (G.method != YAQS_GATE_EXACT && (G.method != YAQS_GATE_WINDOW_TDVP || G.kind != YAQS_GATE_2Q)) {
+            g_err = "bad gate method";
+            return YAQS_EINVAL;
+        }
         gates[g].s0 = (int)G.site0;
         gates[g].s1 = (int)G.site1;
         if (G.kind == YAQS_GATE_1Q) load_c(G.matrix, gates[g].m, 4);

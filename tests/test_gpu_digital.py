@@ -92,3 +92,51 @@ def test_brickwork_unraveled_noise_parity(ctx, oracle_lib):
     assert st["jumps"] == rst["jumps"] and st["jumps"] > 0
     assert np.abs(g - r).max() < 1e-9
     assert np.array_equal(gb, rb)
+
+
+def _generator_circuit(L, depth, seed):
+    """Layers of Rxx/Ryy/Rzz/CZ/CPhase/CX on alternating bonds (random site
+    order) between single-qubit rotations."""
+    G = Y.gates
+    rng = np.random.default_rng(seed)
+    c = Y.DigitalCircuit(L)
+    for q in range(1, L + 1):
+        c.add_gate(G.RyGate(rng.uniform(0, np.pi)), [q])
+    for layer in range(depth):
+        for a in range(1 + layer % 2, L, 2):
+            th = rng.uniform(-np.pi, np.pi)
+            op = [G.RxxGate(th), G.RyyGate(th), G.RzzGate(th), G.CZGate(), G.CPhaseGate(th),
+                  G.CXGate()][rng.integers(6)]
+            c.add_gate(op, [a, a + 1] if rng.random() < 0.5 else [a + 1, a])
+        for q in range(1, L + 1):
+            c.add_gate(G.RzGate(rng.uniform(-1, 1)), [q])
+    return c
+
+
+def _run_opts(ctx, oracle_lib, L, circ, cfg, noise, opts, seed=0):
+    hg = _abi.pack_digital(Y.MPS(L), circ, cfg, noise, seed=seed, alg_options=opts)
+    ncol = int(ctx.lib.yaqs_digital_num_samples(hg.job))
+    jid = ctx.submit_digital(hg)
+    g, gm, gb, st = ctx.wait(jid, len(cfg.observables), ncol, cfg.num_traj, want_bonds=True)
+    hr = _abi.pack_digital(Y.MPS(L), circ, cfg, noise, seed=seed, alg_options=opts)
+    r, rm, rb, rst = oracle_lib.run_digital(hr, nthreads=8)
+    return g, gm, gb, st, r, rm, rb, rst
+
+
+@pytest.mark.parametrize("noisy", [False, True])
+def test_window_tdvp_gates_parity(ctx, oracle_lib, noisy):
+    """local_method :TDVP (the reference default): 2-qubit gates as
+    exp(-i h) of their window Hamiltonian with the TDVP split (keep max(k, 2)),
+    against the oracle's Krylov window step, per trajectory."""
+    print(f"case: digital TDVP windows L=8 depth 8 noisy={noisy}", flush=True)
+    L = 8
+    circ = _generator_circuit(L, 8, seed=12)
+    obs = [Y.Observable(f"z{i}", Y.gates.ZGate(), i) for i in range(1, L + 1)]
+    cfg = Y.StrongMeasurementConfig(obs, num_traj=16 if noisy else 1, max_bond_dim=16, truncation_threshold=1e-9)
+    noise = _depolarizing(L, 0.05) if noisy else None
+    g, gm, gb, st, r, rm, rb, rst = _run_opts(ctx, oracle_lib, L, circ, cfg, noise, Y.TJMOptions(), seed=3)
+    assert st["jumps"] == rst["jumps"]
+    if noisy:
+        assert st["jumps"] > 0
+    assert np.abs(g - r).max() < 1e-9
+    assert np.array_equal(gb, rb)

@@ -1,0 +1,164 @@
+"""The oracle's DigitalTJM with local_method :TDVP (the reference default,
+src/DigitalTJM.jl:76, 335-358): a 2-qubit gate on an adjacent pair is the
+edge step of the circuit sweep on a two-site window, Krylov exp(-i h) of the
+window Hamiltonian h = coeff g_a (x) g_b, then the TDVP split (keep max(k, 2)).
+
+Pinned by:
+- the reference's own digital tests, which run with the default options
+  (test/test_DigitalTJM.jl:50-90: H, Rzz(pi/2) gives <Z1> = 0, <Z2> = 1;
+  H H Rzz(pi/2) gives <Z1 Z2> = 0);
+- the dense state vector of circuits of generator gates without truncation,
+  each gate applied as the unitary it denotes (Rxx/Ryy/Rzz/CZ/CPhase are
+  symmetric in their sites; CX evolved under pi/4 (I-Z) (x) (I-X) controls on
+  the site listed first — in the TEBD path the same CX matrix controls on
+  the higher site, a difference of the reference reproduced here);
+- the split rule: the TDVP split keeps at least two singular values where
+  TEBD keeps one (bond dimensions of a product state under Rzz(0)).
+CPU only."""
+import numpy as np
+import pytest
+
+import yaqs_julia_amd as Y
+from yaqs_julia_amd import _abi
+
+G = Y.gates
+I2 = np.eye(2, dtype=complex)
+X = np.array([[0, 1], [1, 0]], complex)
+Z = np.diag([1.0, -1.0]).astype(complex)
+P0, P1 = np.diag([1.0, 0]).astype(complex), np.diag([0, 1.0]).astype(complex)
+
+
+def _oracle(oracle_lib, L, circ, obs, opts, noise=None, ntraj=1, thr=1e-12, chi=64, seed=0, mps=None):
+    cfg = Y.StrongMeasurementConfig(obs, num_traj=ntraj, max_bond_dim=chi, truncation_threshold=thr)
+    h = _abi.pack_digital(mps or Y.MPS(L), circ, cfg, noise, seed=seed, alg_options=opts)
+    return oracle_lib.run_digital(h, nthreads=4)
+
+
+def test_reference_closed_forms_with_default_options(oracle_lib):
+    opts = Y.TJMOptions()
+    assert opts.local_method == "TDVP" and opts.long_range_method == "TDVP"
+    c = Y.DigitalCircuit(2)
+    c.add_gate(G.HGate(), [1])
+    c.add_gate(G.RzzGate(np.pi / 2), [1, 2])
+    obs = [Y.Observable("z1", G.ZGate(), 1), Y.Observable("z2", G.ZGate(), 2)]
+    _, m, _, _ = _oracle(oracle_lib, 2, c, obs, opts)
+    assert abs(m[0, -1]) < 1e-10 and abs(m[1, -1] - 1.0) < 1e-10
+    c = Y.DigitalCircuit(2)
+    c.add_gate(G.HGate(), [1])
+    c.add_gate(G.HGate(), [2])
+    c.add_gate(G.RzzGate(np.pi / 2), [1, 2])
+    _, m, _, _ = _oracle(oracle_lib, 2, c, [Y.Observable("zz", G.MatrixGate(np.kron(Z, Z)), [1, 2])], opts)
+    assert abs(m[0, -1]) < 1e-10
+
+
+@pytest.mark.parametrize("sites,z", [([1, 2], (-1.0, -1.0)), ([2, 1], (-1.0, 1.0))])
+def test_cx_controls_on_first_listed_site(oracle_lib, sites, z):
+    """X on site 1, then CX: through the TDVP window the control is
+    gate.sites[0] (src/DigitalTJM.jl:209-222 maps generator element 1 to it);
+    through TEBD the reference's CX matrix (src/GateLibrary.jl:144-150, pair
+    index s_a + 2 s_b) flips s_a when s_b = 1, for either site order."""
+    c = Y.DigitalCircuit(2)
+    c.add_gate(G.XGate(), [1])
+    c.add_gate(G.CXGate(), sites)
+    obs = [Y.Observable("z1", G.ZGate(), 1), Y.Observable("z2", G.ZGate(), 2)]
+    _, m, _, _ = _oracle(oracle_lib, 2, c, obs, Y.TJMOptions())
+    assert abs(m[0, -1] - z[0]) < 1e-10 and abs(m[1, -1] - z[1]) < 1e-10
+    _, mt, _, _ = _oracle(oracle_lib, 2, c, obs, Y.TJMOptions(local_method="TEBD"))
+    assert abs(mt[0, -1] + 1.0) < 1e-10 and abs(mt[1, -1] - 1.0) < 1e-10
+
+
+def _pair_unitary(g):
+    """The unitary a generator gate denotes, on the pair index s_a + 2 s_b
+    (s_a the lower site), from its definition."""
+    op = g.op
+    if isinstance(op, G.CXGate):
+        ctrl_low = g.sites[0] < g.sites[1]
+        # control c, target t: |0><0|_c + |1><1|_c X_t
+        if ctrl_low:
+            return np.kron(I2, P0) + np.kron(X, P1)
+        return np.kron(P0, I2) + np.kron(P1, X)
+    return op.matrix  # Rxx, Ryy, Rzz, CZ, CPhase: symmetric under the site swap
+
+
+def random_generator_circuit(L, depth, seed):
+    rng = np.random.default_rng(seed)
+    c = Y.DigitalCircuit(L)
+    for q in range(1, L + 1):
+        c.add_gate(G.RyGate(rng.uniform(0, np.pi)), [q])
+    for layer in range(depth):
+        for a in range(1 + layer % 2, L, 2):
+            kind = rng.integers(6)
+            th = rng.uniform(-np.pi, np.pi)
+            op = [G.RxxGate(th), G.RyyGate(th), G.RzzGate(th), G.CZGate(), G.CPhaseGate(th), G.CXGate()][kind]
+            sites = [a, a + 1] if rng.random() < 0.5 else [a + 1, a]
+            c.add_gate(op, sites)
+        for q in range(1, L + 1):
+            c.add_gate(G.RzGate(rng.uniform(-1, 1)), [q])
+    return c
+
+
+def _dense(L, circ):
+    psi = np.zeros(2 ** L, complex)
+    psi[0] = 1.0
+    for g in circ.gates:
+        if not g.sites:
+            continue
+        if len(g.sites) == 1:
+            a = g.sites[0] - 1
+            t = psi.reshape(2 ** a, 2, 2 ** (L - a - 1))
+            psi = np.einsum("pq,lqr->lpr", g.op.matrix, t).reshape(-1)
+        else:
+            U = _pair_unitary(g)
+            a = min(g.sites) - 1
+            t = psi.reshape(2 ** a, 2, 2, 2 ** (L - a - 2)).transpose(0, 2, 1, 3).reshape(2 ** a, 4, -1)
+            t = np.einsum("pq,lqr->lpr", U, t)
+            psi = t.reshape(2 ** a, 2, 2, -1).transpose(0, 2, 1, 3).reshape(-1)
+    return psi
+
+
+@pytest.mark.parametrize("L,depth,seed", [(4, 5, 1), (6, 6, 2), (7, 4, 3)])
+def test_generator_circuits_match_statevector(oracle_lib, L, depth, seed):
+    circ = random_generator_circuit(L, depth, seed)
+    obs = [Y.Observable(f"z{i}", G.ZGate(), i) for i in range(1, L + 1)] + \
+          [Y.Observable(f"x{i}", G.XGate(), i) for i in range(1, L + 1)]
+    _, m, _, _ = _oracle(oracle_lib, L, circ, obs, Y.TJMOptions(), thr=0.0, chi=2 ** (L // 2))
+    psi = _dense(L, circ)
+    for i in range(L):
+        for k, P in ((0, Z), (1, X)):
+            t = psi.reshape(2 ** i, 2, -1)
+            ref = np.einsum("lpr,pq,lqr->", t.conj(), P, t).real
+            assert abs(m[k * L + i, -1] - ref) < 1e-10, (i, k, m[k * L + i, -1], ref)
+
+
+def test_tdvp_split_keeps_two(oracle_lib):
+    """Rzz(0) on |0000>: TEBD drops the zero singular value (bond 1), the TDVP
+    split keeps max(k, 2) = 2 (src/Algorithms.jl:426-438)."""
+    L = 4
+    c = Y.DigitalCircuit(L)
+    c.add_gate(G.RzzGate(0.0), [2, 3])
+    c.add_sample_barrier()
+    obs = [Y.Observable("z1", G.ZGate(), 1)]
+    _, _, b_tdvp, _ = _oracle(oracle_lib, L, c, obs, Y.TJMOptions(), thr=1e-9)
+    _, _, b_tebd, _ = _oracle(oracle_lib, L, c, obs, Y.TJMOptions(local_method="TEBD"), thr=1e-9)
+    assert b_tdvp.max() == 2 and b_tebd.max() == 1
+
+
+def test_options_and_errors():
+    with pytest.raises(ValueError):
+        Y.TJMOptions(local_method="foo")
+    assert Y.TJMOptions(local_method=":TEBD").local_method == "TEBD"
+    c = Y.DigitalCircuit(2)
+    c.add_gate(G.SWAPGate(), [1, 2])  # no generator in the reference's GateLibrary
+    cfg = Y.StrongMeasurementConfig([Y.Observable("z1", G.ZGate(), 1)], num_traj=1)
+    with pytest.raises(NotImplementedError):
+        _abi.pack_digital(Y.MPS(2), c, cfg, None, alg_options=Y.TJMOptions())
+    _abi.pack_digital(Y.MPS(2), c, cfg, None, alg_options=Y.TJMOptions(local_method="TEBD"))
+    # an explicit generator overrides the gate's (add_gate!(...; generator=...))
+    c2 = Y.DigitalCircuit(2)
+    c2.add_gate(G.SWAPGate(), [2, 1], generator=[X, Z])
+    h = _abi.pack_digital(Y.MPS(2), c2, cfg, None, alg_options=Y.TJMOptions())
+    g = h.job.gates[0]
+    assert g.method == _abi.GATE_WINDOW_TDVP
+    m = np.array(g.matrix[:32]).view(complex).reshape(4, 4, order="F")
+    # sites [2, 1]: element 1 (X) belongs to site 2, the high factor; coeff 1.0
+    assert np.allclose(m, np.kron(X, Z))

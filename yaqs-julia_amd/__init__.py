@@ -6,14 +6,15 @@ the HIP C-ABI library `build/libyaqs_hip.so`; this package only builds job
 descriptions and reads results back.
 """
 from . import gates
-from .circuit import DigitalCircuit, haar_brickwork
+from .circuit import DigitalCircuit, TJMOptions, haar_brickwork
 from .configs import Observable, StrongMeasurementConfig, TimeEvolutionConfig
 from .networks import MPO, MPS, init_general_hamiltonian, init_ising
 from .noise import LocalNoiseProcess, NoiseModel, relaxation_dephasing
 from .simulator import Context, YaqsError, run
 
 __all__ = [
-    "gates", "DigitalCircuit", "haar_brickwork", "Observable", "StrongMeasurementConfig", "TimeEvolutionConfig",
+    "gates", "DigitalCircuit", "TJMOptions", "haar_brickwork", "Observable", "StrongMeasurementConfig",
+    "TimeEvolutionConfig",
     "MPO", "MPS", "init_general_hamiltonian", "init_ising", "LocalNoiseProcess", "NoiseModel",
     "relaxation_dephasing", "Context", "YaqsError", "run",
 ]

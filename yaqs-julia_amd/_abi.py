@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "build", "libyaqs_hip.so")
 YAQS_OK, YAQS_EINVAL, YAQS_ENOMEM_DEVICE, YAQS_EDEVICE, YAQS_ENUMERIC, YAQS_ENOTFOUND, YAQS_EAGAIN = range(7)
 PREC_FP64, PREC_MIXED = 0, 1
 GATE_1Q, GATE_2Q, GATE_SAMPLE = 1, 2, 3
+GATE_EXACT, GATE_WINDOW_TDVP = 0, 1  # yaqs_gate_method
 NOISE_KIND = {"lowering": 0, "raising": 1, "pauli_x": 2, "pauli_y": 3, "pauli_z": 4}
 NOISE_CUSTOM = 5
 
@@ -55,7 +56,7 @@ class AnalogJob(ct.Structure):
 
 
 class Gate(ct.Structure):
-    _fields_ = [("kind", ct.c_int32), ("_pad", ct.c_int32), ("site0", ct.c_int64), ("site1", ct.c_int64),
+    _fields_ = [("kind", ct.c_int32), ("method", ct.c_int32), ("site0", ct.c_int64), ("site1", ct.c_int64),
                 ("matrix", ct.c_double * 32)]
 
 
@@ -186,8 +187,12 @@ def pack_analog(mps, mpo, cfg, noise_model, *, precision: int = PREC_FP64, seed:
 
 
 def pack_digital(mps, circuit, cfg, noise_model, *, precision: int = PREC_FP64, seed: int = 0,
-                 traj_offset: int = 0, num_traj: Optional[int] = None) -> _Holder:
+                 traj_offset: int = 0, num_traj: Optional[int] = None, alg_options=None) -> _Holder:
+    """`alg_options` (TJMOptions) picks how 2-qubit gates are applied; None
+    applies every gate matrix (local_method TEBD)."""
+    from .circuit import window_hamiltonian
     from .gates import Barrier
+    tdvp = alg_options is not None and alg_options.local_method == "TDVP"
     h = _Holder()
     mps_arr = _pack_mps(h, mps)
     gates = circuit.gates
@@ -202,7 +207,11 @@ def pack_digital(mps, circuit, cfg, noise_model, *, precision: int = PREC_FP64, 
         q.kind = GATE_1Q if len(g.sites) == 1 else GATE_2Q
         q.site0 = g.sites[0]
         q.site1 = g.sites[1] if len(g.sites) == 2 else g.sites[0]
-        _fill_matrix(q.matrix, g.op.matrix)
+        if tdvp and q.kind == GATE_2Q:
+            q.method = GATE_WINDOW_TDVP
+            _fill_matrix(q.matrix, window_hamiltonian(g))
+        else:
+            _fill_matrix(q.matrix, g.op.matrix)
     h.keep.append(garr)
     noise_arr, n_noise = _pack_noise(h, noise_model)
     obs_arr, n_obs = _pack_obs(h, cfg.observables)

@@ -13,6 +13,39 @@ from .gates import AbstractOperator, Barrier, MatrixGate, haar_su4
 class DigitalGate:
     op: AbstractOperator
     sites: List[int]
+    generator: Optional[list] = None  # overrides op.generator() (src/DigitalTJM.jl:40-44)
+
+
+@dataclass
+class TJMOptions:
+    """TJMOptions (src/DigitalTJM.jl:70-76): how 2-qubit gates are applied.
+    "TDVP" (the reference default) evolves the gate's window under its
+    generator; "TEBD" applies the gate matrix.  Only nearest-neighbour gates
+    run on the GPU path, so long_range_method is recorded but has no gates
+    to act on."""
+    local_method: str = "TDVP"
+    long_range_method: str = "TDVP"
+
+    def __post_init__(self):
+        for m in (self.local_method, self.long_range_method):
+            if str(m).lstrip(":").upper() not in ("TDVP", "TEBD"):
+                raise ValueError(f"unknown method {m!r} (TDVP or TEBD)")
+        self.local_method = str(self.local_method).lstrip(":").upper()
+        self.long_range_method = str(self.long_range_method).lstrip(":").upper()
+
+
+def window_hamiltonian(gate: DigitalGate) -> np.ndarray:
+    """h = coeff * g_a (x) g_b of construct_window_mpo (src/DigitalTJM.jl:192-236)
+    for a 2-qubit gate, g_a on the lower site: generator element 1 belongs to
+    gate.sites[0], so the factors swap when the sites are given in
+    descending order.  Indexed like gate matrices (p_a + 2 p_b)."""
+    gen = gate.generator if gate.generator is not None else gate.op.generator()
+    if gen is None or len(gen) != 2:
+        raise ValueError("a 2-qubit gate needs a two-factor generator for the TDVP window")
+    ga, gb = (np.asarray(gen[0], dtype=complex), np.asarray(gen[1], dtype=complex))
+    if gate.sites[0] != min(gate.sites):
+        ga, gb = gb, ga
+    return gate.op.hamiltonian_coeff() * np.kron(gb, ga)
 
 
 class DigitalCircuit:
@@ -20,10 +53,10 @@ class DigitalCircuit:
         self.num_qubits = int(num_qubits)
         self.gates: List[DigitalGate] = []
 
-    def add_gate(self, op: AbstractOperator, sites) -> None:
+    def add_gate(self, op: AbstractOperator, sites, generator=None) -> None:
         """add_gate! (src/DigitalTJM.jl:64-66)."""
         sites = [int(sites)] if np.ndim(sites) == 0 else [int(s) for s in sites]
-        self.gates.append(DigitalGate(op, sites))
+        self.gates.append(DigitalGate(op, sites, generator))
 
     def add_sample_barrier(self) -> None:
         self.gates.append(DigitalGate(Barrier("SAMPLE_OBSERVABLES"), []))

@@ -7,6 +7,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
+#include <complex>
 #include <cstdlib>
 #include <condition_variable>
 #include <deque>
@@ -38,6 +40,46 @@ yq::c64 mkc(double r, double i) {
     z.re = r;
     z.im = i;
     return z;
+}
+
+// exp(-i h) of a 4x4 matrix in place (column-major), by scaling and squaring
+// of the Taylor series: h / 2^s has 1-norm <= 1/2, the series runs until the
+// term's 1-norm falls below 1e-18 of the sum, then s squarings.
+void expm_minus_i4(yq::c64 *m) {
+    using cd = std::complex<double>;
+    cd A[16];
+    double nrm = 0;
+    for (int c = 0; c < 4; ++c) {
+        double col = 0;
+        for (int r = 0; r < 4; ++r) col += std::abs(cd(m[r + 4 * c].re, m[r + 4 * c].im));
+        nrm = std::max(nrm, col);
+    }
+    if (!std::isfinite(nrm)) throw std::invalid_argument("window Hamiltonian is not finite");
+    int sq = 0;
+    while (nrm > 0.5 && sq < 60) { nrm *= 0.5; ++sq; }
+    const double f = std::ldexp(1.0, -sq);
+    for (int k = 0; k < 16; ++k) A[k] = cd(0, -1) * cd(m[k].re, m[k].im) * f;
+    cd E[16], T[16], U[16];
+    for (int k = 0; k < 16; ++k) E[k] = T[k] = (k % 5 == 0) ? cd(1, 0) : cd(0, 0);
+    auto mul = [](const cd *X, const cd *Y, cd *Z) {
+        for (int c = 0; c < 4; ++c)
+            for (int r = 0; r < 4; ++r) {
+                cd acc = 0;
+                for (int k = 0; k < 4; ++k) acc += X[r + 4 * k] * Y[k + 4 * c];
+                Z[r + 4 * c] = acc;
+            }
+    };
+    for (int n = 1; n <= 40; ++n) {
+        mul(T, A, U);
+        double tn = 0;
+        for (int k = 0; k < 16; ++k) { T[k] = U[k] / double(n); E[k] += T[k]; tn += std::abs(T[k]); }
+        if (tn < 1e-18) break;
+    }
+    for (int q = 0; q < sq; ++q) {
+        mul(E, E, U);
+        for (int k = 0; k < 16; ++k) E[k] = U[k];
+    }
+    for (int k = 0; k < 16; ++k) m[k] = mkc(E[k].real(), E[k].imag());
 }
 
 // Julia column-major (Dl, d, Dr) -> engine row-major [a][s][b]
@@ -158,6 +200,12 @@ yq::JobIn convert_digital(const yaqs_digital_job *job) {
         for (int k = 0; k < 16; ++k) q.m[k] = mkc(0, 0);
         if (q.kind != 3)
             for (int k = 0; k < d * d; ++k) q.m[k] = mkc(G.matrix[2 * k], G.matrix[2 * k + 1]);
+        if (G.method != YAQS_GATE_EXACT) {
+            if (G.method != YAQS_GATE_WINDOW_TDVP || q.kind != 2)
+                throw std::invalid_argument("bad gate method (window TDVP applies to 2-qubit gates only)");
+            expm_minus_i4(q.m);
+            q.rule = 0;
+        }
         J.gates.push_back(q);
     }
     load_noise(J, job->has_noise_model, job->n_noise, job->noise, J.L);

@@ -1492,7 +1492,7 @@ void Engine<T>::run_digital(int64_t t0, int nb_, double *d_out, int64_t *h_bonds
             move_centre(a);
             merge(PHI, a);
             local_op4<T>(theta.as<CT>(), nmax, PHI.wd[a], PHI.wd[a + 2], gops + gate_off[g], nb, s);
-            split(PHI, a, true, 2, J.thr);  // rule T-D
+            split(PHI, a, true, G2.rule, J.thr);  // T-D (TEBD) or T-A (window TDVP)
             centre = a + 1;
             PairCand &pc = pair_cand[a];
             const bool noisy = J.has_noise && !J.noise.empty() && pc.n > 0;
@@ -1631,7 +1631,7 @@ void validate_job(const JobIn &job) {
         if (g.kind == 3) continue;
         if (g.s0 < 0 || g.s0 >= job.L || g.s1 < 0 || g.s1 >= job.L) throw std::invalid_argument("gate site out of range");
         if (g.kind == 2 && std::abs(g.s1 - g.s0) != 1)
-            throw std::invalid_argument("2-qubit gates must act on adjacent sites (TEBD)");
+            throw std::invalid_argument("2-qubit gates must act on adjacent sites");
     }
 }
 

@@ -31,6 +31,7 @@ struct ObsIn {
 struct GateIn {
     int kind;  // 1: 1q, 2: 2q, 3: sample barrier
     int s0, s1;
+    int rule = 2;  // truncation rule of the 2q split: 2 (TEBD) or 0 (window TDVP)
     c64 m[16];  // reference order
 };
 

@@ -26,6 +26,15 @@ class AbstractOperator:
     def matrix(self) -> np.ndarray:
         raise NotImplementedError
 
+    def generator(self):
+        """The factors [g_1(, g_2)] with gate = exp(-i coeff g_1 (x) g_2)
+        (src/GateLibrary.jl:24-26: gates without one raise)."""
+        raise NotImplementedError(f"generator() not implemented for {type(self).__name__}")
+
+    def hamiltonian_coeff(self) -> float:
+        """src/GateLibrary.jl:28-36: 1.0 unless the gate defines it."""
+        return 1.0
+
 
 class _Fixed(AbstractOperator):
     _vals: tuple = ()
@@ -39,17 +48,29 @@ class _Fixed(AbstractOperator):
 class IdGate(_Fixed):
     _vals = (1, 0, 0, 1)
 
+    def generator(self):
+        return [IdGate().matrix]
+
 
 class XGate(_Fixed):  # src/GateLibrary.jl:48-50
     _vals = (0, 1, 1, 0)
+
+    def generator(self):
+        return [XGate().matrix]
 
 
 class YGate(_Fixed):  # src/GateLibrary.jl:52-54
     _vals = (0, 1j, -1j, 0)
 
+    def generator(self):
+        return [YGate().matrix]
+
 
 class ZGate(_Fixed):  # src/GateLibrary.jl:56-58
     _vals = (1, 0, 0, -1)
+
+    def generator(self):
+        return [ZGate().matrix]
 
 
 class HGate(_Fixed):
@@ -93,6 +114,12 @@ class RxGate(_Param):
         c, s = np.cos(self.theta / 2), -1j * np.sin(self.theta / 2)
         return _cm((c, s, s, c), 2)
 
+    def generator(self):  # src/GateLibrary.jl:90-120
+        return [XGate().matrix]
+
+    def hamiltonian_coeff(self):
+        return self.theta / 2
+
 
 class RyGate(_Param):
     @property
@@ -100,11 +127,23 @@ class RyGate(_Param):
         c, s = np.cos(self.theta / 2), np.sin(self.theta / 2)
         return _cm((c, s, -s, c), 2)
 
+    def generator(self):  # src/GateLibrary.jl:90-120
+        return [YGate().matrix]
+
+    def hamiltonian_coeff(self):
+        return self.theta / 2
+
 
 class RzGate(_Param):
     @property
     def matrix(self):
         return _cm((np.exp(-1j * self.theta / 2), 0, 0, np.exp(1j * self.theta / 2)), 2)
+
+    def generator(self):  # src/GateLibrary.jl:90-120
+        return [ZGate().matrix]
+
+    def hamiltonian_coeff(self):
+        return self.theta / 2
 
 
 class PhaseGate(_Param):
@@ -131,6 +170,12 @@ class _Two(_Fixed):
 class CXGate(_Two):  # src/GateLibrary.jl:144-150
     _vals = (1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0)
 
+    def generator(self):  # pi/4 (I - Z) (x) (I - X), src/GateLibrary.jl:151-158
+        return [_cm((0, 0, 0, 2), 2), _cm((1, -1, -1, 1), 2)]
+
+    def hamiltonian_coeff(self):
+        return np.pi / 4
+
 
 class CYGate(_Two):
     _vals = (1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1j, 0, 0, -1j, 0)
@@ -138,6 +183,12 @@ class CYGate(_Two):
 
 class CZGate(_Two):
     _vals = (1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, -1)
+
+    def generator(self):  # pi/4 (I - Z) (x) (I - Z), src/GateLibrary.jl:176-188
+        return [_cm((0, 0, 0, 2), 2), _cm((0, 0, 0, 2), 2)]
+
+    def hamiltonian_coeff(self):
+        return np.pi / 4
 
 
 class SWAPGate(_Two):
@@ -157,12 +208,24 @@ class CPhaseGate(_Two_Param):
     def matrix(self):
         return np.diag([1, 1, 1, np.exp(1j * self.theta)]).astype(C)
 
+    def generator(self):  # -theta/4 (I - Z) (x) (I - Z), src/GateLibrary.jl:229-233
+        return [_cm((0, 0, 0, 2), 2), _cm((0, 0, 0, 2), 2)]
+
+    def hamiltonian_coeff(self):
+        return -self.theta / 4
+
 
 class RxxGate(_Two_Param):  # src/GateLibrary.jl:260-272
     @property
     def matrix(self):
         c, s = np.cos(self.theta / 2), -1j * np.sin(self.theta / 2)
         return _cm((c, 0, 0, s, 0, c, s, 0, 0, s, c, 0, s, 0, 0, c), 4)
+
+    def generator(self):  # src/GateLibrary.jl:267-301
+        return [XGate().matrix, XGate().matrix]
+
+    def hamiltonian_coeff(self):
+        return self.theta / 2
 
 
 class RyyGate(_Two_Param):
@@ -171,12 +234,24 @@ class RyyGate(_Two_Param):
         c, s = np.cos(self.theta / 2), -1j * np.sin(self.theta / 2)
         return _cm((c, 0, 0, -s, 0, c, s, 0, 0, s, c, 0, -s, 0, 0, c), 4)
 
+    def generator(self):  # src/GateLibrary.jl:267-301
+        return [YGate().matrix, YGate().matrix]
+
+    def hamiltonian_coeff(self):
+        return self.theta / 2
+
 
 class RzzGate(_Two_Param):  # exp(-i θ/2 Z⊗Z), src/GateLibrary.jl:286-300
     @property
     def matrix(self):
         a, b = np.exp(-1j * self.theta / 2), np.exp(1j * self.theta / 2)
         return np.diag([a, b, b, a]).astype(C)
+
+    def generator(self):  # src/GateLibrary.jl:267-301
+        return [ZGate().matrix, ZGate().matrix]
+
+    def hamiltonian_coeff(self):
+        return self.theta / 2
 
 
 class MatrixGate(AbstractOperator):

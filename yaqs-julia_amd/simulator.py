@@ -14,7 +14,7 @@ from typing import Optional, Union
 import numpy as np
 
 from . import _abi
-from .circuit import DigitalCircuit
+from .circuit import DigitalCircuit, TJMOptions
 from .configs import StrongMeasurementConfig, TimeEvolutionConfig
 from .networks import MPO, MPS
 from .noise import NoiseModel
@@ -104,12 +104,14 @@ def _all_zero(noise_model: Optional[NoiseModel]) -> bool:
 
 def run(initial_state: MPS, operator_or_circuit: Union[MPO, DigitalCircuit], sim_params,
         noise_model: Optional[NoiseModel] = None, *, precision: str = "fp64", seed: int = 0,
-        ctx: Optional[Context] = None, parallel: bool = True):
-    """Simulator.run(ψ0, H_or_circuit, cfg, noise).  Fills
+        ctx: Optional[Context] = None, parallel: bool = True, alg_options: Optional[TJMOptions] = None):
+    """Simulator.run(ψ0, H_or_circuit, cfg, noise; alg_options).  Fills
     `obs.trajectories` (num_traj x num_cols, real parts stored as complex)
     and `obs.results` for every observable, as src/Simulator.jl:45-68 does.
     Returns job statistics (analog) or per-trajectory bond dimensions
-    (digital, src/Simulator.jl:82-124)."""
+    (digital, src/Simulator.jl:82-124).  Digital runs default to
+    TJMOptions() (TDVP windows, src/DigitalTJM.jl:366), as the reference's
+    run_digital_tjm does."""
     initial_state.normalize()  # src/Simulator.jl:128
     ctx = ctx or default_context()
     prec = _PREC[precision]
@@ -132,7 +134,7 @@ def run(initial_state: MPS, operator_or_circuit: Union[MPO, DigitalCircuit], sim
         if _all_zero(noise_model):
             sim_params.num_traj = 1
         h = _abi.pack_digital(initial_state, operator_or_circuit, sim_params, noise_model, precision=prec,
-                              seed=seed)
+                              seed=seed, alg_options=alg_options if alg_options is not None else TJMOptions())
         ncol = int(ctx.lib.yaqs_digital_num_samples(ct.byref(h.job)))
         for o in sim_params.observables:
             o.initialize(sim_params.num_traj, ncol)
