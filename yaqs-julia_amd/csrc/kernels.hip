@@ -511,41 +511,53 @@ __device__ double2 tridiag_exp_lane(const double *alpha, const double *beta, int
 // partials (w, stored unnormalised, becomes V_{j+1} with s_{j+1} = 1/β),
 // c = exp(-i dt T) e0 and the stopping rule β0·β·|c_j| <= tol (shared with
 // the oracle).  A trajectory that stops keeps nit = j + 1 and leaves the
-// active set; trajectories that go on are counted in nactive[j].
-__global__ __launch_bounds__(256) void lz_finalize_kernel(int j, int kmax, double dt, double tol, const double *partn,
-                                                          int nchunk, const double *galpha, double *alpha, double *beta,
-                                                          const double *beta0, c64 *coef, int *nit, int *active,
-                                                          int *nactive, int B) {
+// active set; trajectories that go on are counted in nactive[j], first per
+// workgroup in LDS, then with one global atomic per workgroup (a thousand
+// same-address atomics from single waves cost about 10 us).
+#ifndef LZ_FIN_THREADS
+#define LZ_FIN_THREADS 256
+#endif
+__global__ __launch_bounds__(LZ_FIN_THREADS) void lz_finalize_kernel(int j, int kmax, double dt, double tol,
+                                                                     const double *partn, int nchunk,
+                                                                     const double *galpha, double *alpha,
+                                                                     double *beta, const double *beta0, c64 *coef,
+                                                                     int *nit, int *active, int *nactive, int B) {
+    __shared__ int going;
     const int lane = threadIdx.x & 63;
-    const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (t >= B || !active[t]) return;
-    double s = 0;
-    for (int i = lane; i < nchunk; i += 64) s += partn[(int64_t)t * nchunk + i];
-    s = wave_sum(s);
-    const double b = sqrt(s);
-    double *al = alpha + (int64_t)t * kmax;
-    double *be = beta + (int64_t)t * (kmax + 1);
-    const double aj = galpha[t];
-    if (lane == 0) al[j] = aj;
-    const double2 cv = tridiag_exp_lane(al, be, j + 1, dt, lane, aj);
-    const double cjx = lane_read(cv.x, j), cjy = lane_read(cv.y, j);
-    const double err = beta0[t] * b * sqrt(cjx * cjx + cjy * cjy);
-    const bool done = (err <= tol) || (b == 0.0) || (j + 1 == kmax);
-    if (done) {
-        if (lane <= j) {
-            c64 o;
-            o.re = cv.x;
-            o.im = cv.y;
-            coef[(int64_t)t * kmax + lane] = o;
+    const int t = blockIdx.x * (LZ_FIN_THREADS / 64) + (threadIdx.x >> 6);
+    if (threadIdx.x == 0) going = 0;
+    __syncthreads();
+    if (t < B && active[t]) {
+        double s = 0;
+        for (int i = lane; i < nchunk; i += 64) s += partn[(int64_t)t * nchunk + i];
+        s = wave_sum(s);
+        const double b = sqrt(s);
+        double *al = alpha + (int64_t)t * kmax;
+        double *be = beta + (int64_t)t * (kmax + 1);
+        const double aj = galpha[t];
+        if (lane == 0) al[j] = aj;
+        const double2 cv = tridiag_exp_lane(al, be, j + 1, dt, lane, aj);
+        const double cjx = lane_read(cv.x, j), cjy = lane_read(cv.y, j);
+        const double err = beta0[t] * b * sqrt(cjx * cjx + cjy * cjy);
+        const bool done = (err <= tol) || (b == 0.0) || (j + 1 == kmax);
+        if (done) {
+            if (lane <= j) {
+                c64 o;
+                o.re = cv.x;
+                o.im = cv.y;
+                coef[(int64_t)t * kmax + lane] = o;
+            }
+            if (lane == 0) {
+                nit[t] = j + 1;
+                active[t] = 0;
+            }
+        } else if (lane == 0) {
+            be[j + 1] = b;
+            atomicAdd(&going, 1);
         }
-        if (lane == 0) {
-            nit[t] = j + 1;
-            active[t] = 0;
-        }
-    } else if (lane == 0) {
-        be[j + 1] = b;
-        atomicAdd(nactive + j, 1);
     }
+    __syncthreads();
+    if (threadIdx.x == 0 && going) atomicAdd(nactive + j, going);
 }
 
 // x = β0 Σ_{q<nit} c_q v_q = β0 Σ c_q s_q V_q  (trajectories with β0 > 0); the basis is read
@@ -625,8 +637,9 @@ void lz_finalize(int j, int kmax, double dt, double tol, const double *partn, in
                  double *alpha, double *beta, const double *beta0, c64 *coef, int *nit, int *active, int *nactive, int B,
                  hipStream_t s) {
     const int nchunk = (int)((n + LZ_CHUNK - 1) / LZ_CHUNK);
-    hipLaunchKernelGGL(lz_finalize_kernel, dim3((B + 3) / 4), dim3(256), 0, s, j, kmax, dt, tol, partn, nchunk, galpha,
-                       alpha, beta, beta0, coef, nit, active, nactive, B);
+    constexpr int per = LZ_FIN_THREADS / 64;
+    hipLaunchKernelGGL(lz_finalize_kernel, dim3((B + per - 1) / per), dim3(LZ_FIN_THREADS), 0, s, j, kmax, dt, tol,
+                       partn, nchunk, galpha, alpha, beta, beta0, coef, nit, active, nactive, B);
     YQ_HIP(hipGetLastError());
 }
 template <typename T>
