@@ -469,8 +469,10 @@ class Engine {
     bool any_obs2 = false;
     std::vector<char> obs2_at;
     int *h_pinned = nullptr;
-    hipEvent_t lz_ev[2] = {nullptr, nullptr};  // Lanczos active counts copied to h_pinned[1 + (j & 1)]
-    // The count a device-to-host copy writes over the -1 left in
+    int *d_pinned = nullptr;  // device address of h_pinned
+    DevBuf fin_done;          // lz_finalize's last-workgroup ticket
+    hipEvent_t lz_ev[2] = {nullptr, nullptr};  // Lanczos active counts land in h_pinned[1 + (j & 1)]
+    // The count the finalize kernel stores over the -1 left in
     // h_pinned[slot]: the host spins on the pinned word (the next launches
     // go out as soon as the count lands, without the event's wake-up), then
     // falls back to the event after some tens of milliseconds.
@@ -665,7 +667,12 @@ void Engine<T>::setup() {
     mask.alloc(B * sizeof(int));
     rngk.alloc(B * sizeof(uint64_t));
     count.alloc(sizeof(int));
-    YQ_HIP(hipHostMalloc((void **)&h_pinned, 16 * sizeof(int), hipHostMallocDefault));
+    fin_done.alloc(sizeof(int));
+    fill_int(fin_done.as<int>(), 0, 1, s);
+    // coherent: the Lanczos finalize kernel stores the active count straight
+    // into h_pinned[1 + (j & 1)], visible to the host's spin before the kernel ends
+    YQ_HIP(hipHostMalloc((void **)&h_pinned, 16 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+    YQ_HIP(hipHostGetDevicePointer((void **)&d_pinned, h_pinned, 0));
     for (auto &e : lz_ev) YQ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
     // ---- noise: dissipation operators per site and jump candidates
@@ -989,12 +996,11 @@ void Engine<T>::expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, Jo
         lz_dots<T>(V.as<CT>(), sV(), nmax, n, j, part.as<c64>(), kdim + 1, active.as<int>(), nb, s);
         lz_update<T>(V.as<CT>(), sV(), nmax, n, j, part.as<c64>(), kdim + 1, partn.as<double>(), galpha.as<double>(),
                      active.as<int>(), beta.as<double>(), kdim, nb, s);
+        // the kernel's count overwrites the -1 left in the host slot (see wait_pinned)
+        __atomic_store_n(h_pinned + 1 + (j & 1), -1, __ATOMIC_RELEASE);
         lz_finalize(j, kdim, dt, J.ktol, partn.as<double>(), n, galpha.as<double>(), alpha.as<double>(),
                     beta.as<double>(), beta0.as<double>(), coef.as<c64>(), nit.as<int>(), active.as<int>(),
-                    nactive.as<int>(), nb, s);
-        // the copy overwrites the -1 left in the host slot (see wait_pinned)
-        __atomic_store_n(h_pinned + 1 + (j & 1), -1, __ATOMIC_RELEASE);
-        YQ_HIP(hipMemcpyAsync(h_pinned + 1 + (j & 1), nactive.as<int>() + j, sizeof(int), hipMemcpyDeviceToHost, s));
+                    nactive.as<int>(), nb, fin_done.as<int>(), d_pinned + 1 + (j & 1), s);
         YQ_HIP(hipEventRecord(lz_ev[j & 1], s));
     };
     // With `ahead`, iteration j+1 is queued before the host waits for the

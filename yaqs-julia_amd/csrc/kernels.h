@@ -56,10 +56,13 @@ void lz_update(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, const c64
                double *galpha, const int *active, const double *beta, int kmax, int B, hipStream_t s);
 // β = ‖w‖ (stored in beta[j+1] when the trajectory goes on) and the
 // stopping rule; trajectories that go on are counted in nactive[j] (cleared
-// by lz_init for every slot).
+// by lz_init for every slot).  With `host_count` (a device pointer to a
+// coherent pinned host int) the last workgroup to finish also stores the
+// final nactive[j] there; `fin_done` is a device int that is zero between
+// launches (the last workgroup resets it).
 void lz_finalize(int j, int kmax, double dt, double tol, const double *partn, int64_t n, const double *galpha,
                  double *alpha, double *beta, const double *beta0, c64 *coef, int *nit, int *active, int *nactive, int B,
-                 hipStream_t s);
+                 int *fin_done, int *host_count, hipStream_t s);
 template <typename T>
 void lz_combine(cplx<T> *x, int64_t sx, const cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, const double *beta0,
                 const c64 *coef, const int *nit, const double *beta, int kmax, const int *mask, int B, hipStream_t s);

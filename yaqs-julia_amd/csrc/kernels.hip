@@ -513,7 +513,10 @@ __device__ double2 tridiag_exp_lane(const double *alpha, const double *beta, int
 // the oracle).  A trajectory that stops keeps nit = j + 1 and leaves the
 // active set; trajectories that go on are counted in nactive[j], first per
 // workgroup in LDS, then with one global atomic per workgroup (a thousand
-// same-address atomics from single waves cost about 10 us).
+// same-address atomics from single waves cost about 10 us).  The last
+// workgroup to count (device-scope release/acquire around the `fin_done`
+// ticket) reads the total and stores it into the host's pinned word, so the
+// host sees the count when this kernel ends, without a copy.
 #ifndef LZ_FIN_THREADS
 #define LZ_FIN_THREADS 256
 #endif
@@ -521,7 +524,8 @@ __global__ __launch_bounds__(LZ_FIN_THREADS) void lz_finalize_kernel(int j, int 
                                                                      const double *partn, int nchunk,
                                                                      const double *galpha, double *alpha,
                                                                      double *beta, const double *beta0, c64 *coef,
-                                                                     int *nit, int *active, int *nactive, int B) {
+                                                                     int *nit, int *active, int *nactive, int B,
+                                                                     int *fin_done, int *host_count) {
     __shared__ int going;
     const int lane = threadIdx.x & 63;
     const int t = blockIdx.x * (LZ_FIN_THREADS / 64) + (threadIdx.x >> 6);
@@ -557,7 +561,18 @@ __global__ __launch_bounds__(LZ_FIN_THREADS) void lz_finalize_kernel(int j, int 
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0 && going) atomicAdd(nactive + j, going);
+    if (threadIdx.x == 0) {
+        if (going) atomicAdd(nactive + j, going);
+        if (host_count) {
+            __threadfence();
+            if (atomicAdd(fin_done, 1) == (int)gridDim.x - 1) {
+                __threadfence();
+                const int total = atomicAdd(nactive + j, 0);
+                atomicExch(fin_done, 0);
+                __hip_atomic_store(host_count, total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
+    }
 }
 
 // x = β0 Σ_{q<nit} c_q v_q = β0 Σ c_q s_q V_q  (trajectories with β0 > 0); the basis is read
@@ -635,11 +650,13 @@ void lz_update(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, const c64
 }
 void lz_finalize(int j, int kmax, double dt, double tol, const double *partn, int64_t n, const double *galpha,
                  double *alpha, double *beta, const double *beta0, c64 *coef, int *nit, int *active, int *nactive, int B,
-                 hipStream_t s) {
+                 int *fin_done, int *host_count, hipStream_t s) {
+    if (host_count && !fin_done) throw std::invalid_argument("lz_finalize: host_count needs fin_done");
     const int nchunk = (int)((n + LZ_CHUNK - 1) / LZ_CHUNK);
     constexpr int per = LZ_FIN_THREADS / 64;
     hipLaunchKernelGGL(lz_finalize_kernel, dim3((B + per - 1) / per), dim3(LZ_FIN_THREADS), 0, s, j, kmax, dt, tol,
-                       partn, nchunk, galpha, alpha, beta, beta0, coef, nit, active, nactive, B);
+                       partn, nchunk, galpha, alpha, beta, beta0, coef, nit, active, nactive, B, fin_done,
+                       host_count);
     YQ_HIP(hipGetLastError());
 }
 template <typename T>
