@@ -157,13 +157,17 @@ typedef enum {
 typedef struct {
     int32_t kind;
     int32_t method;          /* yaqs_gate_method; 2Q gates only, 0 otherwise */
-    int64_t site0, site1;    /* 1-based; for 2Q gates sites must be adjacent */
+    int64_t site0, site1;    /* 1-based; non-adjacent 2Q gates: SWAP network, EXACT only */
     double matrix[32];
 } yaqs_gate;
 
 /* Digital TJM job: run_digital_tjm (src/DigitalTJM.jl:363-450) with the
- * local method chosen per 2-qubit gate (yaqs_gate_method); long-range
- * gates (SWAP networks or long windows) are not supported. */
+ * local method chosen per 2-qubit gate (yaqs_gate_method).  A non-adjacent
+ * 2-qubit gate with YAQS_GATE_EXACT runs as the TEBD SWAP network of
+ * apply_window! (src/DigitalTJM.jl:314-330: SWAPs move the higher site next
+ * to the lower one, the gate acts there, the SWAPs unwind, each step an
+ * apply_local_gate_exact!); no noise process may act on its sites.  TDVP
+ * windows of long-range gates are not supported. */
 typedef struct {
     int64_t L;
     const yaqs_mps_tensor *mps;

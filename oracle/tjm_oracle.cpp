@@ -14,8 +14,8 @@
 //   Dissipation        src/Dissipation.jl:14-159
 //   Jumps              src/StochasticProcess.jl:35-330
 //   Simulator loop     src/Simulator.jl:22-125
-//   DigitalTJM         src/DigitalTJM.jl:86-305, 335-358, 363-450 (TEBD and
-//                      TDVP windows of adjacent pairs)
+//   DigitalTJM         src/DigitalTJM.jl:86-305, 307-358, 363-450 (TEBD, TDVP
+//                      windows of adjacent pairs, SWAP networks)
 // The dense kernels the reference takes from LAPACK/KrylovKit are written
 // here from their textbook definitions: one-sided (Hestenes) Jacobi SVD,
 // Householder QR/LQ, Lanczos with full re-orthogonalisation, and a cyclic
@@ -1352,8 +1352,17 @@ void run_digital_trajectory(const yaqs_digital_job *job, const MPSState &psi0, c
             // apply_window! sorts the sites and applies the matrix unchanged
             // on (min, max) (src/DigitalTJM.jl:307-334)
             int a = std::min(G.s0, G.s1), b = std::max(G.s0, G.s1);
-            if (G.method == YAQS_GATE_WINDOW_TDVP) window_tdvp_gate(st, G.m, a, b, thr, mb);
-            else tebd_gate(st, G.m, a, b, thr, mb);
+            if (b > a + 1) {
+                // long_range_method :TEBD: SWAP network (src/DigitalTJM.jl:314-330)
+                static const cd SW[16] = {1, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0, 0, 1};
+                for (int k = b - 1; k >= a + 1; --k) tebd_gate(st, SW, k, k + 1, thr, mb);
+                tebd_gate(st, G.m, a, a + 1, thr, mb);
+                for (int k = a + 1; k <= b - 1; ++k) tebd_gate(st, SW, k, k + 1, thr, mb);
+            } else if (G.method == YAQS_GATE_WINDOW_TDVP) {
+                window_tdvp_gate(st, G.m, a, b, thr, mb);
+            } else {
+                tebd_gate(st, G.m, a, b, thr, mb);
+            }
             if (noise && !noise->empty()) {
                 std::vector<Proc> local;  // create_local_noise_model (:172-188)
                 for (const Proc &p : *noise) {
@@ -1499,8 +1508,8 @@ int oracle_run_digital(const yaqs_digital_job *job, double *traj_out, double *me
         if (G.kind == YAQS_GATE_1Q) load_c(G.matrix, gates[g].m, 4);
         if (G.kind == YAQS_GATE_2Q) {
             load_c(G.matrix, gates[g].m, 16);
-            if (std::abs(G.site1 - G.site0) != 1) {
-                g_err = "TEBD gates must act on adjacent sites";
+            if (G.site1 == G.site0 || (std::abs(G.site1 - G.site0) != 1 && G.method != YAQS_GATE_EXACT)) {
+                g_err = "long-range gates run only as TEBD SWAP networks";
                 return YAQS_EINVAL;
             }
         }

@@ -193,6 +193,7 @@ def pack_digital(mps, circuit, cfg, noise_model, *, precision: int = PREC_FP64, 
     from .circuit import window_hamiltonian
     from .gates import Barrier
     tdvp = alg_options is not None and alg_options.local_method == "TDVP"
+    tdvp_long = alg_options is not None and alg_options.long_range_method == "TDVP"
     h = _Holder()
     mps_arr = _pack_mps(h, mps)
     gates = circuit.gates
@@ -207,7 +208,11 @@ def pack_digital(mps, circuit, cfg, noise_model, *, precision: int = PREC_FP64, 
         q.kind = GATE_1Q if len(g.sites) == 1 else GATE_2Q
         q.site0 = g.sites[0]
         q.site1 = g.sites[1] if len(g.sites) == 2 else g.sites[0]
-        if tdvp and q.kind == GATE_2Q:
+        long_range = q.kind == GATE_2Q and abs(q.site1 - q.site0) > 1
+        if long_range and tdvp_long:
+            raise NotImplementedError("long-range TDVP windows are not supported by the GPU path; "
+                                      "use TJMOptions(long_range_method='TEBD') for SWAP networks")
+        if tdvp and q.kind == GATE_2Q and not long_range:
             q.method = GATE_WINDOW_TDVP
             _fill_matrix(q.matrix, window_hamiltonian(g))
         else:

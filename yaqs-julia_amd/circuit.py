@@ -20,9 +20,10 @@ class DigitalGate:
 class TJMOptions:
     """TJMOptions (src/DigitalTJM.jl:70-76): how 2-qubit gates are applied.
     "TDVP" (the reference default) evolves the gate's window under its
-    generator; "TEBD" applies the gate matrix.  Only nearest-neighbour gates
-    run on the GPU path, so long_range_method is recorded but has no gates
-    to act on."""
+    generator; "TEBD" applies the gate matrix.  For long-range gates the GPU
+    path runs the TEBD SWAP network (src/DigitalTJM.jl:314-330) only:
+    long_range_method "TDVP" (the reference default) raises when the
+    circuit has a long-range gate."""
     local_method: str = "TDVP"
     long_range_method: str = "TDVP"
 

@@ -490,6 +490,7 @@ class Engine {
     std::map<int, PairCand> pair_cand;  // digital candidate tables per gate pair
     DevBuf gate_ops;  // digital: gate matrices (engine order), 16 c64 per gate
     std::vector<int64_t> gate_off;
+    int64_t swap_off = -1;  // SWAP matrix in gate_ops
     DevBuf dissd_ops;  // digital dissipation (dt = 1) per gate pair and site: [(2 a + r) * 4]
     std::vector<int> dgt_active;
     // gate pairs whose two dissipators are multiples of the identity (Pauli
@@ -885,6 +886,14 @@ void Engine<T>::setup() {
                     gate_off[g] = (int64_t)gops.size();
                     gops.insert(gops.end(), e, e + 16);
                 }
+            }
+            {  // the SWAP of the long-range TEBD network (src/GateLibrary.jl SWAPGate)
+                c64 sw[16], e[16];
+                for (int k = 0; k < 16; ++k) sw[k] = mk(0, 0);
+                sw[0] = sw[6] = sw[9] = sw[15] = mk(1, 0);
+                op4_engine(sw, e);
+                swap_off = (int64_t)gops.size();
+                gops.insert(gops.end(), e, e + 16);
             }
             upload(gate_ops, gops, s);
         }
@@ -1494,7 +1503,29 @@ void Engine<T>::run_digital(int64_t t0, int nb_, double *d_out, int64_t *h_bonds
         for (int g : layers[li]) {
             const GateIn &G2 = J.gates[g];
             if (G2.kind != 2) continue;
-            const int a = std::min(G2.s0, G2.s1);
+            const int a = std::min(G2.s0, G2.s1), b = std::max(G2.s0, G2.s1);
+            if (b > a + 1) {
+                // long-range gate, TEBD method: SWAP network (src/DigitalTJM.jl:314-330)
+                // moves site b next to a, the gate acts on (a, a+1), the SWAPs unwind;
+                // every step is apply_local_gate_exact! (rule T-D)
+                auto nn = [&](int k, const c64 *op) {
+                    move_centre(k);
+                    merge(PHI, k);
+                    local_op4<T>(theta.as<CT>(), nmax, PHI.wd[k], PHI.wd[k + 2], op, nb, s);
+                    split(PHI, k, true, 2, J.thr);
+                    centre = k + 1;
+                };
+                for (int k = b - 1; k >= a + 1; --k) nn(k, gops + swap_off);
+                nn(a, gops + gate_off[g]);
+                for (int k = a + 1; k <= b - 1; ++k) nn(k, gops + swap_off);
+                // no noise process acts on a or b (checked at submission): normalize!
+                const int64_t nc = (int64_t)PHI.wd[b] * 2 * PHI.wd[b + 1];
+                norm2<T>(A(PHI, b), slotA[b], nc, norm2v.as<double>(), nullptr, nb, s);
+                jump_decide(norm2v.as<double>(), 0, 0, rngk.as<uint64_t>(), 0, opidx.as<int>(), fac.as<double>(), nb,
+                            s);
+                scale<T>(A(PHI, b), slotA[b], nc, fac.as<double>(), nullptr, nb, s);
+                continue;
+            }
             move_centre(a);
             merge(PHI, a);
             local_op4<T>(theta.as<CT>(), nmax, PHI.wd[a], PHI.wd[a + 2], gops + gate_off[g], nb, s);
@@ -1636,8 +1667,16 @@ void validate_job(const JobIn &job) {
     for (auto &g : job.gates) {
         if (g.kind == 3) continue;
         if (g.s0 < 0 || g.s0 >= job.L || g.s1 < 0 || g.s1 >= job.L) throw std::invalid_argument("gate site out of range");
-        if (g.kind == 2 && std::abs(g.s1 - g.s0) != 1)
-            throw std::invalid_argument("2-qubit gates must act on adjacent sites");
+        if (g.kind == 2 && g.s1 == g.s0) throw std::invalid_argument("2-qubit gate on a single site");
+        if (g.kind == 2 && std::abs(g.s1 - g.s0) != 1) {
+            if (g.rule != 2)
+                throw std::invalid_argument("long-range 2-qubit gates run only as TEBD SWAP networks (no TDVP windows)");
+            if (job.has_noise)
+                for (auto &p : job.noise)
+                    if (p.s0 == g.s0 || p.s0 == g.s1 || (p.nsites == 2 && (p.s1 == g.s0 || p.s1 == g.s1)))
+                        throw std::invalid_argument(
+                            "noise processes on the sites of a long-range gate are not supported by the GPU path");
+        }
     }
 }
 
