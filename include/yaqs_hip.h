@@ -166,8 +166,8 @@ typedef struct {
  * 2-qubit gate with YAQS_GATE_EXACT runs as the TEBD SWAP network of
  * apply_window! (src/DigitalTJM.jl:314-330: SWAPs move the higher site next
  * to the lower one, the gate acts there, the SWAPs unwind, each step an
- * apply_local_gate_exact!); no noise process may act on its sites.  TDVP
- * windows of long-range gates are not supported. */
+ * apply_local_gate_exact!), then the gate's local noise acts on its two
+ * sites.  TDVP windows of long-range gates are not supported. */
 typedef struct {
     int64_t L;
     const yaqs_mps_tensor *mps;

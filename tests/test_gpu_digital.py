@@ -158,36 +158,26 @@ def _long_range_circuit(L, depth, seed, pairs):
     return c
 
 
-@pytest.mark.parametrize("noisy", [False, True])
-def test_swap_network_parity(ctx, oracle_lib, noisy):
+@pytest.mark.parametrize("noise_sites", [None, (2, 3, 6, 7), (1, 2, 3, 4, 5, 6, 7, 8)])
+def test_swap_network_parity(ctx, oracle_lib, noise_sites):
     """Long-range gates as TEBD SWAP networks (src/DigitalTJM.jl:314-330),
-    per trajectory against the oracle; noise acts on the sites the long-range
-    gates leave alone (2, 3, 6, 7)."""
-    print(f"case: digital SWAP networks L=8 noisy={noisy}", flush=True)
+    per trajectory against the oracle: noise-free, noise only on the sites
+    the long-range gates leave alone, and noise on every site (dissipation
+    and jumps on the two sites of the long-range gate after the network)."""
+    print(f"case: digital SWAP networks L=8 noise on {noise_sites}", flush=True)
     L = 8
     circ = _long_range_circuit(L, 10, seed=21, pairs=[(1, 4), (5, 8), (1, 8), (4, 5)])
     obs = [Y.Observable(f"z{i}", Y.gates.ZGate(), i) for i in range(1, L + 1)]
-    cfg = Y.StrongMeasurementConfig(obs, num_traj=16 if noisy else 1, max_bond_dim=16, truncation_threshold=1e-9)
+    cfg = Y.StrongMeasurementConfig(obs, num_traj=16 if noise_sites else 1, max_bond_dim=16,
+                                    truncation_threshold=1e-9)
     noise = None
-    if noisy:
-        noise = Y.NoiseModel([{"name": n, "sites": [i], "strength": 0.03} for i in (2, 3, 6, 7)
-                              for n in ("pauli_x", "pauli_z")], L)
+    if noise_sites:
+        noise = Y.NoiseModel([{"name": n, "sites": [i], "strength": 0.03} for i in noise_sites
+                              for n in ("pauli_x", "lowering")], L)
     opts = Y.TJMOptions(local_method="TEBD", long_range_method="TEBD")
     g, gm, gb, st, r, rm, rb, rst = _run_opts(ctx, oracle_lib, L, circ, cfg, noise, opts, seed=5)
     assert st["jumps"] == rst["jumps"]
-    if noisy:
+    if noise_sites:
         assert st["jumps"] > 0
     assert np.abs(g - r).max() < 1e-9
     assert np.array_equal(gb, rb)
-
-
-def test_swap_network_with_noise_on_its_sites_is_rejected(ctx):
-    print("case: digital long-range gate with noise on its sites", flush=True)
-    L = 4
-    c = Y.DigitalCircuit(L)
-    c.add_gate(Y.gates.CXGate(), [1, 3])
-    cfg = Y.StrongMeasurementConfig([Y.Observable("z1", Y.gates.ZGate(), 1)], num_traj=2, max_bond_dim=16)
-    noise = Y.NoiseModel([{"name": "pauli_z", "sites": [3], "strength": 0.1}], L)
-    h = _abi.pack_digital(Y.MPS(L), c, cfg, noise, alg_options=Y.TJMOptions(long_range_method="TEBD"))
-    with pytest.raises(Y.YaqsError, match="long-range"):
-        ctx.submit_digital(h)

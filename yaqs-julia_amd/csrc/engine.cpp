@@ -488,6 +488,9 @@ class Engine {
     uint64_t traj0 = 0;
     int64_t nchunk_max = 0;
     std::map<int, PairCand> pair_cand;  // digital candidate tables per gate pair
+    std::map<int, PairCand> lr_cand;    // per long-range gate pair (a, b), key a * L + b
+    DevBuf dsite_ops;                   // digital 1-site dissipation (dt = 1) per site: [q * 4]
+    std::vector<int> dsite_active;
     DevBuf gate_ops;  // digital: gate matrices (engine order), 16 c64 per gate
     std::vector<int64_t> gate_off;
     int64_t swap_off = -1;  // SWAP matrix in gate_ops
@@ -871,6 +874,49 @@ void Engine<T>::setup() {
                 sops[(size_t)a * 4 + 3] = mk(f, 0);
             }
             upload(dscal_ops, sops, s);
+            // long-range gates (SWAP networks): candidates of the processes on a, then
+            // on b (create_local_noise_model, src/DigitalTJM.jl:172-188), and each
+            // site's 1-site dissipator (the r = 0 part of the pair operators above)
+            std::vector<c64> dso((size_t)L * 4, mk(0, 0));
+            dsite_active.assign((size_t)L, 0);
+            for (int q = 0; q < L; ++q) {
+                c64 *M = &dso[(size_t)q * 4];
+                M[0] = mk(1, 0);
+                M[3] = mk(1, 0);
+                if (!J.has_noise || allzero) continue;
+                for (auto &p : J.noise) {
+                    if (p.nsites != 1 || p.s0 != q) continue;
+                    c64 D[4];
+                    if (p.pauli_name) {
+                        const double f = std::exp(-0.5 * p.gamma);
+                        D[0] = mk(f, 0); D[1] = mk(0, 0); D[2] = mk(0, 0); D[3] = mk(f, 0);
+                    } else {
+                        c64 Le[4];
+                        op2_engine(p.m, Le);
+                        damp2(Le, 0.5 * p.gamma, D);
+                    }
+                    matmul2(D, M, M);
+                    dsite_active[q] = 1;
+                }
+            }
+            upload(dsite_ops, dso, s);
+            for (const GateIn &G2 : J.gates) {
+                if (G2.kind != 2) continue;
+                const int la = std::min(G2.s0, G2.s1), lb = std::max(G2.s0, G2.s1);
+                if (lb <= la + 1 || lr_cand.count(la * L + lb)) continue;
+                cs.clear(); co.clear(); cs2.clear(); co2.clear(); ldl.clear(); cg.clear();
+                for (int site : {la, lb})
+                    for (size_t q = 0; q < J.noise.size(); ++q)
+                        if (J.noise[q].nsites == 1 && J.noise[q].s0 == site) add(q, site, false);
+                PairCand &pc = lr_cand[la * L + lb];
+                pc.n = (int)cs.size();
+                upload(pc.site, cs, s);
+                upload(pc.op, co, s);
+                upload(pc.site2, cs2, s);
+                upload(pc.op2, co2, s);
+                upload(pc.LdL, ldl, s);
+                upload(pc.gamma, cg, s);
+            }
             std::vector<c64> gops;
             gate_off.assign(J.gates.size(), -1);
             for (size_t g = 0; g < J.gates.size(); ++g) {
@@ -1518,8 +1564,62 @@ void Engine<T>::run_digital(int64_t t0, int nb_, double *d_out, int64_t *h_bonds
                 for (int k = b - 1; k >= a + 1; --k) nn(k, gops + swap_off);
                 nn(a, gops + gate_off[g]);
                 for (int k = a + 1; k <= b - 1; ++k) nn(k, gops + swap_off);
-                // no noise process acts on a or b (checked at submission): normalize!
                 const int64_t nc = (int64_t)PHI.wd[b] * 2 * PHI.wd[b + 1];
+                auto lci = lr_cand.find(a * L + b);
+                if (J.has_noise && !J.noise.empty() && lci != lr_cand.end() && lci->second.n > 0) {
+                    const PairCand &lc = lci->second;
+                    // dissipation (dt = 1) on b (the centre) and on a, then QR steps a..b-1
+                    // bring the norm back into the centre
+                    if (dsite_active[b])
+                        local_op<T>(A(PHI, b), slotA[b], PHI.wd[b], PHI.wd[b + 1], dsite_ops.as<c64>() + (int64_t)b * 4,
+                                    nullptr, nullptr, nb, s);
+                    if (dsite_active[a]) {
+                        local_op<T>(A(PHI, a), slotA[a], PHI.wd[a], PHI.wd[a + 1], dsite_ops.as<c64>() + (int64_t)a * 4,
+                                    nullptr, nullptr, nb, s);
+                        for (int k = a; k < b; ++k) qr_step(PHI, k, nullptr);
+                    }
+                    norm2<T>(A(PHI, b), slotA[b], nc, norm2v.as<double>(), nullptr, nb, s);
+                    jump_decide(norm2v.as<double>(), J.seed, traj0, rngk.as<uint64_t>(), 1, jstate.as<int>(),
+                                fac.as<double>(), nb, s);
+                    mask_from_state(jstate.as<int>(), 0, mask.as<int>(), nullptr, nb, s);
+                    scale<T>(A(PHI, b), slotA[b], nc, fac.as<double>(), mask.as<int>(), nb, s);
+                    fill_int(count.as<int>(), 0, 1, s);
+                    mask_from_state(jstate.as<int>(), 1, mask.as<int>(), count.as<int>(), nb, s);
+                    if (sync_count(count.as<int>()) > 0) {
+                        // ρ_b with the centre at b, ρ_a with the centre at a
+                        auto rho_site = [&](int q) {
+                            const int cl = PHI.wd[q], cr = PHI.wd[q + 1];
+                            copy2d<T, double>(A(PHI, q), slotA[q], 2 * cr, Xbuf.as<c64>(), slotX, 2 * cr, cl, 2 * cr,
+                                              mask.as<int>(), nb, s);
+                            rho<T>(Xbuf.as<c64>(), slotX, A(PHI, q), slotA[q], cl, 2, cr, rho1.as<c64>() + (int64_t)q * 4,
+                                   (int64_t)L * 4, mask.as<int>(), nb, s);
+                        };
+                        rho_site(b);
+                        move_centre(a);
+                        rho_site(a);
+                        jump_choose(rho1.as<c64>(), (int64_t)L * 4, L, lc.n, lc.site.as<int>(), lc.LdL.as<c64>(),
+                                    lc.gamma.as<double>(), 1.0, J.seed, traj0, rngk.as<uint64_t>(), jstate.as<int>(),
+                                    chosen.as<int>(), wbuf.as<double>(), nb, s);
+                        for (int q : {a, b}) {
+                            jump_opidx(jstate.as<int>(), chosen.as<int>(), lc.site.as<int>(), lc.op.as<int>(),
+                                       lc.site2.as<int>(), lc.op2.as<int>(), q, opidx.as<int>(), nb, s);
+                            local_op<T>(A(PHI, q), slotA[q], PHI.wd[q], PHI.wd[q + 1], jump_ops.as<c64>(),
+                                        opidx.as<int>(), nullptr, nb, s);
+                        }
+                        fill_int(count.as<int>(), 0, 1, s);
+                        mask_from_state(jstate.as<int>(), 1, mask.as<int>(), count.as<int>(), nb, s);
+                        res.jumps += sync_count(count.as<int>());
+                        mask_from_state(jstate.as<int>(), -1, mask.as<int>(), nullptr, nb, s);
+                        for (int k = a; k < b; ++k) qr_step(PHI, k, nullptr);  // centre back at b
+                        centre = b;
+                        norm2<T>(A(PHI, b), slotA[b], nc, norm2v.as<double>(), mask.as<int>(), nb, s);
+                        jump_decide(norm2v.as<double>(), 0, 0, rngk.as<uint64_t>(), 0, opidx.as<int>(),
+                                    fac.as<double>(), nb, s);
+                        scale<T>(A(PHI, b), slotA[b], nc, fac.as<double>(), mask.as<int>(), nb, s);
+                    }
+                    continue;
+                }
+                // no noise process acts on a or b: normalize!
                 norm2<T>(A(PHI, b), slotA[b], nc, norm2v.as<double>(), nullptr, nb, s);
                 jump_decide(norm2v.as<double>(), 0, 0, rngk.as<uint64_t>(), 0, opidx.as<int>(), fac.as<double>(), nb,
                             s);
@@ -1671,11 +1771,6 @@ void validate_job(const JobIn &job) {
         if (g.kind == 2 && std::abs(g.s1 - g.s0) != 1) {
             if (g.rule != 2)
                 throw std::invalid_argument("long-range 2-qubit gates run only as TEBD SWAP networks (no TDVP windows)");
-            if (job.has_noise)
-                for (auto &p : job.noise)
-                    if (p.s0 == g.s0 || p.s0 == g.s1 || (p.nsites == 2 && (p.s1 == g.s0 || p.s1 == g.s1)))
-                        throw std::invalid_argument(
-                            "noise processes on the sites of a long-range gate are not supported by the GPU path");
         }
     }
 }
