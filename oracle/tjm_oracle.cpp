@@ -15,7 +15,7 @@
 //   Jumps              src/StochasticProcess.jl:35-330
 //   Simulator loop     src/Simulator.jl:22-125
 //   DigitalTJM         src/DigitalTJM.jl:86-305, 307-358, 363-450 (TEBD, TDVP
-//                      windows of adjacent pairs, SWAP networks)
+//                      windows, SWAP networks)
 // The dense kernels the reference takes from LAPACK/KrylovKit are written
 // here from their textbook definitions: one-sided (Hestenes) Jacobi SVD,
 // Householder QR/LQ, Lanczos with full re-orthogonalisation, and a cyclic
@@ -1325,6 +1325,61 @@ void window_tdvp_gate(MPSState &s, const cd *h, int a, int b, double thr, int ma
     s.oc = b;
 }
 
+// apply_window! :TDVP on a long-range pair a < b - 1 (src/DigitalTJM.jl:335-358):
+// the window a..b with the window MPO (src/DigitalTJM.jl:192-236: coeff g_a on
+// a, g_b on b, identities between, bonds of dimension 1) and the circuit
+// sweep (src/Algorithms.jl:606-627): forward 2-site updates with +1 and the
+// 1-site back-evolution by -1, then the edge step (+1, split right).  `h` is
+// coeff g_a (x) g_b in the gate-matrix index order; its Kronecker factors
+// (unique up to a scalar that cancels in every effective operator) give
+// the two MPO tensors.
+void window_tdvp_long(MPSState &s, const cd *h, int a, int b, double thr, int max_bond) {
+    int im = 0;
+    for (int q = 1; q < 16; ++q)
+        if (std::abs(h[q]) > std::abs(h[im])) im = q;
+    const int r0 = im % 4, c0 = im / 4;  // row p1 + 2 p2, column p1' + 2 p2'
+    const int p1s = r0 % 2, p2s = r0 / 2, p1cs = c0 % 2, p2cs = c0 / 2;
+    T4 Wa, Wb, Wi;
+    for (T4 *W : {&Wa, &Wb, &Wi}) {
+        W->dl = W->dr = 1;
+        W->po = W->pi = 2;
+        W->v.assign(4, 0.0);
+    }
+    Wi.v[0] = Wi.v[3] = 1.0;
+    const cd piv = h[im];
+    for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y) {
+            Wa.v[x + 2 * y] = h[(x + 2 * p2s) + 4 * (y + 2 * p2cs)];          // A[x, y] B[p2s, p2cs]
+            Wb.v[x + 2 * y] = h[(p1s + 2 * x) + 4 * (p1cs + 2 * y)] / piv;    // A[p1s, p1cs] B[x, y] / (A B)[piv]
+        }
+    shift_oc(s, a);
+    const int Lw = b - a + 1;
+    MPSState w;
+    w.L = Lw;
+    w.oc = 1;
+    for (int q = a; q <= b; ++q) w.t.push_back(s.t[q - 1]);
+    std::vector<T4> H(Lw, Wi);
+    H[0] = Wa;
+    H[Lw - 1] = Wb;
+    TdvpCtx C;
+    C.H = &H;
+    C.cfg.dt = 1.0;
+    C.cfg.max_bond = max_bond;
+    C.cfg.thr = thr;
+    C.cfg.kdim = 25;
+    C.cfg.ktol = 1e-10;
+    C.cnt = nullptr;
+    std::vector<Env> EL(Lw + 1), ER(Lw + 1);
+    const int rb = w.t[Lw - 1].dr, lb = w.t[0].dl;
+    ER[Lw] = Env{rb, 1, rb, identity_env(rb, 1)};
+    for (int i = Lw; i >= 2; --i) ER[i - 1] = update_right_env(w.t[i - 1], H[i - 1], ER[i]);
+    EL[0] = Env{lb, 1, lb, identity_env(lb, 1)};
+    for (int i = 1; i <= Lw - 2; ++i) two_site_forward(w, C, EL, ER, i, 1.0, true);
+    two_site_forward(w, C, EL, ER, Lw - 1, 1.0, false);  // _two_site_update_edge_circuit!
+    for (int q = 0; q < Lw; ++q) s.t[a - 1 + q] = w.t[q];
+    s.oc = b;
+}
+
 void run_digital_trajectory(const yaqs_digital_job *job, const MPSState &psi0, const std::vector<Gate> &gates,
                             const std::vector<std::vector<int>> &layers, const std::vector<int> &sample_after,
                             bool sample_at_start, int nsamp, const std::vector<Proc> *noise,
@@ -1352,7 +1407,9 @@ void run_digital_trajectory(const yaqs_digital_job *job, const MPSState &psi0, c
             // apply_window! sorts the sites and applies the matrix unchanged
             // on (min, max) (src/DigitalTJM.jl:307-334)
             int a = std::min(G.s0, G.s1), b = std::max(G.s0, G.s1);
-            if (b > a + 1) {
+            if (b > a + 1 && G.method == YAQS_GATE_WINDOW_TDVP) {
+                window_tdvp_long(st, G.m, a, b, thr, mb);
+            } else if (b > a + 1) {
                 // long_range_method :TEBD: SWAP network (src/DigitalTJM.jl:314-330)
                 static const cd SW[16] = {1, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0, 0, 1};
                 for (int k = b - 1; k >= a + 1; --k) tebd_gate(st, SW, k, k + 1, thr, mb);
@@ -1508,8 +1565,8 @@ int oracle_run_digital(const yaqs_digital_job *job, double *traj_out, double *me
         if (G.kind == YAQS_GATE_1Q) load_c(G.matrix, gates[g].m, 4);
         if (G.kind == YAQS_GATE_2Q) {
             load_c(G.matrix, gates[g].m, 16);
-            if (G.site1 == G.site0 || (std::abs(G.site1 - G.site0) != 1 && G.method != YAQS_GATE_EXACT)) {
-                g_err = "long-range gates run only as TEBD SWAP networks";
+            if (G.site1 == G.site0) {
+                g_err = "2-qubit gate on a single site";
                 return YAQS_EINVAL;
             }
         }

@@ -5,7 +5,7 @@ nearest-neighbour SWAPs, applies the gate on (a, a+1) and unwinds.
 Pinned by the dense state vector: without truncation the network is the
 gate acting on the pair index s_a + 2 s_b of the original sites (closed
 forms for CX, Haar circuits mixing long-range and nearest-neighbour gates
-to 1e-10); the host rejects long-range TDVP windows.  CPU only."""
+to 1e-10); TJMOptions picks the method per gate.  CPU only."""
 import numpy as np
 import pytest
 
@@ -82,11 +82,12 @@ def test_swap_network_matches_statevector(oracle_lib, L, depth, seed):
         assert abs(m[i, -1] - zi) < 1e-10, (i, m[i, -1], zi)
 
 
-def test_long_range_tdvp_window_rejected():
+def test_long_range_method_selects_the_gate_method():
     c = Y.DigitalCircuit(4)
     c.add_gate(G.RzzGate(0.3), [1, 3])
+    c.add_gate(G.RzzGate(0.3), [3, 4])
     cfg = Y.StrongMeasurementConfig([Y.Observable("z1", G.ZGate(), 1)], num_traj=1)
-    with pytest.raises(NotImplementedError):
-        _abi.pack_digital(Y.MPS(4), c, cfg, None, alg_options=Y.TJMOptions())
+    h = _abi.pack_digital(Y.MPS(4), c, cfg, None, alg_options=Y.TJMOptions(local_method="TEBD"))
+    assert h.job.gates[0].method == _abi.GATE_WINDOW_TDVP and h.job.gates[1].method == _abi.GATE_EXACT
     h = _abi.pack_digital(Y.MPS(4), c, cfg, None, alg_options=Y.TJMOptions(long_range_method="TEBD"))
-    assert h.job.gates[0].method == _abi.GATE_EXACT
+    assert h.job.gates[0].method == _abi.GATE_EXACT and h.job.gates[1].method == _abi.GATE_WINDOW_TDVP

@@ -162,3 +162,58 @@ def test_options_and_errors():
     m = np.array(g.matrix[:32]).view(complex).reshape(4, 4, order="F")
     # sites [2, 1]: element 1 (X) belongs to site 2, the high factor; coeff 1.0
     assert np.allclose(m, np.kron(X, Z))
+
+
+def _saturated_mps(L, seed):
+    """A random state as an MPS with every bond at its maximal dimension."""
+    rng = np.random.default_rng(seed)
+    psi = rng.standard_normal(2 ** L) + 1j * rng.standard_normal(2 ** L)
+    psi /= np.linalg.norm(psi)
+    tensors, T, cl = [], psi.reshape(1, -1), 1
+    for _ in range(L - 1):
+        U, S, Vh = np.linalg.svd(T.reshape(cl * 2, -1), full_matrices=False)
+        tensors.append(U.reshape(cl, 2, len(S)))
+        T, cl = S[:, None] * Vh, len(S)
+    tensors.append(T.reshape(cl, 2, 1))
+    return Y.MPS(L, tensors=tensors), psi
+
+
+def _apply_pair(psi, L, U, a, b):
+    """U on the pair index s_a + 2 s_b of sites a < b (0-based), site 0 the
+    most significant qubit of psi."""
+    t = np.moveaxis(psi.reshape([2] * L), (a, b), (0, 1))
+    sh = t.shape
+    t = t.transpose(1, 0, *range(2, L)).reshape(4, -1)
+    t = (U @ t).reshape(2, 2, *sh[2:]).transpose(1, 0, *range(2, L))
+    return np.moveaxis(t, (0, 1), (a, b)).reshape(-1)
+
+
+def test_long_range_windows_exact_on_saturated_bonds(oracle_lib):
+    """With every bond at its maximal dimension the projectors of the circuit
+    sweep are identities, so the forward +1 / back -1 steps cancel and the
+    window applies exp(-i h) exactly: long-range and nearest-neighbour
+    generator gates (both site orders, CX controlling on the site listed
+    first) against the dense state vector."""
+    L = 5
+    mps, psi = _saturated_mps(L, 7)
+    gates = [(G.RzzGate(0.9), [1, 3]), (G.RxxGate(-1.3), [2, 5]), (G.CPhaseGate(2.1), [5, 1]),
+             (G.CXGate(), [4, 2]), (G.RyyGate(0.4), [3, 4]), (G.CZGate(), [1, 4])]
+    c = Y.DigitalCircuit(L)
+    for op, sites in gates:
+        c.add_gate(op, sites)
+        c.add_sample_barrier()  # one gate per layer: the dense order is the circuit order
+    obs = [Y.Observable(f"z{i}", G.ZGate(), i) for i in range(1, L + 1)] + \
+          [Y.Observable(f"x{i}", G.XGate(), i) for i in range(1, L + 1)]
+    cfg = Y.StrongMeasurementConfig(obs, num_traj=1, max_bond_dim=64, truncation_threshold=0.0)
+    h = _abi.pack_digital(mps, c, cfg, None, alg_options=Y.TJMOptions())
+    assert all(g.method == _abi.GATE_WINDOW_TDVP for g in h.job.gates[:h.job.n_gates] if g.kind == _abi.GATE_2Q)
+    _, m, _, _ = oracle_lib.run_digital(h, nthreads=1)
+    for k, (op, sites) in enumerate(gates):
+        a, b = min(sites) - 1, max(sites) - 1
+        U = _pair_unitary(Y.circuit.DigitalGate(op, sites)) if isinstance(op, G.CXGate) else op.matrix
+        psi = _apply_pair(psi, L, U, a, b)
+        for i in range(L):
+            for q, P in ((0, Z), (1, X)):
+                t = psi.reshape(2 ** i, 2, -1)
+                ref = np.einsum("lpr,pq,lqr->", t.conj(), P, t).real
+                assert abs(m[q * L + i, k] - ref) < 1e-9, (k, i, q, m[q * L + i, k], ref)

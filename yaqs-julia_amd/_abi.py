@@ -209,10 +209,7 @@ def pack_digital(mps, circuit, cfg, noise_model, *, precision: int = PREC_FP64, 
         q.site0 = g.sites[0]
         q.site1 = g.sites[1] if len(g.sites) == 2 else g.sites[0]
         long_range = q.kind == GATE_2Q and abs(q.site1 - q.site0) > 1
-        if long_range and tdvp_long:
-            raise NotImplementedError("long-range TDVP windows are not supported by the GPU path; "
-                                      "use TJMOptions(long_range_method='TEBD') for SWAP networks")
-        if tdvp and q.kind == GATE_2Q and not long_range:
+        if q.kind == GATE_2Q and (tdvp_long if long_range else tdvp):
             q.method = GATE_WINDOW_TDVP
             _fill_matrix(q.matrix, window_hamiltonian(g))
         else:
