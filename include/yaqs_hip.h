@@ -148,7 +148,12 @@ typedef enum {
  *   vectors, so the engine applies exp(-i h) directly), then
  *   split_mps_tensor_svd (src/Algorithms.jl:409-445): keep max(k, 2) where
  *   k is the first tail index at which the discarded weight reaches the
- *   threshold, at most max_bond_dim. */
+ *   threshold, at most max_bond_dim.  On a non-adjacent pair a < b the
+ *   window is a..b: the window MPO carries the Kronecker factors of h on a
+ *   and b and identities between (bond dimension 1), and the circuit sweep
+ *   runs forward 2-site steps by +1 with the 1-site back-evolution by -1,
+ *   then the edge step by +1 (src/Algorithms.jl:631-672), with identity
+ *   environments at the window's outer bonds; h must be a Kronecker product. */
 typedef enum {
     YAQS_GATE_EXACT = 0,
     YAQS_GATE_WINDOW_TDVP = 1
@@ -157,7 +162,7 @@ typedef enum {
 typedef struct {
     int32_t kind;
     int32_t method;          /* yaqs_gate_method; 2Q gates only, 0 otherwise */
-    int64_t site0, site1;    /* 1-based; non-adjacent 2Q gates: SWAP network, EXACT only */
+    int64_t site0, site1;    /* 1-based; non-adjacent 2Q gates: SWAP network or long window */
     double matrix[32];
 } yaqs_gate;
 
@@ -167,7 +172,7 @@ typedef struct {
  * apply_window! (src/DigitalTJM.jl:314-330: SWAPs move the higher site next
  * to the lower one, the gate acts there, the SWAPs unwind, each step an
  * apply_local_gate_exact!), then the gate's local noise acts on its two
- * sites.  TDVP windows of long-range gates are not supported. */
+ * sites; with YAQS_GATE_WINDOW_TDVP it runs the TDVP window a..b. */
 typedef struct {
     int64_t L;
     const yaqs_mps_tensor *mps;

@@ -113,12 +113,13 @@ def _generator_circuit(L, depth, seed):
     return c
 
 
-def _run_opts(ctx, oracle_lib, L, circ, cfg, noise, opts, seed=0):
-    hg = _abi.pack_digital(Y.MPS(L), circ, cfg, noise, seed=seed, alg_options=opts)
+def _run_opts(ctx, oracle_lib, L, circ, cfg, noise, opts, seed=0, mps=None):
+    mps = mps or Y.MPS(L)
+    hg = _abi.pack_digital(mps, circ, cfg, noise, seed=seed, alg_options=opts)
     ncol = int(ctx.lib.yaqs_digital_num_samples(hg.job))
     jid = ctx.submit_digital(hg)
     g, gm, gb, st = ctx.wait(jid, len(cfg.observables), ncol, cfg.num_traj, want_bonds=True)
-    hr = _abi.pack_digital(Y.MPS(L), circ, cfg, noise, seed=seed, alg_options=opts)
+    hr = _abi.pack_digital(mps, circ, cfg, noise, seed=seed, alg_options=opts)
     r, rm, rb, rst = oracle_lib.run_digital(hr, nthreads=8)
     return g, gm, gb, st, r, rm, rb, rst
 
@@ -180,4 +181,60 @@ def test_swap_network_parity(ctx, oracle_lib, noise_sites):
     if noise_sites:
         assert st["jumps"] > 0
     assert np.abs(g - r).max() < 1e-9
+    assert np.array_equal(gb, rb)
+
+
+def _long_range_generator_circuit(L, depth, seed):
+    G = Y.gates
+    rng = np.random.default_rng(seed)
+    c = Y.DigitalCircuit(L)
+    for q in range(1, L + 1):
+        c.add_gate(G.RyGate(rng.uniform(0, np.pi)), [q])
+    for _ in range(depth):
+        a, b = sorted(int(x) for x in rng.choice(np.arange(1, L + 1), 2, replace=False))
+        th = rng.uniform(-np.pi, np.pi)
+        op = [G.RxxGate(th), G.RyyGate(th), G.RzzGate(th), G.CZGate(), G.CPhaseGate(th), G.CXGate()][rng.integers(6)]
+        c.add_gate(op, [a, b] if rng.random() < 0.5 else [b, a])
+        q = int(rng.integers(1, L + 1))
+        c.add_gate(G.RxGate(rng.uniform(-1, 1)), [q])
+    return c
+
+
+def _generic_mps(L, chi, seed):
+    """Random MPS in right-canonical form whose bonds all have full rank
+    (dimension min(2^i, 2^(L-i), chi))."""
+    rng = np.random.default_rng(seed)
+    dims = [min(2 ** i, 2 ** (L - i), chi) for i in range(L + 1)]
+    ts = [rng.standard_normal((dims[i], 2, dims[i + 1])) + 1j * rng.standard_normal((dims[i], 2, dims[i + 1]))
+          for i in range(L)]
+    m = Y.MPS(L, tensors=ts)
+    m.normalize()
+    return m
+
+
+@pytest.mark.parametrize("noisy", [False, True])
+def test_long_range_tdvp_windows_parity(ctx, oracle_lib, noisy):
+    """long_range_method :TDVP (the reference default): the circuit sweep over
+    the window of each long-range generator gate (window MPO of bond
+    dimension 1, forward 2-site steps with 1-site back-evolution, edge step),
+    per trajectory against the oracle's sweep.  The sweep projects onto the
+    tangent space of the window, which depends on the basis a split picks for
+    exactly zero singular values (the TDVP split keeps at least two): from a
+    product state the reference's result depends on its SVD's null-space
+    completion.  The run therefore starts from a random MPS whose bonds have
+    full rank (dimension 4 inside), where the tangent spaces are fixed by the
+    state."""
+    print(f"case: digital long-range TDVP windows L=7 noisy={noisy}", flush=True)
+    L = 7
+    circ = _long_range_generator_circuit(L, 12, seed=31)
+    obs = [Y.Observable(f"z{i}", Y.gates.ZGate(), i) for i in range(1, L + 1)] + \
+          [Y.Observable(f"x{i}", Y.gates.XGate(), i) for i in range(1, L + 1)]
+    cfg = Y.StrongMeasurementConfig(obs, num_traj=12 if noisy else 1, max_bond_dim=16, truncation_threshold=1e-9)
+    noise = _depolarizing(L, 0.03) if noisy else None
+    g, gm, gb, st, r, rm, rb, rst = _run_opts(ctx, oracle_lib, L, circ, cfg, noise, Y.TJMOptions(), seed=9,
+                                              mps=_generic_mps(L, 4, 17))
+    assert st["jumps"] == rst["jumps"]
+    if noisy:
+        assert st["jumps"] > 0
+    assert np.abs(g - r).max() < 1e-8
     assert np.array_equal(gb, rb)

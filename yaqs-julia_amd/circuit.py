@@ -20,10 +20,9 @@ class DigitalGate:
 class TJMOptions:
     """TJMOptions (src/DigitalTJM.jl:70-76): how 2-qubit gates are applied.
     "TDVP" (the reference default) evolves the gate's window under its
-    generator; "TEBD" applies the gate matrix.  For long-range gates the GPU
-    path runs the TEBD SWAP network (src/DigitalTJM.jl:314-330) only:
-    long_range_method "TDVP" (the reference default) raises when the
-    circuit has a long-range gate."""
+    generator (for a long-range gate: the circuit sweep over the window
+    s1..s2); "TEBD" applies the gate matrix (for a long-range gate: the SWAP
+    network, src/DigitalTJM.jl:314-330)."""
     local_method: str = "TDVP"
     long_range_method: str = "TDVP"
 

@@ -203,8 +203,9 @@ yq::JobIn convert_digital(const yaqs_digital_job *job) {
         if (G.method != YAQS_GATE_EXACT) {
             if (G.method != YAQS_GATE_WINDOW_TDVP || q.kind != 2)
                 throw std::invalid_argument("bad gate method (window TDVP applies to 2-qubit gates only)");
-            expm_minus_i4(q.m);
             q.rule = 0;
+            if (std::abs(q.s1 - q.s0) == 1) expm_minus_i4(q.m);  // the edge step alone
+            else q.win_long = true;                               // the circuit sweep of the window
         }
         J.gates.push_back(q);
     }

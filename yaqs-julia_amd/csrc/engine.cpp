@@ -494,6 +494,7 @@ class Engine {
     DevBuf gate_ops;  // digital: gate matrices (engine order), 16 c64 per gate
     std::vector<int64_t> gate_off;
     int64_t swap_off = -1;  // SWAP matrix in gate_ops
+    std::map<int, std::vector<CsrDev>> win_csr;  // per long-range TDVP gate: left, right, pair tables
     DevBuf dissd_ops;  // digital dissipation (dt = 1) per gate pair and site: [(2 a + r) * 4]
     std::vector<int> dgt_active;
     // gate pairs whose two dissipators are multiples of the identity (Pauli
@@ -604,7 +605,9 @@ void Engine<T>::setup() {
     if (!J.digital) alloc_state(PSI);
     EL.resize(L + 1);
     ER.resize(L + 1);
-    if (!J.digital) {
+    bool need_win = false;  // digital: long-range TDVP windows run the TDVP steps with window MPOs
+    for (auto &G2 : J.gates) need_win = need_win || G2.win_long;
+    if (!J.digital || need_win) {
         for (int b = 0; b <= L; ++b) {
             EL[b].alloc(slotE[b] * B * sizeof(CT));
             ER[b].alloc(slotE[b] * B * sizeof(CT));
@@ -612,13 +615,40 @@ void Engine<T>::setup() {
         csrL.resize(L);
         csrR.resize(L);
         csr2.resize(std::max(0, L - 1));
-        for (int i = 0; i < L; ++i) {
-            upload_csr(csrL[i], csr_left(J.mpo[i], Dm[i], 2, Dm[i + 1]), s);
-            upload_csr(csrR[i], csr_right(J.mpo[i], Dm[i], 2, Dm[i + 1]), s);
+        if (!J.digital) {
+            for (int i = 0; i < L; ++i) {
+                upload_csr(csrL[i], csr_left(J.mpo[i], Dm[i], 2, Dm[i + 1]), s);
+                upload_csr(csrR[i], csr_right(J.mpo[i], Dm[i], 2, Dm[i + 1]), s);
+            }
+            for (int i = 0; i + 1 < L; ++i)
+                upload_csr(csr2[i], csr_left(merge_mpo(J.mpo[i], J.mpo[i + 1], Dm[i], Dm[i + 1], Dm[i + 2]), Dm[i],
+                                             4, Dm[i + 2]), s);
+        } else {
+            // window MPO of each long-range TDVP gate (construct_window_mpo,
+            // src/DigitalTJM.jl:192-236): the Kronecker factors of h on a and b
+            // (unique up to a scalar that cancels in every effective operator),
+            // identities between, bonds of dimension 1
+            const std::vector<c64> WI = {mk(1, 0), mk(0, 0), mk(0, 0), mk(1, 0)};
+            for (size_t g = 0; g < J.gates.size(); ++g) {
+                const GateIn &G2 = J.gates[g];
+                if (!G2.win_long) continue;
+                const int a = std::min(G2.s0, G2.s1), b = std::max(G2.s0, G2.s1), n = b - a + 1;
+                c64 e4[16], Af[4], Bf[4];
+                op4_engine(G2.m, e4);
+                if (!kron_factor(e4, Af, Bf))
+                    throw std::invalid_argument("the window Hamiltonian of a long-range gate must be g_a (x) g_b");
+                const std::vector<c64> WA(Af, Af + 4), WB(Bf, Bf + 4);
+                auto W = [&](int q) -> const std::vector<c64> & { return q == 0 ? WA : (q == n - 1 ? WB : WI); };
+                std::vector<CsrDev> v((size_t)3 * n - 1);
+                for (int q = 0; q < n; ++q) {
+                    upload_csr(v[q], csr_left(W(q), 1, 2, 1), s);
+                    upload_csr(v[n + q], csr_right(W(q), 1, 2, 1), s);
+                }
+                for (int q = 0; q + 1 < n; ++q)
+                    upload_csr(v[2 * n + q], csr_left(merge_mpo(W(q), W(q + 1), 1, 1, 1), 1, 4, 1), s);
+                win_csr[(int)g] = std::move(v);
+            }
         }
-        for (int i = 0; i + 1 < L; ++i)
-            upload_csr(csr2[i], csr_left(merge_mpo(J.mpo[i], J.mpo[i + 1], Dm[i], Dm[i + 1], Dm[i + 2]), Dm[i], 4,
-                                         Dm[i + 2]), s);
         V.alloc((size_t)B * sV() * sizeof(CT));
         T1.alloc((size_t)B * slotS * sizeof(CT));
         T2.alloc((size_t)B * slotS * sizeof(CT));
@@ -1553,7 +1583,8 @@ void Engine<T>::run_digital(int64_t t0, int nb_, double *d_out, int64_t *h_bonds
             if (b > a + 1) {
                 // long-range gate, TEBD method: SWAP network (src/DigitalTJM.jl:314-330)
                 // moves site b next to a, the gate acts on (a, a+1), the SWAPs unwind;
-                // every step is apply_local_gate_exact! (rule T-D)
+                // every step is apply_local_gate_exact! (rule T-D).  TDVP method: the
+                // window sweep below.
                 auto nn = [&](int k, const c64 *op) {
                     move_centre(k);
                     merge(PHI, k);
@@ -1561,9 +1592,43 @@ void Engine<T>::run_digital(int64_t t0, int nb_, double *d_out, int64_t *h_bonds
                     split(PHI, k, true, 2, J.thr);
                     centre = k + 1;
                 };
-                for (int k = b - 1; k >= a + 1; --k) nn(k, gops + swap_off);
-                nn(a, gops + gate_off[g]);
-                for (int k = a + 1; k <= b - 1; ++k) nn(k, gops + swap_off);
+                if (G2.win_long) {
+                    // long_range_method :TDVP: the circuit sweep over the window a..b
+                    // (src/DigitalTJM.jl:335-358, src/Algorithms.jl:606-672): forward
+                    // 2-site steps by +1 with the 1-site back-evolution by -1, then
+                    // the edge step by +1 split right; identity environments at the
+                    // window's outer bonds (src/Algorithms.jl:801-818)
+                    std::vector<CsrDev> &wv = win_csr.at(g);
+                    const int n = b - a + 1;
+                    auto swap_tables = [&]() {
+                        for (int q = 0; q < n; ++q) {
+                            std::swap(csrL[a + q], wv[q]);
+                            std::swap(csrR[a + q], wv[n + q]);
+                        }
+                        for (int q = 0; q + 1 < n; ++q) std::swap(csr2[a + q], wv[2 * n + q]);
+                    };
+                    swap_tables();
+                    move_centre(a);
+                    identity_env<T>(ER[b + 1].as<CT>(), slotE[b + 1], PHI.wd[b + 1], 1, 1, nb, s);
+                    for (int i = b; i >= a + 2; --i) env_right(PHI, i);
+                    identity_env<T>(EL[a].as<CT>(), slotE[a], PHI.wd[a], 1, 0, nb, s);
+                    for (int i = a; i + 2 <= b; ++i) {
+                        merge(PHI, i);
+                        expm(PHI, true, i, theta.as<CT>(), nmax, 1.0, res);
+                        split(PHI, i, true, 0, J.thr);
+                        env_left(PHI, i);
+                        expm(PHI, false, i + 1, A(PHI, i + 1), slotA[i + 1], -1.0, res);
+                    }
+                    merge(PHI, b - 1);
+                    expm(PHI, true, b - 1, theta.as<CT>(), nmax, 1.0, res);
+                    split(PHI, b - 1, true, 0, J.thr);
+                    centre = b;
+                    swap_tables();  // ownership only: the launched kernels keep their table pointers
+                } else {
+                    for (int k = b - 1; k >= a + 1; --k) nn(k, gops + swap_off);
+                    nn(a, gops + gate_off[g]);
+                    for (int k = a + 1; k <= b - 1; ++k) nn(k, gops + swap_off);
+                }
                 const int64_t nc = (int64_t)PHI.wd[b] * 2 * PHI.wd[b + 1];
                 auto lci = lr_cand.find(a * L + b);
                 if (J.has_noise && !J.noise.empty() && lci != lr_cand.end() && lci->second.n > 0) {
@@ -1769,8 +1834,8 @@ void validate_job(const JobIn &job) {
         if (g.s0 < 0 || g.s0 >= job.L || g.s1 < 0 || g.s1 >= job.L) throw std::invalid_argument("gate site out of range");
         if (g.kind == 2 && g.s1 == g.s0) throw std::invalid_argument("2-qubit gate on a single site");
         if (g.kind == 2 && std::abs(g.s1 - g.s0) != 1) {
-            if (g.rule != 2)
-                throw std::invalid_argument("long-range 2-qubit gates run only as TEBD SWAP networks (no TDVP windows)");
+            if (g.rule != 2 && !g.win_long)
+                throw std::invalid_argument("bad method for a long-range 2-qubit gate");
         }
     }
 }

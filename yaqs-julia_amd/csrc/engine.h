@@ -32,6 +32,7 @@ struct GateIn {
     int kind;  // 1: 1q, 2: 2q, 3: sample barrier
     int s0, s1;
     int rule = 2;  // truncation rule of the 2q split: 2 (TEBD) or 0 (window TDVP)
+    bool win_long = false;  // long-range TDVP window: m holds h = coeff g_a (x) g_b
     c64 m[16];  // reference order
 };
 
