@@ -556,7 +556,9 @@ int64_t Engine<T>::footprint(const JobIn &job, int B) {
     }
     const bool noise = job.has_noise;
     int64_t elems = sites * (job.digital ? 1 : (noise ? 3 : 2));  // PHI (+ PSI, + scratch)
-    if (!job.digital) elems += 2 * envs + (int64_t)(job.kdim + 1) * nmax + 2 * smax;
+    bool windows = false;  // digital: long-range TDVP windows use the environments and Lanczos basis too
+    for (auto &g : job.gates) windows = windows || g.win_long;
+    if (!job.digital || windows) elems += 2 * envs + (int64_t)(job.kdim + 1) * nmax + 2 * smax;
     elems += nmax + cm * cm + sites;  // theta, Cf, absorb (bounded by the state size)
     elems += cm * 32;                 // CGS2 projection scratch
     int64_t bytes = elems * (int64_t)sizeof(cplx<T>);
