@@ -6,16 +6,22 @@ dt=0.05, chi_max=64, threshold 1e-9, Krylov dim <= 25, observables Z_i and
 X_i, 1000 trajectories, complex-FP32 contractions with FP64 Lanczos / Gram /
 norms ("mixed").  One step = one TJM time step of every trajectory (the
 step_through and sample of analog_tjm_2: two TDVP sweeps, dissipation,
-jumps, 60 observables).  The job starts at t=0; the first W steps are the
-untimed warm-up, the next K steps are timed (stream-synchronised at both
-ends).  Metric: trajectory-site-steps per second = ntraj * L * K / time.
+jumps, 60 observables).  The job starts at t=0; the timed window is K steps
+(--steps) that start at saturated bonds: the untimed warm-up is
+max(--warmup, 19) steps, since the bulk bonds reach chi=64 by step 19 and
+every later step of the T=4 job costs the same.  Metric:
+trajectory-site-steps per second = ntraj * L * K / window seconds.
 
-Also reported on the same window: the plain per-trajectory FP64 GPU path
-(one trajectory per batch, on a trajectory subset) and the FP64 CPU oracle
-(on a bounded trajectory subset, OpenMP, one trajectory per thread), plus the
-roofline of the dominant kernel (the complex-FP32 MFMA GEMM) from HIP events
-around every launch in the window (in a second run of the job as one wave
-when the context ran it as concurrent waves, whose kernels share the device).
+Also reported, on the same steps: batched FP64 on the GPU (the precision
+gain, separated from the batching gain), the plain per-trajectory FP64 GPU
+path (one trajectory per batch, on a trajectory subset), the ensemble
+<Z_i> and <X_i> error of the mixed run against batched FP64 over all
+trajectories, and the FP64 CPU oracle (a bounded trajectory subset, one
+trajectory per thread, OpenBLAS zgemm / LAPACK zgesdd as the reference
+links them), plus the roofline of the dominant kernel (the complex-FP32
+MFMA GEMM) from HIP events around every launch in the window (in a second
+run of the job as one wave when the context ran it as concurrent waves,
+whose kernels share the device).
 """
 import argparse
 import ctypes as ct
@@ -30,6 +36,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "yaqs-julia_amd")
 FP32_MFMA_PEAK = 157.3e12  # MI355X dense FP32 matrix peak, FLOP/s
+SATURATED_WARMUP = 19      # steps from t=0 until the bulk bonds reach chi_max=64
+METRIC = "traj-site-steps/s (L=30,\u03c7=64, \u27e8Z\u27e9 err\u22641e-4) vs plain GPU; p50/p99 job latency"
 
 
 def _pkg():
@@ -84,15 +92,40 @@ def gpu_window(Y, lib, ctx, ntraj, warmup, steps, precision):
     return out, mean.reshape(nobs, ncol)
 
 
+def host_info():
+    """Host CPU model, logical CPUs of the machine and CPUs this process may
+    run on (the box's share)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count() or 1, "cpus_allowed": allowed}
+
+
 def cpu_window(Y, ntraj, warmup, steps, nthreads):
     """FP64 oracle on the same job restricted to `ntraj` trajectories (one per
-    OpenMP thread): wall time of steps warmup+1 .. warmup+steps, measured
-    inside the oracle for every trajectory (longest one reported)."""
+    OpenMP thread) with the reference's BLAS/LAPACK (OpenBLAS zgemm, LAPACK
+    zgesdd, one thread per call): wall time of steps warmup+1 ..
+    warmup+steps, measured inside the oracle for every trajectory (longest
+    one reported)."""
     if ROOT not in sys.path:
         sys.path.insert(0, ROOT)
     from oracle import oracle as O
-    h, _ = c2_job(Y, ntraj, warmup + steps, "fp64")
-    return O.run_analog_window(h, warmup, nthreads=nthreads)
+    blas = O.use_blas(True)
+    try:
+        h, _ = c2_job(Y, ntraj, warmup + steps, "fp64")
+        return O.run_analog_window(h, warmup, nthreads=nthreads), blas
+    finally:
+        O.use_blas(False)
 
 
 def c3_job(Y, ntraj, layers=40, L=40, chi=128, precision="mixed", p=0.01):
@@ -249,7 +282,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
-    ap.add_argument("--warmup", type=int, default=19, help="untimed steps from t=0 (19 reaches saturated chi=64)")
+    ap.add_argument("--warmup", type=int, default=SATURATED_WARMUP,
+                    help=f"untimed steps from t=0 (at least {SATURATED_WARMUP}: the window starts at saturated chi=64)")
     ap.add_argument("--ntraj", type=int, default=1000)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--plain-traj", type=int, default=2, help="trajectories for the plain FP64 per-trajectory path")
@@ -257,6 +291,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle timed steps (bounded CPU sample)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the batched FP64 leg")
+    ap.add_argument("--only", default="all", choices=["all", "roofline"],
+                    help="roofline: run the one-wave roofline window alone (for profiling)")
     ap.add_argument("--serve-jobs", type=int, default=12, help="jobs of the serving stream (0 = skip)")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3"],
                     help="c2: the analog headline line; c3: the digital brickwork line")
@@ -279,38 +316,54 @@ def main():
         c3_main(Y, args)
         return
     lib = Y._abi.load_library()
-    ctx = Y.Context(0)
-    print(f"[bench] C2 window: warmup={args.warmup} steps={args.steps} ntraj={args.ntraj} {args.precision}",
-          flush=True)
-    rep, mean = gpu_window(Y, lib, ctx, args.ntraj, args.warmup, args.steps, args.precision)
     L = 30
-    value = args.ntraj * L * args.steps / rep["timed_s"]
+    K = args.steps
+    win0 = max(args.warmup, SATURATED_WARMUP)
+    Kaux = min(K, 4)  # steps of the FP64, plain and roofline legs
+    window = {"first_step": win0 + 1, "last_step": win0 + K, "t": [win0 * 0.05, (win0 + K) * 0.05],
+              "rule": f"timed window starts at saturated bonds: warm-up = max(--warmup, {SATURATED_WARMUP}) steps"}
+    if args.only == "roofline":
+        # the roofline window alone (one wave), as profiled under rocprofv3
+        rctx = Y.Context(0, max_batch=args.ntraj)
+        rrep, _ = gpu_window(Y, lib, rctx, args.ntraj, win0, K, args.precision)
+        rctx.close()
+        print(json.dumps({"roofline_window": window, "timed_s": rrep["timed_s"],
+                          "gemm_launches": int(rrep["gemm_launches"]), "gemm_ms": rrep["gemm_ms"],
+                          "gemm_flop": rrep["gemm_flop"],
+                          "achieved_tflops": rrep["gemm_flop"] / (rrep["gemm_ms"] * 1e-3) / 1e12}), flush=True)
+        return
+    ctx = Y.Context(0)
+    print(f"[bench] C2 window: steps {win0 + 1}..{win0 + K} (warm-up {win0}, asked {args.warmup}) "
+          f"ntraj={args.ntraj} {args.precision}", flush=True)
+    rep, mean = gpu_window(Y, lib, ctx, args.ntraj, win0, K, args.precision)
+    value = args.ntraj * L * K / rep["timed_s"]
     print(f"[bench] GPU window {rep['timed_s']:.2f}s -> {value:.4g} traj-site-steps/s, max bond {rep['max_bond']}",
           flush=True)
     # The roofline comes from a window where the GEMMs have the device to
     # their wave: with concurrent waves, the HIP events around a launch also
     # span kernels of the other waves' streams.  So a job the context ran as
     # several concurrent waves is run once more as one wave (max_batch =
-    # ntraj) for the roofline figures.
+    # ntraj), over the first Kaux steps of the window, for the roofline.
     rrep, roof_note = rep, "the timed window above"
     if rep["concurrent_waves"] > 1:
         rctx = Y.Context(0, max_batch=args.ntraj)
-        print("[bench] roofline window: the same job as one wave", flush=True)
-        rrep, _ = gpu_window(Y, lib, rctx, args.ntraj, args.warmup, args.steps, args.precision)
+        print(f"[bench] roofline window: steps {win0 + 1}..{win0 + Kaux} as one wave", flush=True)
+        rrep, _ = gpu_window(Y, lib, rctx, args.ntraj, win0, Kaux, args.precision)
         rctx.close()
-        rv = args.ntraj * L * args.steps / rrep["timed_s"]
+        rv = args.ntraj * L * Kaux / rrep["timed_s"]
         print(f"[bench] one wave: {rrep['timed_s']:.2f}s -> {rv:.4g} traj-site-steps/s", flush=True)
-        roof_note = (f"the same window run as one wave of {args.ntraj} trajectories ({rv:.0f} traj-site-steps/s); "
-                     f"the timed run used {int(rep['concurrent_waves'])} concurrent waves")
+        roof_note = (f"steps {win0 + 1}..{win0 + Kaux} run as one wave of {args.ntraj} trajectories "
+                     f"({rv:.0f} traj-site-steps/s; the timed run used {int(rep['concurrent_waves'])} concurrent "
+                     f"waves); reproduce with: python bench.py --only roofline --steps {Kaux}")
     achieved = rrep["gemm_flop"] / (rrep["gemm_ms"] * 1e-3) if rrep["gemm_ms"] > 0 else 0.0
     out = {
-        "metric": "traj-site-steps/s (L=30, chi=64, <Z> err<=1e-4) vs plain GPU; p50/p99 job latency",
+        "metric": METRIC,
         "value": value,
         "unit": "traj-site-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * rep["timed_s"] / args.steps,
+        "ms_per_step": 1e3 * rep["timed_s"] / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -320,10 +373,10 @@ def main():
             "workload": "C2: Heisenberg XXX L=30 Neel, lowering+pauli_z gamma=0.01, 2-site TDVP dt=0.05, "
                         "chi_max=64, thr 1e-9, Krylov<=25, 60 observables",
             "ntraj": args.ntraj, "L": L, "chi_max": 64, "precision": args.precision,
-            "window_t": [args.warmup * 0.05, (args.warmup + args.steps) * 0.05],
+            "window": window,
             "max_bond_reached": int(rep["max_bond"]),
             "concurrent_waves": int(rep["concurrent_waves"]),
-            "flop_model_per_traj_site_step": rep["flop_model"] / (args.ntraj * L * args.steps),
+            "flop_model_per_traj_site_step": rep["flop_model"] / (args.ntraj * L * K),
             "matvecs": int(rep["matvecs"]),
         },
         "roofline": {
@@ -331,45 +384,65 @@ def main():
             "achieved": achieved / 1e12, "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
             "frac": achieved / FP32_MFMA_PEAK, "traffic": _pmc_traffic(),
             "launches": int(rrep["gemm_launches"]),
+            "flop_per_launch": rrep["gemm_flop"] / max(1, rrep["gemm_launches"]),
             "avg_launch_us": 1e3 * rrep["gemm_ms"] / max(1, rrep["gemm_launches"]),
             "gemm_share_of_window": rrep["gemm_ms"] * 1e-3 / rrep["timed_s"],
             "window": roof_note,
         },
     }
+    if args.precision == "mixed" and not args.no_fp64:
+        # batched FP64 on the same steps: the precision gain apart from the
+        # batching gain, and the reference for the metric's error bound
+        fctx = Y.Context(0)
+        print(f"[bench] batched FP64 GPU, steps {win0 + 1}..{win0 + Kaux}", flush=True)
+        frep, fmean = gpu_window(Y, lib, fctx, args.ntraj, win0, Kaux, "fp64")
+        fctx.close()
+        fv = args.ntraj * L * Kaux / frep["timed_s"]
+        out["batched_fp64_gpu"] = {"value": fv, "unit": "traj-site-steps/s", "ntraj": args.ntraj,
+                                   "steps": [win0 + 1, win0 + Kaux], "mixed_speedup": value / fv}
+        print(f"[bench] batched FP64 GPU: {fv:.4g} traj-site-steps/s", flush=True)
+        nc = win0 + Kaux + 1
+        ez = float(np.abs(mean[:L, :nc] - fmean[:L, :nc]).max())
+        ex = float(np.abs(mean[L:, :nc] - fmean[L:, :nc]).max())
+        dt_traj = float(np.abs(rep["traj"][:, :nc, :] - frep["traj"][:, :nc, :]).max())
+        out["accuracy"] = {"max_abs_err_Z": ez, "max_abs_err_X": ex, "max_abs_err_per_trajectory": dt_traj,
+                           "trajectories": args.ntraj, "times": [0.0, (nc - 1) * 0.05], "tolerance": 1e-4,
+                           "vs": "batched FP64 GPU path on the same seeds (it matches the FP64 oracle within 1e-10 "
+                                 "per trajectory in tests/test_gpu_c2.py); ensemble <Z_i>, <X_i> at every time"}
+        print(f"[bench] mixed vs FP64 ensemble over {args.ntraj} trajectories: <Z_i> {ez:.2e} <X_i> {ex:.2e}",
+              flush=True)
     if not args.no_plain and args.plain_traj > 0:
         pctx = Y.Context(0, max_batch=1)
         print(f"[bench] plain FP64 per-trajectory GPU path, {args.plain_traj} trajectories", flush=True)
-        prep, _ = gpu_window(Y, lib, pctx, args.plain_traj, args.warmup, args.steps, "fp64")
+        prep, _ = gpu_window(Y, lib, pctx, args.plain_traj, win0, Kaux, "fp64")
         pctx.close()
-        pv = args.plain_traj * L * args.steps / prep["timed_s"]
+        pv = args.plain_traj * L * Kaux / prep["timed_s"]
         out["plain_fp64_gpu"] = {"value": pv, "unit": "traj-site-steps/s", "ntraj": args.plain_traj,
-                                 "batch": 1, "speedup_of_batched": value / pv}
+                                 "batch": 1, "steps": [win0 + 1, win0 + Kaux], "speedup_of_batched": value / pv}
         print(f"[bench] plain FP64 per-trajectory GPU: {pv:.4g} traj-site-steps/s", flush=True)
-        # the metric's <Z> error: the same trajectories (same seeds) of the
-        # timed job against the FP64 path, every <Z_i> at every time step
-        k = min(args.plain_traj, args.ntraj)
-        dz = np.abs(rep["traj"][:L, :, :k] - prep["traj"][:L, :, :k])
-        out["accuracy"] = {"max_abs_err_Z": float(dz.max()), "trajectories": k, "tolerance": 1e-4,
-                           "vs": "plain FP64 GPU path (FP64 oracle parity 1e-10 in tests), same seeds, "
-                                 "every <Z_i> at every time step of trajectories 0..%d" % (k - 1)}
-        print(f"[bench] mixed vs FP64, <Z_i> of {k} trajectories: max |err| {dz.max():.2e}", flush=True)
     if args.serve_jobs > 0:
         print(f"[bench] serving stream, {args.serve_jobs} jobs", flush=True)
         out["serving"] = serve_stream(Y, args.serve_jobs)
         print(f"[bench] serving p50 {out['serving']['p50_s']:.2f}s p99 {out['serving']['p99_s']:.2f}s, "
               f"{out['serving']['throughput']:.4g} traj-site-steps/s", flush=True)
-    if not args.no_cpu:
-        nthreads = args.cpu_traj or min(16, os.cpu_count() or 1)
-        csteps = args.cpu_steps
-        print(f"[bench] CPU oracle, {nthreads} trajectories on {nthreads} threads, {args.warmup}+{csteps} steps",
-              flush=True)
-        t = cpu_window(Y, nthreads, args.warmup, csteps, nthreads)
-        cv = nthreads * L * csteps / t
-        out["cpu_baseline"] = {"value": cv, "unit": "traj-site-steps/s", "cores": nthreads, "kind": "port",
-                               "sample": f"FP64 oracle, {nthreads} trajectories (one per thread), steps "
-                                         f"{args.warmup + 1}..{args.warmup + csteps} of the same job"}
-        print(f"[bench] CPU oracle: {cv:.4g} traj-site-steps/s on {nthreads} threads", flush=True)
     ctx.close()
+    if not args.no_cpu:
+        hi = host_info()
+        nthreads = args.cpu_traj or min(16, hi["cpus_allowed"])
+        csteps = args.cpu_steps
+        print(f"[bench] CPU oracle, {nthreads} trajectories on {nthreads} threads, steps "
+              f"{win0 + 1}..{win0 + csteps} after {win0} untimed", flush=True)
+        t, blas = cpu_window(Y, nthreads, win0, csteps, nthreads)
+        cv = nthreads * L * csteps / t
+        out["cpu_baseline"] = {
+            "value": cv, "unit": "traj-site-steps/s", "cores": nthreads, "kind": "port",
+            "sample": f"FP64 oracle (CPU restatement of Yaqs.jl; OpenBLAS 0.3.29 zgemm/zgesdd, one thread each: "
+                      f"{'bound' if blas else 'not found, own loops'}), trajectories 0..{nthreads - 1} one per "
+                      f"thread, steps {win0 + 1}..{win0 + csteps} of the same job (the first step(s) of the GPU "
+                      f"window), slowest trajectory's time",
+            **hi}
+        print(f"[bench] CPU oracle: {cv:.4g} traj-site-steps/s on {nthreads} threads ({hi['cpu_model']}, "
+              f"nproc {hi['nproc']}, allowed {hi['cpus_allowed']})", flush=True)
     print(json.dumps(out), flush=True)
 
 

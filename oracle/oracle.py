@@ -52,8 +52,36 @@ def lib():
         L.oracle_last_window_seconds.restype = ct.c_double
         L.oracle_tdvp_evolve.argtypes = [ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_int, ct.c_int,
                                          ct.c_double, ct.c_int, ct.c_void_p, c_dp]
+        L.oracle_use_blas.argtypes = [ct.c_char_p, ct.c_char_p, ct.c_char_p]
+        L.oracle_blas_bound.restype = ct.c_int
+        L.oracle_use_lapack_svd.argtypes = [ct.c_char_p, ct.c_char_p]
         _lib = L
     return _lib
+
+
+def openblas_path():
+    """numpy's bundled OpenBLAS (0.3.29, 64-bit integers: the BLAS version of
+    the reference's Manifest), or None."""
+    import glob
+    d = os.path.join(os.path.dirname(os.path.dirname(np.__file__)), "numpy.libs")
+    hits = sorted(glob.glob(os.path.join(d, "libscipy_openblas64_*.so")))
+    return hits[0] if hits else None
+
+
+def use_blas(on: bool = True) -> bool:
+    """Timing baseline only: route the oracle's larger products through
+    OpenBLAS zgemm and its SVDs through LAPACK zgesdd (the library and
+    version the reference links, one thread per call), or back to its own
+    loops and Jacobi SVD, which the parity tests pin (the default).  Returns
+    whether BLAS is bound."""
+    L = lib()
+    path = openblas_path() if on else None
+    if on and path:
+        L.oracle_use_blas(path.encode(), b"scipy_zgemm_64_", b"scipy_openblas_set_num_threads64_")
+        L.oracle_use_lapack_svd(path.encode(), b"scipy_zgesdd_64_")
+    else:
+        L.oracle_use_blas(None, None, None)
+    return bool(L.oracle_blas_bound())
 
 
 def _ptr(a):

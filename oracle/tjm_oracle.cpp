@@ -37,6 +37,8 @@
 #include <omp.h>
 #endif
 
+#include <dlfcn.h>
+
 #include "../include/yaqs_hip.h"
 
 namespace {
@@ -78,8 +80,34 @@ struct TrajRng {
 // ---------------------------------------------------------- dense utils ----
 // Column-major complex matrix product C(m,n) = op(A) * op(B); ta/tb: 'N',
 // 'T' or 'C' (conjugate transpose).  Inner loop runs over contiguous rows.
+// Optional BLAS: the reference's tensor contractions run on OpenBLAS zgemm
+// (LinearAlgebra / TensorOperations, OpenBLAS 0.3.29 in its Manifest); when
+// oracle_use_blas() has bound a Fortran-ABI zgemm with 64-bit integers
+// (numpy's bundled OpenBLAS 0.3.29), products above a small size go there,
+// single-threaded inside each trajectory's thread as the reference runs
+// them (BLAS.set_num_threads(1)).  Same mathematics, another summation
+// order: results move at the 1e-15 level.
+typedef void (*zgemm64_fn)(const char *, const char *, const int64_t *, const int64_t *, const int64_t *,
+                           const cd *, const cd *, const int64_t *, const cd *, const int64_t *, const cd *, cd *,
+                           const int64_t *, long, long);
+zgemm64_fn g_zgemm = nullptr;
+// Optional LAPACK zgesdd (the reference's `svd`, LinearAlgebra's default
+// divide-and-conquer driver, src/Algorithms.jl:420) for the timing baseline
+// only: the parity tests keep the Jacobi SVD below, whose null-space
+// completion the tests pin.
+typedef void (*zgesdd64_fn)(const char *, const int64_t *, const int64_t *, cd *, const int64_t *, double *, cd *,
+                            const int64_t *, cd *, const int64_t *, cd *, const int64_t *, double *, int64_t *,
+                            int64_t *, long);
+zgesdd64_fn g_zgesdd = nullptr;
+
 void gemm(char ta, char tb, int m, int n, int k, const cd *A, int lda, const cd *B, int ldb,
           cd *C, int ldc, bool accumulate = false) {
+    if (g_zgemm && (int64_t)m * n * k >= 4096) {
+        const int64_t M = m, N = n, K = k, LA = std::max(lda, 1), LB = std::max(ldb, 1), LC = std::max(ldc, 1);
+        const cd one(1.0, 0.0), beta(accumulate ? 1.0 : 0.0, 0.0);
+        g_zgemm(&ta, &tb, &M, &N, &K, &one, A, &LA, B, &LB, &beta, C, &LC, 1, 1);
+        return;
+    }
     if (!accumulate)
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < m; ++i) C[i + (size_t)ldc * j] = 0.0;
@@ -302,6 +330,23 @@ void svd_tall(const cvec &A, int m, int n, cvec &U, std::vector<double> &S, cvec
 // Thin SVD A = U S Vt of a column-major m x n matrix (Julia `svd`,
 // src/Algorithms.jl:420).  U (m x r), S (r), Vt (r x n), r = min(m, n).
 void svd_thin(const cvec &A, int m, int n, cvec &U, std::vector<double> &S, cvec &Vt) {
+    if (g_zgesdd && m > 0 && n > 0) {
+        const int64_t M = m, N = n, r = std::min(m, n), mx = std::max(m, n);
+        cvec X = A, work(1);
+        U.assign((size_t)m * r, 0.0);
+        Vt.assign((size_t)r * n, 0.0);
+        S.assign(r, 0.0);
+        std::vector<double> rwork((size_t)(2 * mx * r + 5 * r * r + 7 * r + 1));
+        std::vector<int64_t> iwork((size_t)8 * r);
+        int64_t lwork = -1, info = 0, ldu = m, ldvt = r;
+        g_zgesdd("S", &M, &N, X.data(), &M, S.data(), U.data(), &ldu, Vt.data(), &ldvt, work.data(), &lwork,
+                 rwork.data(), iwork.data(), &info, 1);
+        lwork = std::max<int64_t>(1, (int64_t)work[0].real());
+        work.resize((size_t)lwork);
+        g_zgesdd("S", &M, &N, X.data(), &M, S.data(), U.data(), &ldu, Vt.data(), &ldvt, work.data(), &lwork,
+                 rwork.data(), iwork.data(), &info, 1);
+        if (info == 0) return;
+    }
     if (m >= n) {
         cvec V;
         svd_tall(A, m, n, U, S, V);
@@ -1527,6 +1572,44 @@ int oracle_run_analog(const yaqs_analog_job *job, double *traj_out, double *mean
 // reported is the longest over trajectories (they run concurrently, one per
 // thread).
 void oracle_set_window(int warmup_steps) { g_window_warmup = warmup_steps; }
+
+// Binds zgemm (symbol `sym`, Fortran ABI with 64-bit integers) from the
+// shared library at `path`, and sets its thread count to one through
+// `threads_sym` when given.  path == NULL unbinds.  Returns 0 on success.
+int oracle_use_blas(const char *path, const char *sym, const char *threads_sym) {
+    if (!path) {
+        g_zgemm = nullptr;
+        g_zgesdd = nullptr;
+        return 0;
+    }
+    void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return 1;
+    void *f = dlsym(h, sym);
+    if (!f) return 2;
+    if (threads_sym) {
+        typedef void (*setn_fn)(int64_t);
+        if (void *t = dlsym(h, threads_sym)) reinterpret_cast<setn_fn>(t)(1);
+    }
+    g_zgemm = reinterpret_cast<zgemm64_fn>(f);
+    return 0;
+}
+
+// Binds zgesdd (symbol `sym`, Fortran ABI with 64-bit integers) from the
+// library at `path` for the SVDs; NULL unbinds.  Returns 0 on success.
+int oracle_use_lapack_svd(const char *path, const char *sym) {
+    if (!path) {
+        g_zgesdd = nullptr;
+        return 0;
+    }
+    void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return 1;
+    void *f = dlsym(h, sym);
+    if (!f) return 2;
+    g_zgesdd = reinterpret_cast<zgesdd64_fn>(f);
+    return 0;
+}
+
+int oracle_blas_bound(void) { return g_zgemm ? 1 : 0; }
 double oracle_last_window_seconds(void) { return g_last_window; }
 
 int64_t oracle_digital_num_samples(const yaqs_digital_job *job) {

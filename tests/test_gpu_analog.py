@@ -1,13 +1,15 @@
 """GPU parity of the batched analog TJM engine against the FP64 oracle.
 
 FP64 mode runs the same algorithm on the same per-trajectory seeds, so every
-trajectory (jump sequence included) must match the oracle: tolerance 1e-8 on
-each <O>(t) (sums of ~1e-13 Lanczos/Gram rounding over hundreds of sweeps).
-Exception: after a jump that leaves a bond exactly rank-deficient (raising on
-|0>, for instance) the reference's gauge steps complete the null space with
-whatever orthonormal vectors LAPACK returns, and 2-site TDVP's tangent space
-depends on that choice; the oracle (Householder) and the engine (CGS2 with
-dead directions) then agree to the TDVP projection error only (~1e-5).
+trajectory (jump sequence included) must match the oracle: tolerance 1e-10 on
+each <O>(t) (measured 1e-14 .. 5e-12; each test prints its deviation).
+Exception: after a raising jump the site tensor [0 | A_0] of a bond with
+more rows than columns is exactly rank-deficient; the gauge step's
+orthonormal completion of the null space is then fixed by rounding noise (in
+the reference by LAPACK's Householder reflectors of a residual of size
+1e-16), and 2-site TDVP's tangent space depends on it.  The oracle and the
+engine pick different, equally valid completions and agree to the TDVP
+projection error only (~1e-5; median 1e-13).
 Mixed mode (complex-FP32 contractions) is compared on ensemble means within
 1e-4 where the jump sequences coincide, and within sampling error otherwise.
 """
@@ -19,7 +21,7 @@ from yaqs_julia_amd import _abi
 
 pytestmark = pytest.mark.gpu
 
-TOL_FP64 = 1e-8
+TOL_FP64 = 1e-10
 TOL_RANK_DEFICIENT = 1e-4
 
 
@@ -44,6 +46,14 @@ def _oracle(oracle_lib, mps, H, cfg, noise, seed=0):
     return oracle_lib.run_analog(h, nthreads=8)
 
 
+def _err(tag, g, r):
+    """Largest |gpu - oracle| over every observable, time and trajectory,
+    printed so each run records the measured deviation."""
+    e = float(np.abs(np.asarray(g) - np.asarray(r)).max())
+    print(f"  {tag}: max|gpu-oracle| = {e:.3e}", flush=True)
+    return e
+
+
 def _zx(L):
     return [Y.Observable(f"z{i}", Y.gates.ZGate(), i) for i in range(1, L + 1)] + \
            [Y.Observable(f"x{i}", Y.gates.XGate(), i) for i in range(1, L + 1)]
@@ -56,7 +66,7 @@ def test_noise_free_tdvp_matches_oracle(ctx, oracle_lib):
     H = Y.init_ising(L, 1.0, 0.5)
     g, gm, st = _gpu(ctx, Y.MPS(L), H, cfg, None)
     r, rm, _ = _oracle(oracle_lib, Y.MPS(L), H, cfg, None)
-    assert np.abs(g - r).max() < TOL_FP64
+    assert _err("g", g, r) < TOL_FP64
     assert st["matvecs"] > 0 and st["max_bond"] >= 4
 
 
@@ -71,8 +81,8 @@ def test_c1_per_trajectory_parity(ctx, oracle_lib):
     g, gm, st = _gpu(ctx, Y.MPS(L), H, cfg, noise, seed=11)
     r, rm, rst = _oracle(oracle_lib, Y.MPS(L), H, cfg, noise, seed=11)
     assert st["jumps"] == rst["jumps"] and st["jumps"] > 0
-    assert np.abs(g - r).max() < TOL_FP64
-    assert np.abs(gm - rm).max() < TOL_FP64
+    assert _err("g", g, r) < TOL_FP64
+    assert _err("gm", gm, rm) < TOL_FP64
 
 
 def test_heisenberg_neel_parity(ctx, oracle_lib):
@@ -86,7 +96,7 @@ def test_heisenberg_neel_parity(ctx, oracle_lib):
     g, gm, st = _gpu(ctx, Y.MPS(L, state="Neel"), H, cfg, noise, seed=5)
     r, rm, rst = _oracle(oracle_lib, Y.MPS(L, state="Neel"), H, cfg, noise, seed=5)
     assert st["jumps"] == rst["jumps"]
-    assert np.abs(g - r).max() < TOL_FP64
+    assert _err("g", g, r) < TOL_FP64
 
 
 def test_first_order_and_final_only(ctx, oracle_lib):
@@ -100,7 +110,7 @@ def test_first_order_and_final_only(ctx, oracle_lib):
         g, _, _ = _gpu(ctx, Y.MPS(L), H, cfg, noise, seed=2)
         r, _, _ = _oracle(oracle_lib, Y.MPS(L), H, cfg, noise, seed=2)
         assert g.shape == r.shape
-        assert np.abs(g - r).max() < TOL_FP64, (order, sample)
+        assert _err("g", g, r) < TOL_FP64, (order, sample)
 
 
 def test_two_site_observables_and_raising(ctx, oracle_lib):
@@ -117,7 +127,7 @@ def test_two_site_observables_and_raising(ctx, oracle_lib):
     r, _, rst = _oracle(oracle_lib, Y.MPS(L), H, cfg, noise, seed=9)
     assert st["jumps"] == rst["jumps"]
     # raising jumps on |0> sites leave exactly rank-deficient bonds
-    assert np.abs(g - r).max() < TOL_RANK_DEFICIENT
+    assert _err("g", g, r) < TOL_RANK_DEFICIENT
     assert np.median(np.abs(g - r)) < TOL_FP64
 
 
@@ -134,7 +144,8 @@ def test_mixed_precision_ensemble(ctx, oracle_lib):
     same = np.all(np.abs(g - r).max(axis=(0, 1)) < 1e-2)
     assert same or st["jumps"] != rst["jumps"]
     if st["jumps"] == rst["jumps"]:
-        assert np.abs(gm[:L] - rm[:L]).max() < 1e-4
+        assert _err("<Z_i> mean", gm[:L], rm[:L]) < 1e-4
+        assert _err("<X_i> mean", gm[L:], rm[L:]) < 1e-4
     else:
         assert np.abs(gm[:L] - rm[:L]).max() < 0.05
 
@@ -171,12 +182,7 @@ def test_unraveled_dephasing_parity(ctx, oracle_lib, unravel, kw):
     g, gm, st = _gpu(ctx, Y.MPS(L, state="x+"), H, cfg, noise, seed=4)
     r, rm, rst = _oracle(oracle_lib, Y.MPS(L, state="x+"), H, cfg, noise, seed=4)
     assert st["jumps"] == rst["jumps"] and st["jumps"] > 0
-    if unravel == "projector":
-        # (I -+ Z)/2 projections make bonds exactly rank-deficient
-        assert np.abs(g - r).max() < TOL_RANK_DEFICIENT
-        assert np.median(np.abs(g - r)) < TOL_FP64
-    else:
-        assert np.abs(g - r).max() < TOL_FP64
+    assert _err("g", g, r) < TOL_FP64
 
 
 def test_unraveled_crosstalk_is_rejected(ctx):

@@ -21,6 +21,13 @@ def ctx():
     c.close()
 
 
+def _err(tag, g, r):
+    """Largest |gpu - oracle|, printed so each run records the deviation."""
+    e = float(np.abs(np.asarray(g) - np.asarray(r)).max())
+    print(f"  {tag}: max|gpu-oracle| = {e:.3e}", flush=True)
+    return e
+
+
 def _depolarizing(L, p):
     procs = []
     for i in range(1, L + 1):
@@ -47,7 +54,7 @@ def test_brickwork_noise_free(ctx, oracle_lib):
     cfg = Y.StrongMeasurementConfig(obs, num_traj=1, max_bond_dim=64, truncation_threshold=1e-12)
     g, gm, gb, st, r, rm, rb, rst = _run(ctx, oracle_lib, L, circ, cfg, None)
     assert g.shape == r.shape == (L, 7, 1)
-    assert np.abs(g - r).max() < 1e-10
+    assert _err("g", g, r) < 1e-10
     assert np.array_equal(gb, rb)
 
 
@@ -60,7 +67,7 @@ def test_brickwork_depolarizing_parity(ctx, oracle_lib):
     cfg = Y.StrongMeasurementConfig(obs, num_traj=16, max_bond_dim=16, truncation_threshold=1e-9)
     g, gm, gb, st, r, rm, rb, rst = _run(ctx, oracle_lib, L, circ, cfg, _depolarizing(L, 0.05), seed=7)
     assert st["jumps"] == rst["jumps"] and st["jumps"] > 0
-    assert np.abs(g - r).max() < 1e-9
+    assert _err("g", g, r) < 1e-9
     assert np.array_equal(gb, rb)
 
 
@@ -73,7 +80,7 @@ def test_digital_mixed_precision(ctx, oracle_lib):
     g, gm, gb, st, r, rm, rb, rst = _run(ctx, oracle_lib, L, circ, cfg, _depolarizing(L, 0.01),
                                          precision=_abi.PREC_MIXED, seed=1)
     if st["jumps"] == rst["jumps"]:
-        assert np.abs(gm - rm).max() < 1e-4
+        assert _err("gm", gm, rm) < 1e-4
 
 
 def test_brickwork_unraveled_noise_parity(ctx, oracle_lib):
@@ -90,7 +97,7 @@ def test_brickwork_unraveled_noise_parity(ctx, oracle_lib):
     noise = Y.NoiseModel(procs, L)
     g, gm, gb, st, r, rm, rb, rst = _run(ctx, oracle_lib, L, circ, cfg, noise, seed=8)
     assert st["jumps"] == rst["jumps"] and st["jumps"] > 0
-    assert np.abs(g - r).max() < 1e-9
+    assert _err("g", g, r) < 1e-9
     assert np.array_equal(gb, rb)
 
 
@@ -139,7 +146,7 @@ def test_window_tdvp_gates_parity(ctx, oracle_lib, noisy):
     assert st["jumps"] == rst["jumps"]
     if noisy:
         assert st["jumps"] > 0
-    assert np.abs(g - r).max() < 1e-9
+    assert _err("g", g, r) < 1e-9
     assert np.array_equal(gb, rb)
 
 
@@ -180,7 +187,7 @@ def test_swap_network_parity(ctx, oracle_lib, noise_sites):
     assert st["jumps"] == rst["jumps"]
     if noise_sites:
         assert st["jumps"] > 0
-    assert np.abs(g - r).max() < 1e-9
+    assert _err("g", g, r) < 1e-9
     assert np.array_equal(gb, rb)
 
 
@@ -236,5 +243,5 @@ def test_long_range_tdvp_windows_parity(ctx, oracle_lib, noisy):
     assert st["jumps"] == rst["jumps"]
     if noisy:
         assert st["jumps"] > 0
-    assert np.abs(g - r).max() < 1e-8
+    assert _err("g", g, r) < 1e-8
     assert np.array_equal(gb, rb)

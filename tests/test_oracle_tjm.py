@@ -169,3 +169,41 @@ def test_jump_weights_brute_force(oracle_lib):
     w = np.array(w) / np.sum(w)
     assert np.allclose(p, w, atol=1e-12)
     assert [tuple(x) for x in c[:2]] == [(1, 1, 0), (1, 1, 1)]
+
+
+def test_f2_reference_large_ising(oracle_lib):
+    """Noise-free 2-site TDVP at L=30 (Ising J=1 g=0.5, |0..0>, chi=32,
+    dt=0.05, T=2) against the reference's own output large_ising_results.csv
+    (test/simulate_large_ising.jl:31-48).  The run is truncated at chi=32, so
+    the comparison is not exact: the reference's Julia and Python outputs
+    differ by 3.7e-7 at t <= 0.45; the oracle stays within 5e-7 there and
+    within 4e-6 at every time."""
+    ref = {}
+    with open(os.path.join(GOLDEN, "f2_large_ising.csv")) as f:
+        for r in csv.DictReader(f):
+            ref[(round(float(r["Time"]), 4), int(r["Site"]))] = float(r["ExpVal"])
+    sites = (1, 16, 30)
+    obs = [Y.Observable(f"z{i}", Y.gates.ZGate(), i) for i in sites]
+    cfg = Y.TimeEvolutionConfig(obs, 2.0, dt=0.05, num_traj=1, max_bond_dim=32)
+    h = _abi.pack_analog(Y.MPS(30, state="zeros"), Y.init_ising(30, 1.0, 0.5), cfg, None)
+    _, mean, _ = oracle_lib.run_analog(h)
+    errs = np.array([[t, abs(mean[q, j] - ref[(round(t, 4), s)])] for j, t in enumerate(cfg.times)
+                     for q, s in enumerate(sites)])
+    assert len(errs) == 41 * 3
+    assert errs[errs[:, 0] <= 0.45 + 1e-9, 1].max() < 5e-7
+    assert errs[:, 1].max() < 4e-6
+
+
+def test_c2_fixture_reproduced(oracle_lib):
+    """tests/golden/c2_oracle_traj0-3.npy (the oracle's trajectories 0..3 of
+    the L=30 Heisenberg workload, written by make_oracle_fixtures.py) is what
+    the oracle computes: trajectory 0 rerun for its first 6 steps."""
+    import sys
+    sys.path.insert(0, GOLDEN)
+    from make_oracle_fixtures import c2_job
+    _, _, mps, H, cfg, noise = c2_job(steps=6, ntraj=1)
+    h = _abi.pack_analog(mps, H, cfg, noise, seed=0)
+    traj, _, _ = oracle_lib.run_analog(h)
+    fix = np.load(os.path.join(GOLDEN, "c2_oracle_traj0-3.npy"))
+    assert fix.shape == (60, 25, 4)
+    assert np.abs(traj[:, :, 0] - fix[:, :7, 0]).max() < 1e-12
