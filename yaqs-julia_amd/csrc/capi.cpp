@@ -395,6 +395,14 @@ void worker_main(yaqs_ctx *c) {
     if (hipSetDevice(c->device) != hipSuccess) return;
     hipStream_t stream = nullptr;
     if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return;
+    // the worker's pinned words and events, reused by every wave it runs
+    yq::StreamRes sres;
+    try {
+        sres.create();
+    } catch (const std::exception &) {
+        (void)hipStreamDestroy(stream);
+        return;
+    }
     for (;;) {
         Wave w;
         {
@@ -411,7 +419,7 @@ void worker_main(yaqs_ctx *c) {
         std::string err;
         yq::JobResult wr;
         try {
-            auto eng = yq::make_engine(job.in, w.nb, stream);
+            auto eng = yq::make_engine(job.in, w.nb, stream, &sres);
             eng->run(w.t0, w.nb, job.d_out, job.in.digital ? job.out.bonds.data() : nullptr, wr);
             YQ_HIP(hipStreamSynchronize(stream));
         } catch (const yq::HipError &e) {
@@ -470,6 +478,8 @@ void worker_main(yaqs_ctx *c) {
         }
         c->cv_work.notify_all();
     }
+    (void)hipStreamSynchronize(stream);
+    sres.destroy();
     (void)hipStreamDestroy(stream);
 }
 
@@ -666,12 +676,14 @@ int yaqs_bench_analog(yaqs_ctx *ctx, const yaqs_analog_job *job, int64_t warmup_
         std::vector<std::string> errs(nconc);
         auto lane = [&](int w) {
             hipStream_t stream = nullptr;
+            yq::StreamRes sres;
             try {
                 YQ_HIP(hipSetDevice(ctx->device));
                 YQ_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+                sres.create();
                 for (int64_t t0 = (int64_t)w * b; t0 < ntraj; t0 += (int64_t)nconc * b) {
                     const int nb = (int)std::min<int64_t>(b, ntraj - t0);
-                    auto eng = yq::make_engine(in, (int)b, stream);
+                    auto eng = yq::make_engine(in, (int)b, stream, &sres);
                     eng->run(t0, nb, d_out, nullptr, res[w]);
                 }
                 YQ_HIP(hipStreamSynchronize(stream));
@@ -679,6 +691,8 @@ int yaqs_bench_analog(yaqs_ctx *ctx, const yaqs_analog_job *job, int64_t warmup_
                 errs[w] = e.what();
                 if (in.wsync) in.wsync->release();
             }
+            if (stream) (void)hipStreamSynchronize(stream);
+            sres.destroy();
             if (stream) (void)hipStreamDestroy(stream);
         };
         if (nconc == 1) {

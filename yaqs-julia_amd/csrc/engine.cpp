@@ -40,24 +40,39 @@ double host_now() {
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool owned = false;  // hipMalloc'd
     DevBuf() {}
     explicit DevBuf(size_t b) { alloc(b); }
     void alloc(size_t b) {
         release();
         bytes = b ? b : 16;
         YQ_HIP(hipMalloc(&p, bytes));
+        owned = true;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        owned = false;
     }
     ~DevBuf() { release(); }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes), owned(o.owned) {
+        o.p = nullptr;
+        o.bytes = 0;
+        o.owned = false;
+    }
     DevBuf &operator=(DevBuf &&o) noexcept {
-        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            owned = o.owned;
+            o.p = nullptr;
+            o.bytes = 0;
+            o.owned = false;
+        }
         return *this;
     }
     template <typename X>
@@ -424,11 +439,33 @@ class Engine {
   public:
     typedef cplx<T> CT;
 
-    Engine(const JobIn &job, int B, hipStream_t s) : J(job), B(B), s(s) { setup(); }
+    Engine(const JobIn &job, int B, hipStream_t s, StreamRes *res = nullptr) : J(job), B(B), s(s) {
+        if (res) {
+            h_pinned = res->h_pinned;
+            d_pinned = res->d_pinned;
+            lz_ev[0] = res->ev[0];
+            lz_ev[1] = res->ev[1];
+        } else {
+            own.create();
+            h_pinned = own.h_pinned;
+            d_pinned = own.d_pinned;
+            lz_ev[0] = own.ev[0];
+            lz_ev[1] = own.ev[1];
+        }
+        try {
+            setup();
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            own.destroy();
+            throw;
+        }
+    }
     ~Engine() {
-        for (auto &e : lz_ev)
-            if (e) (void)hipEventDestroy(e);
-        if (h_pinned) (void)hipHostFree(h_pinned);
+        // nothing of this engine may still be in flight when its buffers,
+        // pinned words and events go (an exception can end a wave between
+        // launches)
+        (void)hipStreamSynchronize(s);
+        own.destroy();
     }
     Engine(const Engine &) = delete;
     Engine &operator=(const Engine &) = delete;
@@ -468,6 +505,7 @@ class Engine {
     DevBuf obs_site, obs_n, obs_op;
     bool any_obs2 = false;
     std::vector<char> obs2_at;
+    StreamRes own;            // the engine's own host objects when none were passed
     int *h_pinned = nullptr;
     int *d_pinned = nullptr;  // device address of h_pinned
     DevBuf fin_done;          // lz_finalize's last-workgroup ticket
@@ -705,11 +743,6 @@ void Engine<T>::setup() {
     count.alloc(sizeof(int));
     fin_done.alloc(sizeof(int));
     fill_int(fin_done.as<int>(), 0, 1, s);
-    // coherent: the Lanczos finalize kernel stores the active count straight
-    // into h_pinned[1 + (j & 1)], visible to the host's spin before the kernel ends
-    YQ_HIP(hipHostMalloc((void **)&h_pinned, 16 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
-    YQ_HIP(hipHostGetDevicePointer((void **)&d_pinned, h_pinned, 0));
-    for (auto &e : lz_ev) YQ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
     // ---- noise: dissipation operators per site and jump candidates
     auto site_ops = [&](double dt, std::vector<c64> &ops, std::vector<int> *act) {
@@ -1871,11 +1904,28 @@ int64_t footprint_bytes(const JobIn &job, int64_t batch) {
     return job.precision == 1 ? Engine<float>::footprint(job, (int)batch) : Engine<double>::footprint(job, (int)batch);
 }
 
+void StreamRes::create() {
+    // coherent: the Lanczos finalize kernel stores the active count straight
+    // into h_pinned[1 + (j & 1)], visible to the host's spin before the kernel ends
+    YQ_HIP(hipHostMalloc((void **)&h_pinned, 16 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+    YQ_HIP(hipHostGetDevicePointer((void **)&d_pinned, h_pinned, 0));
+    for (auto &e : ev) YQ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+
+void StreamRes::destroy() {
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    ev[0] = ev[1] = nullptr;
+    if (h_pinned) (void)hipHostFree(h_pinned);
+    h_pinned = d_pinned = nullptr;
+}
+
 namespace {
 template <typename T>
 class WaveEngineT : public WaveEngine {
   public:
-    WaveEngineT(const JobIn &job, int batch, hipStream_t s) : eng(job, batch, s), digital(job.digital) {}
+    WaveEngineT(const JobIn &job, int batch, hipStream_t s, StreamRes *res)
+        : eng(job, batch, s, res), digital(job.digital) {}
     void run(int64_t t0, int nb, double *d_out, int64_t *h_bonds, JobResult &res) override {
         if (digital) eng.run_digital(t0, nb, d_out, h_bonds, res);
         else eng.run_analog(t0, nb, d_out, res);
@@ -1907,10 +1957,10 @@ void WindowSync::release() {
     cv.notify_all();
 }
 
-std::unique_ptr<WaveEngine> make_engine(const JobIn &job, int batch, hipStream_t stream) {
+std::unique_ptr<WaveEngine> make_engine(const JobIn &job, int batch, hipStream_t stream, StreamRes *res) {
     validate_job(job);
-    if (job.precision == 1) return std::unique_ptr<WaveEngine>(new WaveEngineT<float>(job, batch, stream));
-    return std::unique_ptr<WaveEngine>(new WaveEngineT<double>(job, batch, stream));
+    if (job.precision == 1) return std::unique_ptr<WaveEngine>(new WaveEngineT<float>(job, batch, stream, res));
+    return std::unique_ptr<WaveEngine>(new WaveEngineT<double>(job, batch, stream, res));
 }
 
 void run_job(const JobIn &job, int64_t batch, hipStream_t stream, JobResult &out, std::atomic<bool> *cancel) {

@@ -85,6 +85,18 @@ struct JobResult {
     int64_t timed_steps = 0, gemm_launches = 0, win_matvecs = 0;
 };
 
+// Host objects one stream reuses across the engines it runs: the coherent
+// pinned words the Lanczos finalize kernel stores its active counts into
+// (and their device address) and two events.  A worker creates them once,
+// so waves neither allocate pinned memory nor create events.
+struct StreamRes {
+    int *h_pinned = nullptr;
+    int *d_pinned = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    void create();   // hipHostMalloc + hipEventCreate (throws HipError)
+    void destroy();  // after the stream that used them is synchronised
+};
+
 // Throws std::invalid_argument for malformed jobs, HipError for device errors.
 void validate_job(const JobIn &job);
 int64_t num_columns(const JobIn &job);
@@ -102,7 +114,9 @@ class WaveEngine {
     virtual ~WaveEngine() = default;
     virtual void run(int64_t t0, int nb, double *d_out, int64_t *h_bonds, JobResult &res) = 0;
 };
-std::unique_ptr<WaveEngine> make_engine(const JobIn &job, int batch, hipStream_t stream);
+// res: the stream's reusable host objects, or nullptr (the engine creates
+// and destroys its own).
+std::unique_ptr<WaveEngine> make_engine(const JobIn &job, int batch, hipStream_t stream, StreamRes *res = nullptr);
 
 double rng_uniform_host(uint64_t seed, uint64_t traj, uint64_t k);
 
