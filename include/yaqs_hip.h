@@ -280,6 +280,25 @@ typedef struct {
 int yaqs_bench_analog(yaqs_ctx *ctx, const yaqs_analog_job *job, int64_t warmup_steps, double *traj_out,
                       double *mean_out, yaqs_bench_report *report);
 
+/* Allocation-free variant of yaqs_run_analog.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same results as yaqs_run_analog (same kernels; the launch
+ * tuner only takes choices it has already cached).  It calls no hipMalloc,
+ * hipFree or other allocating routine: every device buffer of the job is
+ * carved from one per-device scratch array of yaqs_noalloc_scratch_bytes()
+ * (256 MiB) that each GPU the library runs on holds until the process ends;
+ * its device address and a stream are set up by yaqs_create.  Runs on the
+ * calling thread and returns when done; trajectories run in waves that fit
+ * the scratch.  A job of which one trajectory does not fit is rejected with
+ * YAQS_ENOMEM_DEVICE before anything is launched.  Calls on one GPU share its
+ * scratch, which keeps the last call's contents: calls from one process are
+ * serialised, calls from several processes (or contexts in several
+ * libraries) on one GPU race. */
+int yaqs_run_analog_noalloc(yaqs_ctx *ctx, const yaqs_analog_job *job, double *traj_out, double *mean_out,
+                            yaqs_job_stats *stats);
+int64_t yaqs_noalloc_scratch_bytes(void);
+
 /* Counter-based RNG of the trajectory seeding contract: draw k of global
  * trajectory t under `seed` is a pure function of (seed, t, k).  The
  * reference draws Random.default_rng() at src/StochasticProcess.jl:221-244;

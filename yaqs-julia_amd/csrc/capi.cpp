@@ -21,6 +21,9 @@
 #include <vector>
 
 #include "engine.h"
+#include "kernels.h"
+#include "noalloc.h"
+#include "tune.h"
 
 namespace {
 
@@ -251,6 +254,12 @@ struct yaqs_ctx {
     uint64_t next_id = 1;
     bool stop = false;
     std::vector<std::thread> workers;
+    // allocation-free entry point: device address of the device's scratch
+    // array, and a stream with its pinned words and events, all set up by
+    // yaqs_create
+    void *na_scratch = nullptr;
+    hipStream_t na_stream = nullptr;
+    yq::StreamRes na_res;
 };
 
 namespace {
@@ -549,6 +558,22 @@ int yaqs_create(yaqs_ctx **out, int device, const yaqs_ctx_opts *opts) {
     if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return fail(YAQS_EDEVICE, "hipMemGetInfo failed");
     auto *c = new yaqs_ctx();
     c->device = device;
+    if (yq::noalloc_init(device) != 0) {
+        delete c;
+        return fail(YAQS_EDEVICE, "cannot look up the device scratch of the allocation-free entry point");
+    }
+    c->na_scratch = yq::noalloc_scratch(device);
+    if (hipStreamCreateWithFlags(&c->na_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(YAQS_EDEVICE, "hipStreamCreate failed");
+    }
+    try {
+        c->na_res.create();
+    } catch (const std::exception &e) {
+        (void)hipStreamDestroy(c->na_stream);
+        delete c;
+        return fail(YAQS_EDEVICE, e.what());
+    }
     c->budget = (opts && opts->hbm_budget_bytes > 0) ? opts->hbm_budget_bytes : (int64_t)(0.9 * (double)freeb);
     c->max_batch = opts ? opts->max_batch : 0;
     c->verbose = opts ? opts->verbose : 0;
@@ -571,6 +596,12 @@ void yaqs_destroy(yaqs_ctx *ctx) {
     }
     ctx->cv_work.notify_all();
     for (auto &t : ctx->workers) t.join();
+    if (ctx->na_stream) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->na_stream);
+        ctx->na_res.destroy();
+        (void)hipStreamDestroy(ctx->na_stream);
+    }
     delete ctx;
 }
 
@@ -645,6 +676,113 @@ int64_t yaqs_job_footprint_bytes(const yaqs_analog_job *job) {
 }
 
 double yaqs_rng_uniform(uint64_t seed, uint64_t traj, uint64_t k) { return yq::rng_uniform_host(seed, traj, k); }
+
+int64_t yaqs_noalloc_scratch_bytes(void) { return (int64_t)yq::kNoAllocScratchBytes; }
+
+int yaqs_run_analog_noalloc(yaqs_ctx *ctx, const yaqs_analog_job *job, double *traj_out, double *mean_out,
+                            yaqs_job_stats *stats) {
+    if (!ctx) return fail(YAQS_EINVAL, "ctx is NULL");
+    if (!ctx->na_scratch) return fail(YAQS_EDEVICE, "the device scratch was not set up");
+    const double t_submit = now_s();
+    yq::JobIn in;
+    try {
+        in = convert_analog(job);
+        yq::validate_job(in);
+    } catch (const std::exception &e) {
+        return fail(YAQS_EINVAL, e.what());
+    }
+    const int64_t ntraj = in.num_traj, nobs = (int64_t)in.obs.size(), ncol = yq::num_columns(in);
+    const int64_t nout = std::max<int64_t>(2, nobs * ncol * ntraj);
+    const int64_t out_bytes = ((nout * (int64_t)sizeof(double)) + 255) & ~(int64_t)255;
+    const int64_t cap = (int64_t)yq::kNoAllocScratchBytes;
+    // Slots per wave from the footprint model, with room for the buffers'
+    // 256-byte alignment and the job's own tables (MPS/MPO uploads, sparse
+    // MPO steps); the arena itself stops an underestimate with an error
+    // before anything is launched into memory it does not hold.
+    int64_t tables = 0;
+    for (const auto &m : in.mps) tables += (int64_t)m.size() * (int64_t)sizeof(yq::c64) + 256;
+    for (const auto &w : in.mpo) tables += 6 * ((int64_t)w.size() * (int64_t)sizeof(yq::c64) + 3 * 256);
+    const int64_t slack = ((int64_t)4 << 20) + tables + (12 * in.L + 256) * 256;
+    const int64_t per = yq::footprint_bytes(in, 1);
+    const int64_t room = cap - out_bytes - slack;
+    const int64_t B = room > 0 ? std::min<int64_t>(ntraj, room / std::max<int64_t>(1, per)) : 0;
+    if (B < 1)
+        return fail(YAQS_ENOMEM_DEVICE,
+                    "job exceeds the allocation-free entry point's scratch: one trajectory needs " +
+                        std::to_string(per + out_bytes + slack) + " bytes, the scratch holds " + std::to_string(cap) +
+                        " bytes (use yaqs_run_analog)");
+    std::lock_guard<std::mutex> lk(yq::noalloc_lock(ctx->device));
+    hipStream_t s = ctx->na_stream;
+    double *d_out = static_cast<double *>(ctx->na_scratch);
+    char *arena = static_cast<char *>(ctx->na_scratch) + out_bytes;
+    yq::JobResult res;
+    try {
+        YQ_HIP(hipSetDevice(ctx->device));
+        // the scratch keeps the previous call's contents: clear the output
+        yq::fill_zero<double>(reinterpret_cast<yq::c64 *>(d_out), 0, (nout + 1) / 2, 1, s);
+        yq::tune_freeze(true);
+        for (int64_t t0 = 0; t0 < ntraj; t0 += B) {
+            const int nb = (int)std::min<int64_t>(B, ntraj - t0);
+            yq::ArenaScope scope(arena, (size_t)(cap - out_bytes));
+            auto eng = yq::make_engine(in, (int)B, s, &ctx->na_res);
+            eng->run(t0, nb, d_out, nullptr, res);
+            YQ_HIP(hipStreamSynchronize(s));
+        }
+        yq::tune_freeze(false);
+        if (traj_out && nobs)
+            YQ_HIP(hipMemcpyAsync(traj_out, d_out, (size_t)nobs * ncol * ntraj * sizeof(double),
+                                  hipMemcpyDeviceToHost, s));
+        YQ_HIP(hipStreamSynchronize(s));
+    } catch (const yq::ArenaFull &e) {
+        yq::tune_freeze(false);
+        (void)hipStreamSynchronize(s);
+        return fail(YAQS_ENOMEM_DEVICE, e.what());
+    } catch (const yq::HipError &e) {
+        yq::tune_freeze(false);
+        (void)hipStreamSynchronize(s);
+        return fail(YAQS_EDEVICE, e.what());
+    } catch (const std::invalid_argument &e) {
+        yq::tune_freeze(false);
+        (void)hipStreamSynchronize(s);
+        return fail(YAQS_EINVAL, e.what());
+    } catch (const std::exception &e) {
+        yq::tune_freeze(false);
+        (void)hipStreamSynchronize(s);
+        return fail(YAQS_EDEVICE, e.what());
+    }
+    if (mean_out && nobs) {
+        for (int64_t q = 0; q < nobs; ++q)
+            for (int64_t c = 0; c < ncol; ++c) mean_out[q * ncol + c] = 0.0;
+        if (traj_out) {
+            for (int64_t q = 0; q < nobs; ++q)
+                for (int64_t c = 0; c < ncol; ++c) {
+                    double sum = 0;
+                    for (int64_t t = 0; t < ntraj; ++t) sum += traj_out[(q * ncol + c) * ntraj + t];
+                    mean_out[q * ncol + c] = sum / (double)ntraj;
+                }
+        } else {
+            std::vector<double> h((size_t)nobs * ncol * ntraj);
+            if (hipMemcpy(h.data(), d_out, h.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+                return fail(YAQS_EDEVICE, "copy of the job output failed");
+            for (int64_t q = 0; q < nobs; ++q)
+                for (int64_t c = 0; c < ncol; ++c) {
+                    double sum = 0;
+                    for (int64_t t = 0; t < ntraj; ++t) sum += h[(size_t)(q * ncol + c) * ntraj + t];
+                    mean_out[q * ncol + c] = sum / (double)ntraj;
+                }
+        }
+    }
+    if (stats) {
+        stats->wall_s = now_s() - t_submit;
+        stats->queue_s = 0;
+        stats->flop_model = res.flop;
+        stats->matvecs = res.matvecs;
+        stats->jumps = res.jumps;
+        stats->max_bond = res.max_bond;
+        stats->num_samples = ncol;
+    }
+    return YAQS_OK;
+}
 
 int yaqs_bench_analog(yaqs_ctx *ctx, const yaqs_analog_job *job, int64_t warmup_steps, double *traj_out,
                       double *mean_out, yaqs_bench_report *report) {

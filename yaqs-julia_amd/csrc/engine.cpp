@@ -37,15 +37,32 @@ double host_now() {
 }
 
 // ------------------------------------------------------------ host utils ---
+// Bump arena of the calling thread (ArenaScope), or none.
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0, used = 0;
+};
+thread_local Arena *g_arena = nullptr;
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
-    bool owned = false;  // hipMalloc'd
+    bool owned = false;  // hipMalloc'd (else carved from an arena)
     DevBuf() {}
     explicit DevBuf(size_t b) { alloc(b); }
     void alloc(size_t b) {
         release();
         bytes = b ? b : 16;
+        if (g_arena) {
+            const size_t off = (g_arena->used + 255) & ~(size_t)255;
+            if (off + bytes > g_arena->size)
+                throw ArenaFull("scratch arena exhausted: " + std::to_string(off + bytes) + " bytes needed, " +
+                                std::to_string(g_arena->size) + " available");
+            p = g_arena->base + off;
+            g_arena->used = off + bytes;
+            owned = false;
+            return;
+        }
         YQ_HIP(hipMalloc(&p, bytes));
         owned = true;
     }
@@ -1919,6 +1936,18 @@ void StreamRes::destroy() {
     if (h_pinned) (void)hipHostFree(h_pinned);
     h_pinned = d_pinned = nullptr;
 }
+
+ArenaScope::ArenaScope(void *base, size_t bytes) {
+    if (g_arena) throw std::logic_error("nested scratch arenas");
+    g_arena = new Arena{static_cast<char *>(base), bytes, 0};
+}
+
+ArenaScope::~ArenaScope() {
+    delete g_arena;
+    g_arena = nullptr;
+}
+
+size_t ArenaScope::used() const { return g_arena ? g_arena->used : 0; }
 
 namespace {
 template <typename T>

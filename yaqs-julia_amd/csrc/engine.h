@@ -97,6 +97,21 @@ struct StreamRes {
     void destroy();  // after the stream that used them is synchronised
 };
 
+// Device memory of engines built on this thread comes from `base` (bump
+// allocation, 256-byte aligned) instead of hipMalloc while an ArenaScope is
+// alive; running out throws ArenaFull before anything is launched into it.
+struct ArenaFull : std::runtime_error {
+    explicit ArenaFull(const std::string &s) : std::runtime_error(s) {}
+};
+class ArenaScope {
+  public:
+    ArenaScope(void *base, size_t bytes);
+    ~ArenaScope();
+    size_t used() const;
+    ArenaScope(const ArenaScope &) = delete;
+    ArenaScope &operator=(const ArenaScope &) = delete;
+};
+
 // Throws std::invalid_argument for malformed jobs, HipError for device errors.
 void validate_job(const JobIn &job);
 int64_t num_columns(const JobIn &job);

@@ -20,6 +20,7 @@ std::set<std::pair<int, TuneKey>> g_busy;         // keys being measured
 int g_enabled = -1;
 bool g_loaded = false;
 thread_local bool g_measuring = false;
+thread_local bool g_frozen = false;  // never measure on this thread (cached choices only)
 
 // YAQS_TUNE_CACHE=<file>: choices are read from the file on first use and
 // every new choice is appended, so later processes launch the same
@@ -72,6 +73,8 @@ bool tune_enabled() {
 
 bool tune_measuring() { return g_measuring; }
 
+void tune_freeze(bool on) { g_frozen = on; }
+
 int cached_variant(const TuneKey &key) {
     int dev = 0;
     YQ_HIP(hipGetDevice(&dev));
@@ -84,6 +87,7 @@ int cached_variant(const TuneKey &key) {
 
 int tuned_variant(const TuneKey &key, int n, const std::function<void(int)> &launch, hipStream_t s) {
     if (n <= 1) return 0;
+    if (g_frozen) return cached_variant(key);
     int dev = 0;
     YQ_HIP(hipGetDevice(&dev));
     const auto ck = std::make_pair(dev, key);

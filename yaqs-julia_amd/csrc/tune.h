@@ -29,6 +29,9 @@ int tuned_variant(const TuneKey &key, int n, const std::function<void(int)> &lau
 int cached_variant(const TuneKey &key);
 
 bool tune_enabled();
+// While on, tuned_variant on the calling thread returns the cached choice
+// (or 0) without measuring: no events are created, nothing extra launched.
+void tune_freeze(bool on);
 // True on the calling thread while a measurement runs (profilers skip it).
 bool tune_measuring();
 
