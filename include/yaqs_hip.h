@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define YAQS_ABI_VERSION 1
+#define YAQS_ABI_VERSION 2
 
 /* Return codes.  Mirrors the reference's failure modes: Julia `error(...)` /
  * `@assert` on bad input (src/Simulator.jl:136,139), plus device failures. */
@@ -201,6 +201,8 @@ typedef struct {
     int64_t jumps;              /* quantum jumps applied                      */
     int64_t max_bond;           /* largest bond reached                       */
     int64_t num_samples;        /* columns of the output (times or samples)   */
+    int64_t fp64_retries;       /* mixed-precision trajectories rerun in FP64
+                                   after NaN/Inf before a split              */
 } yaqs_job_stats;
 
 typedef struct {

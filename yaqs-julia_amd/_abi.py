@@ -72,7 +72,7 @@ class DigitalJob(ct.Structure):
 class JobStats(ct.Structure):
     _fields_ = [("wall_s", ct.c_double), ("queue_s", ct.c_double), ("flop_model", ct.c_double),
                 ("matvecs", ct.c_int64), ("jumps", ct.c_int64), ("max_bond", ct.c_int64),
-                ("num_samples", ct.c_int64)]
+                ("num_samples", ct.c_int64), ("fp64_retries", ct.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

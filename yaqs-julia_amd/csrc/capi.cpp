@@ -454,6 +454,7 @@ void worker_main(yaqs_ctx *c) {
             job.out.matvecs += wr.matvecs;
             job.out.jumps += wr.jumps;
             job.out.max_bond = std::max(job.out.max_bond, wr.max_bond);
+            job.out.failed.insert(job.out.failed.end(), wr.failed.begin(), wr.failed.end());
             if (st != YAQS_OK && !job.failed) {
                 job.failed = true;
                 job.status_code_pending = st;
@@ -465,7 +466,48 @@ void worker_main(yaqs_ctx *c) {
             // only this worker can reach here for the job: no wave is running or pending
             int fst = job.failed ? job.status_code_pending : YAQS_OK;
             std::string ferr = job.err;
-            if (!job.failed) {
+            if (!job.failed && !job.out.failed.empty()) {
+                // NaN/Inf before a split (src/Algorithms.jl:414-417): a
+                // mixed-precision trajectory is run again in FP64 on its
+                // own; what still fails (or failed in FP64) ends the job
+                // with YAQS_ENUMERIC
+                std::vector<int64_t> bad = job.out.failed;
+                std::sort(bad.begin(), bad.end());
+                bad.erase(std::unique(bad.begin(), bad.end()), bad.end());
+                const bool mixed = job.in.precision == YAQS_PREC_MIXED;
+                std::vector<int64_t> still;
+                if (mixed) {
+                    yq::JobIn f = job.in;
+                    f.precision = YAQS_PREC_FP64;
+                    f.wsync = nullptr;
+                    f.bench_warmup = -1;
+                    try {
+                        for (int64_t t : bad) {
+                            yq::JobResult rr;
+                            auto eng = yq::make_engine(f, 1, stream, &sres);
+                            eng->run(t, 1, job.d_out, job.in.digital ? job.out.bonds.data() : nullptr, rr);
+                            YQ_HIP(hipStreamSynchronize(stream));
+                            if (!rr.failed.empty()) still.push_back(t);
+                            job.out.jumps += rr.jumps;
+                            job.out.matvecs += rr.matvecs;
+                            job.out.flop += rr.flop;
+                        }
+                    } catch (const std::exception &e) {
+                        fst = YAQS_EDEVICE;
+                        ferr = std::string("FP64 retry of a trajectory with NaN/Inf failed: ") + e.what();
+                    }
+                } else {
+                    still = bad;
+                }
+                job.out.retried = mixed ? (int64_t)bad.size() : 0;
+                if (fst == YAQS_OK && !still.empty()) {
+                    fst = YAQS_ENUMERIC;
+                    ferr = "NaN/Inf in the two-site tensor before a split in " + std::to_string(still.size()) +
+                           " trajectories (first: " + std::to_string(still.front()) + ")" +
+                           (mixed ? ", also in the FP64 retry" : "");
+                }
+            }
+            if (!job.failed && fst == YAQS_OK) {
                 if (hipMemcpy(job.out.traj.data(), job.d_out, job.out.traj.size() * sizeof(double),
                               hipMemcpyDeviceToHost) != hipSuccess) {
                     fst = YAQS_EDEVICE;
@@ -536,6 +578,7 @@ int collect(yaqs_ctx *ctx, uint64_t id, double *traj_out, double *mean_out, int6
         stats->jumps = r.jumps;
         stats->max_bond = r.max_bond;
         stats->num_samples = r.ncol;
+        stats->fp64_retries = r.retried;
     }
     return YAQS_OK;
 }
@@ -729,6 +772,10 @@ int yaqs_run_analog_noalloc(yaqs_ctx *ctx, const yaqs_analog_job *job, double *t
             YQ_HIP(hipStreamSynchronize(s));
         }
         yq::tune_freeze(false);
+        if (!res.failed.empty())
+            return fail(YAQS_ENUMERIC, "NaN/Inf in the two-site tensor before a split in " +
+                                           std::to_string(res.failed.size()) + " trajectories (first: " +
+                                           std::to_string(res.failed.front()) + ")");
         if (traj_out && nobs)
             YQ_HIP(hipMemcpyAsync(traj_out, d_out, (size_t)nobs * ncol * ntraj * sizeof(double),
                                   hipMemcpyDeviceToHost, s));
@@ -780,6 +827,7 @@ int yaqs_run_analog_noalloc(yaqs_ctx *ctx, const yaqs_analog_job *job, double *t
         stats->jumps = res.jumps;
         stats->max_bond = res.max_bond;
         stats->num_samples = ncol;
+        stats->fp64_retries = 0;
     }
     return YAQS_OK;
 }

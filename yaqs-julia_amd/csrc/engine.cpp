@@ -342,6 +342,12 @@ struct GemmProf {
     }
 };
 thread_local GemmProf *g_prof = nullptr;
+
+// Test hook (yaqs_debug_poison_theta): NaN stored into the two-site tensor
+// of one trajectory before its given split, for engines of one precision.
+std::atomic<int64_t> g_poison_traj{-1};
+std::atomic<int64_t> g_poison_split{-1};
+std::atomic<int> g_poison_prec{-1};
 // Trajectories active in the current masked launch (Lanczos), -1 = all, -2 =
 // not known yet (a Lanczos iteration queued ahead of its count): the
 // profiler counts algorithmic FLOP of active trajectories only.
@@ -488,6 +494,8 @@ class Engine {
     Engine &operator=(const Engine &) = delete;
 
     void run_analog(int64_t t0, int nb, double *d_out, JobResult &res);
+    // appends the trajectories the NaN/Inf guard flagged in this wave
+    void collect_failed(JobResult &res);
     void run_digital(int64_t t0, int nb, double *d_out, int64_t *h_bonds, JobResult &res);
 
     static int64_t footprint(const JobIn &job, int B);
@@ -526,6 +534,8 @@ class Engine {
     int *h_pinned = nullptr;
     int *d_pinned = nullptr;  // device address of h_pinned
     DevBuf fin_done;          // lz_finalize's last-workgroup ticket
+    DevBuf nfail;             // [B] NaN/Inf guard flags of the wave
+    int64_t nsplit = 0;       // splits since the wave started (test hook)
     hipEvent_t lz_ev[2] = {nullptr, nullptr};  // Lanczos active counts land in h_pinned[1 + (j & 1)]
     // The count the finalize kernel stores over the -1 left in
     // h_pinned[slot]: the host spins on the pinned word (the next launches
@@ -760,6 +770,7 @@ void Engine<T>::setup() {
     count.alloc(sizeof(int));
     fin_done.alloc(sizeof(int));
     fill_int(fin_done.as<int>(), 0, 1, s);
+    nfail.alloc((size_t)B * sizeof(int));
 
     // ---- noise: dissipation operators per site and jump candidates
     auto site_ops = [&](double dt, std::vector<c64> &ops, std::vector<int> *act) {
@@ -1198,12 +1209,18 @@ void Engine<T>::split(State &S, int i, bool right, int rule, double thr) {
     const int m = 2 * cl, n = 2 * cr;
     const int ns = std::min(m, n);
     const int gn = right ? m : n;
+    if (g_poison_traj.load() >= 0 && g_poison_prec.load() == J.precision && g_poison_split.load() == nsplit) {
+        const int64_t t = g_poison_traj.load() - (int64_t)traj0;
+        if (t >= 0 && t < nb) poison_slot<T>(theta.as<CT>(), nmax, (int)t, s);
+    }
+    ++nsplit;
     if (right)
         gemm<T, T, double, double>(gd(m, m, n, OP_N, OP_C, n, n, m, nmax, nmax, slotG, nb), theta.as<CT>(),
                                    theta.as<CT>(), G.as<c64>(), s);
     else
         gemm<T, T, double, double>(gd(n, n, m, OP_C, OP_N, n, n, n, nmax, nmax, slotG, nb), theta.as<CT>(),
                                    theta.as<CT>(), G.as<c64>(), s);
+    guard_gram(G.as<c64>(), slotG, gn, nfail.as<int>(), nb, s);
     const int mb = (int)std::min<int64_t>(J.max_bond, 1 << 30);
     double2 *Pg = Pbuf.p ? Pbuf.as<double2>() : nullptr;
     int *fb = Hfb.p ? Hfb.as<int>() : nullptr;
@@ -1466,6 +1483,8 @@ template <typename T>
 void Engine<T>::run_analog(int64_t t0, int nb_, double *d_out, JobResult &res) {
     nb = nb_;
     traj0 = J.traj_offset + (uint64_t)t0;
+    fill_int(nfail.as<int>(), 0, B, s);
+    nsplit = 0;
     YQ_HIP(hipMemsetAsync(rngk.p, 0, (size_t)B * sizeof(uint64_t), s));
     init_state(PHI);
     const int nt = (int)J.num_times;
@@ -1565,6 +1584,8 @@ template <typename T>
 void Engine<T>::run_digital(int64_t t0, int nb_, double *d_out, int64_t *h_bonds, JobResult &res) {
     nb = nb_;
     traj0 = J.traj_offset + (uint64_t)t0;
+    fill_int(nfail.as<int>(), 0, B, s);
+    nsplit = 0;
     YQ_HIP(hipMemsetAsync(rngk.p, 0, (size_t)B * sizeof(uint64_t), s));
     init_state(PHI);
     // greedy layering (src/DigitalTJM.jl:86-145)
@@ -1828,6 +1849,15 @@ void Engine<T>::run_digital(int64_t t0, int nb_, double *d_out, int64_t *h_bonds
 
 }  // namespace
 
+template <typename T>
+void Engine<T>::collect_failed(JobResult &res) {
+    std::vector<int> h(nb);
+    YQ_HIP(hipMemcpyAsync(h.data(), nfail.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+    YQ_HIP(hipStreamSynchronize(s));
+    for (int t = 0; t < nb; ++t)
+        if (h[t]) res.failed.push_back((int64_t)(traj0 - J.traj_offset) + t);
+}
+
 // ----------------------------------------------------------------- API -----
 double rng_uniform_host(uint64_t seed, uint64_t traj, uint64_t k) {
     uint32_t c[4] = {(uint32_t)k, (uint32_t)(k >> 32), (uint32_t)traj, (uint32_t)(traj >> 32)};
@@ -1958,6 +1988,7 @@ class WaveEngineT : public WaveEngine {
     void run(int64_t t0, int nb, double *d_out, int64_t *h_bonds, JobResult &res) override {
         if (digital) eng.run_digital(t0, nb, d_out, h_bonds, res);
         else eng.run_analog(t0, nb, d_out, res);
+        eng.collect_failed(res);
     }
 
   private:
@@ -2034,3 +2065,9 @@ void run_job(const JobIn &job, int64_t batch, hipStream_t stream, JobResult &out
 }
 
 }  // namespace yq
+
+extern "C" void yaqs_debug_poison_theta(int64_t traj, int64_t split, int precision) {
+    yq::g_poison_split = split;
+    yq::g_poison_prec = precision;
+    yq::g_poison_traj = traj;
+}

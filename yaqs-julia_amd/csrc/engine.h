@@ -80,6 +80,10 @@ struct JobResult {
     std::vector<int64_t> bonds;   // [traj][col] (digital)
     double flop = 0;
     int64_t matvecs = 0, jumps = 0, max_bond = 0;
+    // job-level indices of trajectories whose two-site tensor held NaN/Inf
+    // before a split (their results are not valid)
+    std::vector<int64_t> failed;
+    int64_t retried = 0;  // mixed-precision trajectories rerun in FP64
     // measurement window (bench_warmup >= 0)
     double timed_s = 0, gemm_ms = 0, gemm_flop = 0, win_flop = 0;
     int64_t timed_steps = 0, gemm_launches = 0, win_matvecs = 0;

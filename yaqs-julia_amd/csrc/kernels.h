@@ -135,5 +135,14 @@ void observe(const c64 *rho1, int64_t sR1, const c64 *rho2, int64_t sR2, int nob
              hipStream_t s);
 void set_dims(int *dims, int ld, int b, const int *keep, int val, const int *mask, int B, hipStream_t s);
 void fill_int(int *p, int v, int n, hipStream_t s);
+// NaN/Inf guard before a bond split (src/Algorithms.jl:414-417): a Gram
+// matrix G (n x n per trajectory, stride sG) whose diagonal is not finite
+// comes from a two-site tensor with a NaN/Inf entry; its trajectory is
+// flagged (fail[t] = 1) and the matrix replaced by e0 e0^T, so the eigensolver only ever
+// sees finite input.
+void guard_gram(c64 *G, int64_t sG, int n, int *fail, int B, hipStream_t s);
+// Test hook: stores NaN into element 0 of slot t of A (t < B).
+template <typename T>
+void poison_slot(cplx<T> *A, int64_t sA, int t, hipStream_t s);
 
 }  // namespace yq

@@ -418,8 +418,11 @@ __global__ __launch_bounds__(256) void lz_update_kernel(cplx<T> *V, int64_t sV, 
 __device__ double2 tridiag_exp_taylor(double a, double bl, double br, int k, double tau, int lane) {
     double rown = fabs(a) + fabs(bl) + fabs(br);
     rown = wave_max(rown);
-    int s = (int)ceil(fabs(tau) * rown / 0.5);
-    if (s < 1) s = 1;
+    // sub-steps with |h| rown <= 1/2; NaN gives one, and the count is capped
+    // so that a non-finite T (a trajectory the split guard flags) cannot
+    // keep the wavefront looping
+    const double sd = ceil(fabs(tau) * rown / 0.5);
+    const int s = sd >= 1.0 ? (sd < 65536.0 ? (int)sd : 65536) : 1;
     const double h = tau / s;
     double2 v = make_double2(lane == 0 ? 1.0 : 0.0, 0.0);
     for (int st = 0; st < s; ++st) {
@@ -1337,6 +1340,47 @@ void fill_int(int *p, int v, int n, hipStream_t s) {
     hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
     YQ_HIP(hipGetLastError());
 }
+
+// One workgroup per trajectory: any non-finite diagonal entry flags the
+// trajectory and replaces its matrix by e0 e0^T (finite, rank one, so the
+// split keeps at most two columns and the wave's working dimension does not
+// grow for it).
+__global__ void guard_gram_kernel(c64 *G, int64_t sG, int n, int *fail) {
+    __shared__ int bad;
+    const int t = blockIdx.x;
+    c64 *g = G + (int64_t)t * sG;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const c64 v = g[(int64_t)i * n + i];
+        if (!isfinite(v.re) || !isfinite(v.im)) bad = 1;
+    }
+    __syncthreads();
+    if (!bad) return;
+    if (threadIdx.x == 0) fail[t] = 1;
+    const int64_t nn = (int64_t)n * n;
+    for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) g[e] = cmake(e == 0 ? 1.0 : 0.0, 0.0);
+}
+
+void guard_gram(c64 *G, int64_t sG, int n, int *fail, int B, hipStream_t s) {
+    if (B <= 0 || n <= 0) return;
+    hipLaunchKernelGGL(guard_gram_kernel, dim3(B), dim3(256), 0, s, G, sG, n, fail);
+    YQ_HIP(hipGetLastError());
+}
+
+template <typename T>
+__global__ void poison_kernel(cplx<T> *A) {
+    A->re = (T)NAN;
+    A->im = (T)NAN;
+}
+
+template <typename T>
+void poison_slot(cplx<T> *A, int64_t sA, int t, hipStream_t s) {
+    hipLaunchKernelGGL(poison_kernel<T>, dim3(1), dim3(1), 0, s, A + (int64_t)t * sA);
+    YQ_HIP(hipGetLastError());
+}
+template void poison_slot<float>(c32 *, int64_t, int, hipStream_t);
+template void poison_slot<double>(c64 *, int64_t, int, hipStream_t);
 
 // one wavefront: c = exp(-i tau T) e0 of the Lanczos stopping rule, for the
 // test hook below (alpha: k values, beta: k values, beta[0] unused)
