@@ -183,16 +183,3 @@ def test_unraveled_dephasing_parity(ctx, oracle_lib, unravel, kw):
     r, rm, rst = _oracle(oracle_lib, Y.MPS(L, state="x+"), H, cfg, noise, seed=4)
     assert st["jumps"] == rst["jumps"] and st["jumps"] > 0
     assert _err("g", g, r) < TOL_FP64
-
-
-def test_unraveled_crosstalk_is_rejected(ctx):
-    """Unravelings of adjacent 2-site processes give non-product jump
-    operators (I + X(x)X, exp(i theta X(x)X)); the engine applies 2-site jumps
-    as two 1-site factors, so job submission fails with a clear error."""
-    print("case: unraveled crosstalk rejected", flush=True)
-    L = 4
-    cfg = Y.TimeEvolutionConfig(_zx(L), 0.2, dt=0.05, num_traj=2, max_bond_dim=8)
-    for unravel in ("projector", "unitary_2pt"):
-        noise = Y.NoiseModel([{"name": "crosstalk_xx", "sites": [2, 3], "strength": 0.1, "unraveling": unravel}], L)
-        with pytest.raises(Y.YaqsError, match="2-site non-Pauli"):
-            _gpu(ctx, Y.MPS(L), Y.init_ising(L, 1.0, 0.5), cfg, noise)

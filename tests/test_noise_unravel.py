@@ -164,3 +164,55 @@ def test_oracle_ensemble_matches_lindblad(oracle_lib, unravel, kw):
     ex = E.lindblad(H, jumps, np.outer(psi, psi.conj()), dt, steps, ops)
     dev = np.abs(mean - ex)
     assert dev.max() < 0.07 and dev.mean() < 0.015, (unravel, dev.max(), dev.mean())
+
+
+def _pair_op(M, s, L):
+    """A 4x4 operator in the reference's pair index (p1 + 2 p2, p1 on site s)
+    as a dense operator on sites (s, s+1) of the exact solver (site 1 most
+    significant)."""
+    import scipy.sparse as sp
+    P = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], dtype=complex)
+    Mstd = P @ np.asarray(M, complex) @ P
+    return sp.kron(sp.kron(sp.identity(2 ** (s - 1), format="csr"), sp.csr_matrix(Mstd)),
+                   sp.identity(2 ** (L - s - 1), format="csr"), format="csr")
+
+
+@pytest.mark.parametrize("name", ["custom", "raising_two", "crosstalk_xy"])
+def test_oracle_two_site_ensemble_matches_lindblad(oracle_lib, name):
+    """Adjacent 2-site processes (merged-pair dissipation with the T-C split,
+    merged-pair weights, local_2 jumps): Ising L=4 (J=1, g=0.5) from |++++>,
+    lowering gamma=0.1 on every site and the 2-site process gamma=0.2 on
+    every pair, dt=0.05, T=1, 300 trajectories: the oracle's mean <Z_i>,
+    <X_i> against the exact Lindblad solution within sampling error
+    (measured mean deviations 0.009-0.013; against Lindblad solutions with
+    twice the pair rate, the pair's sites swapped or the operator transposed
+    the same ensembles give 0.024-0.11 wherever those change the model)."""
+    L, ntraj, dt, steps = 4, 300, 0.05, 20
+    rng = np.random.default_rng(11)
+    M = (rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4))) / 2 if name == "custom" else None
+    procs = [{"name": "lowering", "sites": [i], "strength": 0.1} for i in range(1, L + 1)]
+    for i in range(1, L):
+        p = {"name": name, "sites": [i, i + 1], "strength": 0.2}
+        if M is not None:
+            p["matrix"] = M
+        procs.append(p)
+    noise = Y.NoiseModel(procs, L)
+    obs = [Y.Observable(f"z{s}", Y.gates.ZGate(), s) for s in range(1, L + 1)] + \
+          [Y.Observable(f"x{s}", Y.gates.XGate(), s) for s in range(1, L + 1)]
+    cfg = Y.TimeEvolutionConfig(obs, dt * steps, dt=dt, num_traj=ntraj, max_bond_dim=16)
+    h = _abi.pack_analog(Y.MPS(L, state="x+"), Y.init_ising(L, 1.0, 0.5), cfg, noise, seed=5)
+    traj, mean, st = oracle_lib.run_analog(h, nthreads=8)
+    assert st["jumps"] > 0
+    H = E.ising(L, 1.0, 0.5)
+    op2 = noise.processes[-1].matrix
+    jumps = [(0.1, E.site_op(E.LOWER, s, L)) for s in range(1, L + 1)] + \
+            [(0.2, _pair_op(op2, s, L)) for s in range(1, L)]
+    plus = np.ones(2) / np.sqrt(2)
+    psi = plus
+    for _ in range(L - 1):
+        psi = np.kron(psi, plus)
+    ops = [E.site_op(E.Z, s, L) for s in range(1, L + 1)] + [E.site_op(E.X, s, L) for s in range(1, L + 1)]
+    ex = E.lindblad(H, jumps, np.outer(psi, psi.conj()), dt, steps, ops)
+    dev = np.abs(mean - ex)
+    print(name, dev.max(), dev.mean())
+    assert dev.max() < 0.07 and dev.mean() < 0.015, (name, dev.max(), dev.mean())

@@ -151,6 +151,55 @@ void damp2(const c64 *Le, double scale, c64 *out) {
     for (int i = 0; i < 4; ++i) out[i] = fromc(R[i]);
 }
 
+// exp(-scale * L^†L) for a 4x4 L (engine row-major) by scaling and squaring
+// of the Taylor series (L^†L is Hermitian PSD; scale*||L^†L|| / 2^s <= 1/2,
+// terms until they fall below 1e-18 of the sum): the 4x4 `exp` of
+// src/Dissipation.jl:94-95.
+void damp4(const c64 *Le, double scale, c64 *out) {
+    cd M[16];
+    double nrm = 0;
+    for (int r = 0; r < 4; ++r) {
+        double row = 0;
+        for (int c = 0; c < 4; ++c) {
+            cd acc = 0;
+            for (int k = 0; k < 4; ++k) acc += std::conj(toc(Le[k * 4 + r])) * toc(Le[k * 4 + c]);
+            M[r * 4 + c] = -scale * acc;
+            row += std::abs(M[r * 4 + c]);
+        }
+        nrm = std::max(nrm, row);
+    }
+    int sq = 0;
+    while (nrm > 0.5 && sq < 60) { nrm *= 0.5; ++sq; }
+    const double f = std::ldexp(1.0, -sq);
+    for (auto &z : M) z *= f;
+    cd E[16], Tm[16], U[16];
+    for (int k = 0; k < 16; ++k) E[k] = Tm[k] = (k % 5 == 0) ? cd(1, 0) : cd(0, 0);
+    auto mul = [](const cd *X, const cd *Y, cd *Z) {
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c) {
+                cd acc = 0;
+                for (int k = 0; k < 4; ++k) acc += X[r * 4 + k] * Y[k * 4 + c];
+                Z[r * 4 + c] = acc;
+            }
+    };
+    for (int n = 1; n <= 40; ++n) {
+        mul(Tm, M, U);
+        double tn = 0, en = 0;
+        for (int k = 0; k < 16; ++k) {
+            Tm[k] = U[k] / double(n);
+            E[k] += Tm[k];
+            tn += std::abs(Tm[k]);
+            en += std::abs(E[k]);
+        }
+        if (tn <= 1e-18 * en) break;
+    }
+    for (int q = 0; q < sq; ++q) {
+        mul(E, E, U);
+        for (int k = 0; k < 16; ++k) E[k] = U[k];
+    }
+    for (int k = 0; k < 16; ++k) out[k] = fromc(E[k]);
+}
+
 void matmul2(const c64 *A, const c64 *B, c64 *C) {  // C = A B (2x2 row-major)
     c64 r[4];
     for (int i = 0; i < 2; ++i)
@@ -520,6 +569,23 @@ class Engine {
     DevBuf beta0, galpha, alpha, beta, coef, nit, active, part, partn, nactive;
     DevBuf norm2v, fac, jstate, chosen, opidx, mask, rngk, wbuf, count;
     DevBuf diss_half, diss_full, diss_one;  // per-site 2x2 dissipation operators
+    // Adjacent 2-site non-Pauli processes (analog): the dissipation runs the
+    // reference's sweep (merge, 4x4 exp(-dt/2 γ L†L), relative-threshold
+    // split, centre shift per site) and jumps on a pair are merged-pair
+    // jumps; d4_at[i]: indices (of 4x4 operators) of the processes whose
+    // upper site is i, in process order
+    bool has2 = false;
+    std::vector<std::vector<int>> d4_at;
+    DevBuf diss4_half, diss4_full;  // 16 c64 per 2-site non-Pauli process
+    bool scr_ready = false;         // SCR holds ψ~ of the last sweep (see stochastic)
+    // merged-pair jump candidates: cand_pair[c] = index of the candidate's
+    // 4x4 operator in pair_ops (-1: not a pair jump), cand_kind[c] = 1 when
+    // its weight is Tr(L†L ρ2) (cand_LdL4), 0 for Tr(L†L ρ1)
+    DevBuf cand_pair, cand_kind, cand_LdL4, pair_ops, opidx4;
+    std::vector<char> cand2_at;        // pairs (i, i+1) whose ρ2 the weights need
+    std::vector<char> pair_cand_site;  // pairs with merged-pair jump candidates
+    std::vector<int> h_cand_site, h_cand_pair;
+    DevBuf rho2b, mask2;
     DevBuf invL_half, invL_full;            // inverse of the last site's operator
     bool last_site_damped = false;
     std::vector<int> diss_site_active;
@@ -581,14 +647,18 @@ class Engine {
     void matvec(State &S, bool two, int i, const CT *x, int64_t sx, CT *y, int64_t sy, const int *act);
     void expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, JobResult &res);
     void merge(State &S, int i);
-    void split(State &S, int i, bool right, int rule, double thr);
+    void split(State &S, int i, bool right, int rule, double thr, const int *msk = nullptr);
     void env_left(State &S, int i);
     void env_right(State &S, int i);
     void tdvp(State &S, JobResult &res);
     void lq_sweep(State &S, int from, const int *msk);
     void qr_step(State &S, int i, const int *msk);
-    void dissipate(State &S, const DevBuf &ops);
-    void transfer(State &S, const int *msk, c64 *rho_out);
+    void dissipate(State &S, bool half, int field = 0);
+    void dissipate_sweep(State &S, bool half, int field);
+    void lq_step(State &S, int i, const int *msk, bool keep_factor = true);
+    void regrow(State &S, int b, int w);
+    void transfer(State &S, const int *msk, c64 *rho_out, c64 *rho2_out = nullptr,
+                  const std::vector<char> *at2 = nullptr);
     void measure(State &S, int col, int ncol, double *d_out);
     void stochastic(State &S, double dt, const DevBuf *invL, JobResult &res);
     void absorb_left(State &S, int i, const int *msk);
@@ -820,9 +890,32 @@ void Engine<T>::setup() {
     site_ops(J.dt, hops, nullptr);
     upload(diss_full, hops, s);
     inv_last(hops, invL_full);
-    if (!J.digital && J.has_noise && last_site_damped) {
+    // adjacent 2-site non-Pauli processes: 4x4 dissipators (dt/2 and dt)
+    d4_at.assign(L, {});
+    {
+        bool allzero = true;
+        for (auto &p : J.noise) allzero = allzero && p.gamma == 0.0;
+        std::vector<c64> h4, f4;
+        int n4 = 0;
+        if (!J.digital && J.has_noise && !allzero)
+            for (auto &p : J.noise) {
+                if (p.nsites != 2 || p.pauli_name) continue;
+                c64 e4[16], M[16];
+                op4_engine(p.m, e4);
+                damp4(e4, 0.5 * (J.dt / 2) * p.gamma, M);
+                h4.insert(h4.end(), M, M + 16);
+                damp4(e4, 0.5 * J.dt * p.gamma, M);
+                f4.insert(f4.end(), M, M + 16);
+                d4_at[std::max(p.s0, p.s1)].push_back(n4++);
+                has2 = true;
+            }
+        upload(diss4_half, h4, s);
+        upload(diss4_full, f4, s);
+    }
+    if (!J.digital && J.has_noise && (last_site_damped || has2)) {
         alloc_state(SCR);
         rho1b.alloc((size_t)B * L * 4 * sizeof(c64));
+        rho2b.alloc((size_t)B * L * 16 * sizeof(c64));
     }
     site_ops(1.0, hops, nullptr);
     upload(diss_one, hops, s);
@@ -837,6 +930,8 @@ void Engine<T>::setup() {
             op2_engine(p.m, e);
             proc_op[q] = (int)(jops.size() / 4);
             jops.insert(jops.end(), e, e + 4);
+        } else if (!J.digital) {
+            // analog: a merged-pair jump (src/StochasticProcess.jl:266-309)
         } else {
             c64 e4[16], Af[4], Bf[4];
             op4_engine(p.m, e4);
@@ -852,8 +947,8 @@ void Engine<T>::setup() {
     upload(jump_ops, jops, s);
     // analog candidates in the reference's order (src/StochasticProcess.jl:97-167)
     {
-        std::vector<int> cs, co, cs2, co2;
-        std::vector<c64> ldl;
+        std::vector<int> cs, co, cs2, co2, cpair, ckind;
+        std::vector<c64> ldl, ldl4, pops;
         std::vector<double> cg;
         auto add = [&](size_t q, int site, bool two) {
             const ProcIn &p = J.noise[q];
@@ -861,6 +956,32 @@ void Engine<T>::setup() {
             co.push_back(proc_op[q]);
             cs2.push_back(two ? proc_site2[q] : -1);
             co2.push_back(two ? proc_op2[q] : -1);
+            // analog pair candidates: the 4x4 jump operator, and L†L for the
+            // merged-pair weight of processes that is_pauli_proc
+            // (src/StochasticProcess.jl:343-353) does not call Pauli: names
+            // with pauli/crosstalk (not projector) or a unitary matrix weigh
+            // by the centre norm instead (src/StochasticProcess.jl:143-147)
+            int pidx = -1, kind = 0;
+            c64 M4[16];
+            for (auto &z : M4) z = mk(0, 0);
+            if (two && !J.digital) {
+                c64 e4[16];
+                op4_engine(p.m, e4);
+                pidx = (int)(pops.size() / 16);
+                pops.insert(pops.end(), e4, e4 + 16);
+                bool unitary = true;
+                for (int r = 0; r < 4; ++r)
+                    for (int c = 0; c < 4; ++c) {
+                        cd acc = 0;
+                        for (int k = 0; k < 4; ++k) acc += std::conj(toc(e4[k * 4 + r])) * toc(e4[k * 4 + c]);
+                        M4[r * 4 + c] = fromc(acc);
+                        unitary = unitary && std::abs(acc - (r == c ? 1.0 : 0.0)) <= 1e-10;
+                    }
+                if (!p.pauli_name && !unitary) kind = 1;
+            }
+            cpair.push_back(pidx);
+            ckind.push_back(kind);
+            ldl4.insert(ldl4.end(), M4, M4 + 16);
             c64 M[4];
             if (!two) {
                 c64 e[4];
@@ -883,6 +1004,20 @@ void Engine<T>::setup() {
             }
         }
         ncand = (int)cs.size();
+        cand2_at.assign(L, 0);
+        pair_cand_site.assign(L, 0);
+        for (int c = 0; c < ncand; ++c) {
+            if (ckind[c] == 1) cand2_at[cs[c]] = 1;
+            if (cpair[c] >= 0) pair_cand_site[cs[c]] = 1;
+        }
+        h_cand_site = cs;
+        h_cand_pair = cpair;
+        mask2.alloc((size_t)B * sizeof(int));
+        upload(cand_pair, cpair, s);
+        upload(cand_kind, ckind, s);
+        upload(cand_LdL4, ldl4, s);
+        upload(pair_ops, pops, s);
+        opidx4.alloc((size_t)B * sizeof(int));
         upload(cand_site, cs, s);
         upload(cand_op, co, s);
         upload(cand_site2, cs2, s);
@@ -1204,8 +1339,9 @@ void Engine<T>::merge(State &S, int i) {
 // the FP64 Gram matrix and a Jacobi eigensolver.  right = centre moves right
 // (A_i = U, A_{i+1} = U^† Θ); else A_{i+1} = V^†, A_i = Θ V.
 template <typename T>
-void Engine<T>::split(State &S, int i, bool right, int rule, double thr) {
+void Engine<T>::split(State &S, int i, bool right, int rule, double thr, const int *msk) {
     const int cl = S.wd[i], cr = S.wd[i + 2];
+    const int w_old = S.wd[i + 1];
     const int m = 2 * cl, n = 2 * cr;
     const int ns = std::min(m, n);
     const int gn = right ? m : n;
@@ -1215,18 +1351,19 @@ void Engine<T>::split(State &S, int i, bool right, int rule, double thr) {
     }
     ++nsplit;
     if (right)
-        gemm<T, T, double, double>(gd(m, m, n, OP_N, OP_C, n, n, m, nmax, nmax, slotG, nb), theta.as<CT>(),
+        gemm<T, T, double, double>(gd(m, m, n, OP_N, OP_C, n, n, m, nmax, nmax, slotG, nb, msk), theta.as<CT>(),
                                    theta.as<CT>(), G.as<c64>(), s);
     else
-        gemm<T, T, double, double>(gd(n, n, m, OP_C, OP_N, n, n, n, nmax, nmax, slotG, nb), theta.as<CT>(),
+        gemm<T, T, double, double>(gd(n, n, m, OP_C, OP_N, n, n, n, nmax, nmax, slotG, nb, msk), theta.as<CT>(),
                                    theta.as<CT>(), G.as<c64>(), s);
-    guard_gram(G.as<c64>(), slotG, gn, nfail.as<int>(), nb, s);
+    guard_gram(G.as<c64>(), slotG, gn, nfail.as<int>(), msk, nb, s);
+    if (msk) fill_int(keep.as<int>(), 0, nb, s);  // masked-out slots report rank 0
     const int mb = (int)std::min<int64_t>(J.max_bond, 1 << 30);
     double2 *Pg = Pbuf.p ? Pbuf.as<double2>() : nullptr;
     int *fb = Hfb.p ? Hfb.as<int>() : nullptr;
     const bool multi = heev(G.as<c64>(), slotG, U.as<c64>(), slotG, Zbuf.as<double>(), slotZ, Pg, slotP, Hwork.p, slotH,
                             fb, gn, ns, evals.as<double>(), perm.as<int>(), gdim + 2, keep.as<int>(), rule, thr, mb,
-                            nullptr, nb, s, false);
+                            msk, nb, s, false);
     // new working dimension = max kept rank over the wave; the kept ranks and
     // the inverse-iteration breakdown flags come back in one readback, and the
     // single-workgroup solver runs only when a matrix needs it
@@ -1236,26 +1373,32 @@ void Engine<T>::split(State &S, int i, bool right, int rule, double thr) {
     YQ_HIP(hipStreamSynchronize(s));
     if (multi && std::any_of(hf.begin(), hf.end(), [](int f) { return f != 0; })) {
         heev_fixup(G.as<c64>(), slotG, U.as<c64>(), slotG, Zbuf.as<double>(), slotZ, Pg, slotP, fb, gn, ns,
-                   evals.as<double>(), perm.as<int>(), gdim + 2, keep.as<int>(), rule, thr, mb, nullptr, nb, s);
+                   evals.as<double>(), perm.as<int>(), gdim + 2, keep.as<int>(), rule, thr, mb, msk, nb, s);
         YQ_HIP(hipMemcpyAsync(hk.data(), keep.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
         YQ_HIP(hipStreamSynchronize(s));
     }
     int kw = 1;
     for (int v : hk) kw = std::max(kw, v);
     if (kw > cap[i + 1]) throw std::runtime_error("bond dimension exceeds capacity");
+    if (msk) {
+        // slots outside the mask keep their tensors: the working dimension
+        // only grows, and their tensors are re-laid out for it first
+        if (kw > w_old) regrow(S, i + 1, kw);
+        kw = std::max(kw, w_old);
+    }
     if (right) {
-        write_iso<T>(U.as<c64>(), slotG, perm.as<int>(), gdim + 2, keep.as<int>(), m, kw, 0, A(S, i), slotA[i], nullptr,
+        write_iso<T>(U.as<c64>(), slotG, perm.as<int>(), gdim + 2, keep.as<int>(), m, kw, 0, A(S, i), slotA[i], msk,
                      nb, s);
-        gemmT<T>(gd(kw, n, m, OP_C, OP_N, kw, n, n, slotA[i], nmax, slotA[i + 1], nb), A(S, i), theta.as<CT>(),
+        gemmT<T>(gd(kw, n, m, OP_C, OP_N, kw, n, n, slotA[i], nmax, slotA[i + 1], nb, msk), A(S, i), theta.as<CT>(),
                  A(S, i + 1), s);
     } else {
         write_iso<T>(U.as<c64>(), slotG, perm.as<int>(), gdim + 2, keep.as<int>(), n, kw, 1, A(S, i + 1), slotA[i + 1],
-                     nullptr, nb, s);
-        gemm<T, T, T, T>(gd(m, kw, n, OP_N, OP_C, n, n, kw, nmax, slotA[i + 1], slotA[i], nb), theta.as<CT>(),
+                     msk, nb, s);
+        gemm<T, T, T, T>(gd(m, kw, n, OP_N, OP_C, n, n, kw, nmax, slotA[i + 1], slotA[i], nb, msk), theta.as<CT>(),
                          A(S, i + 1), A(S, i), s);
     }
     S.wd[i + 1] = kw;
-    set_dims(S.dims.as<int>(), L + 1, i + 1, keep.as<int>(), 0, nullptr, nb, s);
+    set_dims(S.dims.as<int>(), L + 1, i + 1, keep.as<int>(), 0, msk, nb, s);
 }
 
 // E_left[i+1] from E_left[i] and A_i (update_left_environment, src/Algorithms.jl:199-204)
@@ -1359,17 +1502,96 @@ void Engine<T>::qr_step(State &S, int i, const int *msk) {
     copy2d<T, T>(absorb.as<CT>(), slotA[i + 1], 2 * c2, A(S, i + 1), slotA[i + 1], 2 * c2, c1, 2 * c2, msk, nb, s);
 }
 
+// apply_dissipation (src/Dissipation.jl:37-159).  Without adjacent 2-site
+// non-Pauli processes every operator is local and exact in any gauge, so the
+// per-site 2x2 products are applied in one pass; with them, the reference's
+// sweep runs (dissipate_sweep).  field: the reference's centre field on entry
+// (0 after a TDVP sweep; the last observable's site right after the t=0
+// measurement), where the engine's state has its centre at 0.
 template <typename T>
-void Engine<T>::dissipate(State &S, const DevBuf &ops) {
+void Engine<T>::dissipate(State &S, bool half, int field) {
+    scr_ready = false;
+    if (has2) {
+        dissipate_sweep(S, half, field);
+        return;
+    }
+    const DevBuf &ops = half ? diss_half : diss_full;
     for (int i = 0; i < L; ++i)
         if (diss_site_active[i])
             local_op<T>(A(S, i), slotA[i], S.wd[i], S.wd[i + 1], ops.as<c64>() + 4 * (int64_t)i, nullptr, nullptr, nb, s);
 }
 
+// The reference's dissipation sweep, site i = L..1: the 1-site operators of
+// site i, then each 2-site non-Pauli process with upper site i in process
+// order (merge (i-1, i), apply exp(-dt/2 γ L†L), split with the relative
+// rule T-C: U on i-1, S V† on i), then the centre moves to i-1 (from the
+// field by QR when i = L, by LQ of site i after that).  Pauli-named 2-site
+// processes are the scalar already folded into site i's operator.  At the
+// end ψ~ (site L replaced by the isometry of its LQ) is kept in SCR for the
+// jump weights (see stochastic).
+template <typename T>
+void Engine<T>::dissipate_sweep(State &S, bool half, int field) {
+    const DevBuf &ops = half ? diss_half : diss_full;
+    const DevBuf &ops4 = half ? diss4_half : diss4_full;
+    for (int k = 0; k < field; ++k) qr_step(S, k, nullptr);  // the engine's centre to the field
+    for (int i = L - 1; i >= 0; --i) {
+        if (diss_site_active[i])
+            local_op<T>(A(S, i), slotA[i], S.wd[i], S.wd[i + 1], ops.as<c64>() + 4 * (int64_t)i, nullptr, nullptr, nb, s);
+        if (i == 0) break;
+        for (int q : d4_at[i]) {
+            merge(S, i - 1);
+            local_op4<T>(theta.as<CT>(), nmax, S.wd[i - 1], S.wd[i + 1], ops4.as<c64>() + 16 * (int64_t)q, nb, s);
+            split(S, i - 1, true, 1, J.thr);
+        }
+        if (i == L - 1) {
+            if (field == L - 1) lq_step(S, i, nullptr);
+            else
+                for (int k = field; k + 1 < L - 1; ++k) qr_step(S, k, nullptr);
+        } else {
+            lq_step(S, i, nullptr);
+        }
+    }
+    // ψ~ for the weights: the reference reads the centre norms with site L
+    // as it stands (not re-orthogonalised when the field started below L)
+    if (field != L - 1 && L >= 2) {
+        copy_state(S, SCR);
+        lq_step(SCR, L - 1, nullptr, false);
+        scr_ready = true;
+    }
+}
+
+// LQ of site i (CGS2 on its rows): A_i <- Q and, with keep_factor, the L
+// factor absorbed into site i-1 (the centre moves to i-1); without, the
+// factor is dropped.
+template <typename T>
+void Engine<T>::lq_step(State &S, int i, const int *msk, bool keep_factor) {
+    const int c0 = S.wd[i], c1 = S.wd[i + 1];
+    cgs2<T>(A(S, i), slotA[i], c0, 2 * c1, 2 * c1, 1, Cf.as<CT>(), (int64_t)capmax * capmax, cgsS.as<CT>(), slotCgs,
+            cgsF.as<double>(), msk, nb, s);
+    if (keep_factor) absorb_left(S, i, msk);
+}
+
+// Re-lays out every slot's A_{b-1} (columns) and A_b (rows) for a larger
+// working dimension w at bond b, with zeros in the new columns / rows.
+template <typename T>
+void Engine<T>::regrow(State &S, int b, int w) {
+    const int wo = S.wd[b];
+    if (w <= wo) return;
+    const int r0 = 2 * S.wd[b - 1];          // rows of A_{b-1} ([a][s] flattened)
+    fill_zero<T>(absorb.as<CT>(), slotA[b - 1], (int64_t)r0 * w, nb, s);
+    copy2d<T, T>(A(S, b - 1), slotA[b - 1], wo, absorb.as<CT>(), slotA[b - 1], w, r0, wo, nullptr, nb, s);
+    copy2d<T, T>(absorb.as<CT>(), slotA[b - 1], w, A(S, b - 1), slotA[b - 1], w, r0, w, nullptr, nb, s);
+    const int64_t row = 2 * (int64_t)S.wd[b + 1];  // A_b is [a][s][c]: one bond row = 2 wd[b+1] elements
+    fill_zero<T>(A(S, b) + (int64_t)wo * row, slotA[b], (int64_t)(w - wo) * row, nb, s);
+    S.wd[b] = w;
+}
+
 // Left transfer sweep in FP64 giving every site's reduced matrix ρ_s (and
 // 4x4 ρ for 2-site observables) of a right-canonical state.
 template <typename T>
-void Engine<T>::transfer(State &S, const int *msk, c64 *rho_out) {
+void Engine<T>::transfer(State &S, const int *msk, c64 *rho_out, c64 *rho2_out, const std::vector<char> *at2) {
+    if (!rho2_out) rho2_out = rho2.as<c64>();
+    if (!at2) at2 = &obs2_at;
     c64 *E0 = Ebuf0.as<c64>(), *E1 = Ebuf1.as<c64>();
     const int64_t sE = (int64_t)capmax * capmax;
     identity_env<double>(E0, sE, S.wd[0], 1, 0, nb, s);
@@ -1379,12 +1601,12 @@ void Engine<T>::transfer(State &S, const int *msk, c64 *rho_out) {
                                         E0, A(S, i), Xbuf.as<c64>(), s);
         rho<T>(Xbuf.as<c64>(), slotX, A(S, i), slotA[i], cl, 2, cr, rho_out + (int64_t)i * 4, (int64_t)L * 4, msk, nb,
                s);
-        if (obs2_at[i] && i + 1 < L) {
+        if ((*at2)[i] && i + 1 < L) {
             merge(S, i);
             const int c2 = S.wd[i + 2];
             gemm<double, T, double, double>(gd(cl, 4 * c2, cl, OP_N, OP_N, cl, 4 * c2, 4 * c2, sE, nmax, slotX, nb, msk),
                                             E0, theta.as<CT>(), Xbuf.as<c64>(), s);
-            rho<T>(Xbuf.as<c64>(), slotX, theta.as<CT>(), nmax, cl, 4, c2, rho2.as<c64>() + (int64_t)i * 16,
+            rho<T>(Xbuf.as<c64>(), slotX, theta.as<CT>(), nmax, cl, 4, c2, rho2_out + (int64_t)i * 16,
                    (int64_t)L * 16, msk, nb, s);
             gemm<double, T, double, double>(gd(cl, 2 * cr, cl, OP_N, OP_N, cl, 2 * cr, 2 * cr, sE, slotA[i], slotX, nb,
                                                msk),
@@ -1431,8 +1653,21 @@ void Engine<T>::stochastic(State &S, double dt, const DevBuf *invL, JobResult &r
     mask_from_state(jstate.as<int>(), 1, mask.as<int>(), count.as<int>(), nb, s);
     const int njump = sync_count(count.as<int>());
     if (njump == 0) return;
-    transfer(S, mask.as<int>(), rho1.as<c64>());
-    if (invL && last_site_damped) {
+    transfer(S, mask.as<int>(), rho1.as<c64>(), rho2.as<c64>(), &cand2_at);
+    if (has2) {
+        // ψ~ of the dissipation sweep (site L replaced by its LQ isometry):
+        // 1-site weights of sites < L and merged-pair weights of pairs
+        // below (L-1, L) read it; the others read the true state
+        if (scr_ready) {
+            lq_sweep(SCR, L - 2, mask.as<int>());
+            transfer(SCR, mask.as<int>(), rho1b.as<c64>(), rho2b.as<c64>(), &cand2_at);
+            copy2d<double, double>(rho1b.as<c64>(), (int64_t)L * 4, 4 * (L - 1), rho1.as<c64>(), (int64_t)L * 4,
+                                   4 * (L - 1), 1, 4 * (L - 1), mask.as<int>(), nb, s);
+            if (L >= 3)
+                copy2d<double, double>(rho2b.as<c64>(), (int64_t)L * 16, 16 * (L - 2), rho2.as<c64>(),
+                                       (int64_t)L * 16, 16 * (L - 2), 1, 16 * (L - 2), mask.as<int>(), nb, s);
+        }
+    } else if (invL && last_site_damped) {
         copy_state(S, SCR);
         local_op<T>(A(SCR, L - 1), slotA[L - 1], SCR.wd[L - 1], SCR.wd[L], invL->as<c64>(), nullptr, mask.as<int>(), nb,
                     s);
@@ -1443,7 +1678,7 @@ void Engine<T>::stochastic(State &S, double dt, const DevBuf *invL, JobResult &r
     }
     jump_choose(rho1.as<c64>(), (int64_t)L * 4, L, ncand, cand_site.as<int>(), cand_LdL.as<c64>(),
                 cand_gamma.as<double>(), dt, J.seed, traj0, rngk.as<uint64_t>(), jstate.as<int>(), chosen.as<int>(),
-                wbuf.as<double>(), nb, s);
+                wbuf.as<double>(), nb, s, rho2.as<c64>(), (int64_t)L * 16, cand_LdL4.as<c64>(), cand_kind.as<int>());
     if (std::getenv("YAQS_DEBUG_JUMPS")) {
         std::vector<int> hs(nb), hc(nb);
         std::vector<uint64_t> hk(nb);
@@ -1467,6 +1702,39 @@ void Engine<T>::stochastic(State &S, double dt, const DevBuf *invL, JobResult &r
         jump_opidx(jstate.as<int>(), chosen.as<int>(), cand_site.as<int>(), cand_op.as<int>(), cand_site2.as<int>(),
                    cand_op2.as<int>(), i, opidx.as<int>(), nb, s);
         local_op<T>(A(S, i), slotA[i], S.wd[i], S.wd[i + 1], jump_ops.as<c64>(), opidx.as<int>(), nullptr, nb, s);
+    }
+    // merged-pair jumps (local_2, src/StochasticProcess.jl:266-309): the
+    // centre walks right by QR for the trajectories whose pair lies further
+    // on; at its pair each merges it, applies the 4x4 operator and splits
+    // with the relative rule T-C (U on the first site, S V† on the second)
+    if (std::any_of(pair_cand_site.begin(), pair_cand_site.end(), [](char c) { return c != 0; })) {
+        std::vector<int> hs(nb), hc(nb);
+        YQ_HIP(hipMemcpyAsync(hs.data(), jstate.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        YQ_HIP(hipMemcpyAsync(hc.data(), chosen.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        YQ_HIP(hipStreamSynchronize(s));
+        std::vector<char> at(L, 0);
+        int pmax = -1;
+        for (int t = 0; t < nb; ++t)
+            if (hs[t] == 1 && hc[t] >= 0 && hc[t] < ncand && h_cand_pair[hc[t]] >= 0) {
+                const int p = h_cand_site[hc[t]];
+                at[p] = 1;
+                pmax = std::max(pmax, p);
+            }
+        for (int p = 0; p <= pmax; ++p) {
+            if (at[p]) {
+                jump_pair_mask(jstate.as<int>(), chosen.as<int>(), cand_site.as<int>(), cand_pair.as<int>(), p, 0,
+                               mask2.as<int>(), opidx4.as<int>(), nb, s);
+                merge(S, p);
+                local_op4_sel<T>(theta.as<CT>(), nmax, S.wd[p], S.wd[p + 2], pair_ops.as<c64>(), opidx4.as<int>(), nb,
+                                 s);
+                split(S, p, true, 1, J.thr, mask2.as<int>());
+            }
+            if (p < pmax) {
+                jump_pair_mask(jstate.as<int>(), chosen.as<int>(), cand_site.as<int>(), cand_pair.as<int>(), p, 1,
+                               mask2.as<int>(), nullptr, nb, s);
+                qr_step(S, p, mask2.as<int>());
+            }
+        }
     }
     fill_int(count.as<int>(), 0, 1, s);
     mask_from_state(jstate.as<int>(), 1, mask.as<int>(), count.as<int>(), nb, s);
@@ -1498,7 +1766,7 @@ void Engine<T>::run_analog(int64_t t0, int nb_, double *d_out, JobResult &res) {
         for (int j = 2; j <= nt; ++j) {
             tdvp(PHI, res);
             if (noise) {
-                dissipate(PHI, diss_full);
+                dissipate(PHI, false);
                 stochastic(PHI, J.dt, &invL_full, res);
             }
             record_bond(PHI);
@@ -1511,14 +1779,14 @@ void Engine<T>::run_analog(int64_t t0, int nb_, double *d_out, JobResult &res) {
         // centre field after the t=0 measurement: site of the last observable
         int field = 0;
         if (J.sample_timesteps && !J.obs.empty()) field = std::min(J.obs.back().s0, J.obs.back().s1);
-        dissipate(PHI, diss_half);
+        dissipate(PHI, true, field);
         stochastic(PHI, J.dt, field == L - 1 ? nullptr : &invL_half, res);
     }
     auto sample = [&](int j) {  // src/AnalogTJM.jl:62-81
         copy_state(PHI, PSI);
         tdvp(PSI, res);
         if (noise) {
-            dissipate(PSI, diss_half);
+            dissipate(PSI, true);
             stochastic(PSI, J.dt, &invL_half, res);
         }
         record_bond(PSI);
@@ -1569,7 +1837,7 @@ void Engine<T>::run_analog(int64_t t0, int nb_, double *d_out, JobResult &res) {
     for (int j = 3; j <= nt; ++j) {
         tdvp(PHI, res);  // step_through (src/AnalogTJM.jl:40-55)
         if (noise) {
-            dissipate(PHI, diss_full);
+            dissipate(PHI, false);
             stochastic(PHI, J.dt, &invL_full, res);
         }
         if (J.sample_timesteps || j == nt) sample(j);
@@ -1903,8 +2171,9 @@ void validate_job(const JobIn &job) {
         if (p.s0 < 0 || p.s0 >= job.L || p.s1 < 0 || p.s1 >= job.L) throw std::invalid_argument("noise site out of range");
         if (p.nsites == 2) {
             if (std::abs(p.s1 - p.s0) != 1) throw std::invalid_argument("long-range noise processes are not supported");
-            if (!p.pauli_name)
-                throw std::invalid_argument("adjacent 2-site non-Pauli noise processes are not supported by the GPU path");
+            if (!p.pauli_name && job.digital)
+                throw std::invalid_argument("adjacent 2-site non-Pauli noise processes are not supported by the digital "
+                                            "GPU path");
         }
     }
     for (auto &o : job.obs) {

@@ -107,6 +107,10 @@ void local_op(cplx<T> *A, int64_t sA, int dl, int dr, const c64 *ops, const int 
               hipStream_t s);
 template <typename T>
 void local_op4(cplx<T> *A, int64_t sA, int dl, int dr, const c64 *M, int B, hipStream_t s);
+// local_op4 with the matrix of trajectory t at Mtab + 16 op_idx[t]
+// (op_idx[t] < 0: slot untouched).
+template <typename T>
+void local_op4_sel(cplx<T> *A, int64_t sA, int dl, int dr, const c64 *Mtab, const int *op_idx, int B, hipStream_t s);
 template <typename T>
 void norm2(const cplx<T> *A, int64_t sA, int64_t n, double *out, const int *mask, int B, hipStream_t s);
 template <typename T>
@@ -126,7 +130,12 @@ void jump_decide(const double *norm2v, uint64_t seed, uint64_t traj0, uint64_t *
                  double *fac, int B, hipStream_t s);
 void jump_choose(const c64 *rhov, int64_t sR, int L, int ncand, const int *cand_site, const c64 *cand_LdL,
                  const double *cand_gamma, double dt, uint64_t seed, uint64_t traj0, uint64_t *rng_k, int *state,
-                 int *chosen, double *wbuf, int B, hipStream_t s);
+                 int *chosen, double *wbuf, int B, hipStream_t s, const c64 *rho2 = nullptr, int64_t sR2 = 0,
+                 const c64 *cand_LdL4 = nullptr, const int *cand_kind = nullptr);
+// Masks of trajectories whose chosen jump is a merged-pair candidate on pair
+// (p, p+1) (mode 0, opidx = its operator's index or -1) or beyond p (mode 1).
+void jump_pair_mask(const int *state, const int *chosen, const int *cand_site, const int *cand_pair, int p, int mode,
+                    int *mask, int *opidx, int B, hipStream_t s);
 void jump_opidx(const int *state, const int *chosen, const int *cand_site, const int *cand_op, const int *cand_site2,
                 const int *cand_op2, int site, int *opidx, int B, hipStream_t s);
 void mask_from_state(const int *state, int v, int *mask, int *count, int B, hipStream_t s);
@@ -140,7 +149,7 @@ void fill_int(int *p, int v, int n, hipStream_t s);
 // comes from a two-site tensor with a NaN/Inf entry; its trajectory is
 // flagged (fail[t] = 1) and the matrix replaced by e0 e0^T, so the eigensolver only ever
 // sees finite input.
-void guard_gram(c64 *G, int64_t sG, int n, int *fail, int B, hipStream_t s);
+void guard_gram(c64 *G, int64_t sG, int n, int *fail, const int *mask, int B, hipStream_t s);
 // Test hook: stores NaN into element 0 of slot t of A (t < B).
 template <typename T>
 void poison_slot(cplx<T> *A, int64_t sA, int t, hipStream_t s);

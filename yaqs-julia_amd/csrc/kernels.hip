@@ -996,8 +996,12 @@ template void local_op<double>(c64 *, int64_t, int, int, const c64 *, const int 
 // Θ[a][p][b] <- Σ_p' M[p][p'] Θ[a][p'][b] for a 4x4 M on the combined
 // physical index p = 2 s1 + s2.
 template <typename T>
-__global__ void local_op4_kernel(cplx<T> *A, int64_t sA, int dl, int dr, const c64 *M) {
+__global__ void local_op4_kernel(cplx<T> *A, int64_t sA, int dl, int dr, const c64 *M, const int *op_idx) {
     const int t = blockIdx.y;
+    if (op_idx) {  // per-trajectory matrix from a table of 16-entry matrices, < 0: untouched
+        if (op_idx[t] < 0) return;
+        M += (int64_t)op_idx[t] * 16;
+    }
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (int64_t)dl * dr) return;
     const int a = (int)(e / dr), b = (int)(e % dr);
@@ -1012,11 +1016,21 @@ __global__ void local_op4_kernel(cplx<T> *A, int64_t sA, int dl, int dr, const c
 template <typename T>
 void local_op4(cplx<T> *A, int64_t sA, int dl, int dr, const c64 *M, int B, hipStream_t s) {
     const int64_t tot = (int64_t)dl * dr;
-    hipLaunchKernelGGL(local_op4_kernel<T>, dim3((unsigned)((tot + 255) / 256), B), dim3(256), 0, s, A, sA, dl, dr, M);
+    hipLaunchKernelGGL(local_op4_kernel<T>, dim3((unsigned)((tot + 255) / 256), B), dim3(256), 0, s, A, sA, dl, dr, M,
+                       nullptr);
     YQ_HIP(hipGetLastError());
 }
 template void local_op4<float>(c32 *, int64_t, int, int, const c64 *, int, hipStream_t);
 template void local_op4<double>(c64 *, int64_t, int, int, const c64 *, int, hipStream_t);
+template <typename T>
+void local_op4_sel(cplx<T> *A, int64_t sA, int dl, int dr, const c64 *Mtab, const int *op_idx, int B, hipStream_t s) {
+    const int64_t tot = (int64_t)dl * dr;
+    hipLaunchKernelGGL(local_op4_kernel<T>, dim3((unsigned)((tot + 255) / 256), B), dim3(256), 0, s, A, sA, dl, dr,
+                       Mtab, op_idx);
+    YQ_HIP(hipGetLastError());
+}
+template void local_op4_sel<float>(c32 *, int64_t, int, int, const c64 *, const int *, int, hipStream_t);
+template void local_op4_sel<double>(c64 *, int64_t, int, int, const c64 *, const int *, int, hipStream_t);
 
 // ‖A_t‖² in FP64 (one workgroup per trajectory).
 template <typename T>
@@ -1211,19 +1225,23 @@ __global__ void jump_decide_kernel(const double *norm2, uint64_t seed, uint64_t 
 // candidates in the reference's order; weight = dt γ Tr(L†L ρ_site).
 __global__ void jump_choose_kernel(const c64 *rho, int64_t sR, int L, int ncand, const int *cand_site,
                                    const c64 *cand_LdL, const double *cand_gamma, double dt, uint64_t seed, uint64_t traj0,
-                                   uint64_t *rng_k, int *state, int *chosen, double *wbuf, int B) {
+                                   uint64_t *rng_k, int *state, int *chosen, double *wbuf, int B, const c64 *rho2,
+                                   int64_t sR2, const c64 *cand_LdL4, const int *cand_kind) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= B || state[t] != 1) return;
     double tot = 0;
     double *w = wbuf + (int64_t)t * ncand;
     for (int c = 0; c < ncand; ++c) {
-        const c64 *r = rho + t * sR + (int64_t)cand_site[c] * 4;
-        const c64 *M = cand_LdL + (int64_t)c * 4;
-        // Tr(M ρ) = Σ_{s,s'} M[s'][s] ρ[s][s']
+        // Tr(M ρ) = Σ_{s,s'} M[s'][s] ρ[s][s'], M = L†L: 2x2 on the site's ρ, or
+        // 4x4 on the pair's ρ2 (merged-pair weight, src/StochasticProcess.jl:148-160)
+        const bool pair = cand_kind && cand_kind[c] == 1;
+        const int d = pair ? 4 : 2;
+        const c64 *r = pair ? rho2 + t * sR2 + (int64_t)cand_site[c] * 16 : rho + t * sR + (int64_t)cand_site[c] * 4;
+        const c64 *M = pair ? cand_LdL4 + (int64_t)c * 16 : cand_LdL + (int64_t)c * 4;
         double v = 0;
-        for (int s = 0; s < 2; ++s)
-            for (int s2 = 0; s2 < 2; ++s2) {
-                const c64 m = M[s2 * 2 + s], p = r[s * 2 + s2];
+        for (int s = 0; s < d; ++s)
+            for (int s2 = 0; s2 < d; ++s2) {
+                const c64 m = M[s2 * d + s], p = r[s * d + s2];
                 v += m.re * p.re - m.im * p.im;
             }
         w[c] = dt * cand_gamma[c] * v;
@@ -1253,9 +1271,10 @@ void jump_decide(const double *norm2v, uint64_t seed, uint64_t traj0, uint64_t *
 }
 void jump_choose(const c64 *rhov, int64_t sR, int L, int ncand, const int *cand_site, const c64 *cand_LdL,
                  const double *cand_gamma, double dt, uint64_t seed, uint64_t traj0, uint64_t *rng_k, int *state,
-                 int *chosen, double *wbuf, int B, hipStream_t s) {
+                 int *chosen, double *wbuf, int B, hipStream_t s, const c64 *rho2, int64_t sR2, const c64 *cand_LdL4,
+                 const int *cand_kind) {
     hipLaunchKernelGGL(jump_choose_kernel, dim3((B + 255) / 256), dim3(256), 0, s, rhov, sR, L, ncand, cand_site, cand_LdL,
-                       cand_gamma, dt, seed, traj0, rng_k, state, chosen, wbuf, B);
+                       cand_gamma, dt, seed, traj0, rng_k, state, chosen, wbuf, B, rho2, sR2, cand_LdL4, cand_kind);
     YQ_HIP(hipGetLastError());
 }
 
@@ -1273,6 +1292,33 @@ __global__ void jump_opidx_kernel(const int *state, const int *chosen, const int
     }
     opidx[t] = o;
 }
+// Trajectories whose chosen jump is a merged-pair candidate (cand_pair[c] >= 0:
+// its 4x4 operator's index) on pair (p, p+1) (mode 0; opidx = that index,
+// else -1) or on a pair beyond p (mode 1; opidx untouched).
+__global__ void jump_pair_mask_kernel(const int *state, const int *chosen, const int *cand_site, const int *cand_pair,
+                                      int p, int mode, int *mask, int *opidx, int B) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B) return;
+    int m = 0, o = -1;
+    if (state[t] == 1 && chosen[t] >= 0 && cand_pair[chosen[t]] >= 0) {
+        const int site = cand_site[chosen[t]];
+        if (mode == 0 && site == p) {
+            m = 1;
+            o = cand_pair[chosen[t]];
+        } else if (mode == 1 && site > p) {
+            m = 1;
+        }
+    }
+    mask[t] = m;
+    if (mode == 0) opidx[t] = o;
+}
+void jump_pair_mask(const int *state, const int *chosen, const int *cand_site, const int *cand_pair, int p, int mode,
+                    int *mask, int *opidx, int B, hipStream_t s) {
+    hipLaunchKernelGGL(jump_pair_mask_kernel, dim3((B + 255) / 256), dim3(256), 0, s, state, chosen, cand_site,
+                       cand_pair, p, mode, mask, opidx, B);
+    YQ_HIP(hipGetLastError());
+}
+
 void jump_opidx(const int *state, const int *chosen, const int *cand_site, const int *cand_op, const int *cand_site2,
                 const int *cand_op2, int site, int *opidx, int B, hipStream_t s) {
     hipLaunchKernelGGL(jump_opidx_kernel, dim3((B + 255) / 256), dim3(256), 0, s, state, chosen, cand_site, cand_op,
@@ -1345,9 +1391,10 @@ void fill_int(int *p, int v, int n, hipStream_t s) {
 // trajectory and replaces its matrix by e0 e0^T (finite, rank one, so the
 // split keeps at most two columns and the wave's working dimension does not
 // grow for it).
-__global__ void guard_gram_kernel(c64 *G, int64_t sG, int n, int *fail) {
+__global__ void guard_gram_kernel(c64 *G, int64_t sG, int n, int *fail, const int *mask) {
     __shared__ int bad;
     const int t = blockIdx.x;
+    if (mask && !mask[t]) return;
     c64 *g = G + (int64_t)t * sG;
     if (threadIdx.x == 0) bad = 0;
     __syncthreads();
@@ -1362,9 +1409,9 @@ __global__ void guard_gram_kernel(c64 *G, int64_t sG, int n, int *fail) {
     for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) g[e] = cmake(e == 0 ? 1.0 : 0.0, 0.0);
 }
 
-void guard_gram(c64 *G, int64_t sG, int n, int *fail, int B, hipStream_t s) {
+void guard_gram(c64 *G, int64_t sG, int n, int *fail, const int *mask, int B, hipStream_t s) {
     if (B <= 0 || n <= 0) return;
-    hipLaunchKernelGGL(guard_gram_kernel, dim3(B), dim3(256), 0, s, G, sG, n, fail);
+    hipLaunchKernelGGL(guard_gram_kernel, dim3(B), dim3(256), 0, s, G, sG, n, fail, mask);
     YQ_HIP(hipGetLastError());
 }
 

@@ -136,8 +136,9 @@ class NoiseModel:
     All of these average to the same Lindblad dissipator.  Long-range
     (non-adjacent, MPO) processes are not part of the batched TJM path and
     raise NotImplementedError; adjacent pairs with a non-Pauli operator
-    (the expansions of crosstalk processes) are built here but rejected by
-    the GPU engine when a job is created."""
+    (raising_two, lowering_two, explicit 4x4 matrices, the expansions of
+    crosstalk processes) run on the analog GPU path as merged-pair
+    dissipation and jumps, and are rejected by the digital path."""
 
     def __init__(self, processes: Sequence[Any] = (), num_qubits: Optional[int] = None, *,
                  theta0: float = np.pi / 2, dt: Optional[float] = None, sigma: float = 1.0,
