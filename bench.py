@@ -196,54 +196,78 @@ def c3_main(Y, args):
     print(json.dumps(out), flush=True)
 
 
-def serve_stream(Y, njobs, seed=0, mean_gap_s=0.25):
-    """Serving stream shaped like the mixed-request workload, scaled to finish
-    in about a minute: Poisson arrivals of analog jobs drawn from the seed
-    (Ising g in [0.2, 1.5], XY or Heisenberg; L in 10..32; chi_max in
-    {16, 32, 64}; relaxation + dephasing gamma log-uniform in [1e-3, 1e-1];
-    64..256 trajectories; T=0.5, dt=0.05; mixed precision).  Latency is
-    submit -> done as measured by the library."""
+def c4_jobs(Y, njobs, seed=0, T=2.0, ntraj_range=(64, 1024), force=None):
+    """Jobs of the mixed-request workload, drawn from `seed`: model in {Ising
+    g ~ U[0.2, 1.5] (D=3), XY = init_general_hamiltonian(L,1,1,0,0,0,0),
+    Heisenberg (D=5)}; L ~ U{10..64}; chi_max in {16, 32, 64, 128, 256};
+    relaxation + dephasing gamma log-uniform in [1e-3, 1e-1]; trajectories
+    ~ U{ntraj_range}; T (2 in the workload), dt=0.05; Neel start for
+    Heisenberg, |0..0> otherwise; observables Z_i; mixed precision; seeds
+    drawn from the stream.  `force` overrides fields of job 0 (e.g. the
+    largest L and chi).  Returns (holder, cfg, L, ntraj, desc) per job."""
     rng = np.random.default_rng(seed)
-    ctx = Y.Context(0)
-    specs = []
-    for _ in range(njobs):
-        L = int(rng.integers(10, 33))
+    out = []
+    for k in range(njobs):
+        L = int(rng.integers(10, 65))
         model = int(rng.integers(0, 3))
+        g = float(rng.uniform(0.2, 1.5))
+        chi = int(rng.choice([16, 32, 64, 128, 256]))
+        gamma = float(10 ** rng.uniform(-3, -1))
+        ntraj = int(rng.integers(ntraj_range[0], ntraj_range[1] + 1))
+        jseed = int(rng.integers(0, 2 ** 31))
+        if k == 0 and force:
+            L = force.get("L", L)
+            chi = force.get("chi", chi)
+            model = force.get("model", model)
         if model == 0:
-            H = Y.init_ising(L, 1.0, float(rng.uniform(0.2, 1.5)))
+            H = Y.init_ising(L, 1.0, g)
         elif model == 1:
             H = Y.init_general_hamiltonian(L, 1, 1, 0, 0, 0, 0)
         else:
             H = Y.init_general_hamiltonian(L, 1, 1, 1, 0, 0, 0)
-        chi = int(rng.choice([16, 32, 64]))
-        gamma = float(10 ** rng.uniform(-3, -1))
-        ntraj = int(rng.integers(64, 257))
         obs = [Y.Observable(f"z{i}", Y.gates.ZGate(), i) for i in range(1, L + 1)]
-        cfg = Y.TimeEvolutionConfig(obs, 0.5, dt=0.05, num_traj=ntraj, max_bond_dim=chi, truncation_threshold=1e-9)
+        cfg = Y.TimeEvolutionConfig(obs, T, dt=0.05, num_traj=ntraj, max_bond_dim=chi, truncation_threshold=1e-9)
         mps = Y.MPS(L, state="Neel" if model == 2 else "zeros")
         mps.normalize()
         h = Y._abi.pack_analog(mps, H, cfg, Y.relaxation_dephasing(L, gamma), precision=Y._abi.PREC_MIXED,
-                               seed=int(rng.integers(0, 2 ** 31)))
-        specs.append((h, cfg, L, ntraj))
+                               seed=jseed)
+        desc = f"{['Ising', 'XY', 'Heisenberg'][model]} L={L} chi={chi} gamma={gamma:.1e} ntraj={ntraj}"
+        out.append((h, cfg, L, ntraj, desc))
+    return out
+
+
+def serve_stream(Y, njobs, seed=0, mean_gap_s=0.25, T=0.5, ntraj_range=(64, 256)):
+    """Poisson arrivals (mean gap `mean_gap_s`, short enough to keep the
+    device saturated) of `njobs` jobs of the mixed-request workload
+    (c4_jobs: models, L 10-64, chi 16-256 and rates as in the workload; by
+    default T=0.5 and 64-256 trajectories instead of T=2 and 64-1024, which
+    keeps the stream to about a minute: one Heisenberg L=40 chi=128 job of
+    the full workload alone takes about a minute); latency is submit -> done
+    as measured by the library."""
+    rng = np.random.default_rng(seed + 1)
+    ctx = Y.Context(0)
+    specs = c4_jobs(Y, njobs, seed=seed, T=T, ntraj_range=ntraj_range)
     gaps = rng.exponential(mean_gap_s, njobs)
     t0 = time.perf_counter()
     ids = []
-    for (h, cfg, L, ntraj), gap in zip(specs, gaps):
+    for (h, cfg, L, ntraj, _), gap in zip(specs, gaps):
         time.sleep(gap)
         ids.append(ctx.submit_analog(h))
     lat, work = [], 0
-    for (h, cfg, L, ntraj), jid in zip(specs, ids):
+    for (h, cfg, L, ntraj, desc), jid in zip(specs, ids):
         _, _, _, st = ctx.wait(jid, len(cfg.observables), len(cfg.times), ntraj)
         lat.append(st["wall_s"])
         work += ntraj * L * (len(cfg.times) - 1)
+        print(f"[bench] serving: {desc} done, latency {st['wall_s']:.2f}s", flush=True)
     span = time.perf_counter() - t0
     ctx.close()
     lat = np.array(lat)
     return {"jobs": njobs, "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "max_s": float(lat.max()), "throughput": work / span, "unit": "traj-site-steps/s",
             "makespan_s": span, "mean_interarrival_s": mean_gap_s,
-            "stream": "Poisson; Ising/XY/Heisenberg, L 10-32, chi 16/32/64, gamma 1e-3..1e-1, 64-256 traj, "
-                      "T=0.5 dt=0.05, mixed"}
+            "stream": f"Poisson; Ising/XY/Heisenberg, L 10-64, chi 16-256, gamma 1e-3..1e-1, "
+                      f"{ntraj_range[0]}-{ntraj_range[1]} traj, T={T} dt=0.05, mixed",
+            "job_list": [d for (_, _, _, _, d) in specs]}
 
 
 def trace_summary(path, last, kernel="gemm_mfma_kernel<float"):
