@@ -270,36 +270,85 @@ def serve_stream(Y, njobs, seed=0, mean_gap_s=0.25, T=0.5, ntraj_range=(64, 256)
             "job_list": [d for (_, _, _, _, d) in specs]}
 
 
+def _window_rows(rows, last, kernel):
+    """rows: (order key, name, ...) sorted by the key; the window starts at
+    the first of the last `last` dispatches whose name contains `kernel`."""
+    idx = [i for i, r in enumerate(rows) if kernel in r[1]]
+    if not idx:
+        return [], []
+    sel = idx[-last:] if last > 0 else idx
+    return rows[sel[0]:], [rows[i] for i in sel]
+
+
 def trace_summary(path, last, kernel="gemm_mfma_kernel<float"):
-    """Average duration of the last `last` dispatches of `kernel` in a
-    rocprofv3 --kernel-trace CSV (the timed window is the end of the run)."""
+    """rocprofv3 --kernel-trace CSV: average duration of the last `last`
+    dispatches of `kernel` (the timed window is the end of the run), every
+    kernel's calls and time from the first of them on, and the device's idle
+    time there (gaps over 2 us between the union of dispatch intervals)."""
     import csv
-    rows, allk = [], []
+    rows = []
     with open(path) as f:
         for r in csv.DictReader(f):
-            s, e = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
-            name = r["Kernel_Name"].split("(")[0]
-            allk.append((s, e, name))
-            if kernel in r["Kernel_Name"]:
-                rows.append((s, e))
+            rows.append((int(r["Start_Timestamp"]), r["Kernel_Name"].split("(")[0], int(r["End_Timestamp"])))
     rows.sort()
-    sel = rows[-last:] if last > 0 else rows
-    dur = [e - s for s, e in sel]
-    # every kernel dispatched from the first GEMM of the window on
-    t0 = sel[0][0] if sel else 0
+    win, sel = _window_rows(rows, last, kernel)
+    dur = [e - s for s, _, e in sel]
     per = {}
-    for s, e, name in allk:
-        if s >= t0:
-            c = per.setdefault(name, [0, 0])
-            c[0] += 1
-            c[1] += e - s
-    span = (max(e for s, e, _ in allk) - t0) if allk else 0
-    top = sorted(per.items(), key=lambda kv: -kv[1][1])[:20]
-    return {"kernel": kernel, "dispatches": len(rows), "window_dispatches": len(sel),
+    for s, name, e in win:
+        c = per.setdefault(name, [0, 0])
+        c[0] += 1
+        c[1] += e - s
+    idle, end = 0, win[0][0] if win else 0
+    for s, _, e in win:
+        if s > end + 2000:
+            idle += s - end
+        end = max(end, e)
+    span = (end - win[0][0]) if win else 0
+    top = sorted(per.items(), key=lambda kv: -kv[1][1])[:24]
+    return {"kernel": kernel, "window_dispatches": len(sel),
             "avg_us_window": sum(dur) / max(1, len(dur)) / 1e3,
-            "avg_us_all": sum(e - s for s, e in rows) / max(1, len(rows)) / 1e3,
-            "window_span_s": span / 1e9,
-            "window_kernels": [{"name": k, "calls": v[0], "total_s": v[1] / 1e9} for k, v in top]}
+            "window_span_s": span / 1e9, "idle_s": idle / 1e9,
+            "window_kernels": [{"name": k, "calls": v[0], "total_s": v[1] / 1e9, "avg_us": v[1] / v[0] / 1e3,
+                                "share": v[1] / max(1, span)} for k, v in top]}
+
+
+def pmc_summary(path, last, kernel="gemm_mfma_kernel<float", trace=None):
+    """rocprofv3 --pmc CSV: per kernel name, the average per dispatch of
+    every counter over the window (the dispatches from the first of the last
+    `last` dispatches of `kernel` on); with the kernel trace of the same
+    command, the average duration per dispatch and counter-bytes / duration
+    (FETCH_SIZE and WRITE_SIZE are in KiB per dispatch)."""
+    import csv
+    disp = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            d = disp.setdefault(int(r["Dispatch_Id"]), {"name": r["Kernel_Name"].split("(")[0], "c": {}})
+            d["c"][r["Counter_Name"]] = d["c"].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    rows = sorted((k, v["name"], v["c"]) for k, v in disp.items())
+    win, _ = _window_rows(rows, last, kernel)
+    per = {}
+    for _, name, c in win:
+        a = per.setdefault(name, {"calls": 0})
+        a["calls"] += 1
+        for k, v in c.items():
+            a[k] = a.get(k, 0.0) + v
+    durs = {}
+    if trace:
+        t = trace_summary(trace, last, kernel)
+        durs = {w["name"]: w["avg_us"] for w in t["window_kernels"]}
+    out = []
+    for name, a in sorted(per.items(), key=lambda kv: -kv[1]["calls"]):
+        e = {"name": name, "calls": a["calls"]}
+        for k, v in a.items():
+            if k != "calls":
+                e[k] = v / a["calls"]
+        if name in durs:
+            e["avg_us"] = durs[name]
+            kib = e.get("FETCH_SIZE", 0.0) + e.get("WRITE_SIZE", 0.0)
+            if kib:
+                e["raw_GBps"] = kib * 1024 / (durs[name] * 1e-6) / 1e9
+        out.append(e)
+    return {"kernel": kernel, "window_dispatches_from": last, "per_kernel": out[:30]}
 
 
 def main():
@@ -325,9 +374,23 @@ def main():
     ap.add_argument("--cpu-layers", type=int, default=7, help="c3: layers of the bounded oracle sample")
     ap.add_argument("--trace-summary", metavar="CSV", help="summarise a rocprofv3 kernel trace and exit")
     ap.add_argument("--trace-last", type=int, default=0, help="dispatches of the timed window (from the bench line)")
+    ap.add_argument("--pmc-summary", metavar="CSV", help="summarise a rocprofv3 --pmc counter CSV and exit")
+    ap.add_argument("--trace", metavar="CSV", help="with --pmc-summary: the kernel trace of the same command")
+    ap.add_argument("--kernel", default="gemm_mfma_kernel<float", help="kernel that marks the window")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="launch a plain copy kernel of BYTES (4 times) and exit (counter calibration)")
     args = ap.parse_args()
     if args.trace_summary:
-        print(json.dumps(trace_summary(args.trace_summary, args.trace_last)))
+        print(json.dumps(trace_summary(args.trace_summary, args.trace_last, args.kernel)))
+        return
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(args.pmc_summary, args.trace_last, args.kernel, args.trace)))
+        return
+    if args.calibrate_copy:
+        lib = _pkg()._abi.load_library()
+        lib.yaqs_debug_copy_bytes.argtypes = [ct.c_int64, ct.c_int]
+        rc = lib.yaqs_debug_copy_bytes(args.calibrate_copy, 4)
+        print(json.dumps({"copy_bytes": args.calibrate_copy, "launches": 4, "rc": rc}))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); trajectories shard across GPUs "

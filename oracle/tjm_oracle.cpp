@@ -99,6 +99,10 @@ typedef void (*zgesdd64_fn)(const char *, const int64_t *, const int64_t *, cd *
                             const int64_t *, cd *, const int64_t *, cd *, const int64_t *, double *, int64_t *,
                             int64_t *, long);
 zgesdd64_fn g_zgesdd = nullptr;
+// Dead directions of QR/LQ dropped (qr_thin); off only to measure how much
+// the rounding-decided completion moves results (oracle_set_drop_dead).
+bool g_drop_dead = true;
+thread_local bool tl_keep_all = false;  // set during measurements (see expect)
 
 void gemm(char ta, char tb, int m, int n, int k, const cd *A, int lda, const cd *B, int ldb,
           cd *C, int ldc, bool accumulate = false) {
@@ -176,10 +180,21 @@ cd dotc(const cd *a, const cd *b, size_t n) {  // Σ conj(a)·b (Julia `dot`)
     return cd(re, im);
 }
 
-// Householder QR of a column-major m x n matrix: A = Q R, Q (m x r), R (r x n),
-// r = min(m, n).  Thin factors as Julia's `Matrix(qr(A).Q)`, `qr(A).R`
-// (src/MPS.jl:200-205).
-void qr_thin(const cvec &A, int m, int n, cvec &Q, cvec &R, int &r) {
+// Thin QR of a column-major m x n matrix: A = Q R, Q (m x r), R (r x n),
+// r = min(m, n), as Julia's `Matrix(qr(A).Q)`, `qr(A).R` (src/MPS.jl:
+// 200-205), up to the phase of each column of Q (the freedom of a QR; the
+// state Q R is the same).  Columns are orthonormalised in order by
+// classical Gram-Schmidt applied twice.  Dead directions: a column whose
+// residual is at most 1e-12 ||A||_F depends on the earlier ones; LAPACK
+// builds a reflector from its rounding noise there, so the column of Q it
+// returns is decided by that noise (and so is the 2-site TDVP tangent space
+// that follows).  This restatement drops such a direction instead: a zero
+// column of Q and a zero row of R (later columns are orthogonalised against
+// the live ones only), the same state with the bond at its numerical rank
+// -- the rule the GPU engine's gauge steps follow.  With g_drop_dead off it
+// runs Householder reflectors as LAPACK does (used to measure how far the
+// noise-decided completion moves results).
+void qr_householder(const cvec &A, int m, int n, cvec &Q, cvec &R, int &r) {
     r = std::min(m, n);
     cvec W = A;
     std::vector<cvec> vs(r);
@@ -224,6 +239,38 @@ void qr_thin(const cvec &A, int m, int n, cvec &Q, cvec &R, int &r) {
             for (int i = 0; i < len; ++i) s += std::conj(v[i]) * Q[(k + i) + (size_t)m * j];
             s *= taus[k];
             for (int i = 0; i < len; ++i) Q[(k + i) + (size_t)m * j] -= v[i] * s;
+        }
+    }
+}
+
+void qr_thin(const cvec &A, int m, int n, cvec &Q, cvec &R, int &r) {
+    if (!g_drop_dead || tl_keep_all) {
+        qr_householder(A, m, n, Q, R, r);
+        return;
+    }
+    r = std::min(m, n);
+    double fro = 0;
+    for (auto &x : A) fro += std::norm(x);
+    fro = std::sqrt(fro);
+    const double dead = 1e-12 * fro;
+    Q.assign((size_t)m * r, 0.0);
+    R.assign((size_t)r * n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        cvec v(A.begin() + (size_t)m * j, A.begin() + (size_t)m * (j + 1));
+        const int kmax = std::min(j, r);  // earlier columns of Q
+        for (int pass = 0; pass < 2; ++pass)
+            for (int k = 0; k < kmax; ++k) {
+                const cd *q = &Q[(size_t)m * k];
+                const cd h = dotc(q, v.data(), m);
+                R[k + (size_t)r * j] += h;
+                for (int i = 0; i < m; ++i) v[i] -= h * q[i];
+            }
+        if (j < r) {
+            const double nr = nrm2(v);
+            if (nr > dead && nr > 0.0) {
+                R[j + (size_t)r * j] = nr;
+                for (int i = 0; i < m; ++i) Q[i + (size_t)m * j] = v[i] / nr;
+            }
         }
     }
 }
@@ -1029,6 +1076,12 @@ struct Candidate {
 // create_probability_distribution (src/StochasticProcess.jl:89-212).
 void prob_distribution(MPSState &s, const std::vector<Proc> &noise, double dt, std::vector<double> &probs,
                        std::vector<Candidate> &cands) {
+    // its centre walk is a gauge move the GPU engine does not make (it reads
+    // the weights from transfer matrices): it keeps every direction
+    struct KeepAll {
+        KeepAll() { tl_keep_all = true; }
+        ~KeepAll() { tl_keep_all = false; }
+    } keep;
     probs.clear();
     cands.clear();
     if (noise.empty()) return;
@@ -1133,7 +1186,16 @@ struct Obs {
     cd op[16];
 };
 
+// The centre shifts of a measurement are gauge moves on the state itself
+// (src/MPS.jl:608-651; analog_tjm_1 measures the evolving state): they keep
+// every direction (Householder, as LAPACK), where the sweeps of the
+// algorithm drop dead ones; the GPU engine measures through transfer
+// matrices and leaves the state's gauge alone.
 double expect(MPSState &s, const Obs &o) {  // src/SimulationConfigs.jl:42-56
+    struct KeepAll {
+        KeepAll() { tl_keep_all = true; }
+        ~KeepAll() { tl_keep_all = false; }
+    } keep;
     if (o.nsites == 1) return local_expect(s, o.op, o.s0).real();
     int a = std::min(o.s0, o.s1);
     return local_expect2(s, o.op, a).real();
@@ -1610,6 +1672,8 @@ int oracle_use_lapack_svd(const char *path, const char *sym) {
 }
 
 int oracle_blas_bound(void) { return g_zgemm ? 1 : 0; }
+
+void oracle_set_drop_dead(int on) { g_drop_dead = on != 0; }
 double oracle_last_window_seconds(void) { return g_last_window; }
 
 int64_t oracle_digital_num_samples(const yaqs_digital_job *job) {

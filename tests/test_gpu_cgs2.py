@@ -2,7 +2,7 @@
 column blocks, LQ of (chi, 2 chi) row blocks) against its definition:
 A = Q Cf^T (column k of A is sum_j Cf[k][j] q_j), Cf lower triangular with a
 real non-negative diagonal, live vectors orthonormal, dead directions
-(remaining norm <= 1e-14 ||A||_F) exactly zero.  Shapes cross the 32-vector
+(remaining norm <= 1e-12 ||A||_F) exactly zero.  Shapes cross the 32-vector
 panels; FP64 to 1e-12, FP32 to 2e-5 (relative to ||A||)."""
 import ctypes as ct
 

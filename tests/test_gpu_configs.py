@@ -3,9 +3,8 @@
 - C1 at full size: Ising L=6 (J=1, g=0.5) from |000000>, lowering + pauli_z
   gamma=0.1 on every site, dt=0.1, T=5, chi_max=16, threshold 1e-10, 100
   trajectories seeds 0..99, FP64: every trajectory against the oracle
-  (median 1e-12; the few left nearly a product state by a lowering jump
-  within 1e-8, see the test), and the ensemble against the exact Lindblad
-  solution within sampling error.
+  (1e-10), and the ensemble against the exact Lindblad solution within
+  sampling error.
 - C3: the L=40 Haar SU(4) brickwork, 40 layers, depolarizing p=0.01 as
   pauli_x/y/z with p/3 on each gate's qubits, TEBD threshold 1e-9,
   chi_max=128, 512 trajectories, mixed: finite <Z_i> in [-1, 1], bonds
@@ -64,12 +63,10 @@ def test_c1_full_size(ctx, oracle_lib):
     e = float(d.max())
     print(f"  max|gpu-oracle| = {e:.3e}, median over trajectories {np.median(d):.3e}, "
           f"{int((d > 1e-10).sum())} above 1e-10; jumps {st['jumps']} / {int(rst['jumps'])}", flush=True)
-    # Trajectories a lowering jump leaves almost a product state keep a
+    # trajectories a lowering jump leaves nearly a product state keep a
     # second singular value far below 1e-8 (the T-A rule keeps at least two):
-    # its singular vectors come from the Gram eigenvectors in the engine and
-    # from the oracle's SVD, and agree only to ~1e-9 in the observables
-    # (with pauli_x/pauli_z noise every trajectory agrees to 1e-11).
-    assert st["jumps"] == rst["jumps"] and e < 1e-8 and np.median(d) < 1e-11 and (d > 1e-10).sum() <= 10
+    # the engine resolves its singular vectors with a deflated second Gram
+    assert st["jumps"] == rst["jumps"] and e < 1e-10
     # ensemble against the exact Lindblad evolution
     jumps = [(0.1, E.site_op(E.LOWER, s, L)) for s in range(1, L + 1)] + \
             [(0.1, E.site_op(E.Z, s, L)) for s in range(1, L + 1)]

@@ -45,6 +45,40 @@ def test_qr_lq(oracle_lib, m, n):
     assert np.abs(np.triu(Lm, 1)).max(initial=0) == 0
 
 
+def test_qr_lq_against_lapack(oracle_lib):
+    """Full rank: the thin factors equal numpy's LAPACK ones up to the phase
+    of each column of Q (rows of the LQ factor), the freedom of the QR."""
+    for m, n in ((12, 5), (5, 12), (9, 9)):
+        A = crand(m, n)
+        Q, R = oracle_lib.qr(A)
+        Qn, Rn = np.linalg.qr(A)
+        ph = np.diag(Qn.conj().T @ Q)
+        assert np.allclose(np.abs(ph), 1, atol=1e-12)
+        assert np.allclose(Q, Qn * ph, atol=1e-12) and np.allclose(R, ph.conj()[:, None] * Rn, atol=1e-12)
+
+
+def test_qr_lq_dead_directions(oracle_lib):
+    """Exactly dependent columns (rows): LAPACK completes Q from a reflector
+    of rounding noise (numpy's Q column differs under a 1e-15 change of A),
+    the oracle drops the direction: zero column of Q and zero row of R, the
+    live columns orthonormal, Q R = A.  The same rule as the GPU engine's
+    gauge steps (a residual below 1e-12 of ||A||_F is dead)."""
+    a = crand(10, 3)
+    A = np.c_[a[:, :2], a[:, 0] * (0.3 - 0.2j) + a[:, 1] * 0.7, a[:, 2:]]  # column 2 dependent
+    Q, R = oracle_lib.qr(A)
+    assert np.abs(Q[:, 2]).max() == 0 and np.abs(R[2]).max() == 0
+    live = [0, 1, 3]
+    assert np.allclose(Q[:, live].conj().T @ Q[:, live], np.eye(3), atol=1e-12)
+    assert np.allclose(Q @ R, A, atol=1e-12)
+    # LAPACK's completion of the dead column follows the rounding noise
+    Qn0, _ = np.linalg.qr(A)
+    Qn1, _ = np.linalg.qr(A + 1e-15 * crand(10, 4))
+    assert np.abs(np.abs(np.vdot(Qn0[:, 2], Qn1[:, 2])) - 1) > 1e-6
+    Lm, Q2 = oracle_lib.lq(A.conj().T)
+    assert np.abs(Q2[2]).max() == 0 and np.abs(Lm[:, 2]).max() == 0
+    assert np.allclose(Lm @ Q2, A.conj().T, atol=1e-12)
+
+
 def test_tridiagonal_exponential(oracle_lib):
     for k in (1, 2, 7, 25):
         a = rng.standard_normal(k)

@@ -21,6 +21,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <type_traits>
 #include <stdexcept>
 
 #include "kernels.h"
@@ -601,6 +602,11 @@ class Engine {
     int *d_pinned = nullptr;  // device address of h_pinned
     DevBuf fin_done;          // lz_finalize's last-workgroup ticket
     DevBuf nfail;             // [B] NaN/Inf guard flags of the wave
+    // FP64 splits: kept singular values below 1e-6.5 of the largest are
+    // resolved by a deflated second Gram (see split)
+    bool defl = false;
+    DevBuf Dq, Dcnt, Dflag, keepR, dflag_count;
+    int64_t slotDq = 0;
     int64_t nsplit = 0;       // splits since the wave started (test hook)
     hipEvent_t lz_ev[2] = {nullptr, nullptr};  // Lanczos active counts land in h_pinned[1 + (j & 1)]
     // The count the finalize kernel stores over the -1 left in
@@ -648,6 +654,7 @@ class Engine {
     void expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, JobResult &res);
     void merge(State &S, int i);
     void split(State &S, int i, bool right, int rule, double thr, const int *msk = nullptr);
+    void deflate_tail(State &S, int i, bool right, int kw, int mb);
     void env_left(State &S, int i);
     void env_right(State &S, int i);
     void tdvp(State &S, JobResult &res);
@@ -703,6 +710,7 @@ int64_t Engine<T>::footprint(const JobIn &job, int B) {
     bytes += heev_work_bytes((int)gd, (int)std::min<int64_t>(gd, job.max_bond)) + (int64_t)sizeof(int);  // eigensolver work
     bytes += (2 * cm * cm + std::max<int64_t>(nmax, 2 * cm * cm)) * (int64_t)sizeof(c64);  // transfer
     bytes += (int64_t)L * 20 * sizeof(c64) * 2 + (job.kdim + 2) * 64 * 16 + 256 + 8;
+    if (std::is_same<T, double>::value && !job.digital) bytes += gd * cm * (int64_t)sizeof(cplx<T>) + 16;  // Dq
     return bytes * B;
 }
 
@@ -841,6 +849,19 @@ void Engine<T>::setup() {
     fin_done.alloc(sizeof(int));
     fill_int(fin_done.as<int>(), 0, 1, s);
     nfail.alloc((size_t)B * sizeof(int));
+    static const bool defl_off = [] {
+        const char *e = std::getenv("YAQS_SPLIT_DEFLATE");
+        return e && e[0] == '0';
+    }();
+    defl = std::is_same<T, double>::value && !J.digital && !defl_off;
+    if (defl) {
+        slotDq = gdim * capmax;
+        Dq.alloc((size_t)B * slotDq * sizeof(CT));
+        Dcnt.alloc((size_t)B * sizeof(int));
+        Dflag.alloc((size_t)B * sizeof(int));
+        keepR.alloc((size_t)B * sizeof(int));
+        dflag_count.alloc(sizeof(int));
+    }
 
     // ---- noise: dissipation operators per site and jump candidates
     auto site_ops = [&](double dt, std::vector<c64> &ops, std::vector<int> *act) {
@@ -1368,12 +1389,28 @@ void Engine<T>::split(State &S, int i, bool right, int rule, double thr, const i
     // the inverse-iteration breakdown flags come back in one readback, and the
     // single-workgroup solver runs only when a matrix needs it
     std::vector<int> hk(nb), hf(multi ? nb : 0);
+    int nflag = 0;
+    // Kept eigenvalues below kTailCut of the largest (singular values below
+    // 3e-7 of the largest, e.g. the second one the T-A rule keeps in a nearly
+    // product state) lie at the rounding level of the Gram matrix, so its
+    // eigenvectors there are not the singular vectors; those trajectories get
+    // a second, deflated Gram below (FP64 only: complex-FP32 tensors do not
+    // resolve such singular values either way)
+    constexpr double kTailCut = 1e-13;
+    auto flag_tails = [&]() {
+        fill_int(dflag_count.as<int>(), 0, 1, s);
+        tail_flags(evals.as<double>(), gdim + 2, keep.as<int>(), kTailCut, Dcnt.as<int>(), Dflag.as<int>(),
+                   dflag_count.as<int>(), nb, s);
+        YQ_HIP(hipMemcpyAsync(&nflag, dflag_count.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    };
+    if (defl) flag_tails();
     YQ_HIP(hipMemcpyAsync(hk.data(), keep.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
     if (multi) YQ_HIP(hipMemcpyAsync(hf.data(), fb, nb * sizeof(int), hipMemcpyDeviceToHost, s));
     YQ_HIP(hipStreamSynchronize(s));
     if (multi && std::any_of(hf.begin(), hf.end(), [](int f) { return f != 0; })) {
         heev_fixup(G.as<c64>(), slotG, U.as<c64>(), slotG, Zbuf.as<double>(), slotZ, Pg, slotP, fb, gn, ns,
                    evals.as<double>(), perm.as<int>(), gdim + 2, keep.as<int>(), rule, thr, mb, msk, nb, s);
+        if (defl) flag_tails();
         YQ_HIP(hipMemcpyAsync(hk.data(), keep.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
         YQ_HIP(hipStreamSynchronize(s));
     }
@@ -1389,16 +1426,57 @@ void Engine<T>::split(State &S, int i, bool right, int rule, double thr, const i
     if (right) {
         write_iso<T>(U.as<c64>(), slotG, perm.as<int>(), gdim + 2, keep.as<int>(), m, kw, 0, A(S, i), slotA[i], msk,
                      nb, s);
+        if (nflag > 0) deflate_tail(S, i, true, kw, mb);
         gemmT<T>(gd(kw, n, m, OP_C, OP_N, kw, n, n, slotA[i], nmax, slotA[i + 1], nb, msk), A(S, i), theta.as<CT>(),
                  A(S, i + 1), s);
     } else {
         write_iso<T>(U.as<c64>(), slotG, perm.as<int>(), gdim + 2, keep.as<int>(), n, kw, 1, A(S, i + 1), slotA[i + 1],
                      msk, nb, s);
+        if (nflag > 0) deflate_tail(S, i, false, kw, mb);
         gemm<T, T, T, T>(gd(m, kw, n, OP_N, OP_C, n, n, kw, nmax, slotA[i + 1], slotA[i], nb, msk), theta.as<CT>(),
                          A(S, i + 1), A(S, i), s);
     }
     S.wd[i + 1] = kw;
     set_dims(S.dims.as<int>(), L + 1, i + 1, keep.as<int>(), 0, msk, nb, s);
+}
+
+// The kept tail of a split whose singular values the Gram route cannot
+// resolve (flagged trajectories only, FP64): with Q_d the resolved kept
+// columns (right = true: left vectors U_d; else right vectors V_d), the
+// residual R = Θ - U_d U_d^† Θ (or Θ - Θ V_d V_d^†) holds the rest of Θ at
+// its own scale, so the leading eigenvectors of R R^† (R^† R) are the tail's
+// singular vectors to working accuracy; they replace the tail columns
+// (rows) of the isometry just written.
+template <typename T>
+void Engine<T>::deflate_tail(State &S, int i, bool right, int kw, int mb) {
+    const int cl = S.wd[i], cr = S.wd[i + 2];
+    const int m = 2 * cl, n = 2 * cr, ns = std::min(m, n), gn = right ? m : n;
+    const int *fl = Dflag.as<int>();
+    CT *th = theta.as<CT>(), *q = Dq.as<CT>(), *t1 = T1.as<CT>(), *t2 = T2.as<CT>(), *r = V.as<CT>();
+    write_iso<T>(U.as<c64>(), slotG, perm.as<int>(), gdim + 2, Dcnt.as<int>(), gn, kw, right ? 0 : 1, q, slotDq, fl,
+                 nb, s);
+    if (right) {
+        gemm<T, T, T, T>(gd(kw, n, m, OP_C, OP_N, kw, n, n, slotDq, nmax, slotS, nb, fl), q, th, t1, s);  // U_d^† Θ
+        gemm<T, T, T, T>(gd(m, n, kw, OP_N, OP_N, kw, n, n, slotDq, slotS, slotS, nb, fl), q, t1, t2, s);
+    } else {
+        gemm<T, T, T, T>(gd(m, kw, n, OP_N, OP_C, n, n, kw, nmax, slotDq, slotS, nb, fl), th, q, t1, s);  // Θ V_d
+        gemm<T, T, T, T>(gd(m, n, kw, OP_N, OP_N, kw, n, n, slotS, slotDq, slotS, nb, fl), t1, q, t2, s);
+    }
+    sub<T>(th, nmax, t2, slotS, r, sV(), (int64_t)m * n, fl, nb, s);
+    if (right)
+        gemm<T, T, double, double>(gd(m, m, n, OP_N, OP_C, n, n, m, sV(), sV(), slotG, nb, fl), r, r, G.as<c64>(), s);
+    else
+        gemm<T, T, double, double>(gd(n, n, m, OP_C, OP_N, n, n, n, sV(), sV(), slotG, nb, fl), r, r, G.as<c64>(), s);
+    double2 *Pg = Pbuf.p ? Pbuf.as<double2>() : nullptr;
+    (void)heev(G.as<c64>(), slotG, U.as<c64>(), slotG, Zbuf.as<double>(), slotZ, Pg, slotP, Hwork.p, slotH,
+               Hfb.p ? Hfb.as<int>() : nullptr, gn, ns, evals.as<double>(), perm.as<int>(), gdim + 2, keepR.as<int>(),
+               0, 0.0, mb, fl, nb, s, true);
+    if (right)
+        tail_fix<T>(A(S, i), slotA[i], m, kw, 0, Dcnt.as<int>(), keep.as<int>(), U.as<c64>(), slotG, perm.as<int>(),
+                    gdim + 2, fl, nb, s);
+    else
+        tail_fix<T>(A(S, i + 1), slotA[i + 1], n, kw, 1, Dcnt.as<int>(), keep.as<int>(), U.as<c64>(), slotG,
+                    perm.as<int>(), gdim + 2, fl, nb, s);
 }
 
 // E_left[i+1] from E_left[i] and A_i (update_left_environment, src/Algorithms.jl:199-204)

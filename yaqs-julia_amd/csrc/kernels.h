@@ -150,6 +150,21 @@ void fill_int(int *p, int v, int n, hipStream_t s);
 // flagged (fail[t] = 1) and the matrix replaced by e0 e0^T, so the eigensolver only ever
 // sees finite input.
 void guard_gram(c64 *G, int64_t sG, int n, int *fail, const int *mask, int B, hipStream_t s);
+// Split of a Gram matrix with kept eigenvalues below tau * lambda_max: per
+// trajectory the count above the cut (dcount), a flag when kept ones fall
+// below it (at least one above), and the number of flagged trajectories
+// added to *count.
+void tail_flags(const double *evals, int64_t sE, const int *keep, double tau, int *dcount, int *flag, int *count, int B,
+                hipStream_t s);
+// D = A - C on n elements per slot (masked).
+template <typename T>
+void sub(const cplx<T> *A, int64_t sA, const cplx<T> *C, int64_t sC, cplx<T> *D, int64_t sD, int64_t n, const int *mask,
+         int B, hipStream_t s);
+// Columns (mode 0) / conjugated rows (mode 1) k in [dcount, keep) of an
+// isometry written by write_iso, from the leading eigenvectors of U.
+template <typename T>
+void tail_fix(cplx<T> *Out, int64_t sO, int n, int kw, int mode, const int *dcount, const int *keep, const c64 *U,
+              int64_t sU, const int *perm, int64_t sE, const int *mask, int B, hipStream_t s);
 // Test hook: stores NaN into element 0 of slot t of A (t < B).
 template <typename T>
 void poison_slot(cplx<T> *A, int64_t sA, int t, hipStream_t s);

@@ -723,7 +723,7 @@ template void write_iso<double>(const c64 *, int64_t, const int *, int64_t, cons
 // Orthonormalises nvec vectors v_k[e] = A[k*vs + e*es] (e < len) in order by
 // classical Gram-Schmidt applied twice, writing the coefficients
 // Cf[k][j] = <q_j, a_k> (j <= k, row-major nvec x nvec, zero above the
-// diagonal).  A vector whose remaining norm is <= 1e-14 ‖A‖_F becomes
+// diagonal).  A vector whose remaining norm is <= 1e-12 ‖A‖_F becomes
 // exactly zero (a dead direction, e.g. a zero-padded bond index).  Rows give
 // LQ (vs = row stride, es = 1), columns QR (vs = 1, es = row stride).
 //
@@ -793,7 +793,7 @@ __global__ __launch_bounds__(256) void cgs2_panel_kernel(cplx<T> *A, int64_t sA,
     if (mask && !mask[t]) return;
     cplx<T> *M = A + t * sA;
     cplx<T> *C = Cf + t * sC;
-    const double dead = 1e-14 * fro[t];
+    const double dead = 1e-12 * fro[t];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
     // load, coalesced along whichever index is contiguous
     if (vs == 1) {
@@ -1515,3 +1515,115 @@ extern "C" int yaqs_debug_cgs2(int prec, int nvec, int len, int lq, int count, d
         return 3;
     }
 }
+
+// Calibration hook (not part of the public header): copies `bytes` (a
+// multiple of 16) between two device buffers with one plain 16-byte-per-lane
+// copy kernel, `reps` times, so that hardware traffic counters (FETCH_SIZE,
+// WRITE_SIZE) can be scaled against a launch of known traffic.
+__global__ __launch_bounds__(256) void copy16_kernel(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                     int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+extern "C" int yaqs_debug_copy_bytes(int64_t bytes, int reps) {
+    using namespace yq;
+    if (bytes <= 0 || (bytes & 15) || reps < 1) return 1;
+    try {
+        double2 *a = nullptr, *b = nullptr;
+        YQ_HIP(hipMalloc(&a, bytes));
+        YQ_HIP(hipMalloc(&b, bytes));
+        YQ_HIP(hipMemset(a, 0, bytes));
+        const int64_t n = bytes / 16;
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(copy16_kernel, dim3(4096), dim3(256), 0, 0, a, b, n);
+        YQ_HIP(hipGetLastError());
+        YQ_HIP(hipDeviceSynchronize());
+        (void)hipFree(a);
+        (void)hipFree(b);
+        return 0;
+    } catch (const std::exception &) {
+        return 3;
+    }
+}
+
+namespace yq {
+
+// ------------------------------------------------ split: deflated tail --
+// Kept eigenvalues of a Gram matrix below tau * lambda_max (sorted
+// descending) carry singular values below sqrt(tau) of the largest, whose
+// eigenvectors the Gram route cannot resolve (their eigenvalues sit at the
+// rounding level of G).  dcount[t] = kept eigenvalues above the cut, flag[t]
+// = 1 when some kept ones are below it (and at least one above), counted in
+// *count.
+__global__ void tail_flags_kernel(const double *evals, int64_t sE, const int *keep, double tau, int *dcount, int *flag,
+                                  int *count, int B) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B) return;
+    const double *ev = evals + t * sE;
+    const int k = keep[t];
+    int d = 0;
+    while (d < k && ev[d] > tau * ev[0]) ++d;
+    const int f = (d >= 1 && d < k) ? 1 : 0;
+    dcount[t] = d;
+    flag[t] = f;
+    if (f) atomicAdd(count, 1);
+}
+void tail_flags(const double *evals, int64_t sE, const int *keep, double tau, int *dcount, int *flag, int *count, int B,
+                hipStream_t s) {
+    hipLaunchKernelGGL(tail_flags_kernel, dim3((B + 255) / 256), dim3(256), 0, s, evals, sE, keep, tau, dcount, flag,
+                       count, B);
+    YQ_HIP(hipGetLastError());
+}
+
+// D = A - C (n elements per slot, masked)
+template <typename T>
+__global__ void sub_kernel(const cplx<T> *A, int64_t sA, const cplx<T> *C, int64_t sC, cplx<T> *D, int64_t sD, int64_t n,
+                           const int *mask) {
+    const int t = blockIdx.y;
+    if (mask && !mask[t]) return;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) st2(D[t * sD + e], csub(ld2(A[t * sA + e]), ld2(C[t * sC + e])));
+}
+template <typename T>
+void sub(const cplx<T> *A, int64_t sA, const cplx<T> *C, int64_t sC, cplx<T> *D, int64_t sD, int64_t n, const int *mask,
+         int B, hipStream_t s) {
+    hipLaunchKernelGGL(sub_kernel<T>, dim3((unsigned)((n + 255) / 256), B), dim3(256), 0, s, A, sA, C, sC, D, sD, n, mask);
+    YQ_HIP(hipGetLastError());
+}
+template void sub<float>(const c32 *, int64_t, const c32 *, int64_t, c32 *, int64_t, int64_t, const int *, int, hipStream_t);
+template void sub<double>(const c64 *, int64_t, const c64 *, int64_t, c64 *, int64_t, int64_t, const int *, int,
+                          hipStream_t);
+
+// Columns (mode 0) or rows (mode 1, conjugated) k in [dcount, keep) of Out
+// from the leading eigenvectors of U (sorted by perm): the isometry written
+// by write_iso with its resolved part kept and its tail replaced.
+template <typename T>
+__global__ void tail_fix_kernel(cplx<T> *Out, int64_t sO, int n, int kw, int mode, const int *dcount, const int *keep,
+                                const c64 *U, int64_t sU, const int *perm, int64_t sE, const int *mask) {
+    const int t = blockIdx.y;
+    if (mask && !mask[t]) return;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * kw) return;
+    int r, k;
+    if (mode == 0) { r = (int)(e / kw); k = (int)(e % kw); }
+    else { k = (int)(e / n); r = (int)(e % n); }
+    const int d = dcount[t];
+    if (k < d || k >= keep[t]) return;
+    const c64 u = U[t * sU + (int64_t)r * n + perm[t * sE + (k - d)]];
+    st2(Out[t * sO + e], make_double2(u.re, mode == 0 ? u.im : -u.im));
+}
+template <typename T>
+void tail_fix(cplx<T> *Out, int64_t sO, int n, int kw, int mode, const int *dcount, const int *keep, const c64 *U,
+              int64_t sU, const int *perm, int64_t sE, const int *mask, int B, hipStream_t s) {
+    const int64_t tot = (int64_t)n * kw;
+    hipLaunchKernelGGL(tail_fix_kernel<T>, dim3((unsigned)((tot + 255) / 256), B), dim3(256), 0, s, Out, sO, n, kw, mode,
+                       dcount, keep, U, sU, perm, sE, mask);
+    YQ_HIP(hipGetLastError());
+}
+template void tail_fix<float>(c32 *, int64_t, int, int, int, const int *, const int *, const c64 *, int64_t, const int *,
+                              int64_t, const int *, int, hipStream_t);
+template void tail_fix<double>(c64 *, int64_t, int, int, int, const int *, const int *, const c64 *, int64_t, const int *,
+                               int64_t, const int *, int, hipStream_t);
+
+}  // namespace yq
