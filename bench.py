@@ -365,8 +365,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the batched FP64 leg")
-    ap.add_argument("--only", default="all", choices=["all", "roofline"],
-                    help="roofline: run the one-wave roofline window alone (for profiling)")
+    ap.add_argument("--only", default="all", choices=["all", "roofline", "serve"],
+                    help="roofline: run the one-wave roofline window alone; serve: the serving stream alone "
+                         "(both for profiling)")
     ap.add_argument("--serve-jobs", type=int, default=12, help="jobs of the serving stream (0 = skip)")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3"],
                     help="c2: the analog headline line; c3: the digital brickwork line")
@@ -409,6 +410,10 @@ def main():
     Kaux = min(K, 4)  # steps of the FP64, plain and roofline legs
     window = {"first_step": win0 + 1, "last_step": win0 + K, "t": [win0 * 0.05, (win0 + K) * 0.05],
               "rule": f"timed window starts at saturated bonds: warm-up = max(--warmup, {SATURATED_WARMUP}) steps"}
+    if args.only == "serve":
+        rep = serve_stream(Y, args.serve_jobs)
+        print(json.dumps({"serving": rep}), flush=True)
+        return
     if args.only == "roofline":
         # the roofline window alone (one wave), as profiled under rocprofv3
         rctx = Y.Context(0, max_batch=args.ntraj)

@@ -67,7 +67,7 @@ def test_brickwork_depolarizing_parity(ctx, oracle_lib):
     cfg = Y.StrongMeasurementConfig(obs, num_traj=16, max_bond_dim=16, truncation_threshold=1e-9)
     g, gm, gb, st, r, rm, rb, rst = _run(ctx, oracle_lib, L, circ, cfg, _depolarizing(L, 0.05), seed=7)
     assert st["jumps"] == rst["jumps"] and st["jumps"] > 0
-    assert _err("g", g, r) < 1e-9
+    assert _err("g", g, r) < 1e-10
     assert np.array_equal(gb, rb)
 
 
@@ -97,7 +97,7 @@ def test_brickwork_unraveled_noise_parity(ctx, oracle_lib):
     noise = Y.NoiseModel(procs, L)
     g, gm, gb, st, r, rm, rb, rst = _run(ctx, oracle_lib, L, circ, cfg, noise, seed=8)
     assert st["jumps"] == rst["jumps"] and st["jumps"] > 0
-    assert _err("g", g, r) < 1e-9
+    assert _err("g", g, r) < 1e-10
     assert np.array_equal(gb, rb)
 
 
@@ -146,7 +146,7 @@ def test_window_tdvp_gates_parity(ctx, oracle_lib, noisy):
     assert st["jumps"] == rst["jumps"]
     if noisy:
         assert st["jumps"] > 0
-    assert _err("g", g, r) < 1e-9
+    assert _err("g", g, r) < 1e-10
     assert np.array_equal(gb, rb)
 
 
@@ -187,7 +187,7 @@ def test_swap_network_parity(ctx, oracle_lib, noise_sites):
     assert st["jumps"] == rst["jumps"]
     if noise_sites:
         assert st["jumps"] > 0
-    assert _err("g", g, r) < 1e-9
+    assert _err("g", g, r) < 1e-10
     assert np.array_equal(gb, rb)
 
 
@@ -243,5 +243,5 @@ def test_long_range_tdvp_windows_parity(ctx, oracle_lib, noisy):
     assert st["jumps"] == rst["jumps"]
     if noisy:
         assert st["jumps"] > 0
-    assert _err("g", g, r) < 1e-8
+    assert _err("g", g, r) < 1e-10
     assert np.array_equal(gb, rb)

@@ -2,13 +2,14 @@
 // splits (the SVD of split_mps_tensor_svd, src/Algorithms.jl:409-445).
 //
 // Fast path (n <= 128, i.e. bond dimensions up to 64), four kernels:
-//   1. hetrd_kernel: one 256-thread workgroup per matrix reduces the packed
-//      lower triangle (LDS, <= 144 KB) to a real symmetric tridiagonal T with
-//      LAPACK zhetd2/zlarfg conventions (Q = H_0 ... H_{n-2},
-//      H_k = I - tau_k v_k v_k^H, subdiagonal real).  Three barriers per
-//      column: the Hermitian product p = tau A22 v (two column halves per
-//      row), the reduction of p^H v, and the rank-2 update, which also
-//      extracts the next column and its norm.
+//   1. Householder reduction to a real symmetric tridiagonal T with LAPACK
+//      zhetd2/zlarfg conventions (Q = H_0 ... H_{n-2},
+//      H_k = I - tau_k v_k v_k^H, subdiagonal real), one workgroup per
+//      matrix: hetrd_reg_kernel (n > 32) keeps the whole matrix in the
+//      registers of 1024 threads, two barriers per column; hetrd_kernel
+//      (n <= 32) keeps the packed lower triangle in LDS, 512 threads, three
+//      barriers per column (Hermitian product, p^H v, rank-2 update with the
+//      next column's extraction).
 //   2. stebz_kernel: every eigenvalue of T by bisection on Sturm counts, one
 //      thread per eigenvalue (no serial chain across eigenvalues), to an
 //      absolute width of 2 eps ||T||; sorted output and the kept rank.
@@ -630,6 +631,215 @@ __global__ __launch_bounds__(TRD_WAVES * 64) void hetrd_kernel(const c64 *__rest
         dg[n - 1] = P[pidx(n - 1, n - 1)].x;
         eg[n - 1] = 0.0;
     }
+}
+
+// 1b. The same reduction with the whole matrix in registers (n > 32): 1024
+// threads, thread (r = tid / 8, cc = tid % 8) holds row r at columns
+// cc + 8b, b < 16 (complex FP64, full Hermitian storage), so a column step
+// reads no matrix element from memory.  Column k (k1 = k + 1),
+// two barriers:
+//   A: q = A22 y (y = x except y_k1 = alpha - beta) as A22 x minus
+//      beta A[:, k1]: each thread its columns, the 8 lanes of a row summed
+//      by DPP; p = tau inv q, v = inv y, per-wave partial of p^H v;
+//   B: A22 -= v w^H + w v^H (w = p + al v) folded per row into
+//      a_rc -= s1 conj(p_c) + s2 conj(y_c), s1 = v_r,
+//      s2 = v_r conj(al inv) + w_r conj(inv).  Then row k1 writes the next
+//      column (x_c = A[c][k1] = conj(A[k1][c]), c > k1), its tail norm and
+//      d[k1], and row k1 + 1 writes A[c][k1 + 1] for the next correction.
+// Every LDS vector is zero outside the trailing block, so partially covered
+// column groups need no masks; waves whose rows are all reduced (or past n)
+// and groups of 32 columns (4 blocks, their LDS reads issued together)
+// outside the trailing block are skipped by uniform branches, so the
+// per-column work shrinks with the trailing block.  The kernel is bound by
+// the latency of its per-column chain (LDS round trips, DPP reductions, the
+// scalar Householder arithmetic, two barriers); a variant with 4 lanes per
+// row (512 threads, 32 values each) halves the per-wave scalar work but
+// hides less latency and measured slower.  Conventions and outputs as
+// hetrd_kernel.
+constexpr int TRR_THREADS = 1024;
+__device__ __forceinline__ double sum_row(double v) {  // over the 8 lanes of a row
+    v += dpp_move<0xb1>(v);                             // xor 1
+    v += dpp_move<0x4e>(v);                             // xor 2
+    v += dpp_move<0x141>(v);                            // half-row mirror: the other quad
+    return v;
+}
+// sum over lanes 0..15 (the other lanes' values are ignored), uniform
+__device__ __forceinline__ double sum16(double v) {
+    v += dpp_move<0xb1>(v);   // xor 1
+    v += dpp_move<0x4e>(v);   // xor 2
+    v += dpp_move<0x141>(v);  // half-row mirror
+    v += dpp_move<0x140>(v);  // row mirror
+    return lane_read(v, 0);
+}
+__global__ __launch_bounds__(TRR_THREADS) void hetrd_reg_kernel(const c64 *__restrict__ G, int64_t sG,
+                                                                char *__restrict__ Wk, int64_t sW, int n,
+                                                                const int *__restrict__ mask) {
+    constexpr int LPR = 8, NB = 128 / LPR, RPW = 64 / LPR, NWV = TRR_THREADS / 64, GC = 4 * LPR;
+    const int t = blockIdx.x;
+    if (mask && !mask[t]) return;
+    __shared__ double2 xb[2][128];  // x of column k by parity: A[c][k] for c > k, else 0
+    __shared__ double2 cb[2][128];  // A[c][k1] for c >= k1, else 0
+    __shared__ double2 xy[128];     // y for the update
+    __shared__ double2 pb[128];     // p, 0 outside the trailing block
+    __shared__ double sxs[2];       // tail norm of x
+    __shared__ double2 rpv[NWV];
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int r = tid / LPR, cc = tid % LPR;
+    const HeevLay lay = heev_lay(n);
+    char *base = Wk + t * sW;
+    double2 *Vg = (double2 *)(base + lay.V);
+    double2 *taug = (double2 *)(base + lay.tau);
+    double *dg = (double *)(base + lay.d);
+    double *eg = (double *)(base + lay.e);
+    const c64 *A = G + t * sG;
+    const double2 zero = make_double2(0.0, 0.0);
+    double2 a[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int c = cc + LPR * b;
+        a[b] = zero;
+        if (r < n && c < n) a[b] = make_double2(A[(int64_t)r * n + c].re, A[(int64_t)r * n + c].im);
+    }
+    if (tid < 128) {
+        xb[1][tid] = zero;
+        cb[1][tid] = zero;
+        xy[tid] = zero;
+        pb[tid] = zero;
+    }
+    // column 0 from row 0, the first correction from row 1
+    if (r == 0) {
+        double xsp = 0;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int c = cc + LPR * b;
+            xb[0][c] = c > 0 && c < n ? c_conj(a[b]) : zero;
+            if (c >= 2) xsp += a[b].x * a[b].x + a[b].y * a[b].y;
+        }
+        if (cc == 0) dg[0] = a[0].x;
+        xsp = sum_row(xsp);
+        if (cc == 0) sxs[0] = xsp;
+    } else if (r == 1) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int c = cc + LPR * b;
+            cb[0][c] = c >= 1 && c < n ? c_conj(a[b]) : zero;
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k + 1 < n; ++k) {
+        const int par = k & 1, k1 = k + 1;
+        const bool wact = RPW * w + RPW - 1 >= k1 && RPW * w < n;
+        double2 vr = zero, pr = zero, tk = zero, inv = zero;
+        double beta = 0.0;
+        if (wact) {
+            const double2 alpha = xb[par][k1];
+            const double xs = sxs[par];
+            const double2 ck = cb[par][r];
+            const double2 xr = xb[par][r];
+            beta = alpha.x;
+            if (!(xs == 0.0 && alpha.y == 0.0)) {
+                const double nr = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xs);
+                beta = alpha.x >= 0 ? -nr : nr;
+                const double rb = 1.0 / beta;
+                tk = make_double2((beta - alpha.x) * rb, -alpha.y * rb);
+                const double2 den = make_double2(alpha.x - beta, alpha.y);
+                const double rdn = 1.0 / (den.x * den.x + den.y * den.y);
+                inv = make_double2(den.x * rdn, -den.y * rdn);
+            }
+            const double2 ti = c_mul(tk, inv);
+            double2 acc = zero;
+#pragma unroll
+            for (int g = 0; g < NB / 4; ++g) {
+                if (GC * g + GC - 1 >= k1 && GC * g < n) {
+                    double2 xc[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) xc[u] = xb[par][cc + LPR * (4 * g + u)];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const double2 av = a[4 * g + u];
+                        acc.x += av.x * xc[u].x - av.y * xc[u].y;
+                        acc.y += av.x * xc[u].y + av.y * xc[u].x;
+                    }
+                }
+            }
+            acc.x = sum_row(acc.x);
+            acc.y = sum_row(acc.y);
+            acc.x -= beta * ck.x;
+            acc.y -= beta * ck.y;
+            const bool rin = r >= k1 && r < n;
+            if (rin) {
+                pr = c_mul(ti, acc);
+                vr = r == k1 ? make_double2(1.0, 0.0) : c_mul(inv, xr);
+            }
+            if (cc == 0) {
+                pb[r] = pr;
+                if (rin) Vg[(int64_t)k * n + r] = vr;
+                if (r == k1) {
+                    eg[k] = beta;
+                    taug[k] = tk;
+                }
+            } else if (cc == 1) {
+                xy[r] = rin ? (r == k1 ? make_double2(xr.x - beta, xr.y) : xr) : zero;
+            }
+            // per-wave partial of p^H v: one lane per row carries the row's term
+            const double2 z = cc == 0 ? c_mulc(pr, vr) : zero;
+            const double zx = wsum(z.x), zy = wsum(z.y);
+            if (lane == 0) rpv[w] = make_double2(zx, zy);
+        }
+        __syncthreads();
+        if (wact) {
+            // the active waves' partials, one per lane, summed across lanes
+            double2 q = zero;
+            if (lane < NWV && RPW * lane + RPW - 1 >= k1 && RPW * lane < n) q = rpv[lane];
+            double2 pv;
+            pv.x = sum16(q.x);
+            pv.y = sum16(q.y);
+            const double2 al = c_mul(make_double2(-0.5 * tk.x, -0.5 * tk.y), pv);
+            const double2 wr = make_double2(pr.x + al.x * vr.x - al.y * vr.y, pr.y + al.x * vr.y + al.y * vr.x);
+            const double2 s1 = vr;
+            const double2 ai = c_mul(al, inv);
+            const double2 s2 = make_double2(vr.x * ai.x + vr.y * ai.y + wr.x * inv.x + wr.y * inv.y,
+                                            vr.y * ai.x - vr.x * ai.y + wr.y * inv.x - wr.x * inv.y);
+#pragma unroll
+            for (int g = 0; g < NB / 4; ++g) {
+                if (GC * g + GC - 1 >= k1 && GC * g < n) {
+                    double2 yc[4], pc[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        yc[u] = xy[cc + LPR * (4 * g + u)];
+                        pc[u] = pb[cc + LPR * (4 * g + u)];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        double2 &av = a[4 * g + u];
+                        av.x -= s1.x * pc[u].x + s1.y * pc[u].y + s2.x * yc[u].x + s2.y * yc[u].y;
+                        av.y -= s1.y * pc[u].x - s1.x * pc[u].y + s2.y * yc[u].x - s2.x * yc[u].y;
+                    }
+                }
+            }
+            if (r == k1) {
+                double xsp = 0, dk = 0;
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    const int c = cc + LPR * b;
+                    xb[par ^ 1][c] = c > k1 && c < n ? c_conj(a[b]) : zero;
+                    if (c > k1 + 1) xsp += a[b].x * a[b].x + a[b].y * a[b].y;
+                    dk = c == k1 ? a[b].x : dk;
+                }
+                if (cc == k1 % LPR) dg[k1] = dk;
+                xsp = sum_row(xsp);
+                if (cc == 0) sxs[par ^ 1] = xsp;
+            } else if (r == k1 + 1) {
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    const int c = cc + LPR * b;
+                    cb[par ^ 1][c] = c >= k1 + 1 && c < n ? c_conj(a[b]) : zero;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) eg[n - 1] = 0.0;
 }
 
 // 2. Eigenvalues of T by bisection on Sturm counts, one thread per
@@ -1330,7 +1540,10 @@ bool heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, d
     if (fast) {
         const HeevLay lay = heev_lay(n);
         const size_t sh1 = (size_t)(np + (4 + TRD_PARTS) * n) * sizeof(double2);
-        hipLaunchKernelGGL(hetrd_kernel, dim3(B), dim3(TRD_WAVES * 64), sh1, s, G, sG, W, sW, n, mask);
+        if (n > 32)
+            hipLaunchKernelGGL(hetrd_reg_kernel, dim3(B), dim3(TRR_THREADS), 0, s, G, sG, W, sW, n, mask);
+        else
+            hipLaunchKernelGGL(hetrd_kernel, dim3(B), dim3(TRD_WAVES * 64), sh1, s, G, sG, W, sW, n, mask);
         YQ_HIP(hipGetLastError());
         hipLaunchKernelGGL(stebz_kernel<LDS_MAX_N>, dim3(B), dim3(LDS_MAX_N), 0, s, W, sW, lay.d, lay.e, n, ns, evals,
                            perm, sE, keep, rule, thr, max_bond, mask);

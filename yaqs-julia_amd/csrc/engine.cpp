@@ -846,8 +846,8 @@ void Engine<T>::setup() {
     mask.alloc(B * sizeof(int));
     rngk.alloc(B * sizeof(uint64_t));
     count.alloc(sizeof(int));
-    fin_done.alloc(sizeof(int));
-    fill_int(fin_done.as<int>(), 0, 1, s);
+    fin_done.alloc(sizeof(unsigned long long));
+    fill_int(fin_done.as<int>(), 0, 2, s);
     nfail.alloc((size_t)B * sizeof(int));
     static const bool defl_off = [] {
         const char *e = std::getenv("YAQS_SPLIT_DEFLATE");
@@ -1304,7 +1304,7 @@ void Engine<T>::expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, Jo
         __atomic_store_n(h_pinned + 1 + (j & 1), -1, __ATOMIC_RELEASE);
         lz_finalize(j, kdim, dt, J.ktol, partn.as<double>(), n, galpha.as<double>(), alpha.as<double>(),
                     beta.as<double>(), beta0.as<double>(), coef.as<c64>(), nit.as<int>(), active.as<int>(),
-                    nactive.as<int>(), nb, fin_done.as<int>(), d_pinned + 1 + (j & 1), s);
+                    nactive.as<int>(), nb, fin_done.as<unsigned long long>(), d_pinned + 1 + (j & 1), s);
         YQ_HIP(hipEventRecord(lz_ev[j & 1], s));
     };
     // With `ahead`, iteration j+1 is queued before the host waits for the

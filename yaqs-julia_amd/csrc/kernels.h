@@ -58,11 +58,11 @@ void lz_update(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, const c64
 // stopping rule; trajectories that go on are counted in nactive[j] (cleared
 // by lz_init for every slot).  With `host_count` (a device pointer to a
 // coherent pinned host int) the last workgroup to finish also stores the
-// final nactive[j] there; `fin_done` is a device int that is zero between
-// launches (the last workgroup resets it).
+// final nactive[j] there; `fin_done` is a device 64-bit word that is zero
+// between launches (the last workgroup resets it).
 void lz_finalize(int j, int kmax, double dt, double tol, const double *partn, int64_t n, const double *galpha,
                  double *alpha, double *beta, const double *beta0, c64 *coef, int *nit, int *active, int *nactive, int B,
-                 int *fin_done, int *host_count, hipStream_t s);
+                 unsigned long long *fin_done, int *host_count, hipStream_t s);
 template <typename T>
 void lz_combine(cplx<T> *x, int64_t sx, const cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, const double *beta0,
                 const c64 *coef, const int *nit, const double *beta, int kmax, const int *mask, int B, hipStream_t s);

@@ -516,10 +516,14 @@ __device__ double2 tridiag_exp_lane(const double *alpha, const double *beta, int
 // the oracle).  A trajectory that stops keeps nit = j + 1 and leaves the
 // active set; trajectories that go on are counted in nactive[j], first per
 // workgroup in LDS, then with one global atomic per workgroup (a thousand
-// same-address atomics from single waves cost about 10 us).  The last
-// workgroup to count (device-scope release/acquire around the `fin_done`
-// ticket) reads the total and stores it into the host's pinned word, so the
-// host sees the count when this kernel ends, without a copy.
+// same-address atomics from single waves cost about 10 us).  With a host
+// word, each workgroup also adds (1 << 32) | its count to the 64-bit
+// `fin_done`: one atomic on one word carries the ticket (high half) and the
+// running total (low half), so the workgroup that draws the last ticket has
+// every count in its return value, with no fence; it stores the total into
+// the host's pinned word (the host needs the count alone, so a relaxed store
+// suffices: device-scope and system-scope release fences write back the L2
+// of the XCD, which cost most of this kernel's time).
 #ifndef LZ_FIN_THREADS
 #define LZ_FIN_THREADS 256
 #endif
@@ -528,7 +532,7 @@ __global__ __launch_bounds__(LZ_FIN_THREADS) void lz_finalize_kernel(int j, int 
                                                                      const double *galpha, double *alpha,
                                                                      double *beta, const double *beta0, c64 *coef,
                                                                      int *nit, int *active, int *nactive, int B,
-                                                                     int *fin_done, int *host_count) {
+                                                                     unsigned long long *fin_done, int *host_count) {
     __shared__ int going;
     const int lane = threadIdx.x & 63;
     const int t = blockIdx.x * (LZ_FIN_THREADS / 64) + (threadIdx.x >> 6);
@@ -567,12 +571,11 @@ __global__ __launch_bounds__(LZ_FIN_THREADS) void lz_finalize_kernel(int j, int 
     if (threadIdx.x == 0) {
         if (going) atomicAdd(nactive + j, going);
         if (host_count) {
-            __threadfence();
-            if (atomicAdd(fin_done, 1) == (int)gridDim.x - 1) {
-                __threadfence();
-                const int total = atomicAdd(nactive + j, 0);
-                atomicExch(fin_done, 0);
-                __hip_atomic_store(host_count, total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            const unsigned long long old = atomicAdd(fin_done, (1ull << 32) | (unsigned long long)going);
+            if ((unsigned)(old >> 32) == gridDim.x - 1) {
+                const int total = (int)(old & 0xffffffffull) + going;
+                atomicExch(fin_done, 0ull);
+                __hip_atomic_store(host_count, total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             }
         }
     }
@@ -653,7 +656,7 @@ void lz_update(cplx<T> *V, int64_t sV, int64_t nmax, int64_t n, int j, const c64
 }
 void lz_finalize(int j, int kmax, double dt, double tol, const double *partn, int64_t n, const double *galpha,
                  double *alpha, double *beta, const double *beta0, c64 *coef, int *nit, int *active, int *nactive, int B,
-                 int *fin_done, int *host_count, hipStream_t s) {
+                 unsigned long long *fin_done, int *host_count, hipStream_t s) {
     if (host_count && !fin_done) throw std::invalid_argument("lz_finalize: host_count needs fin_done");
     const int nchunk = (int)((n + LZ_CHUNK - 1) / LZ_CHUNK);
     constexpr int per = LZ_FIN_THREADS / 64;
