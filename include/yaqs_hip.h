@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define YAQS_ABI_VERSION 2
+#define YAQS_ABI_VERSION 3
 
 /* Return codes.  Mirrors the reference's failure modes: Julia `error(...)` /
  * `@assert` on bad input (src/Simulator.jl:136,139), plus device failures. */
@@ -104,7 +104,9 @@ typedef struct {
     const yaqs_mps_tensor *mps;        /* L tensors (normalized by the caller, src/Simulator.jl:128) */
     const yaqs_mpo_tensor *mpo;        /* L tensors                                                 */
     int32_t has_noise_model;           /* 0 = `nothing`                                             */
-    int32_t _pad0;
+    int32_t tdvp_sites;                /* 0 or 2: two_site_tdvp! (the reference's analog path);
+                                          1: single_site_tdvp! (src/Algorithms.jl:449-452, 471-532)
+                                          in its place, bond dimensions fixed by the input MPS      */
     int64_t n_noise;
     const yaqs_noise_process *noise;
     int64_t n_obs;

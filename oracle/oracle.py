@@ -52,6 +52,8 @@ def lib():
         L.oracle_last_window_seconds.restype = ct.c_double
         L.oracle_tdvp_evolve.argtypes = [ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_int, ct.c_int,
                                          ct.c_double, ct.c_int, ct.c_void_p, c_dp]
+        L.oracle_tdvp_overlap.argtypes = [ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_int, ct.c_int,
+                                          ct.c_int, ct.c_double, c_dp]
         L.oracle_use_blas.argtypes = [ct.c_char_p, ct.c_char_p, ct.c_char_p]
         L.oracle_blas_bound.restype = ct.c_int
         L.oracle_use_lapack_svd.argtypes = [ct.c_char_p, ct.c_char_p]
@@ -220,6 +222,16 @@ def tdvp_evolve(holder_analog, steps):
                              int(job.max_bond_dim), job.truncation_threshold, nobs, ct.cast(job.obs, ct.c_void_p),
                              _ptr(out))
     return out.reshape(nobs, steps + 1)
+
+
+def tdvp_overlap(holder_analog, steps, sites):
+    """<ψ0|ψ(t)> after `steps` noise-free sweeps of two_site_tdvp! (sites=2)
+    or single_site_tdvp! (sites=1)."""
+    job = holder_analog.job
+    out = np.zeros(2)
+    lib().oracle_tdvp_overlap(job.L, ct.cast(job.mps, ct.c_void_p), ct.cast(job.mpo, ct.c_void_p), job.dt, steps,
+                              sites, int(job.max_bond_dim), job.truncation_threshold, _ptr(out))
+    return complex(out[0], out[1])
 
 
 def stochastic_factor(holder):

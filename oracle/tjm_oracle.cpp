@@ -10,6 +10,7 @@
 // here as well):
 //   AnalogTJM          src/AnalogTJM.jl:21-176
 //   2-site TDVP        src/Algorithms.jl:144-226, 409-445, 580-819
+//   1-site TDVP        src/Algorithms.jl:222-226, 449-452, 471-532
 //   MPS gauge/measure  src/MPS.jl:182-311, 528-651
 //   Dissipation        src/Dissipation.jl:14-159
 //   Jumps              src/StochasticProcess.jl:35-330
@@ -927,6 +928,97 @@ void two_site_tdvp(MPSState &s, const TdvpCtx &C) {
     }
 }
 
+// project_bond (src/Algorithms.jl:222-226): C (k x kr, column-major) ->
+// C_new[bra_l, bra_r] = sum L[bra_l, mpo, k] C[k, ket_r] R[bra_r, mpo, ket_r].
+cvec project_bond(const cvec &Cm, int k, int kr, const Env &Le, const Env &Re) {
+    cvec T1((size_t)Le.b * Le.m * kr);  // [bra_l, mpo, ket_r]
+    gemm('N', 'N', Le.b * Le.m, kr, k, Le.v.data(), Le.b * Le.m, Cm.data(), k, T1.data(), Le.b * Le.m);
+    cvec out((size_t)Le.b * Re.b);
+    gemm('N', 'T', Le.b, Re.b, Le.m * kr, T1.data(), Le.b, Re.v.data(), Re.b, out.data(), Le.b);
+    return out;
+}
+
+cvec expm_bond(const TdvpCtx &C, const cvec &Cm, int k, int kr, const Env &Le, const Env &Re, double dt) {
+    int iters = 0;
+    auto f = [&](const cvec &x) { return project_bond(x, k, kr, Le, Re); };
+    cvec out = expm_krylov(f, Cm, dt, C.cfg.kdim, C.cfg.ktol, &iters);
+    if (C.cnt) C.cnt->flop += 8.0 * 2.0 * Le.m * (double)k * k * kr * iters;
+    return out;
+}
+
+// _tdvp_sweep_hamiltonian_1site! (src/Algorithms.jl:471-532): forward, each
+// site evolves by dt/2, then (but the last) its QR moves the centre right and
+// the R factor evolves by -dt/2 under the bond Hamiltonian before it is
+// absorbed into the next site; backward the same with LQ from site L.
+void single_site_tdvp(MPSState &s, const TdvpCtx &C) {
+    const std::vector<T4> &H = *C.H;
+    shift_oc(s, 1);
+    const int L = s.L;
+    const double dt = C.cfg.dt;
+    std::vector<Env> EL(L + 1), ER(L + 1);
+    {
+        int rb = s.t[L - 1].dr, rm = H[L - 1].dr;
+        ER[L] = Env{rb, rm, rb, identity_env(rb, rm)};
+        for (int i = L; i >= 2; --i) {
+            ER[i - 1] = update_right_env(s.t[i - 1], H[i - 1], ER[i]);
+            if (C.cnt) C.cnt->env++;
+        }
+        int lb = s.t[0].dl, lm = H[0].dl;
+        EL[0] = Env{lb, lm, lb, identity_env(lb, lm)};
+    }
+    for (int i = 1; i <= L; ++i) {  // site i (1-based): EL[i-1], ER[i]
+        {
+            T3 &A = s.t[i - 1];
+            A.v = expm_site(C, A.v, A.dl, A.d, A.dr, EL[i - 1], ER[i], H[i - 1], dt / 2, false);
+        }
+        if (i == L) break;
+        const T3 &A = s.t[i - 1];
+        const int l = A.dl, p = A.d, r = A.dr;
+        int rr;
+        cvec Q, R;
+        qr_thin(A.v, l * p, r, Q, R, rr);
+        T3 nA(l, p, rr);
+        nA.v = Q;
+        s.t[i - 1] = std::move(nA);
+        EL[i] = update_left_env(s.t[i - 1], H[i - 1], EL[i - 1]);
+        if (C.cnt) C.cnt->env++;
+        const cvec Cn = expm_bond(C, R, rr, r, EL[i], ER[i], -dt / 2);
+        const T3 &N = s.t[i];
+        T3 nN(rr, N.d, N.dr);
+        gemm('N', 'N', rr, N.d * N.dr, r, Cn.data(), rr, N.v.data(), r, nN.v.data(), rr);
+        s.t[i] = std::move(nN);
+    }
+    for (int i = L; i >= 1; --i) {
+        {
+            T3 &A = s.t[i - 1];
+            A.v = expm_site(C, A.v, A.dl, A.d, A.dr, EL[i - 1], ER[i], H[i - 1], dt / 2, false);
+        }
+        if (i == 1) break;
+        const T3 &A = s.t[i - 1];
+        const int l = A.dl, p = A.d, r = A.dr;
+        int rr;
+        cvec Lm, Q;
+        lq_thin(A.v, l, p * r, Lm, Q, rr);
+        T3 nA(rr, p, r);
+        nA.v = Q;
+        s.t[i - 1] = std::move(nA);
+        ER[i - 1] = update_right_env(s.t[i - 1], H[i - 1], ER[i]);
+        if (C.cnt) C.cnt->env++;
+        const cvec Cn = expm_bond(C, Lm, l, rr, EL[i - 1], ER[i - 1], -dt / 2);
+        const T3 &P = s.t[i - 2];
+        T3 nP(P.dl, P.d, rr);
+        gemm('N', 'N', P.dl * P.d, rr, l, P.v.data(), P.dl * P.d, Cn.data(), l, nP.v.data(), P.dl * P.d);
+        s.t[i - 2] = std::move(nP);
+    }
+}
+
+// The sweep the analog path runs: two_site_tdvp! in the reference
+// (src/AnalogTJM.jl:44,68,161); single_site_tdvp! when the job asks for it.
+void tdvp_step(MPSState &s, const TdvpCtx &C, int sites) {
+    if (sites == 1) single_site_tdvp(s, C);
+    else two_site_tdvp(s, C);
+}
+
 // ------------------------------------------------------- noise model ------
 struct Proc {
     int nsites;
@@ -1292,7 +1384,7 @@ void run_trajectory(const yaqs_analog_job *job, const MPSState &psi0, const std:
     if (job->order == 1) {
         for (int j = 2; j <= nt; ++j) {
             if (j == g_window_warmup + 2) out.w0 = wall_now();
-            two_site_tdvp(state, C);
+            tdvp_step(state, C, job->tdvp_sites);
             if (noise) {
                 apply_dissipation(state, noise, dt, thr, mb);
                 stoch(state, dt);
@@ -1308,7 +1400,7 @@ void run_trajectory(const yaqs_analog_job *job, const MPSState &psi0, const std:
     stoch(state, dt);
     auto sample = [&](int j) {  // src/AnalogTJM.jl:62-81
         MPSState psi = state;
-        two_site_tdvp(psi, C);
+        tdvp_step(psi, C, job->tdvp_sites);
         apply_dissipation(psi, noise, dt / 2, thr, mb);
         stoch(psi, dt);
         measure(psi, job->sample_timesteps ? j - 1 : 0);
@@ -1316,7 +1408,7 @@ void run_trajectory(const yaqs_analog_job *job, const MPSState &psi0, const std:
     if (job->sample_timesteps && nt > 1) sample(2);
     for (int j = 3; j <= nt; ++j) {
         if (j == g_window_warmup + 2) out.w0 = wall_now();
-        two_site_tdvp(state, C);  // step_through (src/AnalogTJM.jl:40-55)
+        tdvp_step(state, C, job->tdvp_sites);  // step_through (src/AnalogTJM.jl:40-55)
         apply_dissipation(state, noise, dt, thr, mb);
         stoch(state, dt);
         if (job->sample_timesteps || j == nt) sample(j);
@@ -1886,6 +1978,30 @@ int oracle_prob_distribution(int64_t L, const yaqs_mps_tensor *t, int64_t n_nois
         cand[3 * i + 2] = c[i].proc;
     }
     return (int)p.size();
+}
+
+// Noise-free TDVP: `steps` sweeps of two_site_tdvp! (sites = 2) or
+// single_site_tdvp! (sites = 1) from the given state; returns
+// scalar_product(ψ(t), ψ0) = <ψ0|ψ(t)> (src/MPS.jl:528-585) in out[0..1]
+// (test/test_Algorithms.jl:37-93).
+int oracle_tdvp_overlap(int64_t L, const yaqs_mps_tensor *mps, const yaqs_mpo_tensor *mpo, double dt, int steps,
+                        int sites, int max_bond, double thr, double *out) {
+    MPSState s = load_mps(L, mps);
+    const MPSState s0 = s;
+    std::vector<T4> H(L);
+    for (int64_t i = 0; i < L; ++i) {
+        const yaqs_mpo_tensor &w = mpo[i];
+        H[i].dl = (int)w.dl; H[i].po = (int)w.dout; H[i].pi = (int)w.din; H[i].dr = (int)w.dr;
+        H[i].v.resize((size_t)w.dl * w.dout * w.din * w.dr);
+        load_c(w.data, H[i].v.data(), H[i].v.size());
+    }
+    Cfg cfg{dt, max_bond, thr, 25, 1e-10};
+    TdvpCtx C{&H, cfg, nullptr};
+    for (int k = 0; k < steps; ++k) tdvp_step(s, C, sites);
+    const cd ov = scalar_product(s, s0);
+    out[0] = ov.real();
+    out[1] = ov.imag();
+    return 0;
 }
 
 // Noise-free 2-site TDVP: `steps` sweeps of two_site_tdvp! from the given

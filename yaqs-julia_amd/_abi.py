@@ -47,7 +47,7 @@ class ObservableS(ct.Structure):
 
 class AnalogJob(ct.Structure):
     _fields_ = [("L", ct.c_int64), ("mps", ct.POINTER(MpsTensor)), ("mpo", ct.POINTER(MpoTensor)),
-                ("has_noise_model", ct.c_int32), ("_pad0", ct.c_int32), ("n_noise", ct.c_int64),
+                ("has_noise_model", ct.c_int32), ("tdvp_sites", ct.c_int32), ("n_noise", ct.c_int64),
                 ("noise", ct.POINTER(NoiseProcess)), ("n_obs", ct.c_int64), ("obs", ct.POINTER(ObservableS)),
                 ("dt", ct.c_double), ("num_times", ct.c_int64), ("num_traj", ct.c_int64),
                 ("max_bond_dim", ct.c_int64), ("truncation_threshold", ct.c_double), ("order", ct.c_int32),
@@ -150,7 +150,10 @@ def _pack_obs(h: _Holder, observables):
 
 
 def pack_analog(mps, mpo, cfg, noise_model, *, precision: int = PREC_FP64, seed: int = 0, traj_offset: int = 0,
-                krylov_dim: int = 25, krylov_tol: float = 1e-10, num_traj: Optional[int] = None) -> _Holder:
+                krylov_dim: int = 25, krylov_tol: float = 1e-10, num_traj: Optional[int] = None,
+                tdvp_sites: int = 2) -> _Holder:
+    """`tdvp_sites` = 1 runs single_site_tdvp! where the reference's analog
+    path calls two_site_tdvp! (bond dimensions then stay those of `mps`)."""
     h = _Holder()
     mps_arr = _pack_mps(h, mps)
     mpo_arr = (MpoTensor * mpo.length)()
@@ -176,6 +179,7 @@ def pack_analog(mps, mpo, cfg, noise_model, *, precision: int = PREC_FP64, seed:
     job.max_bond_dim = cfg.max_bond_dim
     job.truncation_threshold = cfg.truncation_threshold
     job.order = cfg.order
+    job.tdvp_sites = tdvp_sites
     job.sample_timesteps = 1 if cfg.sample_timesteps else 0
     job.krylov_dim = krylov_dim
     job.precision = precision

@@ -174,6 +174,9 @@ yq::JobIn convert_analog(const yaqs_analog_job *job) {
     J.max_bond = job->max_bond_dim;
     J.thr = job->truncation_threshold;
     J.order = job->order;
+    J.tdvp_sites = job->tdvp_sites == 1 ? 1 : 2;
+    if (job->tdvp_sites != 0 && job->tdvp_sites != 1 && job->tdvp_sites != 2)
+        throw std::invalid_argument("tdvp_sites must be 0, 1 or 2");
     J.sample_timesteps = job->sample_timesteps;
     J.kdim = job->krylov_dim > 0 ? job->krylov_dim : 25;
     J.ktol = job->krylov_tol > 0 ? job->krylov_tol : 1e-10;

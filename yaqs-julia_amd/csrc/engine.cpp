@@ -651,13 +651,15 @@ class Engine {
 
     // primitives
     void matvec(State &S, bool two, int i, const CT *x, int64_t sx, CT *y, int64_t sy, const int *act);
-    void expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, JobResult &res);
+    void matvec0(State &S, int b, const CT *x, int64_t sx, CT *y, int64_t sy, const int *act);
+    void expm(State &S, int sites, int i, CT *x, int64_t sx, double dt, JobResult &res);
     void merge(State &S, int i);
     void split(State &S, int i, bool right, int rule, double thr, const int *msk = nullptr);
     void deflate_tail(State &S, int i, bool right, int kw, int mb);
     void env_left(State &S, int i);
     void env_right(State &S, int i);
     void tdvp(State &S, JobResult &res);
+    void tdvp1(State &S, JobResult &res);
     void lq_sweep(State &S, int from, const int *msk);
     void qr_step(State &S, int i, const int *msk);
     void dissipate(State &S, bool half, int field = 0);
@@ -1280,14 +1282,32 @@ void Engine<T>::matvec(State &S, bool two, int i, const CT *x, int64_t sx, CT *y
              ER[br].as<CT>(), y, s);
 }
 
-// x <- exp(-i dt H_eff) x by batched Lanczos (src/Algorithms.jl:144-187).
+// The bond (0-site) effective Hamiltonian of the 1-site sweep (_bond_op,
+// src/Algorithms.jl:222-226): y[a'][b'] = sum EL[b][a'][w][a] x[a][b] ER[b][w][b][b'],
+// as EL·x then ·ER; x and y have leading dimension wd[b] rounded up to even
+// and a zero padding column.
 template <typename T>
-void Engine<T>::expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, JobResult &res) {
+void Engine<T>::matvec0(State &S, int b, const CT *x, int64_t sx, CT *y, int64_t sy, const int *act) {
+    const int c = S.wd[b], cp = (c + 1) & ~1, D = Dm[b];
+    CT *t1 = T1.as<CT>();
+    gemmT<T>(gd(c * D, c, c, OP_N, OP_N, c, cp, c, slotE[b], sx, slotS, nb, act), EL[b].as<CT>(), x, t1, s);
+    gemmT<T>(gd(c, c, D * c, OP_N, OP_N, D * c, c, cp, slotS, slotE[b], sy, nb, act), t1, ER[b].as<CT>(), y, s);
+    if (cp != c) zero_col<T>(y, sy, c, cp, c, act, nb, s);
+}
+
+// x <- exp(-i dt H_eff) x by batched Lanczos (src/Algorithms.jl:144-187).
+// sites = 2: Θ of sites i, i+1; 1: site i; 0: the bond matrix of bond i
+// (rows wd[i], leading dimension wd[i] rounded up to even, the padding
+// column zero).
+template <typename T>
+void Engine<T>::expm(State &S, int sites, int i, CT *x, int64_t sx, double dt, JobResult &res) {
+    const bool two = sites == 2;
     const int d = two ? 4 : 2;
     const int cl = S.wd[i], cr = S.wd[two ? i + 2 : i + 1];
-    const int64_t n = (int64_t)cl * d * cr;
-    const double fl = two ? 8.0 * (4.0 * Dm[i] * cl * cr * (double)(cl + cr) + 16.0 * Dm[i] * Dm[i] * (double)cl * cr)
-                          : 8.0 * (2.0 * Dm[i] * cl * cr * (double)(cl + cr) + 4.0 * Dm[i] * Dm[i] * (double)cl * cr);
+    const int64_t n = sites == 0 ? (int64_t)cl * ((cl + 1) & ~1) : (int64_t)cl * d * cr;
+    const double fl = sites == 0 ? 8.0 * 2.0 * Dm[i] * (double)cl * cl * cl
+                      : two ? 8.0 * (4.0 * Dm[i] * cl * cr * (double)(cl + cr) + 16.0 * Dm[i] * Dm[i] * (double)cl * cr)
+                            : 8.0 * (2.0 * Dm[i] * cl * cr * (double)(cl + cr) + 4.0 * Dm[i] * Dm[i] * (double)cl * cr);
     lz_init<T>(x, sx, V.as<CT>(), sV(), n, beta0.as<double>(), active.as<int>(), nit.as<int>(), nactive.as<int>(),
                kdim, nb, s);
     // One Lanczos iteration for the still-active trajectories (every kernel is
@@ -1295,7 +1315,8 @@ void Engine<T>::expm(State &S, bool two, int i, CT *x, int64_t sx, double dt, Jo
     // the host slot of its parity.
     auto iteration = [&](int j, int units) {
         g_nact = units;
-        matvec(S, two, i, Vq(j), sV(), Vq(j + 1), sV(), active.as<int>());
+        if (sites == 0) matvec0(S, i, Vq(j), sV(), Vq(j + 1), sV(), active.as<int>());
+        else matvec(S, two, i, Vq(j), sV(), Vq(j + 1), sV(), active.as<int>());
         g_nact = -1;
         lz_dots<T>(V.as<CT>(), sV(), nmax, n, j, part.as<c64>(), kdim + 1, active.as<int>(), nb, s);
         lz_update<T>(V.as<CT>(), sV(), nmax, n, j, part.as<c64>(), kdim + 1, partn.as<double>(), galpha.as<double>(),
@@ -1518,6 +1539,10 @@ void Engine<T>::env_right(State &S, int i) {
 // One symmetric 2-site TDVP sweep (src/Algorithms.jl:580-603, 631-760).
 template <typename T>
 void Engine<T>::tdvp(State &S, JobResult &res) {
+    if (J.tdvp_sites == 1) {
+        tdvp1(S, res);
+        return;
+    }
     if (L < 2) return;
     const double dt = J.dt;
     identity_env<T>(ER[L].as<CT>(), slotE[L], S.wd[L], Dm[L], 1, nb, s);
@@ -1525,24 +1550,71 @@ void Engine<T>::tdvp(State &S, JobResult &res) {
     identity_env<T>(EL[0].as<CT>(), slotE[0], S.wd[0], Dm[0], 0, nb, s);
     for (int i = 0; i + 2 < L; ++i) {  // forward (src/Algorithms.jl:631-672)
         merge(S, i);
-        expm(S, true, i, theta.as<CT>(), nmax, dt / 2, res);
+        expm(S, 2, i, theta.as<CT>(), nmax, dt / 2, res);
         split(S, i, true, 0, J.thr);
         env_left(S, i);
-        expm(S, false, i + 1, A(S, i + 1), slotA[i + 1], -dt / 2, res);
+        expm(S, 1, i + 1, A(S, i + 1), slotA[i + 1], -dt / 2, res);
     }
     {  // edge (src/Algorithms.jl:674-711)
         const int i = L - 2;
         merge(S, i);
-        expm(S, true, i, theta.as<CT>(), nmax, dt, res);
+        expm(S, 2, i, theta.as<CT>(), nmax, dt, res);
         split(S, i, false, 0, J.thr);
         env_right(S, i + 1);
     }
     for (int i = L - 3; i >= 0; --i) {  // backward (src/Algorithms.jl:713-760)
-        expm(S, false, i + 1, A(S, i + 1), slotA[i + 1], -dt / 2, res);
+        expm(S, 1, i + 1, A(S, i + 1), slotA[i + 1], -dt / 2, res);
         merge(S, i);
-        expm(S, true, i, theta.as<CT>(), nmax, dt / 2, res);
+        expm(S, 2, i, theta.as<CT>(), nmax, dt / 2, res);
         split(S, i, false, 0, J.thr);
         env_right(S, i + 1);
+    }
+}
+
+// single_site_tdvp! (_tdvp_sweep_hamiltonian_1site!, src/Algorithms.jl:471-532)
+// on a state with its centre at site 0: forward, each site evolves by dt/2,
+// then (but the last) its QR moves the centre right, the R factor evolves
+// backward by dt/2 under the bond Hamiltonian and is absorbed into the next
+// site; backward the same with LQ, from site L-1 (which so takes dt in two
+// halves) to site 0.  Bond dimensions do not change; the gauge steps drop
+// dead directions as everywhere in the engine.  The bond matrices live in
+// the theta slots (free in this sweep).
+template <typename T>
+void Engine<T>::tdvp1(State &S, JobResult &res) {
+    const double dt = J.dt;
+    const int64_t cap2 = (int64_t)capmax * capmax;
+    CT *cb = theta.as<CT>();
+    identity_env<T>(ER[L].as<CT>(), slotE[L], S.wd[L], Dm[L], 1, nb, s);
+    for (int i = L - 1; i >= 1; --i) env_right(S, i);
+    identity_env<T>(EL[0].as<CT>(), slotE[0], S.wd[0], Dm[0], 0, nb, s);
+    for (int i = 0; i < L; ++i) {  // forward (src/Algorithms.jl:480-505)
+        expm(S, 1, i, A(S, i), slotA[i], dt / 2, res);
+        if (i + 1 == L) break;
+        const int c0 = S.wd[i], c1 = S.wd[i + 1], c2 = S.wd[i + 2], cp = (c1 + 1) & ~1;
+        // A_i = Q R (CGS2 on columns), R[j][k] = Cf[k][j]
+        cgs2<T>(A(S, i), slotA[i], c1, 2 * c0, 1, c1, Cf.as<CT>(), cap2, cgsS.as<CT>(), slotCgs, cgsF.as<double>(),
+                nullptr, nb, s);
+        copy_bond<T>(Cf.as<CT>(), cap2, c1, true, cb, nmax, cp, c1, nullptr, nb, s);
+        env_left(S, i);
+        expm(S, 0, i + 1, cb, nmax, -dt / 2, res);
+        gemmT<T>(gd(c1, 2 * c2, c1, OP_N, OP_N, cp, 2 * c2, 2 * c2, nmax, slotA[i + 1], slotA[i + 1], nb), cb,
+                 A(S, i + 1), absorb.as<CT>(), s);
+        copy2d<T, T>(absorb.as<CT>(), slotA[i + 1], 2 * c2, A(S, i + 1), slotA[i + 1], 2 * c2, c1, 2 * c2, nullptr, nb,
+                     s);
+    }
+    for (int i = L - 1; i >= 0; --i) {  // backward (src/Algorithms.jl:507-531)
+        expm(S, 1, i, A(S, i), slotA[i], dt / 2, res);
+        if (i == 0) break;
+        const int cm = S.wd[i - 1], c0 = S.wd[i], c1 = S.wd[i + 1], cp = (c0 + 1) & ~1;
+        // A_i = L Q (CGS2 on rows), L = Cf
+        cgs2<T>(A(S, i), slotA[i], c0, 2 * c1, 2 * c1, 1, Cf.as<CT>(), cap2, cgsS.as<CT>(), slotCgs, cgsF.as<double>(),
+                nullptr, nb, s);
+        copy_bond<T>(Cf.as<CT>(), cap2, c0, false, cb, nmax, cp, c0, nullptr, nb, s);
+        env_right(S, i);
+        expm(S, 0, i, cb, nmax, -dt / 2, res);
+        gemmT<T>(gd(2 * cm, c0, c0, OP_N, OP_N, c0, cp, c0, slotA[i - 1], nmax, slotA[i - 1], nb), A(S, i - 1), cb,
+                 absorb.as<CT>(), s);
+        copy2d<T, T>(absorb.as<CT>(), slotA[i - 1], c0, A(S, i - 1), slotA[i - 1], c0, 2 * cm, c0, nullptr, nb, s);
     }
 }
 

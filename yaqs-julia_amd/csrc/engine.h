@@ -53,6 +53,7 @@ struct JobIn {
     int64_t num_times = 1, num_traj = 1, max_bond = 4096;
     double thr = 1e-9;
     int order = 2, sample_timesteps = 1, kdim = 25, precision = 0;
+    int tdvp_sites = 2;  // 1: single_site_tdvp! in place of two_site_tdvp!
     double ktol = 1e-10;
     uint64_t seed = 0, traj_offset = 0;
     int64_t bench_warmup = -1;  // >= 0: time the steps after this many (analog)

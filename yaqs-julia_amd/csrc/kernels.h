@@ -120,6 +120,12 @@ void copy2d(const cplx<TS> *S, int64_t sS, int64_t lds, cplx<TD> *D, int64_t sD,
             const int *mask, int B, hipStream_t s);
 template <typename T>
 void fill_zero(cplx<T> *A, int64_t sA, int64_t n, int B, hipStream_t s);
+// D[r][c] = S[r][c] (S[c][r] with tr) for r, c < n, zero in columns n .. ldd-1
+template <typename T>
+void copy_bond(const cplx<T> *S, int64_t sS, int lds, bool tr, cplx<T> *D, int64_t sD, int ldd, int n, const int *mask,
+               int B, hipStream_t s);
+template <typename T>
+void zero_col(cplx<T> *Y, int64_t sY, int rows, int ld, int col, const int *mask, int B, hipStream_t s);
 template <typename T>
 void identity_env(cplx<T> *E, int64_t sE, int dim, int D, int right, int B, hipStream_t s);
 template <typename T>

@@ -1102,6 +1102,51 @@ template void copy2d<float, double>(const c32 *, int64_t, int64_t, c64 *, int64_
 template void copy2d<double, float>(const c64 *, int64_t, int64_t, c32 *, int64_t, int64_t, int, int, const int *, int,
                                     hipStream_t);
 
+// Bond matrix of a gauge step into a Krylov slot: D[r][c] = S[r][c] (or
+// S[c][r] with `tr`) for r, c < n, and zero in columns n .. ldd-1 (Krylov
+// vectors are even-length element pairs).
+template <typename T>
+__global__ void copy_bond_kernel(const cplx<T> *S, int64_t sS, int lds, int tr, cplx<T> *D, int64_t sD, int ldd, int n,
+                                 const int *mask) {
+    const int t = blockIdx.y;
+    if (mask && !mask[t]) return;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * ldd) return;
+    const int r = (int)(e / ldd), c = (int)(e % ldd);
+    double2 v = make_double2(0.0, 0.0);
+    if (c < n) v = ld2(S[t * sS + (tr ? (int64_t)c * lds + r : (int64_t)r * lds + c)]);
+    st2(D[t * sD + (int64_t)r * ldd + c], v);
+}
+template <typename T>
+void copy_bond(const cplx<T> *S, int64_t sS, int lds, bool tr, cplx<T> *D, int64_t sD, int ldd, int n, const int *mask,
+               int B, hipStream_t s) {
+    const int64_t tot = (int64_t)n * ldd;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(copy_bond_kernel<T>, dim3((unsigned)((tot + 255) / 256), B), dim3(256), 0, s, S, sS, lds, tr ? 1 : 0,
+                       D, sD, ldd, n, mask);
+    YQ_HIP(hipGetLastError());
+}
+template void copy_bond<float>(const c32 *, int64_t, int, bool, c32 *, int64_t, int, int, const int *, int, hipStream_t);
+template void copy_bond<double>(const c64 *, int64_t, int, bool, c64 *, int64_t, int, int, const int *, int, hipStream_t);
+
+// Zero column `col` of the rows x ld matrix of every (unmasked) slot.
+template <typename T>
+__global__ void zero_col_kernel(cplx<T> *Y, int64_t sY, int rows, int ld, int col, const int *mask) {
+    const int t = blockIdx.y;
+    if (mask && !mask[t]) return;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < rows) st2(Y[t * sY + (int64_t)r * ld + col], make_double2(0.0, 0.0));
+}
+template <typename T>
+void zero_col(cplx<T> *Y, int64_t sY, int rows, int ld, int col, const int *mask, int B, hipStream_t s) {
+    if (rows <= 0) return;
+    hipLaunchKernelGGL(zero_col_kernel<T>, dim3((unsigned)((rows + 255) / 256), B), dim3(256), 0, s, Y, sY, rows, ld, col,
+                       mask);
+    YQ_HIP(hipGetLastError());
+}
+template void zero_col<float>(c32 *, int64_t, int, int, int, const int *, int, hipStream_t);
+template void zero_col<double>(c64 *, int64_t, int, int, int, const int *, int, hipStream_t);
+
 // Fill with zero / identity-environment patterns.
 template <typename T>
 __global__ void fill_kernel(cplx<T> *A, int64_t sA, int64_t n) {

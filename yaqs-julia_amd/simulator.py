@@ -104,14 +104,18 @@ def _all_zero(noise_model: Optional[NoiseModel]) -> bool:
 
 def run(initial_state: MPS, operator_or_circuit: Union[MPO, DigitalCircuit], sim_params,
         noise_model: Optional[NoiseModel] = None, *, precision: str = "fp64", seed: int = 0,
-        ctx: Optional[Context] = None, parallel: bool = True, alg_options: Optional[TJMOptions] = None):
+        ctx: Optional[Context] = None, parallel: bool = True, alg_options: Optional[TJMOptions] = None,
+        tdvp_sites: int = 2):
     """Simulator.run(ψ0, H_or_circuit, cfg, noise; alg_options).  Fills
     `obs.trajectories` (num_traj x num_cols, real parts stored as complex)
     and `obs.results` for every observable, as src/Simulator.jl:45-68 does.
     Returns job statistics (analog) or per-trajectory bond dimensions
     (digital, src/Simulator.jl:82-124).  Digital runs default to
     TJMOptions() (TDVP windows, src/DigitalTJM.jl:366), as the reference's
-    run_digital_tjm does."""
+    run_digital_tjm does.  Analog runs with `tdvp_sites=1` evolve with
+    single_site_tdvp! (src/Algorithms.jl:449-452) where the reference calls
+    two_site_tdvp!; the bond dimensions then stay those of the initial
+    state."""
     initial_state.normalize()  # src/Simulator.jl:128
     ctx = ctx or default_context()
     prec = _PREC[precision]
@@ -119,7 +123,7 @@ def run(initial_state: MPS, operator_or_circuit: Union[MPO, DigitalCircuit], sim
         if _all_zero(noise_model):
             sim_params.num_traj = 1  # src/Simulator.jl:27-29
         h = _abi.pack_analog(initial_state, operator_or_circuit, sim_params, noise_model, precision=prec,
-                             seed=seed)
+                             seed=seed, tdvp_sites=tdvp_sites)
         ncol = len(sim_params.times) if sim_params.sample_timesteps else 1
         for o in sim_params.observables:
             o.initialize(sim_params.num_traj, ncol)
