@@ -69,3 +69,29 @@ def test_single_site_tdvp_exact_at_full_bond(L, dt):
     ops = [E.site_op(E.Z, i, L) for i in range(1, L + 1)]
     ez = E.schrodinger(E.general(L, 1.0, 0.7, 0.4, 0.3, 0.0, 0.5), psi0, dt, steps, ops)
     assert np.abs(r[:, :, 0] - ez).max() < 1e-10
+
+
+@pytest.mark.parametrize("noise_scale", [1e-8, 1e-4])
+def test_pad_bond_dimension_then_single_site_tdvp(noise_scale):
+    """MPS(L; state, pad) (src/MPS.jl:139-141, 453-485): the padded Néel
+    state has full bonds, equals the product state to O(noise_scale^2), and
+    the 1-site sweep from it follows the exact dense evolution of the padded
+    state (entanglement grows from the padded directions; from bond
+    dimension 1 it cannot)."""
+    L = 6
+    m = Y.MPS(L, state="Neel")
+    m.pad_bond_dimension(8, noise_scale=noise_scale, rng=np.random.default_rng(0))
+    assert [t.shape[2] for t in m.tensors] == [2, 4, 8, 4, 2, 1]
+    psi0 = _dense(m)
+    psi_neel = E.product_state(L, [0, 1, 0, 1, 0, 1])
+    assert abs(abs(np.vdot(psi_neel, psi0)) - 1.0) < 10 * noise_scale ** 2
+    H = Y.init_general_hamiltonian(L, 1, 1, 1, 0, 0, 0)
+    cfg = Y.TimeEvolutionConfig([], 1.0, dt=0.1, num_traj=1, truncation_threshold=0.0)
+    ov = O.tdvp_overlap(_abi.pack_analog(m, H, cfg, None), 10, 1)
+    import scipy.linalg as sla
+    ex = np.vdot(psi0, sla.expm(-1j * E.general(L, 1, 1, 1, 0, 0, 0).toarray()) @ psi0)
+    assert abs(ov - ex) < 1e-10
+    assert abs(ex - 1.0) > 0.1  # the Néel state is not stationary
+    flat = Y.MPS(L, state="Neel")
+    ovf = O.tdvp_overlap(_abi.pack_analog(flat, H, cfg, None), 10, 1)
+    assert abs(ovf - ex) > 0.1  # bond dimension 1: mean-field dynamics

@@ -77,8 +77,53 @@ class MPS:
                     raise ValueError(f"Unknown state: {state}")
                 ts.append(v.reshape(1, pd[i], 1))
         self.tensors = ts
-        if pad is not None:
-            raise NotImplementedError("pad_bond_dimension! is not part of the 2-site TJM path")
+        if pad is not None:  # src/MPS.jl:139-141
+            self.pad_bond_dimension(pad)
+
+    def pad_bond_dimension(self, target_dim: int, noise_scale: float = 1e-8, rng=None) -> None:
+        """pad_bond_dimension! (src/MPS.jl:453-485): normalize, grow every
+        internal bond by appending columns (site i) and rows (site i+1)
+        filled with complex normal noise of amplitude `noise_scale`, then
+        normalize again.  The state changes by O(noise_scale) and every
+        Schmidt value of the padded bonds is non-zero, which the 1-site TDVP
+        sweep needs to grow entanglement.
+
+        Bond b is padded to min(target_dim, 2^min(b, L-b)) for qubits (the
+        largest rank the bond can carry): the reference pads every bond to
+        target_dim, and its final normalize! already trims the right half
+        back to that rank, while the extra left-half dimensions carry only
+        noise of rank <= 2^b; the engine sizes its buffers by that rank.
+        The noise comes from numpy's generator, not Julia's default_rng.
+        The padded Schmidt values are products of the seeds (noise_scale^2
+        and below: near 1e-16 with the default).  The engine's analog runs
+        never move the centre through the padded directions before a sweep
+        evolves them, so the 1-site sweep grows entanglement from them as in
+        the reference (tests/test_gpu_tdvp1.py: the exact evolution of a
+        padded Néel state to 1e-12).  The oracle restates the reference's
+        measurement and dissipation centre moves, whose gauge steps drop
+        directions below 1e-12 of their block as exactly rank-deficient ones
+        (LAPACK in the reference keeps them), so per-trajectory comparisons
+        of 1-site runs use states whose Schmidt values all lie above that."""
+        assert target_dim > 0
+        rng = rng or np.random.default_rng()
+        self.normalize()
+        L = self.length
+        for i in range(L - 1):
+            A, B = self.tensors[i], self.tensors[i + 1]
+            cap = min(int(np.prod(self.phys_dims[:i + 1])), int(np.prod(self.phys_dims[i + 1:])))
+            chi = A.shape[2]
+            new = min(target_dim, cap)
+            if chi >= new:
+                continue
+            nA = np.zeros((A.shape[0], A.shape[1], new), dtype=C)
+            nA[:, :, :chi] = A
+            nA[:, :, chi:] = noise_scale * (rng.standard_normal(nA[:, :, chi:].shape)
+                                            + 1j * rng.standard_normal(nA[:, :, chi:].shape))
+            nB = np.zeros((new, B.shape[1], B.shape[2]), dtype=C)
+            nB[:chi] = B
+            nB[chi:] = noise_scale * (rng.standard_normal(nB[chi:].shape) + 1j * rng.standard_normal(nB[chi:].shape))
+            self.tensors[i], self.tensors[i + 1] = nA, nB
+        self.normalize()
 
     def copy(self) -> "MPS":
         m = MPS(self.length, tensors=[t.copy() for t in self.tensors], physical_dimensions=self.phys_dims)
