@@ -284,6 +284,22 @@ typedef struct {
 int yaqs_bench_analog(yaqs_ctx *ctx, const yaqs_analog_job *job, int64_t warmup_steps, double *traj_out,
                       double *mean_out, yaqs_bench_report *report);
 
+/* measure_shots(ψ, shots) (src/MPS.jl:742-749, one single_shot_measure per
+ * shot, :683-738): `shots` bitstrings sampled site by site from the MPS
+ * (`L` tensors in the layout of yaqs_mps_tensor, physical dimension 2,
+ * L <= 63).  bits_out[t] packs shot t's outcomes with site i (1-based) in
+ * bit i-1, as single_shot_measure returns them; aggregating them into
+ * counts is the caller's (measure_shots' Dict).  Shot t draws the
+ * counter-RNG uniforms (seed, t, i-1) for site i (the reference draws
+ * Julia's global rand()).  The MPS need not be canonical or normalised: the
+ * library right-canonicalises a device copy and draws site 1 from its
+ * normalised weights (the reference moves the centre from its orth_center
+ * field to site 1, which presumes canonical tensors around that field, and
+ * draws from the raw weights, which presumes <ψ|ψ> = 1).  Synchronous;
+ * runs on the context's device. */
+int yaqs_measure_shots(yaqs_ctx *ctx, int64_t L, const yaqs_mps_tensor *mps, int64_t shots, uint64_t seed,
+                       int64_t *bits_out);
+
 /* Allocation-free variant of yaqs_run_analog.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid

@@ -54,6 +54,8 @@ def lib():
                                          ct.c_double, ct.c_int, ct.c_void_p, c_dp]
         L.oracle_tdvp_overlap.argtypes = [ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_int, ct.c_int,
                                           ct.c_int, ct.c_double, c_dp]
+        L.oracle_measure_shots.argtypes = [ct.c_int64, ct.c_void_p, ct.c_int64, ct.c_uint64,
+                                           ct.POINTER(ct.c_int64)]
         L.oracle_use_blas.argtypes = [ct.c_char_p, ct.c_char_p, ct.c_char_p]
         L.oracle_blas_bound.restype = ct.c_int
         L.oracle_use_lapack_svd.argtypes = [ct.c_char_p, ct.c_char_p]
@@ -232,6 +234,18 @@ def tdvp_overlap(holder_analog, steps, sites):
     lib().oracle_tdvp_overlap(job.L, ct.cast(job.mps, ct.c_void_p), ct.cast(job.mpo, ct.c_void_p), job.dt, steps,
                               sites, int(job.max_bond_dim), job.truncation_threshold, _ptr(out))
     return complex(out[0], out[1])
+
+
+def measure_shots_bits(mps, shots, seed):
+    """Per-shot bitstrings of single_shot_measure with the Philox draws
+    (seed, shot, site)."""
+    from yaqs_julia_amd import _abi
+    h = _abi._Holder()
+    arr = _abi._pack_mps(h, mps)
+    out = np.zeros(max(1, shots), dtype=np.int64)
+    lib().oracle_measure_shots(mps.length, ct.cast(arr, ct.c_void_p), shots, seed,
+                               out.ctypes.data_as(ct.POINTER(ct.c_int64)))
+    return out[:shots]
 
 
 def stochastic_factor(holder):

@@ -1980,6 +1980,50 @@ int oracle_prob_distribution(int64_t L, const yaqs_mps_tensor *t, int64_t n_nois
     return (int)p.size();
 }
 
+// measure_shots (src/MPS.jl:742-749): `shots` calls of single_shot_measure
+// (:683-738) on copies of ψ (taken as given, centre at site 1, as the
+// reference's shift_orthogonality_center!(psi, 1) leaves a state whose
+// orth_center field is 1): per site the slice norms of the current tensor
+// are the outcome weights, the draw is the Philox uniform (seed, shot, site),
+// the outcome the first with r <= cumulative weight (else the last), the
+// projected slice is divided by sqrt(weight) when that exceeds 1e-12 and
+// contracted into the next site.  bits: site i (1-based) in bit i-1.
+int oracle_measure_shots(int64_t L, const yaqs_mps_tensor *mps, int64_t shots, uint64_t seed, int64_t *bits) {
+    const MPSState s0 = load_mps(L, mps);
+    for (int64_t t = 0; t < shots; ++t) {
+        cvec v(1, cd(1.0, 0.0));  // the projected row carried into the next site (L = 1 after site 1)
+        int64_t b = 0;
+        for (int i = 0; i < s0.L; ++i) {
+            const T3 &A = s0.t[i];
+            const int R = A.dr, d = A.d, Lb = A.dl;
+            // current tensor row: C[p][r] = sum_a v[a] A[a, p, r]
+            cvec C((size_t)d * R, 0.0);
+            for (int p = 0; p < d; ++p)
+                for (int r = 0; r < R; ++r) {
+                    cd acc = 0;
+                    for (int a = 0; a < Lb; ++a) acc += v[a] * A.v[a + (size_t)Lb * (p + (size_t)d * r)];
+                    C[p + (size_t)d * r] = acc;
+                }
+            std::vector<double> probs(d, 0.0);
+            for (int p = 0; p < d; ++p)
+                for (int r = 0; r < R; ++r) probs[p] += std::norm(C[p + (size_t)d * r]);
+            const double rr = Philox::uniform(seed, (uint64_t)t, (uint64_t)i);
+            double accum = 0;
+            int chosen = d - 1;
+            for (int k = 0; k < d; ++k) {
+                accum += probs[k];
+                if (rr <= accum) { chosen = k; break; }
+            }
+            b += (int64_t)chosen << i;
+            const double nrm = std::sqrt(probs[chosen]);
+            v.assign(R, 0.0);
+            for (int r = 0; r < R; ++r) v[r] = C[chosen + (size_t)d * r] * (nrm > 1e-12 ? 1.0 / nrm : 1.0);
+        }
+        bits[t] = b;
+    }
+    return 0;
+}
+
 // Noise-free TDVP: `steps` sweeps of two_site_tdvp! (sites = 2) or
 // single_site_tdvp! (sites = 1) from the given state; returns
 // scalar_product(ψ(t), ψ0) = <ψ0|ψ(t)> (src/MPS.jl:528-585) in out[0..1]

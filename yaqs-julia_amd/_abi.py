@@ -275,6 +275,8 @@ def load_library(path: Optional[str] = None):
                                       ct.POINTER(BenchReport)]
     lib.yaqs_run_analog_noalloc.argtypes = [ct.c_void_p, ct.POINTER(AnalogJob), c_dp, c_dp, ct.POINTER(JobStats)]
     lib.yaqs_noalloc_scratch_bytes.restype = ct.c_int64
+    lib.yaqs_measure_shots.argtypes = [ct.c_void_p, ct.c_int64, ct.POINTER(MpsTensor), ct.c_int64, ct.c_uint64,
+                                       ct.POINTER(ct.c_int64)]
     lib.yaqs_rng_uniform.argtypes = [ct.c_uint64, ct.c_uint64, ct.c_uint64]
     lib.yaqs_rng_uniform.restype = ct.c_double
     if path is None:
@@ -286,5 +288,5 @@ EXPORTED_SYMBOLS = [
     "yaqs_abi_version", "yaqs_last_error", "yaqs_create", "yaqs_destroy", "yaqs_submit_analog",
     "yaqs_submit_digital", "yaqs_wait", "yaqs_poll", "yaqs_run_analog", "yaqs_run_digital",
     "yaqs_analog_num_samples", "yaqs_digital_num_samples", "yaqs_job_footprint_bytes", "yaqs_bench_analog",
-    "yaqs_rng_uniform", "yaqs_run_analog_noalloc", "yaqs_noalloc_scratch_bytes",
+    "yaqs_rng_uniform", "yaqs_run_analog_noalloc", "yaqs_noalloc_scratch_bytes", "yaqs_measure_shots",
 ]

@@ -51,6 +51,29 @@ class TimeEvolutionConfig:
                 obs.results = obs.trajectories.real.sum(axis=0) / self.num_traj
 
 
+class MeasurementConfig:
+    """MeasurementConfig (src/SimulationConfigs.jl:99-114): shots, bond
+    limits, per-shot measurement dicts and their aggregate."""
+
+    def __init__(self, shots: int, *, max_bond_dim: int = 4096, min_bond_dim: int = 2,
+                 truncation_threshold: float = 1e-9):
+        self.shots = int(shots)
+        self.max_bond_dim = int(max_bond_dim)
+        self.min_bond_dim = int(min_bond_dim)
+        self.truncation_threshold = float(truncation_threshold)
+        self.measurements = [None] * self.shots
+        self.results = {}
+
+    def aggregate_measurements(self) -> None:
+        """aggregate_measurements! (src/SimulationConfigs.jl:198-209)."""
+        total = {}
+        for m in self.measurements:
+            if m is not None:
+                for k, v in m.items():
+                    total[k] = total.get(k, 0) + v
+        self.results = total
+
+
 class StrongMeasurementConfig:
     """StrongMeasurementConfig (src/SimulationConfigs.jl:121-137), the
     configuration of digital runs."""

@@ -23,6 +23,7 @@
 #include "engine.h"
 #include "kernels.h"
 #include "noalloc.h"
+#include "shots.h"
 #include "tune.h"
 
 namespace {
@@ -695,6 +696,38 @@ int yaqs_run_digital(yaqs_ctx *ctx, const yaqs_digital_job *job, double *traj_ou
     int rc = yaqs_submit_digital(ctx, job, &id);
     if (rc != YAQS_OK) return rc;
     return yaqs_wait(ctx, id, traj_out, mean_out, bond_out, stats);
+}
+
+int yaqs_measure_shots(yaqs_ctx *ctx, int64_t L, const yaqs_mps_tensor *mps, int64_t shots, uint64_t seed,
+                       int64_t *bits_out) {
+    if (!ctx) return fail(YAQS_EINVAL, "ctx is NULL");
+    if (shots > 0 && !bits_out) return fail(YAQS_EINVAL, "bits_out is NULL");
+    hipStream_t s = nullptr;
+    try {
+        if (L < 1 || L > 63) throw std::invalid_argument("L must be 1..63");
+        yq::JobIn J;
+        load_mps(J, L, mps);
+        YQ_HIP(hipSetDevice(ctx->device));
+        YQ_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        yq::measure_shots((int)L, J.mps, J.mps_dl, J.mps_dr, shots, seed, bits_out, s);
+        YQ_HIP(hipStreamDestroy(s));
+        return YAQS_OK;
+    } catch (const std::invalid_argument &e) {
+        if (s) (void)hipStreamDestroy(s);
+        return fail(YAQS_EINVAL, e.what());
+    } catch (const yq::HipError &e) {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+        return fail(YAQS_EDEVICE, e.what());
+    } catch (const std::exception &e) {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+        return fail(YAQS_EDEVICE, e.what());
+    }
 }
 
 int64_t yaqs_analog_num_samples(const yaqs_analog_job *job) {

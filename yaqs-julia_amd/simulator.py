@@ -53,6 +53,15 @@ class Context:
         except Exception:
             pass
 
+    def measure_shots_bits(self, mps: MPS, shots: int, seed: int = 0) -> np.ndarray:
+        """yaqs_measure_shots: per-shot bitstrings (site i in bit i-1)."""
+        h = _abi._Holder()
+        arr = _abi._pack_mps(h, mps)
+        out = np.zeros(max(1, shots), dtype=np.int64)
+        self._check(self.lib.yaqs_measure_shots(self.handle, mps.length, arr, shots, seed,
+                                                out.ctypes.data_as(ct.POINTER(ct.c_int64))))
+        return out[:shots]
+
     # -- submission --------------------------------------------------------
     def submit_analog(self, holder) -> int:
         jid = ct.c_uint64()
@@ -150,3 +159,34 @@ def run(initial_state: MPS, operator_or_circuit: Union[MPO, DigitalCircuit], sim
             o.results = mean[q].copy()
         return [list(b) for b in bonds]
     raise TypeError("Simulation requires MPO (Analog) or DigitalCircuit (Digital).")
+
+
+def measure_shots(mps: MPS, shots: int, *, seed: int = 0, ctx: Optional[Context] = None) -> dict:
+    """measure_shots(mps, shots) (src/MPS.jl:742-749): counts of the sampled
+    bitstrings (site i in bit i-1), sampled on the device by
+    yaqs_measure_shots.  Shot t draws the counter-RNG uniforms (seed, t,
+    site); the reference draws Julia's global rand()."""
+    ctx = ctx or default_context()
+    bits = ctx.measure_shots_bits(mps, shots, seed)
+    vals, counts = np.unique(bits, return_counts=True)
+    return {int(v): int(c) for v, c in zip(vals, counts)}
+
+
+def single_shot_measure(mps: MPS, *, seed: int = 0, shot: int = 0, ctx: Optional[Context] = None) -> int:
+    """single_shot_measure(mps) (src/MPS.jl:683-738): one bitstring, the
+    shot `shot` of the stream under `seed`."""
+    ctx = ctx or default_context()
+    if shot == 0:
+        return int(ctx.measure_shots_bits(mps, 1, seed)[0])
+    return int(ctx.measure_shots_bits(mps, shot + 1, seed)[shot])
+
+
+def run_shots(mps: MPS, config, *, seed: int = 0, ctx: Optional[Context] = None) -> dict:
+    """Fills a MeasurementConfig: `config.measurements` holds one
+    {bitstring: 1} dict per shot and `config.results` their aggregate
+    (aggregate_measurements!, src/SimulationConfigs.jl:198-209)."""
+    ctx = ctx or default_context()
+    bits = ctx.measure_shots_bits(mps, config.shots, seed)
+    config.measurements = [{int(b): 1} for b in bits]
+    config.aggregate_measurements()
+    return config.results
