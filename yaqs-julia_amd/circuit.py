@@ -52,6 +52,9 @@ class DigitalCircuit:
     def __init__(self, num_qubits: int):
         self.num_qubits = int(num_qubits)
         self.gates: List[DigitalGate] = []
+        # processed layers (src/DigitalTJM.jl:41-45): filled by circuit
+        # ingestion for inspection; runs layer `gates` themselves
+        self.layers: List[List[DigitalGate]] = []
 
     def add_gate(self, op: AbstractOperator, sites, generator=None) -> None:
         """add_gate! (src/DigitalTJM.jl:64-66)."""

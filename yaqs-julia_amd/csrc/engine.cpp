@@ -653,6 +653,8 @@ class Engine {
     void matvec(State &S, bool two, int i, const CT *x, int64_t sx, CT *y, int64_t sy, const int *act);
     void matvec0(State &S, int b, const CT *x, int64_t sx, CT *y, int64_t sy, const int *act);
     void expm(State &S, int sites, int i, CT *x, int64_t sx, double dt, JobResult &res);
+    // `sites` counts sites (2, 1, 0 = bond): a bool there is a mistake
+    void expm(State &S, bool sites, int i, CT *x, int64_t sx, double dt, JobResult &res) = delete;
     void merge(State &S, int i);
     void split(State &S, int i, bool right, int rule, double thr, const int *msk = nullptr);
     void deflate_tail(State &S, int i, bool right, int kw, int mb);
@@ -2105,13 +2107,13 @@ void Engine<T>::run_digital(int64_t t0, int nb_, double *d_out, int64_t *h_bonds
                     identity_env<T>(EL[a].as<CT>(), slotE[a], PHI.wd[a], 1, 0, nb, s);
                     for (int i = a; i + 2 <= b; ++i) {
                         merge(PHI, i);
-                        expm(PHI, true, i, theta.as<CT>(), nmax, 1.0, res);
+                        expm(PHI, 2, i, theta.as<CT>(), nmax, 1.0, res);
                         split(PHI, i, true, 0, J.thr);
                         env_left(PHI, i);
-                        expm(PHI, false, i + 1, A(PHI, i + 1), slotA[i + 1], -1.0, res);
+                        expm(PHI, 1, i + 1, A(PHI, i + 1), slotA[i + 1], -1.0, res);
                     }
                     merge(PHI, b - 1);
-                    expm(PHI, true, b - 1, theta.as<CT>(), nmax, 1.0, res);
+                    expm(PHI, 2, b - 1, theta.as<CT>(), nmax, 1.0, res);
                     split(PHI, b - 1, true, 0, J.thr);
                     centre = b;
                     swap_tables();  // ownership only: the launched kernels keep their table pointers
