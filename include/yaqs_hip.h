@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define YAQS_ABI_VERSION 3
+#define YAQS_ABI_VERSION 4
 
 /* Return codes.  Mirrors the reference's failure modes: Julia `error(...)` /
  * `@assert` on bad input (src/Simulator.jl:136,139), plus device failures. */
@@ -61,7 +61,8 @@ typedef enum {
     YAQS_NOISE_PAULI_X = 2,
     YAQS_NOISE_PAULI_Y = 3,
     YAQS_NOISE_PAULI_Z = 4,
-    YAQS_NOISE_CUSTOM = 5    /* matrix given explicitly (1 or 2 adjacent sites) */
+    YAQS_NOISE_CUSTOM = 5,   /* matrix given explicitly (1 or 2 adjacent sites) */
+    YAQS_NOISE_LONG_RANGE = 6 /* MPONoiseProcess on two non-adjacent sites (analog jobs) */
 } yaqs_noise_kind;
 
 typedef struct {
@@ -77,7 +78,20 @@ typedef struct {
 /* One LocalNoiseProcess (src/Noise.jl:15-20).  `matrix` holds the 2x2
  * (first 8 doubles) or 4x4 (32 doubles) jump operator, column-major.
  * `pauli_name` is the name-based Pauli test of src/Dissipation.jl:14-21 and
- * src/StochasticProcess.jl:337-353 (names pauli_* / crosstalk_*). */
+ * src/StochasticProcess.jl:337-353 (names pauli_* / crosstalk_*).
+ *
+ * YAQS_NOISE_LONG_RANGE: one MPONoiseProcess (src/Noise.jl:22-33) on sites
+ * site0 < site1 - 1, whose MPO is build_mpo_phys's O = α I + β σ_site0 τ_site1
+ * (src/Noise.jl:77-128): matrix[0..7] = σ, matrix[8..15] = τ (2x2,
+ * column-major), matrix[16..17] = α, matrix[18..19] = β.  Dissipation
+ * scales site1 by exp(-dt/2 γ) (handle_longrange_scalar!,
+ * src/Dissipation.jl:161-204).  `pauli_name` = 1 marks the candidate the
+ * reference tags "pauli_long_range" (is_pauli_proc with factors,
+ * src/StochasticProcess.jl:168-181): weight dt γ ‖ψ‖², and its jump applies
+ * no operator (only the final normalize!, src/StochasticProcess.jl:317-330);
+ * 0 the general MPO jump: weight dt γ ‖normalize!(truncate!(Oψ))‖², jump
+ * apply_mpo_jump! (src/StochasticProcess.jl:46-82) = O applied, truncate!
+ * (src/MPS.jl:320-441), normalize!.  Digital jobs reject this kind. */
 typedef struct {
     int32_t kind;
     int32_t nsites;          /* 1 or 2 (2 = adjacent pair)          */

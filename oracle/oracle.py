@@ -49,6 +49,9 @@ def lib():
         L.oracle_prob_distribution.argtypes = [ct.c_int64, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_double,
                                                c_dp, ct.POINTER(ct.c_int)]
         L.oracle_set_window.argtypes = [ct.c_int]
+        L.oracle_mpo_jump.argtypes = [ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_int, ct.c_int,
+                                      ct.POINTER(ct.c_int64), c_dp, ct.c_int64]
+        L.oracle_mpo_jump.restype = ct.c_int64
         L.oracle_last_window_seconds.restype = ct.c_double
         L.oracle_tdvp_evolve.argtypes = [ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_int, ct.c_int,
                                          ct.c_double, ct.c_int, ct.c_void_p, c_dp]
@@ -251,6 +254,28 @@ def measure_shots_bits(mps, shots, seed):
 def stochastic_factor(holder):
     job = holder.job
     return lib().oracle_stochastic_factor(job.L, ct.cast(job.mps, ct.c_void_p))
+
+
+def mpo_jump(holder, proc_index, thr, max_bond, mode=0):
+    """apply_mpo_jump! of noise process `proc_index` (mode 0) or MPS.truncate!
+    (mode 1) on the holder's MPS (centre field at L); returns the site
+    tensors (dl, d, dr) as numpy arrays."""
+    job = holder.job
+    L = job.L
+    dims = np.zeros(3 * L, dtype=np.int64)
+    cap = 1 << 22
+    data = np.zeros(cap)
+    procp = ct.cast(ct.byref(job.noise[proc_index]), ct.c_void_p) if mode == 0 else None
+    n = lib().oracle_mpo_jump(L, ct.cast(job.mps, ct.c_void_p), procp, thr, max_bond, mode,
+                              dims.ctypes.data_as(ct.POINTER(ct.c_int64)), _ptr(data), cap)
+    assert n >= 0
+    out, off = [], 0
+    for i in range(L):
+        dl, d, dr = (int(x) for x in dims[3 * i:3 * i + 3])
+        m = dl * d * dr
+        out.append(from_cm(data[off:off + 2 * m].copy(), (dl, d, dr)))
+        off += 2 * m
+    return out
 
 
 def prob_distribution(holder, dt):

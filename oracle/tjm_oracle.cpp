@@ -104,6 +104,10 @@ zgesdd64_fn g_zgesdd = nullptr;
 // the rounding-decided completion moves results (oracle_set_drop_dead).
 bool g_drop_dead = true;
 thread_local bool tl_keep_all = false;  // set during measurements (see expect)
+// sim_params.truncation_threshold / max_bond_dim of the running trajectory,
+// read by the weights of long-range MPO jumps (src/StochasticProcess.jl:63-79)
+thread_local double tl_thr = 1e-12;
+thread_local int tl_max_bond = 1 << 30;
 
 void gemm(char ta, char tb, int m, int n, int k, const cd *A, int lda, const cd *B, int ldb,
           cd *C, int ldc, bool accumulate = false) {
@@ -191,8 +195,9 @@ cd dotc(const cd *a, const cd *b, size_t n) {  // Σ conj(a)·b (Julia `dot`)
 // returns is decided by that noise (and so is the 2-site TDVP tangent space
 // that follows).  This restatement drops such a direction instead: a zero
 // column of Q and a zero row of R (later columns are orthogonalised against
-// the live ones only), the same state with the bond at its numerical rank
-// -- the rule the GPU engine's gauge steps follow.  With g_drop_dead off it
+// the live ones only; the next live column takes its slot), the same state
+// with the bond at its numerical rank -- the rule the GPU engine's gauge
+// steps follow.  With g_drop_dead off it
 // runs Householder reflectors as LAPACK does (used to measure how far the
 // noise-decided completion moves results).
 void qr_householder(const cvec &A, int m, int n, cvec &Q, cvec &R, int &r) {
@@ -256,21 +261,25 @@ void qr_thin(const cvec &A, int m, int n, cvec &Q, cvec &R, int &r) {
     const double dead = 1e-12 * fro;
     Q.assign((size_t)m * r, 0.0);
     R.assign((size_t)r * n, 0.0);
+    // a dead column takes no column of Q: the next live one does (with r
+    // slots for r = min(m, n) columns, a live direction after a dead column
+    // would otherwise find no slot); unused slots stay zero
+    int slot = 0;
     for (int j = 0; j < n; ++j) {
         cvec v(A.begin() + (size_t)m * j, A.begin() + (size_t)m * (j + 1));
-        const int kmax = std::min(j, r);  // earlier columns of Q
         for (int pass = 0; pass < 2; ++pass)
-            for (int k = 0; k < kmax; ++k) {
+            for (int k = 0; k < slot; ++k) {
                 const cd *q = &Q[(size_t)m * k];
                 const cd h = dotc(q, v.data(), m);
                 R[k + (size_t)r * j] += h;
                 for (int i = 0; i < m; ++i) v[i] -= h * q[i];
             }
-        if (j < r) {
+        if (slot < r) {
             const double nr = nrm2(v);
             if (nr > dead && nr > 0.0) {
-                R[j + (size_t)r * j] = nr;
-                for (int i = 0; i < m; ++i) Q[i + (size_t)m * j] = v[i] / nr;
+                R[slot + (size_t)r * j] = nr;
+                for (int i = 0; i < m; ++i) Q[i + (size_t)m * slot] = v[i] / nr;
+                ++slot;
             }
         }
     }
@@ -1026,6 +1035,11 @@ struct Proc {
     double gamma;
     bool pauli_name;  // name-based test, src/Dissipation.jl:14-21
     cd m[16];         // 2x2 (first 4) or 4x4, column-major
+    // long-range process (MPONoiseProcess, src/Noise.jl:22-33) whose MPO is
+    // build_mpo_phys's O = alpha I + beta sig_{s0} tau_{s1} (src/Noise.jl:
+    // 77-128); pauli_name then marks the "pauli_long_range" candidate
+    bool lr = false;
+    cd sig[4], tau[4], alpha = 0.0, beta = 0.0;
 };
 
 bool is_pauli_matrix(const Proc &p) {  // src/StochasticProcess.jl:337-342
@@ -1141,7 +1155,13 @@ void apply_dissipation(MPSState &s, const std::vector<Proc> *noise, double dt, d
         if (i > 1) {
             for (const Proc &p : *noise) {
                 if (p.nsites != 2 || std::max(p.s0, p.s1) != i) continue;
-                if (p.pauli_name) {
+                if (p.lr) {
+                    // handle_longrange_scalar! (src/Dissipation.jl:161-204):
+                    // exp(-dt/2 γ) per process (a projector pair's one factor
+                    // exp(-dt/2 (γ+ + γ-)) is the product of its two)
+                    double f = std::exp(-0.5 * dt * p.gamma);
+                    for (auto &x : s.t[i - 1].v) x *= f;
+                } else if (p.pauli_name) {
                     double f = std::exp(-0.5 * dt * p.gamma);
                     for (auto &x : s.t[i - 1].v) x *= f;
                 } else {
@@ -1159,9 +1179,105 @@ void apply_dissipation(MPSState &s, const std::vector<Proc> *noise, double dt, d
     }
 }
 
+// MPS.truncate! (src/MPS.jl:320-441): the centre to site 1, then one SVD
+// sweep 1 -> L-1: Θ = A_i A_{i+1} scaled by 1/max|Θ| for the SVD (singular
+// values scaled back), kept rank by the absolute rule (discarded weight
+// >= threshold, at least 2) capped at max_bond, A_i = U, A_{i+1} = S V^†;
+// then normalize!.  Returns the discarded weight.
+double truncate_mps(MPSState &s, double thr, int max_bond) {
+    shift_oc(s, 1);
+    double err = 0;
+    for (int i = 1; i <= s.L - 1; ++i) {
+        const T3 &A = s.t[i - 1], &B = s.t[i];
+        const int l = A.dl, d1 = A.d, d2 = B.d, r = B.dr;
+        cvec th = merge2(A, B);  // (l d1) x (d2 r), column-major
+        double scale = 0;
+        for (auto &x : th) scale = std::max(scale, std::abs(x));
+        if (!std::isfinite(scale)) throw std::runtime_error("truncate!: non-finite entries encountered in Theta");
+        if (scale != 0)
+            for (auto &x : th) x *= 1.0 / scale;
+        cvec U, Vt;
+        std::vector<double> S;
+        svd_thin(th, l * d1, d2 * r, U, S, Vt);
+        const int ns = (int)S.size();
+        if (scale != 0)
+            for (auto &x : S) x *= scale;
+        double disc = 0;
+        int keep = ns;
+        for (int k = ns; k >= 1; --k) {
+            disc += S[k - 1] * S[k - 1];
+            if (disc >= thr) { keep = std::max(k, 2); break; }
+        }
+        keep = std::min(keep, max_bond);
+        for (int k = keep; k < ns; ++k) err += S[k] * S[k];
+        T3 nA(l, d1, keep), nB(keep, d2, r);
+        for (size_t q = 0; q < (size_t)l * d1 * keep; ++q) nA.v[q] = U[q];
+        for (int j = 0; j < d2 * r; ++j)
+            for (int q = 0; q < keep; ++q) nB.v[q + (size_t)keep * j] = S[q] * Vt[q + (size_t)ns * j];
+        s.t[i - 1] = std::move(nA);
+        s.t[i] = std::move(nB);
+    }
+    s.oc = s.L;
+    normalize_mps(s);
+    return err;
+}
+
+// build_mpo_phys's tensor at site k (1-based) of O = alpha I + beta sig_a
+// tau_b (src/Noise.jl:77-128), (Left, Out, In, Right) column-major.
+T4 lr_mpo_site(const Proc &p, int k) {
+    const int a = std::min(p.s0, p.s1), b = std::max(p.s0, p.s1);
+    const cd I2[4] = {1.0, 0.0, 0.0, 1.0};
+    auto put = [](T4 &W, int l, int r, const cd *op, cd f) {
+        for (int o = 0; o < 2; ++o)
+            for (int in = 0; in < 2; ++in) W.v[l + W.dl * (o + 2 * (in + 2 * r))] = f * op[o + 2 * in];
+    };
+    T4 W;
+    if (k < a || k > b) {
+        W = T4{1, 2, 2, 1, cvec(4)};
+        put(W, 0, 0, I2, 1.0);
+    } else if (k == a) {
+        W = T4{1, 2, 2, 2, cvec(8)};
+        put(W, 0, 0, I2, 1.0);
+        put(W, 0, 1, p.sig, 1.0);
+    } else if (k < b) {
+        W = T4{2, 2, 2, 2, cvec(16)};
+        put(W, 0, 0, I2, 1.0);
+        put(W, 1, 1, I2, 1.0);
+    } else {
+        W = T4{2, 2, 2, 1, cvec(8)};
+        put(W, 0, 0, I2, p.alpha);
+        put(W, 1, 0, p.tau, p.beta);
+    }
+    return W;
+}
+
+// apply_mpo_jump! (src/StochasticProcess.jl:46-82): every site contracted
+// with its MPO tensor over the physical index, C[la, lw, po, ra, rw] =
+// Σ_pi A[la, pi, ra] W[lw, po, pi, rw] reshaped (la·lw, po, ra·rw) (la and
+// ra fastest), then truncate! with the job's threshold and bond cap.
+void apply_mpo_jump(MPSState &s, const Proc &p, double thr, int max_bond) {
+    for (int k = 1; k <= s.L; ++k) {
+        T3 &A = s.t[k - 1];
+        const T4 W = lr_mpo_site(p, k);
+        T3 C(A.dl * W.dl, W.po, A.dr * W.dr);
+        for (int rw = 0; rw < W.dr; ++rw)
+            for (int ra = 0; ra < A.dr; ++ra)
+                for (int po = 0; po < W.po; ++po)
+                    for (int lw = 0; lw < W.dl; ++lw)
+                        for (int pi = 0; pi < W.pi; ++pi) {
+                            const cd w = W.v[lw + W.dl * (po + W.po * (pi + W.pi * rw))];
+                            if (w == 0.0) continue;
+                            for (int la = 0; la < A.dl; ++la)
+                                C.at(la + A.dl * lw, po, ra + A.dr * rw) += A.at(la, pi, ra) * w;
+                        }
+        s.t[k - 1] = std::move(C);
+    }
+    truncate_mps(s, thr, max_bond);
+}
+
 struct Candidate {
     int proc;
-    int type;  // 1 = local_1, 2 = local_2
+    int type;  // 1 = local_1, 2 = local_2, 3 = pauli_long_range, 4 = mpo_general
     int site;
 };
 
@@ -1217,6 +1333,25 @@ void prob_distribution(MPSState &s, const std::vector<Proc> &noise, double dt, s
             }
         }
     }
+    // long-range candidates in process order (src/StochasticProcess.jl:
+    // 168-198), the centre at L: a "pauli_long_range" one weighs the centre
+    // norm; a general one the norm of a copy after apply_mpo_jump!
+    for (size_t q = 0; q < noise.size(); ++q) {
+        const Proc &p = noise[q];
+        if (!p.lr) continue;
+        double nrm = 0;
+        if (p.pauli_name) {
+            for (auto &x : s.t[s.oc - 1].v) nrm += std::norm(x);
+            dps.push_back(dt * p.gamma * nrm);
+            cands.push_back({(int)q, 3, 0});
+        } else {
+            MPSState tmp = s;
+            apply_mpo_jump(tmp, p, tl_thr, tl_max_bond);
+            for (auto &x : tmp.t[tmp.oc - 1].v) nrm += std::norm(x);
+            dps.push_back(dt * p.gamma * nrm);
+            cands.push_back({(int)q, 4, 0});
+        }
+    }
     double tot = 0;
     for (double x : dps) tot += x;
     if (tot == 0) { cands.clear(); return; }
@@ -1227,6 +1362,8 @@ void prob_distribution(MPSState &s, const std::vector<Proc> &noise, double dt, s
 // jump was applied.
 int stochastic_process(MPSState &s, const std::vector<Proc> &noise, double dt, double thr, int max_bond,
                        TrajRng &rng) {
+    tl_thr = thr;
+    tl_max_bond = max_bond;
     double dp = 1.0 - scalar_product(s, s).real();
     if (noise.empty() || rng.next() >= dp) {
         shift_oc(s, 1);
@@ -1256,7 +1393,12 @@ int stochastic_process(MPSState &s, const std::vector<Proc> &noise, double dt, d
         for (double x : probs) fprintf(stderr, " %.15g", x);
         fprintf(stderr, "\n");
     }
-    if (c.type == 1) {
+    if (c.type == 3) {
+        // "pauli_long_range": no branch of the reference applies it
+        // (src/StochasticProcess.jl:255-327); only normalize! follows
+    } else if (c.type == 4) {
+        apply_mpo_jump(s, p, thr, max_bond);
+    } else if (c.type == 1) {
         shift_oc(s, c.site);
         apply_1site_op(s.t[c.site - 1], p.m);
     } else {
@@ -1320,6 +1462,16 @@ std::vector<Proc> load_noise(int64_t n, const yaqs_noise_process *p) {
         q.s1 = (int)(p[i].nsites == 2 ? p[i].site1 : p[i].site0);
         q.gamma = p[i].strength;
         q.pauli_name = p[i].pauli_name != 0;
+        if (p[i].kind == YAQS_NOISE_LONG_RANGE) {
+            q.lr = true;
+            load_c(p[i].matrix, q.sig, 4);
+            load_c(p[i].matrix + 8, q.tau, 4);
+            q.alpha = cd(p[i].matrix[16], p[i].matrix[17]);
+            q.beta = cd(p[i].matrix[18], p[i].matrix[19]);
+            for (auto &z : q.m) z = 0.0;
+            out.push_back(q);
+            continue;
+        }
         int d = q.nsites == 1 ? 2 : 4;
         load_c(p[i].matrix, q.m, (size_t)d * d);
         out.push_back(q);
@@ -1978,6 +2130,34 @@ int oracle_prob_distribution(int64_t L, const yaqs_mps_tensor *t, int64_t n_nois
         cand[3 * i + 2] = c[i].proc;
     }
     return (int)p.size();
+}
+
+// apply_mpo_jump! of one long-range process (mode 0) or MPS.truncate! alone
+// (mode 1, `proc` unused) on a copy of the MPS; writes (dl, d, dr) per site
+// to dims and the tensors (interleaved, column-major, one after the other)
+// to data.  Returns the doubles written, or -1 when `cap` is too small.
+int64_t oracle_mpo_jump(int64_t L, const yaqs_mps_tensor *mps, const yaqs_noise_process *proc, double thr,
+                        int max_bond, int mode, int64_t *dims, double *data, int64_t cap) {
+    MPSState s = load_mps(L, mps);
+    s.oc = (int)L;  // the centre field create_probability_distribution leaves
+    if (mode == 0) {
+        std::vector<Proc> p = load_noise(1, proc);
+        apply_mpo_jump(s, p[0], thr, max_bond);
+    } else {
+        truncate_mps(s, thr, max_bond);
+    }
+    int64_t n = 0;
+    for (int i = 0; i < s.L; ++i) {
+        dims[3 * i] = s.t[i].dl;
+        dims[3 * i + 1] = s.t[i].d;
+        dims[3 * i + 2] = s.t[i].dr;
+        if (n + 2 * (int64_t)s.t[i].v.size() > cap) return -1;
+        for (const cd &z : s.t[i].v) {
+            data[n++] = z.real();
+            data[n++] = z.imag();
+        }
+    }
+    return n;
 }
 
 // measure_shots (src/MPS.jl:742-749): `shots` calls of single_shot_measure

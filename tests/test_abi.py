@@ -40,7 +40,7 @@ def test_struct_layouts():
 
 def test_abi_version_and_rng_contract(oracle_lib):
     lib = _lib()
-    assert lib.yaqs_abi_version() == 3
+    assert lib.yaqs_abi_version() == 4
     for seed, t, k in [(0, 0, 0), (7, 3, 5), (2 ** 40 + 3, 999, 12), (123, 2 ** 33, 2 ** 35)]:
         assert lib.yaqs_rng_uniform(seed, t, k) == oracle_lib.rng_uniform(seed, t, k)
 

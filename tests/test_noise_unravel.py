@@ -46,13 +46,13 @@ def test_reference_unitary_2pt_case():
 
 
 def test_standard_and_long_range():
-    """test/test_Noise.jl:11-22; long-range (MPO) processes are not part of
-    the batched path and raise."""
+    """test/test_Noise.jl:11-22; a long-range process becomes an MPO process
+    (tests/test_oracle_longrange.py covers them)."""
     m = Y.NoiseModel([{"name": "pauli_x", "sites": [1], "strength": 0.1}], 2)
     assert len(m.processes) == 1 and m.processes[0].name == "pauli_x" and m.processes[0].pauli_name
     assert np.allclose(m.processes[0].matrix, X)
-    with pytest.raises(NotImplementedError):
-        Y.NoiseModel([{"name": "crosstalk_xy", "sites": [1, 3], "strength": 0.05, "unraveling": "projector"}], 4)
+    lr = Y.NoiseModel([{"name": "crosstalk_xy", "sites": [1, 3], "strength": 0.05, "unraveling": "projector"}], 4)
+    assert all(hasattr(p, "mpo") for p in lr.processes)
     with pytest.raises(ValueError):
         Y.NoiseModel([{"name": "pauli_x", "sites": [], "strength": 0.1}], 2)
 

@@ -60,23 +60,30 @@ def test_qr_lq_against_lapack(oracle_lib):
 def test_qr_lq_dead_directions(oracle_lib):
     """Exactly dependent columns (rows): LAPACK completes Q from a reflector
     of rounding noise (numpy's Q column differs under a 1e-15 change of A),
-    the oracle drops the direction: zero column of Q and zero row of R, the
-    live columns orthonormal, Q R = A.  The same rule as the GPU engine's
-    gauge steps (a residual below 1e-12 of ||A||_F is dead)."""
+    the oracle drops the direction: the next live column takes its slot, the
+    unused last slot is a zero column of Q and a zero row of R, the live
+    columns orthonormal, Q R = A (a residual below 1e-12 of ||A||_F is
+    dead).  With more columns than rows a live direction after a dead column
+    still finds a slot (Q R = A for a rank-4 4 x 6 matrix whose second column
+    repeats the first)."""
     a = crand(10, 3)
     A = np.c_[a[:, :2], a[:, 0] * (0.3 - 0.2j) + a[:, 1] * 0.7, a[:, 2:]]  # column 2 dependent
     Q, R = oracle_lib.qr(A)
-    assert np.abs(Q[:, 2]).max() == 0 and np.abs(R[2]).max() == 0
-    live = [0, 1, 3]
-    assert np.allclose(Q[:, live].conj().T @ Q[:, live], np.eye(3), atol=1e-12)
+    assert np.abs(Q[:, 3]).max() == 0 and np.abs(R[3]).max() == 0
+    assert np.allclose(Q[:, :3].conj().T @ Q[:, :3], np.eye(3), atol=1e-12)
     assert np.allclose(Q @ R, A, atol=1e-12)
     # LAPACK's completion of the dead column follows the rounding noise
     Qn0, _ = np.linalg.qr(A)
     Qn1, _ = np.linalg.qr(A + 1e-15 * crand(10, 4))
     assert np.abs(np.abs(np.vdot(Qn0[:, 2], Qn1[:, 2])) - 1) > 1e-6
     Lm, Q2 = oracle_lib.lq(A.conj().T)
-    assert np.abs(Q2[2]).max() == 0 and np.abs(Lm[:, 2]).max() == 0
+    assert np.abs(Q2[3]).max() == 0 and np.abs(Lm[:, 3]).max() == 0
     assert np.allclose(Lm @ Q2, A.conj().T, atol=1e-12)
+    b = crand(4, 5)
+    B = np.c_[b[:, :1], 2 * b[:, :1], b[:, 1:]]  # rank 4, dead column 1
+    Q, R = oracle_lib.qr(B)
+    assert np.allclose(Q @ R, B, atol=1e-12)
+    assert np.allclose(Q.conj().T @ Q, np.eye(4), atol=1e-12)
 
 
 def test_tridiagonal_exponential(oracle_lib):

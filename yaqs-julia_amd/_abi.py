@@ -21,6 +21,7 @@ GATE_1Q, GATE_2Q, GATE_SAMPLE = 1, 2, 3
 GATE_EXACT, GATE_WINDOW_TDVP = 0, 1  # yaqs_gate_method
 NOISE_KIND = {"lowering": 0, "raising": 1, "pauli_x": 2, "pauli_y": 3, "pauli_z": 4}
 NOISE_CUSTOM = 5
+NOISE_LONG_RANGE = 6
 
 c_dp = ct.POINTER(ct.c_double)
 
@@ -126,6 +127,19 @@ def _pack_noise(h: _Holder, noise_model):
     arr = (NoiseProcess * max(1, len(procs)))()
     for i, p in enumerate(procs):
         q = arr[i]
+        if hasattr(p, "mpo"):
+            # MPONoiseProcess: O = a I + b sigma tau as (sigma, tau, a, b)
+            p.check_dissipation_name()
+            q.kind = NOISE_LONG_RANGE
+            q.nsites = 2
+            q.site0, q.site1 = min(p.sites), max(p.sites)
+            q.strength = p.strength
+            q.pauli_name = 1 if p.pauli_long_range() else 0
+            vals = list(_cm_interleaved(p.sigma)) + list(_cm_interleaved(p.tau))
+            vals += [complex(p.a).real, complex(p.a).imag, complex(p.b).real, complex(p.b).imag]
+            for k, v in enumerate(vals):
+                q.matrix[k] = v
+            continue
         q.kind = NOISE_KIND.get(p.name, NOISE_CUSTOM) if len(p.sites) == 1 else NOISE_CUSTOM
         q.nsites = len(p.sites)
         q.site0 = p.sites[0]

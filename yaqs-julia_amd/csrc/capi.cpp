@@ -122,6 +122,19 @@ void load_noise(yq::JobIn &J, int32_t has, int64_t n, const yaqs_noise_process *
         q.pauli_name = p[i].pauli_name != 0;
         const int d = q.nsites == 1 ? 2 : 4;
         for (int k = 0; k < 16; ++k) q.m[k] = mkc(0, 0);
+        if (p[i].kind == YAQS_NOISE_LONG_RANGE) {
+            if (q.nsites != 2 || q.s1 < q.s0 + 2)
+                throw std::invalid_argument("a long-range noise process needs sites site0 < site1 - 1");
+            q.lr = true;
+            for (int k = 0; k < 4; ++k) {
+                q.sig[k] = mkc(p[i].matrix[2 * k], p[i].matrix[2 * k + 1]);
+                q.tau[k] = mkc(p[i].matrix[8 + 2 * k], p[i].matrix[8 + 2 * k + 1]);
+            }
+            q.alpha = mkc(p[i].matrix[16], p[i].matrix[17]);
+            q.beta = mkc(p[i].matrix[18], p[i].matrix[19]);
+            J.noise.push_back(q);
+            continue;
+        }
         for (int k = 0; k < d * d; ++k) q.m[k] = mkc(p[i].matrix[2 * k], p[i].matrix[2 * k + 1]);
         J.noise.push_back(q);
     }

@@ -594,6 +594,14 @@ class Engine {
     DevBuf rho1b;
     DevBuf cand_site, cand_op, cand_site2, cand_op2, cand_LdL, cand_gamma, jump_ops;
     int ncand = 0;
+    // general long-range candidates (apply_mpo_jump!): candidate index and
+    // process; lr_ops: their sig, tau, sig^†sig, tau^†tau; lrnu[t][cand]:
+    // ν of the weight dt γ ν; lr_val: two correlators per trajectory
+    struct LrCand {
+        int cand = 0, proc = 0;
+    };
+    std::vector<LrCand> lr_cands;
+    DevBuf lr_ops, lrnu, lr_val;
     DevBuf obs_site, obs_n, obs_op;
     bool any_obs2 = false;
     std::vector<char> obs2_at;
@@ -656,7 +664,7 @@ class Engine {
     // `sites` counts sites (2, 1, 0 = bond): a bool there is a mistake
     void expm(State &S, bool sites, int i, CT *x, int64_t sx, double dt, JobResult &res) = delete;
     void merge(State &S, int i);
-    void split(State &S, int i, bool right, int rule, double thr, const int *msk = nullptr);
+    void split(State &S, int i, bool right, int rule, double thr, const int *msk = nullptr, int max_keep = 0);
     void deflate_tail(State &S, int i, bool right, int kw, int mb);
     void env_left(State &S, int i);
     void env_right(State &S, int i);
@@ -668,12 +676,27 @@ class Engine {
     void dissipate_sweep(State &S, bool half, int field);
     void lq_step(State &S, int i, const int *msk, bool keep_factor = true);
     void regrow(State &S, int b, int w);
+    void shrink_to_dims(State &S);
     void transfer(State &S, const int *msk, c64 *rho_out, c64 *rho2_out = nullptr,
                   const std::vector<char> *at2 = nullptr);
     void measure(State &S, int col, int ncol, double *d_out);
     void stochastic(State &S, double dt, const DevBuf *invL, JobResult &res);
     void absorb_left(State &S, int i, const int *msk);
+    void correlate(State &S, int a, int b, const c64 *opa, const c64 *opb, const int *msk, c64 *out);
+    void mpo_jump(State &S, int li, const int *msk);
 };
+
+// A general long-range jump (apply_mpo_jump!) doubles the bonds between its
+// two sites before truncate! cuts them back: those bonds get twice the
+// capacity (once, whatever the number of processes over them).
+static void lr_double_caps(const JobIn &job, std::vector<int> &cap) {
+    std::vector<char> dbl(cap.size(), 0);
+    for (auto &p : job.noise)
+        if (p.lr && !p.pauli_name)
+            for (int k = std::min(p.s0, p.s1) + 1; k <= std::max(p.s0, p.s1); ++k) dbl[k] = 1;
+    for (size_t k = 0; k < cap.size(); ++k)
+        if (dbl[k]) cap[k] *= 2;
+}
 
 template <typename T>
 int64_t Engine<T>::footprint(const JobIn &job, int B) {
@@ -686,6 +709,7 @@ int64_t Engine<T>::footprint(const JobIn &job, int B) {
         cap[b] = (int)std::min<int64_t>(c, job.max_bond);
         cap[b] = std::max(cap[b], std::max(job.mps_dr[b - 1], job.mps_dl[b]));
     }
+    lr_double_caps(job, cap);
     int Dmax = 1;
     if (!job.digital)
         for (int i = 0; i < L; ++i) Dmax = std::max(Dmax, std::max(job.mpo_dl[i], job.mpo_dr[i]));
@@ -731,6 +755,7 @@ void Engine<T>::setup() {
     }
     cap[0] = J.mps_dl[0];
     cap[L] = J.mps_dr[L - 1];
+    lr_double_caps(J, cap);
     Dm.assign(L + 1, 1);
     Dm[0] = J.mpo.empty() ? 1 : J.mpo_dl[0];
     for (int i = 0; i < L; ++i) Dm[i + 1] = J.mpo.empty() ? 1 : J.mpo_dr[i];
@@ -891,8 +916,10 @@ void Engine<T>::setup() {
                     if (act) (*act)[i] = 1;
                 }
             }
-            for (auto &p : J.noise) {  // 2-site Pauli processes scale site max(s0, s1)
-                if (p.nsites == 2 && std::max(p.s0, p.s1) == i && p.pauli_name) {
+            // 2-site Pauli processes and long-range processes (handle_longrange_scalar!,
+            // src/Dissipation.jl:161-204) scale site max(s0, s1)
+            for (auto &p : J.noise) {
+                if (p.nsites == 2 && std::max(p.s0, p.s1) == i && (p.pauli_name || p.lr)) {
                     const double f = std::exp(-0.5 * dt * p.gamma);
                     for (int k = 0; k < 4; ++k) ops[i * 4 + k] = mk(ops[i * 4 + k].re * f, ops[i * 4 + k].im * f);
                     if (act) (*act)[i] = 1;
@@ -924,7 +951,7 @@ void Engine<T>::setup() {
         int n4 = 0;
         if (!J.digital && J.has_noise && !allzero)
             for (auto &p : J.noise) {
-                if (p.nsites != 2 || p.pauli_name) continue;
+                if (p.nsites != 2 || p.pauli_name || p.lr) continue;
                 c64 e4[16], M[16];
                 op4_engine(p.m, e4);
                 damp4(e4, 0.5 * (J.dt / 2) * p.gamma, M);
@@ -1025,7 +1052,33 @@ void Engine<T>::setup() {
                 if (J.noise[q].nsites == 1 && J.noise[q].s0 == site) add(q, site, false);
             for (size_t q = 0; q < J.noise.size(); ++q) {
                 const ProcIn &p = J.noise[q];
-                if (p.nsites == 2 && std::min(p.s0, p.s1) == site && p.s0 == site && site + 1 < L) add(q, site, true);
+                if (p.nsites == 2 && !p.lr && std::min(p.s0, p.s1) == site && p.s0 == site && site + 1 < L)
+                    add(q, site, true);
+            }
+        }
+        // long-range candidates after the site loop, in process order
+        // (src/StochasticProcess.jl:168-198): "pauli_long_range" weighs
+        // dt γ ‖ψ‖² (Tr ρ of the last site, whose ρ is the true state's) and
+        // applies no operator; a general one weighs dt γ ν with ν = 1 unless
+        // O annihilates the state (lr_nu, per jump step), kind 2
+        for (size_t q = 0; q < J.noise.size(); ++q) {
+            const ProcIn &p = J.noise[q];
+            if (!p.lr) continue;
+            cs.push_back(L - 1);
+            co.push_back(-1);
+            cs2.push_back(-1);
+            co2.push_back(-1);
+            cpair.push_back(-1);
+            ckind.push_back(p.pauli_name ? 0 : 2);
+            for (int k = 0; k < 16; ++k) ldl4.push_back(mk(0, 0));
+            const c64 I2[4] = {mk(1, 0), mk(0, 0), mk(0, 0), mk(1, 0)};
+            ldl.insert(ldl.end(), I2, I2 + 4);
+            cg.push_back(p.gamma);
+            if (!p.pauli_name) {
+                LrCand lc;
+                lc.cand = (int)cs.size() - 1;
+                lc.proc = (int)q;
+                lr_cands.push_back(lc);
             }
         }
         ncand = (int)cs.size();
@@ -1050,6 +1103,29 @@ void Engine<T>::setup() {
         upload(cand_LdL, ldl, s);
         upload(cand_gamma, cg, s);
         wbuf.alloc((size_t)B * std::max(1, ncand) * sizeof(double));
+        if (!lr_cands.empty()) {
+            // per general long-range candidate: sig, tau, sig^†sig, tau^†tau (engine order)
+            std::vector<c64> lops;
+            for (auto &lc : lr_cands) {
+                const ProcIn &p = J.noise[lc.proc];
+                c64 e[4], f[4];
+                op2_engine(p.sig, e);
+                op2_engine(p.tau, f);
+                lops.insert(lops.end(), e, e + 4);
+                lops.insert(lops.end(), f, f + 4);
+                for (const c64 *o : {e, f}) {  // o^† o, row-major
+                    c64 g[4];
+                    for (int i = 0; i < 2; ++i)
+                        for (int j = 0; j < 2; ++j)
+                            g[i * 2 + j] = fromc(std::conj(toc(o[0 * 2 + i])) * toc(o[0 * 2 + j]) +
+                                                 std::conj(toc(o[1 * 2 + i])) * toc(o[1 * 2 + j]));
+                    lops.insert(lops.end(), g, g + 4);
+                }
+            }
+            upload(lr_ops, lops, s);
+            lrnu.alloc((size_t)B * ncand * sizeof(double));
+            lr_val.alloc((size_t)2 * B * sizeof(c64));
+        }
         if (J.digital) {
             // per adjacent pair (s1, s1+1): processes on s1, pair processes, processes on s2
             for (int s1 = 0; s1 + 1 < L; ++s1) {
@@ -1383,7 +1459,7 @@ void Engine<T>::merge(State &S, int i) {
 // the FP64 Gram matrix and a Jacobi eigensolver.  right = centre moves right
 // (A_i = U, A_{i+1} = U^† Θ); else A_{i+1} = V^†, A_i = Θ V.
 template <typename T>
-void Engine<T>::split(State &S, int i, bool right, int rule, double thr, const int *msk) {
+void Engine<T>::split(State &S, int i, bool right, int rule, double thr, const int *msk, int max_keep) {
     const int cl = S.wd[i], cr = S.wd[i + 2];
     const int w_old = S.wd[i + 1];
     const int m = 2 * cl, n = 2 * cr;
@@ -1402,7 +1478,7 @@ void Engine<T>::split(State &S, int i, bool right, int rule, double thr, const i
                                    theta.as<CT>(), G.as<c64>(), s);
     guard_gram(G.as<c64>(), slotG, gn, nfail.as<int>(), msk, nb, s);
     if (msk) fill_int(keep.as<int>(), 0, nb, s);  // masked-out slots report rank 0
-    const int mb = (int)std::min<int64_t>(J.max_bond, 1 << 30);
+    const int mb = max_keep > 0 ? max_keep : (int)std::min<int64_t>(J.max_bond, 1 << 30);
     double2 *Pg = Pbuf.p ? Pbuf.as<double2>() : nullptr;
     int *fb = Hfb.p ? Hfb.as<int>() : nullptr;
     const bool multi = heev(G.as<c64>(), slotG, U.as<c64>(), slotG, Zbuf.as<double>(), slotZ, Pg, slotP, Hwork.p, slotH,
@@ -1738,6 +1814,27 @@ void Engine<T>::regrow(State &S, int b, int w) {
     S.wd[b] = w;
 }
 
+// The working dimension of every bond back down to the largest true
+// dimension of the wave's trajectories (S.dims): A_{b-1} re-laid out with the
+// smaller row length, A_b keeps its leading rows (the dropped columns and
+// rows are zero padding in every slot).
+template <typename T>
+void Engine<T>::shrink_to_dims(State &S) {
+    std::vector<int> hd((size_t)nb * (L + 1));
+    YQ_HIP(hipMemcpyAsync(hd.data(), S.dims.p, hd.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    YQ_HIP(hipStreamSynchronize(s));
+    for (int b = 1; b < L; ++b) {
+        int w = 1;
+        for (int t = 0; t < nb; ++t) w = std::max(w, hd[(size_t)t * (L + 1) + b]);
+        const int wo = S.wd[b];
+        if (w >= wo) continue;
+        const int r0 = 2 * S.wd[b - 1];
+        copy2d<T, T>(A(S, b - 1), slotA[b - 1], wo, absorb.as<CT>(), slotA[b - 1], w, r0, w, nullptr, nb, s);
+        copy2d<T, T>(absorb.as<CT>(), slotA[b - 1], w, A(S, b - 1), slotA[b - 1], w, r0, w, nullptr, nb, s);
+        S.wd[b] = w;
+    }
+}
+
 // Left transfer sweep in FP64 giving every site's reduced matrix ρ_s (and
 // 4x4 ρ for 2-site observables) of a right-canonical state.
 template <typename T>
@@ -1768,6 +1865,68 @@ void Engine<T>::transfer(State &S, const int *msk, c64 *rho_out, c64 *rho2_out, 
                                         Xbuf.as<c64>(), E1, s);
         std::swap(E0, E1);
     }
+}
+
+// <ψ| opa_a opb_b |ψ> per masked trajectory (2x2 operators in engine
+// order on the ket side), by a left transfer sweep in FP64 over the whole
+// chain, which needs no canonical form (the last site closes a bond of
+// dimension 1); out[t] for the masked slots.
+template <typename T>
+void Engine<T>::correlate(State &S, int a, int b, const c64 *opa, const c64 *opb, const int *msk, c64 *out) {
+    c64 *E0 = Ebuf0.as<c64>(), *E1 = Ebuf1.as<c64>();
+    const int64_t sE = (int64_t)capmax * capmax;
+    identity_env<double>(E0, sE, S.wd[0], 1, 0, nb, s);
+    for (int i = 0; i < L; ++i) {
+        const int cl = S.wd[i], cr = S.wd[i + 1];
+        gemm<double, T, double, double>(gd(cl, 2 * cr, cl, OP_N, OP_N, cl, 2 * cr, 2 * cr, sE, slotA[i], slotX, nb, msk),
+                                        E0, A(S, i), Xbuf.as<c64>(), s);
+        if (i == a) local_op<double>(Xbuf.as<c64>(), slotX, cl, cr, opa, nullptr, msk, nb, s);
+        if (i == b) local_op<double>(Xbuf.as<c64>(), slotX, cl, cr, opb, nullptr, msk, nb, s);
+        gemm<T, double, double, double>(gd(cr, cr, 2 * cl, OP_C, OP_N, cr, cr, cr, slotA[i], slotX, sE, nb, msk), A(S, i),
+                                        Xbuf.as<c64>(), E1, s);
+        std::swap(E0, E1);
+    }
+    copy2d<double, double>(E0, sE, 1, out, 1, 1, 1, 1, msk, nb, s);
+}
+
+// apply_mpo_jump! (src/StochasticProcess.jl:46-82) for the masked
+// trajectories, long-range candidate li (O = α I + β σ_a τ_b):
+// 1. the bonds a+1..b double (regrow) and the site tensors take the blocks of
+//    build_mpo_phys's contraction (src/Noise.jl:77-128): site a [A, σA],
+//    sites between diag(A, A), site b [α A; β τA];
+// 2. truncate! (src/MPS.jl:320-441): its centre shift to site 1, an LQ
+//    sweep (CGS2 over every row: a bond keeps its doubled working dimension,
+//    dependent rows become zero rows), then its left-to-right sweep with the
+//    T-A rule, the job's threshold and bond cap (U on the left site, S V^† on
+//    the right), then normalize!.
+template <typename T>
+void Engine<T>::mpo_jump(State &S, int li, const int *msk) {
+    const ProcIn &p = J.noise[lr_cands[li].proc];
+    const int a = std::min(p.s0, p.s1), b = std::max(p.s0, p.s1);
+    bool fits = true;
+    for (int k = a + 1; k <= b; ++k) fits = fits && 2 * S.wd[k] <= cap[k];
+    if (!fits) shrink_to_dims(S);  // an earlier jump of this step left doubled working dimensions
+    const std::vector<int> w0 = S.wd;
+    for (int k = a + 1; k <= b; ++k) {
+        if (2 * w0[k] > cap[k]) throw std::runtime_error("long-range jump: doubled bond exceeds its capacity");
+        regrow(S, k, 2 * w0[k]);
+    }
+    const c64 *ops = lr_ops.as<c64>() + (int64_t)li * 16;
+    const cd al = toc(p.alpha), be = toc(p.beta);
+    mpo_block<T>(A(S, a), slotA[a], S.wd[a + 1], 0, S.wd[a], w0[a + 1], ops, 0, 0, 0, 0, msk, nb, s);
+    for (int k = a + 1; k < b; ++k)
+        mpo_block<T>(A(S, k), slotA[k], S.wd[k + 1], 1, w0[k], w0[k + 1], ops, 0, 0, 0, 0, msk, nb, s);
+    mpo_block<T>(A(S, b), slotA[b], S.wd[b + 1], 2, w0[b], S.wd[b + 1], ops + 4, al.real(), al.imag(), be.real(),
+                 be.imag(), msk, nb, s);
+    lq_sweep(S, L - 1, msk);
+    for (int k = 0; k + 1 < L; ++k) {
+        merge(S, k);
+        split(S, k, true, 0, J.thr, msk);
+    }
+    lq_sweep(S, L - 1, msk);
+    norm2<T>(A(S, 0), slotA[0], (int64_t)S.wd[0] * 2 * S.wd[1], norm2v.as<double>(), msk, nb, s);
+    jump_decide(norm2v.as<double>(), 0, 0, rngk.as<uint64_t>(), 0, opidx.as<int>(), fac.as<double>(), nb, s);
+    scale<T>(A(S, 0), slotA[0], (int64_t)S.wd[0] * 2 * S.wd[1], fac.as<double>(), msk, nb, s);
 }
 
 template <typename T>
@@ -1828,9 +1987,21 @@ void Engine<T>::stochastic(State &S, double dt, const DevBuf *invL, JobResult &r
         copy2d<double, double>(rho1b.as<c64>(), (int64_t)L * 4, 4 * (L - 1), rho1.as<c64>(), (int64_t)L * 4,
                                4 * (L - 1), 1, 4 * (L - 1), mask.as<int>(), nb, s);
     }
+    // general long-range candidates: ν from two correlators of the true
+    // (unnormalised) state, which S holds for the jumping trajectories
+    for (size_t li = 0; li < lr_cands.size(); ++li) {
+        const ProcIn &p = J.noise[lr_cands[li].proc];
+        const int a = std::min(p.s0, p.s1), b = std::max(p.s0, p.s1);
+        const c64 *ops = lr_ops.as<c64>() + (int64_t)li * 16;
+        correlate(S, a, b, ops, ops + 4, mask.as<int>(), lr_val.as<c64>());
+        correlate(S, a, b, ops + 8, ops + 12, mask.as<int>(), lr_val.as<c64>() + nb);
+        lr_nu(lr_val.as<c64>(), rho1.as<c64>(), (int64_t)L * 4, L, p.alpha.re, p.alpha.im, p.beta.re, p.beta.im,
+              lr_cands[li].cand, ncand, jstate.as<int>(), lrnu.as<double>(), nb, s);
+    }
     jump_choose(rho1.as<c64>(), (int64_t)L * 4, L, ncand, cand_site.as<int>(), cand_LdL.as<c64>(),
                 cand_gamma.as<double>(), dt, J.seed, traj0, rngk.as<uint64_t>(), jstate.as<int>(), chosen.as<int>(),
-                wbuf.as<double>(), nb, s, rho2.as<c64>(), (int64_t)L * 16, cand_LdL4.as<c64>(), cand_kind.as<int>());
+                wbuf.as<double>(), nb, s, rho2.as<c64>(), (int64_t)L * 16, cand_LdL4.as<c64>(), cand_kind.as<int>(),
+                lrnu.p ? lrnu.as<double>() : nullptr);
     if (std::getenv("YAQS_DEBUG_JUMPS")) {
         std::vector<int> hs(nb), hc(nb);
         std::vector<uint64_t> hk(nb);
@@ -1886,6 +2057,20 @@ void Engine<T>::stochastic(State &S, double dt, const DevBuf *invL, JobResult &r
                                mask2.as<int>(), nullptr, nb, s);
                 qr_step(S, p, mask2.as<int>());
             }
+        }
+    }
+    // long-range MPO jumps (apply_mpo_jump!), one candidate at a time
+    if (!lr_cands.empty()) {
+        std::vector<int> hs(nb), hc(nb);
+        YQ_HIP(hipMemcpyAsync(hs.data(), jstate.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        YQ_HIP(hipMemcpyAsync(hc.data(), chosen.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        YQ_HIP(hipStreamSynchronize(s));
+        for (size_t li = 0; li < lr_cands.size(); ++li) {
+            bool any = false;
+            for (int t = 0; t < nb && !any; ++t) any = hs[t] == 1 && hc[t] == lr_cands[li].cand;
+            if (!any) continue;
+            mask_chosen(jstate.as<int>(), chosen.as<int>(), lr_cands[li].cand, mask2.as<int>(), nb, s);
+            mpo_jump(S, (int)li, mask2.as<int>());
         }
     }
     fill_int(count.as<int>(), 0, 1, s);
@@ -2321,6 +2506,12 @@ void validate_job(const JobIn &job) {
     if (job.max_bond > 512) throw std::invalid_argument("max_bond_dim > 512 is not supported by the GPU path");
     for (auto &p : job.noise) {
         if (p.s0 < 0 || p.s0 >= job.L || p.s1 < 0 || p.s1 >= job.L) throw std::invalid_argument("noise site out of range");
+        if (p.lr) {
+            if (job.digital) throw std::invalid_argument("long-range noise processes are not supported by the digital GPU path");
+            if (job.max_bond > 256)
+                throw std::invalid_argument("long-range noise processes need max_bond_dim <= 256 (the jump doubles bonds)");
+            continue;
+        }
         if (p.nsites == 2) {
             if (std::abs(p.s1 - p.s0) != 1) throw std::invalid_argument("long-range noise processes are not supported");
             if (!p.pauli_name && job.digital)

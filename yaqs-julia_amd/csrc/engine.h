@@ -20,6 +20,11 @@ struct ProcIn {
     double gamma;
     bool pauli_name;  // name-based Pauli test (src/Dissipation.jl:14-21)
     c64 m[16];        // reference order: column-major, 4x4 on p1 + 2 p2
+    // long-range process (s0 < s1 - 1): O = alpha I + beta sig_s0 tau_s1,
+    // sig/tau in reference order; pauli_name then marks the candidate whose
+    // jump applies no operator ("pauli_long_range")
+    bool lr = false;
+    c64 sig[4], tau[4], alpha, beta;
 };
 
 struct ObsIn {

@@ -134,10 +134,30 @@ void rho(const c64 *X, int64_t sX, const cplx<T> *A, int64_t sA, int dl, int d, 
 
 void jump_decide(const double *norm2v, uint64_t seed, uint64_t traj0, uint64_t *rng_k, int has_procs, int *state,
                  double *fac, int B, hipStream_t s);
+// cand_kind[c]: 0 = Tr(L†L ρ1) at cand_site, 1 = Tr(L†L ρ2) of the pair,
+// 2 = the per-trajectory factor lrnu[t * ncand + c] (long-range MPO jumps)
 void jump_choose(const c64 *rhov, int64_t sR, int L, int ncand, const int *cand_site, const c64 *cand_LdL,
                  const double *cand_gamma, double dt, uint64_t seed, uint64_t traj0, uint64_t *rng_k, int *state,
                  int *chosen, double *wbuf, int B, hipStream_t s, const c64 *rho2 = nullptr, int64_t sR2 = 0,
-                 const c64 *cand_LdL4 = nullptr, const int *cand_kind = nullptr);
+                 const c64 *cand_LdL4 = nullptr, const int *cand_kind = nullptr, const double *lrnu = nullptr);
+// mask[t] = (state[t] == 1 && chosen[t] == c)
+void mask_chosen(const int *state, const int *chosen, int c, int *mask, int B, hipStream_t s);
+// ν of a general long-range candidate c for the jumping trajectories
+// (state 1): with n = Tr ρ of the last site (rho, stride sR, L sites),
+// cs = <σ_a τ_b> (val[t]) and cn = <σ†σ_a τ†τ_b> (val[B + t]),
+// ‖Oψ‖² = |α|² n + |β|² cn + 2 Re(conj(α) β cs); ν = 1 unless that is at
+// most 1e-12 (|α|² n + |β|² cn) (O annihilates ψ), then 0.
+void lr_nu(const c64 *val, const c64 *rho, int64_t sR, int L, double ar, double ai, double br, double bi, int c,
+           int ncand, const int *state, double *lrnu, int B, hipStream_t s);
+// Blocks of O ψ for O = α I + β σ_a τ_b on the bond-doubled layout (site
+// tensor [l][s][r] with ncol columns, its new layout; masked slots):
+// mode 0 (site a): columns R + r <- σ applied to columns r < R, rows l < nl;
+// mode 1 (between): rows nl + l, columns R + r <- rows l, columns r (l < nl, r < R);
+// mode 2 (site b): rows nl + l <- β τ applied to rows l, rows l <- α rows l
+// (l < nl, r < R).  op: σ or τ, 2x2 engine order.
+template <typename T>
+void mpo_block(cplx<T> *A, int64_t sA, int ncol, int mode, int nl, int R, const c64 *op, double ar, double ai,
+               double br, double bi, const int *mask, int B, hipStream_t s);
 // Masks of trajectories whose chosen jump is a merged-pair candidate on pair
 // (p, p+1) (mode 0, opidx = its operator's index or -1) or beyond p (mode 1).
 void jump_pair_mask(const int *state, const int *chosen, const int *cand_site, const int *cand_pair, int p, int mode,

@@ -1253,12 +1253,17 @@ __global__ void jump_decide_kernel(const double *norm2, uint64_t seed, uint64_t 
 __global__ void jump_choose_kernel(const c64 *rho, int64_t sR, int L, int ncand, const int *cand_site,
                                    const c64 *cand_LdL, const double *cand_gamma, double dt, uint64_t seed, uint64_t traj0,
                                    uint64_t *rng_k, int *state, int *chosen, double *wbuf, int B, const c64 *rho2,
-                                   int64_t sR2, const c64 *cand_LdL4, const int *cand_kind) {
+                                   int64_t sR2, const c64 *cand_LdL4, const int *cand_kind, const double *lrnu) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= B || state[t] != 1) return;
     double tot = 0;
     double *w = wbuf + (int64_t)t * ncand;
     for (int c = 0; c < ncand; ++c) {
+        if (cand_kind && cand_kind[c] == 2) {  // long-range MPO jump: dt γ ν
+            w[c] = dt * cand_gamma[c] * lrnu[(int64_t)t * ncand + c];
+            tot += w[c];
+            continue;
+        }
         // Tr(M ρ) = Σ_{s,s'} M[s'][s] ρ[s][s'], M = L†L: 2x2 on the site's ρ, or
         // 4x4 on the pair's ρ2 (merged-pair weight, src/StochasticProcess.jl:148-160)
         const bool pair = cand_kind && cand_kind[c] == 1;
@@ -1299,11 +1304,88 @@ void jump_decide(const double *norm2v, uint64_t seed, uint64_t traj0, uint64_t *
 void jump_choose(const c64 *rhov, int64_t sR, int L, int ncand, const int *cand_site, const c64 *cand_LdL,
                  const double *cand_gamma, double dt, uint64_t seed, uint64_t traj0, uint64_t *rng_k, int *state,
                  int *chosen, double *wbuf, int B, hipStream_t s, const c64 *rho2, int64_t sR2, const c64 *cand_LdL4,
-                 const int *cand_kind) {
+                 const int *cand_kind, const double *lrnu) {
     hipLaunchKernelGGL(jump_choose_kernel, dim3((B + 255) / 256), dim3(256), 0, s, rhov, sR, L, ncand, cand_site, cand_LdL,
-                       cand_gamma, dt, seed, traj0, rng_k, state, chosen, wbuf, B, rho2, sR2, cand_LdL4, cand_kind);
+                       cand_gamma, dt, seed, traj0, rng_k, state, chosen, wbuf, B, rho2, sR2, cand_LdL4, cand_kind, lrnu);
     YQ_HIP(hipGetLastError());
 }
+
+__global__ void mask_chosen_kernel(const int *state, const int *chosen, int c, int *mask, int B) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < B) mask[t] = (state[t] == 1 && chosen[t] == c) ? 1 : 0;
+}
+void mask_chosen(const int *state, const int *chosen, int c, int *mask, int B, hipStream_t s) {
+    hipLaunchKernelGGL(mask_chosen_kernel, dim3((B + 255) / 256), dim3(256), 0, s, state, chosen, c, mask, B);
+    YQ_HIP(hipGetLastError());
+}
+
+__global__ void lr_nu_kernel(const c64 *val, const c64 *rho, int64_t sR, int L, double ar, double ai, double br,
+                             double bi, int c, int ncand, const int *state, double *lrnu, int B) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B || state[t] != 1) return;
+    const c64 *r = rho + t * sR + (int64_t)(L - 1) * 4;
+    const double n = r[0].re + r[3].re;
+    const c64 cs = val[t];
+    const double cn = val[B + t].re;
+    const double a2 = ar * ar + ai * ai, b2 = br * br + bi * bi;
+    // conj(α) β
+    const double xr = ar * br + ai * bi, xi = ar * bi - ai * br;
+    const double base = a2 * n + b2 * cn;
+    const double on2 = base + 2.0 * (xr * cs.re - xi * cs.im);
+    lrnu[(int64_t)t * ncand + c] = on2 > 1e-12 * base ? 1.0 : 0.0;
+}
+void lr_nu(const c64 *val, const c64 *rho, int64_t sR, int L, double ar, double ai, double br, double bi, int c,
+           int ncand, const int *state, double *lrnu, int B, hipStream_t s) {
+    hipLaunchKernelGGL(lr_nu_kernel, dim3((B + 255) / 256), dim3(256), 0, s, val, rho, sR, L, ar, ai, br, bi, c, ncand,
+                       state, lrnu, B);
+    YQ_HIP(hipGetLastError());
+}
+
+template <typename T>
+__global__ void mpo_block_kernel(cplx<T> *A, int64_t sA, int ncol, int mode, int nl, int R, const c64 *op, double ar,
+                                 double ai, double br, double bi, const int *mask) {
+    const int t = blockIdx.y;
+    if (mask && !mask[t]) return;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nl * R) return;
+    const int l = (int)(e / R), r = (int)(e % R);
+    cplx<T> *base = A + t * sA;
+    const int64_t ls = ncol;  // columns of the site tensor (one physical slice of a row)
+    cplx<T> *p0 = base + ((int64_t)l * 2 + 0) * ls + r;
+    cplx<T> *p1 = base + ((int64_t)l * 2 + 1) * ls + r;
+    const double2 x0 = ld2(*p0), x1 = ld2(*p1);
+    if (mode == 1) {
+        st2(base[((int64_t)(nl + l) * 2 + 0) * ls + R + r], x0);
+        st2(base[((int64_t)(nl + l) * 2 + 1) * ls + R + r], x1);
+        return;
+    }
+    const double2 m00 = make_double2(op[0].re, op[0].im), m01 = make_double2(op[1].re, op[1].im);
+    const double2 m10 = make_double2(op[2].re, op[2].im), m11 = make_double2(op[3].re, op[3].im);
+    double2 y0 = cadd(cmul(m00, x0), cmul(m01, x1)), y1 = cadd(cmul(m10, x0), cmul(m11, x1));
+    if (mode == 0) {
+        st2(p0[R], y0);
+        st2(p1[R], y1);
+        return;
+    }
+    const double2 be = make_double2(br, bi), al = make_double2(ar, ai);
+    st2(base[((int64_t)(nl + l) * 2 + 0) * ls + r], cmul(be, y0));
+    st2(base[((int64_t)(nl + l) * 2 + 1) * ls + r], cmul(be, y1));
+    st2(*p0, cmul(al, x0));
+    st2(*p1, cmul(al, x1));
+}
+template <typename T>
+void mpo_block(cplx<T> *A, int64_t sA, int ncol, int mode, int nl, int R, const c64 *op, double ar, double ai,
+               double br, double bi, const int *mask, int B, hipStream_t s) {
+    const int64_t tot = (int64_t)nl * R;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(mpo_block_kernel<T>, dim3((unsigned)((tot + 255) / 256), B), dim3(256), 0, s, A, sA, ncol, mode, nl,
+                       R, op, ar, ai, br, bi, mask);
+    YQ_HIP(hipGetLastError());
+}
+template void mpo_block<float>(c32 *, int64_t, int, int, int, int, const c64 *, double, double, double, double,
+                               const int *, int, hipStream_t);
+template void mpo_block<double>(c64 *, int64_t, int, int, int, int, const c64 *, double, double, double, double,
+                                const int *, int, hipStream_t);
 
 // Per-site op index for the chosen jumps: opidx[t] = op of candidate
 // chosen[t] if it acts on `site` (first or second factor), else -1.
