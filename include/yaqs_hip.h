@@ -62,7 +62,7 @@ typedef enum {
     YAQS_NOISE_PAULI_Y = 3,
     YAQS_NOISE_PAULI_Z = 4,
     YAQS_NOISE_CUSTOM = 5,   /* matrix given explicitly (1 or 2 adjacent sites) */
-    YAQS_NOISE_LONG_RANGE = 6 /* MPONoiseProcess on two non-adjacent sites (analog jobs) */
+    YAQS_NOISE_LONG_RANGE = 6 /* MPONoiseProcess on two non-adjacent sites                */
 } yaqs_noise_kind;
 
 typedef struct {
@@ -91,7 +91,8 @@ typedef struct {
  * no operator (only the final normalize!, src/StochasticProcess.jl:317-330);
  * 0 the general MPO jump: weight dt γ ‖normalize!(truncate!(Oψ))‖², jump
  * apply_mpo_jump! (src/StochasticProcess.jl:46-82) = O applied, truncate!
- * (src/MPS.jl:320-441), normalize!.  Digital jobs reject this kind. */
+ * (src/MPS.jl:320-441), normalize!.  In digital jobs it acts after a gate on
+ * exactly its two sites (create_local_noise_model, src/DigitalTJM.jl:172-188). */
 typedef struct {
     int32_t kind;
     int32_t nsites;          /* 1 or 2 (2 = adjacent pair)          */
@@ -271,6 +272,15 @@ int64_t yaqs_digital_num_samples(const yaqs_digital_job *job);
 
 /* Projected device footprint of a job in bytes (the admission model). */
 int64_t yaqs_job_footprint_bytes(const yaqs_analog_job *job);
+
+/* Check of that model (no counterpart in the reference; for capacity
+ * planning and tests): builds the engine of one wave of `batch` trajectories
+ * of `job` on `device`, reads the free device memory (hipMemGetInfo) before
+ * and after, releases it, and returns the model's bytes for that wave
+ * (*model) and the memory the engine took (*measured).  Meaningful only
+ * while nothing else allocates on the device. */
+int yaqs_engine_device_bytes(int device, const yaqs_analog_job *job, int64_t batch, int64_t *model,
+                             int64_t *measured);
 
 /* Measurement entry point (no counterpart in the reference, whose timing is
  * stdout-only, experiments/benchmark_julia.jl:66-69): runs an analog job

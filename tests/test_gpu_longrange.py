@@ -13,9 +13,11 @@ trajectory within 1e-10 with equal jump counts for the standard crosstalk
 ("pauli_long_range": scalar dissipation, weight dt gamma |psi|^2, no operator
 applied), its projector, two-point and Gaussian unitary unravelings (MPO
 jumps: O applied on doubled bonds, truncate!, normalize!); the mixed
-precision ensemble within 1e-4 ... 2e-2 of the FP64 ensemble on the same
-seeds (it follows the same jumps unless a draw falls within rounding of a
-boundary).
+precision ensemble within 2e-2 of the FP64 ensemble on the same seeds (it
+follows the same jumps unless a draw falls within rounding of a boundary).
+Digital circuits with a long-range gate on (L, 1) and a crosstalk on that
+pair (the reference's paper experiments), its gates as SWAP networks or
+TDVP windows: FP64 per trajectory within 1e-10, jump counts equal.
 """
 import numpy as np
 import pytest
@@ -100,3 +102,55 @@ def test_longrange_mixed_ensemble(ctx, unravel):
     err = float(np.abs(m64 - m32).max())
     print(f"  max|mixed-fp64| ensemble = {err:.3e}", flush=True)
     assert err < 2e-2
+
+
+def _lr_circuit(depth, seed):
+    """Brickwork of Haar gates plus, per layer, a long-range gate on (L, 1)
+    (the periodic bond of the reference's paper experiments,
+    01_PaperExps/experiments_file.jl:540-552) and single-qubit rotations."""
+    rng = np.random.default_rng(seed)
+    c = Y.DigitalCircuit(L)
+    for layer in range(depth):
+        for s in range(1 + layer % 2, L, 2):
+            c.add_gate(Y.gates.MatrixGate(Y.gates.haar_su4(rng)), [s, s + 1])
+        c.add_gate(Y.gates.RzzGate(rng.uniform(-1, 1)), [L, 1])
+        c.add_gate(Y.gates.RxGate(rng.uniform(-1, 1)), [int(rng.integers(1, L + 1))])
+    c.add_sample_barrier()
+    return c
+
+
+def _digital_noise(unravel):
+    procs = [{"name": n, "sites": [i], "strength": 0.01} for i in range(1, L + 1) for n in ("pauli_x", "pauli_z")]
+    d = {"name": "crosstalk_zz", "sites": [L, 1], "strength": 0.08}
+    if unravel != "standard":
+        d["unraveling"] = unravel
+    if unravel == "unitary_2pt":
+        d["theta0"] = np.pi / 3
+    procs.append(d)
+    return Y.NoiseModel(procs, L)
+
+
+@pytest.mark.parametrize("method", ["TEBD", "TDVP"])
+@pytest.mark.parametrize("unravel", ["standard", "projector", "unitary_2pt", "unitary_gauss"])
+def test_longrange_digital_fp64_parity(ctx, oracle_lib, unravel, method):
+    """Digital circuits with a long-range gate on (L, 1) and a long-range
+    crosstalk on the same pair (applied after that gate only,
+    create_local_noise_model, src/DigitalTJM.jl:172-188): FP64 per
+    trajectory against the oracle, jump counts equal."""
+    print(f"case: digital long-range noise {unravel} gates {method}", flush=True)
+    ntraj = 16
+    circ = _lr_circuit(4, 5)
+    noise = _digital_noise(unravel)
+    obs = [Y.Observable(f"z{s}", Y.gates.ZGate(), s) for s in range(1, L + 1)] + \
+          [Y.Observable(f"x{s}", Y.gates.XGate(), s) for s in range(1, L + 1)]
+    cfg = Y.StrongMeasurementConfig(obs, num_traj=ntraj, max_bond_dim=8, truncation_threshold=1e-10)
+    opts = Y.TJMOptions("TEBD", method)  # the Haar gates have no generator
+    hg = _abi.pack_digital(_generic_mps(), circ, cfg, noise, precision=_abi.PREC_FP64, seed=3, alg_options=opts)
+    ncol = int(ctx.lib.yaqs_digital_num_samples(hg.job))
+    g, _, _, st = ctx.wait(ctx.submit_digital(hg), len(obs), ncol, ntraj)
+    hr = _abi.pack_digital(_generic_mps(), circ, cfg, noise, seed=3, alg_options=opts)
+    r, _, _, rst = oracle_lib.run_digital(hr, nthreads=8)
+    err = float(np.abs(g - r).max())
+    print(f"  jumps gpu {st['jumps']} oracle {rst['jumps']}  max|gpu-oracle| = {err:.3e}", flush=True)
+    assert st["jumps"] == rst["jumps"] and st["jumps"] > 0
+    assert err < 1e-10

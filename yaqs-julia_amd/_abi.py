@@ -285,6 +285,8 @@ def load_library(path: Optional[str] = None):
     lib.yaqs_digital_num_samples.restype = ct.c_int64
     lib.yaqs_job_footprint_bytes.argtypes = [ct.POINTER(AnalogJob)]
     lib.yaqs_job_footprint_bytes.restype = ct.c_int64
+    lib.yaqs_engine_device_bytes.argtypes = [ct.c_int, ct.POINTER(AnalogJob), ct.c_int64, ct.POINTER(ct.c_int64),
+                                             ct.POINTER(ct.c_int64)]
     lib.yaqs_bench_analog.argtypes = [ct.c_void_p, ct.POINTER(AnalogJob), ct.c_int64, c_dp, c_dp,
                                       ct.POINTER(BenchReport)]
     lib.yaqs_run_analog_noalloc.argtypes = [ct.c_void_p, ct.POINTER(AnalogJob), c_dp, c_dp, ct.POINTER(JobStats)]
@@ -303,4 +305,5 @@ EXPORTED_SYMBOLS = [
     "yaqs_submit_digital", "yaqs_wait", "yaqs_poll", "yaqs_run_analog", "yaqs_run_digital",
     "yaqs_analog_num_samples", "yaqs_digital_num_samples", "yaqs_job_footprint_bytes", "yaqs_bench_analog",
     "yaqs_rng_uniform", "yaqs_run_analog_noalloc", "yaqs_noalloc_scratch_bytes", "yaqs_measure_shots",
+    "yaqs_engine_device_bytes",
 ]

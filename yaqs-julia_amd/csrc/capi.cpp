@@ -767,6 +767,42 @@ int64_t yaqs_job_footprint_bytes(const yaqs_analog_job *job) {
     }
 }
 
+int yaqs_engine_device_bytes(int device, const yaqs_analog_job *job, int64_t batch, int64_t *model,
+                             int64_t *measured) {
+    if (!model || !measured) return fail(YAQS_EINVAL, "model / measured is NULL");
+    hipStream_t st = nullptr;
+    try {
+        yq::JobIn in = convert_analog(job);
+        yq::validate_job(in);
+        batch = std::max<int64_t>(1, batch);
+        if (hipSetDevice(device) != hipSuccess) return fail(YAQS_EDEVICE, "hipSetDevice failed");
+        YQ_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        size_t f0 = 0, f1 = 0, tot = 0;
+        YQ_HIP(hipDeviceSynchronize());
+        YQ_HIP(hipMemGetInfo(&f0, &tot));
+        {
+            std::unique_ptr<yq::WaveEngine> eng = yq::make_engine(in, (int)batch, st, nullptr);
+            YQ_HIP(hipDeviceSynchronize());
+            YQ_HIP(hipMemGetInfo(&f1, &tot));
+        }
+        YQ_HIP(hipStreamSynchronize(st));
+        YQ_HIP(hipStreamDestroy(st));
+        st = nullptr;
+        *model = yq::footprint_bytes(in, batch);
+        *measured = (int64_t)f0 - (int64_t)f1;
+        return YAQS_OK;
+    } catch (const std::bad_alloc &e) {
+        if (st) (void)hipStreamDestroy(st);
+        return fail(YAQS_ENOMEM_DEVICE, e.what());
+    } catch (const std::invalid_argument &e) {
+        if (st) (void)hipStreamDestroy(st);
+        return fail(YAQS_EINVAL, e.what());
+    } catch (const std::exception &e) {
+        if (st) (void)hipStreamDestroy(st);
+        return fail(YAQS_EDEVICE, e.what());
+    }
+}
+
 double yaqs_rng_uniform(uint64_t seed, uint64_t traj, uint64_t k) { return yq::rng_uniform_host(seed, traj, k); }
 
 int64_t yaqs_noalloc_scratch_bytes(void) { return (int64_t)yq::kNoAllocScratchBytes; }

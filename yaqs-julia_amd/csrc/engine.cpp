@@ -505,6 +505,12 @@ struct State {
 struct PairCand {  // jump candidates of the local noise model of a gate pair
     DevBuf site, op, site2, op2, LdL, gamma;
     int n = 0;
+    // long-range processes on a long-range gate's pair: candidate kinds (0,
+    // or 2 for a general MPO jump), the general ones as (candidate, index
+    // into lr_cands), and the product of their dissipation scalars (dt = 1)
+    DevBuf kind, scal;
+    std::vector<std::pair<int, int>> lr;
+    bool has_scal = false;
 };
 
 template <typename T>
@@ -984,6 +990,8 @@ void Engine<T>::setup() {
             jops.insert(jops.end(), e, e + 4);
         } else if (!J.digital) {
             // analog: a merged-pair jump (src/StochasticProcess.jl:266-309)
+        } else if (p.lr) {
+            // long-range: no operator ("pauli_long_range") or an MPO jump (mpo_jump)
         } else {
             c64 e4[16], Af[4], Bf[4];
             op4_engine(p.m, e4);
@@ -1238,6 +1246,34 @@ void Engine<T>::setup() {
                     for (size_t q = 0; q < J.noise.size(); ++q)
                         if (J.noise[q].nsites == 1 && J.noise[q].s0 == site) add(q, site, false);
                 PairCand &pc = lr_cand[la * L + lb];
+                // long-range processes on exactly this pair (create_local_noise_model,
+                // src/DigitalTJM.jl:172-188), after the 1-site ones as in the reference's
+                // candidate list: weight on ρ_lb (the true state's norm) for
+                // "pauli_long_range", dt γ ν for a general MPO jump
+                std::vector<int> kinds(cs.size(), 0);
+                double fsc = 1.0;
+                for (size_t q = 0; q < J.noise.size(); ++q) {
+                    const ProcIn &p = J.noise[q];
+                    if (!p.lr || std::min(p.s0, p.s1) != la || std::max(p.s0, p.s1) != lb) continue;
+                    if (!allzero) fsc *= std::exp(-0.5 * p.gamma);
+                    cs.push_back(lb);
+                    co.push_back(-1);
+                    cs2.push_back(-1);
+                    co2.push_back(-1);
+                    const c64 I2[4] = {mk(1, 0), mk(0, 0), mk(0, 0), mk(1, 0)};
+                    ldl.insert(ldl.end(), I2, I2 + 4);
+                    cg.push_back(p.gamma);
+                    kinds.push_back(p.pauli_name ? 0 : 2);
+                    if (!p.pauli_name)
+                        for (size_t li = 0; li < lr_cands.size(); ++li)
+                            if (lr_cands[li].proc == (int)q) pc.lr.push_back({(int)cs.size() - 1, (int)li});
+                    pc.has_scal = true;
+                }
+                if (pc.has_scal) {
+                    const std::vector<c64> sc = {mk(fsc, 0), mk(0, 0), mk(0, 0), mk(fsc, 0)};
+                    upload(pc.scal, sc, s);
+                }
+                upload(pc.kind, kinds, s);
                 pc.n = (int)cs.size();
                 upload(pc.site, cs, s);
                 upload(pc.op, co, s);
@@ -1995,7 +2031,7 @@ void Engine<T>::stochastic(State &S, double dt, const DevBuf *invL, JobResult &r
         const c64 *ops = lr_ops.as<c64>() + (int64_t)li * 16;
         correlate(S, a, b, ops, ops + 4, mask.as<int>(), lr_val.as<c64>());
         correlate(S, a, b, ops + 8, ops + 12, mask.as<int>(), lr_val.as<c64>() + nb);
-        lr_nu(lr_val.as<c64>(), rho1.as<c64>(), (int64_t)L * 4, L, p.alpha.re, p.alpha.im, p.beta.re, p.beta.im,
+        lr_nu(lr_val.as<c64>(), rho1.as<c64>(), (int64_t)L * 4, L - 1, p.alpha.re, p.alpha.im, p.beta.re, p.beta.im,
               lr_cands[li].cand, ncand, jstate.as<int>(), lrnu.as<double>(), nb, s);
     }
     jump_choose(rho1.as<c64>(), (int64_t)L * 4, L, ncand, cand_site.as<int>(), cand_LdL.as<c64>(),
@@ -2316,6 +2352,9 @@ void Engine<T>::run_digital(int64_t t0, int nb_, double *d_out, int64_t *h_bonds
                     if (dsite_active[b])
                         local_op<T>(A(PHI, b), slotA[b], PHI.wd[b], PHI.wd[b + 1], dsite_ops.as<c64>() + (int64_t)b * 4,
                                     nullptr, nullptr, nb, s);
+                    if (lc.has_scal)  // long-range processes on (a, b): handle_longrange_scalar! at b
+                        local_op<T>(A(PHI, b), slotA[b], PHI.wd[b], PHI.wd[b + 1], lc.scal.as<c64>(), nullptr, nullptr,
+                                    nb, s);
                     if (dsite_active[a]) {
                         local_op<T>(A(PHI, a), slotA[a], PHI.wd[a], PHI.wd[a + 1], dsite_ops.as<c64>() + (int64_t)a * 4,
                                     nullptr, nullptr, nb, s);
@@ -2338,16 +2377,42 @@ void Engine<T>::run_digital(int64_t t0, int nb_, double *d_out, int64_t *h_bonds
                                    (int64_t)L * 4, mask.as<int>(), nb, s);
                         };
                         rho_site(b);
+                        // general long-range candidates: ν from two correlators (any gauge)
+                        for (auto &cl : lc.lr) {
+                            const ProcIn &p = J.noise[lr_cands[cl.second].proc];
+                            const c64 *ops = lr_ops.as<c64>() + (int64_t)cl.second * 16;
+                            correlate(PHI, a, b, ops, ops + 4, mask.as<int>(), lr_val.as<c64>());
+                            correlate(PHI, a, b, ops + 8, ops + 12, mask.as<int>(), lr_val.as<c64>() + nb);
+                            lr_nu(lr_val.as<c64>(), rho1.as<c64>(), (int64_t)L * 4, b, p.alpha.re, p.alpha.im,
+                                  p.beta.re, p.beta.im, cl.first, lc.n, jstate.as<int>(), lrnu.as<double>(), nb, s);
+                        }
                         move_centre(a);
                         rho_site(a);
                         jump_choose(rho1.as<c64>(), (int64_t)L * 4, L, lc.n, lc.site.as<int>(), lc.LdL.as<c64>(),
                                     lc.gamma.as<double>(), 1.0, J.seed, traj0, rngk.as<uint64_t>(), jstate.as<int>(),
-                                    chosen.as<int>(), wbuf.as<double>(), nb, s);
+                                    chosen.as<int>(), wbuf.as<double>(), nb, s, nullptr, 0, nullptr,
+                                    lc.kind.as<int>(), lrnu.p ? lrnu.as<double>() : nullptr);
                         for (int q : {a, b}) {
                             jump_opidx(jstate.as<int>(), chosen.as<int>(), lc.site.as<int>(), lc.op.as<int>(),
                                        lc.site2.as<int>(), lc.op2.as<int>(), q, opidx.as<int>(), nb, s);
                             local_op<T>(A(PHI, q), slotA[q], PHI.wd[q], PHI.wd[q + 1], jump_ops.as<c64>(),
                                         opidx.as<int>(), nullptr, nb, s);
+                        }
+                        if (!lc.lr.empty()) {
+                            // MPO jumps: O, truncate!, normalize! leave the centre at 0;
+                            // QR steps bring it back to a like the others'
+                            std::vector<int> hs(nb), hc(nb);
+                            YQ_HIP(hipMemcpyAsync(hs.data(), jstate.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+                            YQ_HIP(hipMemcpyAsync(hc.data(), chosen.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+                            YQ_HIP(hipStreamSynchronize(s));
+                            for (auto &cl : lc.lr) {
+                                bool any = false;
+                                for (int t = 0; t < nb && !any; ++t) any = hs[t] == 1 && hc[t] == cl.first;
+                                if (!any) continue;
+                                mask_chosen(jstate.as<int>(), chosen.as<int>(), cl.first, mask2.as<int>(), nb, s);
+                                mpo_jump(PHI, cl.second, mask2.as<int>());
+                                for (int k = 0; k < a; ++k) qr_step(PHI, k, mask2.as<int>());
+                            }
                         }
                         fill_int(count.as<int>(), 0, 1, s);
                         mask_from_state(jstate.as<int>(), 1, mask.as<int>(), count.as<int>(), nb, s);
@@ -2507,7 +2572,6 @@ void validate_job(const JobIn &job) {
     for (auto &p : job.noise) {
         if (p.s0 < 0 || p.s0 >= job.L || p.s1 < 0 || p.s1 >= job.L) throw std::invalid_argument("noise site out of range");
         if (p.lr) {
-            if (job.digital) throw std::invalid_argument("long-range noise processes are not supported by the digital GPU path");
             if (job.max_bond > 256)
                 throw std::invalid_argument("long-range noise processes need max_bond_dim <= 256 (the jump doubles bonds)");
             continue;

@@ -143,11 +143,12 @@ void jump_choose(const c64 *rhov, int64_t sR, int L, int ncand, const int *cand_
 // mask[t] = (state[t] == 1 && chosen[t] == c)
 void mask_chosen(const int *state, const int *chosen, int c, int *mask, int B, hipStream_t s);
 // ν of a general long-range candidate c for the jumping trajectories
-// (state 1): with n = Tr ρ of the last site (rho, stride sR, L sites),
+// (state 1): with n = Tr ρ of site nsite (rho, stride sR; a site whose ρ is
+// the true state's),
 // cs = <σ_a τ_b> (val[t]) and cn = <σ†σ_a τ†τ_b> (val[B + t]),
 // ‖Oψ‖² = |α|² n + |β|² cn + 2 Re(conj(α) β cs); ν = 1 unless that is at
 // most 1e-12 (|α|² n + |β|² cn) (O annihilates ψ), then 0.
-void lr_nu(const c64 *val, const c64 *rho, int64_t sR, int L, double ar, double ai, double br, double bi, int c,
+void lr_nu(const c64 *val, const c64 *rho, int64_t sR, int nsite, double ar, double ai, double br, double bi, int c,
            int ncand, const int *state, double *lrnu, int B, hipStream_t s);
 // Blocks of O ψ for O = α I + β σ_a τ_b on the bond-doubled layout (site
 // tensor [l][s][r] with ncol columns, its new layout; masked slots):

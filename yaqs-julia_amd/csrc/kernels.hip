@@ -1319,11 +1319,11 @@ void mask_chosen(const int *state, const int *chosen, int c, int *mask, int B, h
     YQ_HIP(hipGetLastError());
 }
 
-__global__ void lr_nu_kernel(const c64 *val, const c64 *rho, int64_t sR, int L, double ar, double ai, double br,
+__global__ void lr_nu_kernel(const c64 *val, const c64 *rho, int64_t sR, int nsite, double ar, double ai, double br,
                              double bi, int c, int ncand, const int *state, double *lrnu, int B) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= B || state[t] != 1) return;
-    const c64 *r = rho + t * sR + (int64_t)(L - 1) * 4;
+    const c64 *r = rho + t * sR + (int64_t)nsite * 4;
     const double n = r[0].re + r[3].re;
     const c64 cs = val[t];
     const double cn = val[B + t].re;
@@ -1334,10 +1334,10 @@ __global__ void lr_nu_kernel(const c64 *val, const c64 *rho, int64_t sR, int L, 
     const double on2 = base + 2.0 * (xr * cs.re - xi * cs.im);
     lrnu[(int64_t)t * ncand + c] = on2 > 1e-12 * base ? 1.0 : 0.0;
 }
-void lr_nu(const c64 *val, const c64 *rho, int64_t sR, int L, double ar, double ai, double br, double bi, int c,
+void lr_nu(const c64 *val, const c64 *rho, int64_t sR, int nsite, double ar, double ai, double br, double bi, int c,
            int ncand, const int *state, double *lrnu, int B, hipStream_t s) {
-    hipLaunchKernelGGL(lr_nu_kernel, dim3((B + 255) / 256), dim3(256), 0, s, val, rho, sR, L, ar, ai, br, bi, c, ncand,
-                       state, lrnu, B);
+    hipLaunchKernelGGL(lr_nu_kernel, dim3((B + 255) / 256), dim3(256), 0, s, val, rho, sR, nsite, ar, ai, br, bi, c,
+                       ncand, state, lrnu, B);
     YQ_HIP(hipGetLastError());
 }
 
