@@ -37,6 +37,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "kernels.h"
 #include "reduce.h"
@@ -1323,6 +1324,196 @@ __global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_kernel(const c64 *__
     }
 }
 
+// 1w'. Blocked form of the same reduction (LAPACK zlatrd/zhetrd structure):
+// the columns are taken in panels of NB; inside a panel the trailing matrix
+// is left as it was at the panel's start and the pending rank-2 updates are
+// applied through the panel's reflectors V and vectors W, held in LDS:
+//   column k:  x = A[k:, k] - sum_q (v_q conj(w_q[k]) + w_q conj(v_q[k]))
+//   reflector of x[k+1:] (same conventions as above), v_k into row k
+//   p = tau (A22 v - sum_q v_q (w_q^H v) - sum_q w_q (v_q^H v))
+//   w = p - tau/2 (p^H v) v
+// and after the panel one rank-2 NB update A22 -= V W^H + W V^H of the
+// trailing square.  Each column so reads the trailing matrix once (the
+// product) and the trailing matrix is written once per panel, where the
+// unblocked pass reads and writes it once per column.  Same outputs (rows
+// holding v, d, e, tau) as hetrd_wide_kernel.
+template <int NB, int NMX>
+__global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_blk_kernel(const c64 *__restrict__ G, int64_t sG,
+                                                                       char *__restrict__ Wk, int64_t sW, int n,
+                                                                       int kw, const int *__restrict__ mask) {
+    constexpr int NT = WT_WAVES * 64;
+    const int t = blockIdx.x;
+    if (mask && !mask[t]) return;
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    __shared__ double2 Vs[NB][NMX], Ws[NB][NMX];
+    __shared__ double2 xb[NMX], pb[NMX];
+    __shared__ double2 cw[NB], cv[NB];  // w_q^H v, v_q^H v
+    __shared__ double rs[WT_WAVES];
+    __shared__ double2 rs2[WT_WAVES];
+    __shared__ double2 tk_s, al_s;
+    const HeevLayW lay = heev_lay_wide(n, kw);
+    char *base = Wk + t * sW;
+    double2 *Am = (double2 *)(base + lay.A);
+    double2 *taug = (double2 *)(base + lay.tau);
+    double *dg = (double *)(base + lay.d);
+    double *eg = (double *)(base + lay.e);
+    const c64 *Gt = G + t * sG;
+    for (int64_t e = tid; e < (int64_t)n * n; e += NT) Am[e] = make_double2(Gt[e].re, Gt[e].im);
+    __syncthreads();
+    for (int k0 = 0; k0 + 1 < n; k0 += NB) {
+        const int nbp = min(NB, n - 1 - k0);  // reflectors of this panel (columns k0 .. k0 + nbp - 1)
+        for (int j = 0; j < nbp; ++j) {
+            const int k = k0 + j;
+            // corrected column k, rows k .. n-1 (row k of the full storage, conjugated)
+            for (int r = k + tid; r < n; r += NT) {
+                double2 x = r == k ? Am[(int64_t)k * n + k] : c_conj(Am[(int64_t)k * n + r]);
+                for (int q = 0; q < j; ++q) {
+                    const double2 a = c_mul(Vs[q][r], c_conj(Ws[q][k])), b = c_mul(Ws[q][r], c_conj(Vs[q][k]));
+                    x.x -= a.x + b.x;
+                    x.y -= a.y + b.y;
+                }
+                xb[r] = x;
+            }
+            __syncthreads();
+            // reflector of x[k+1 ..]: sum |x|^2 over rows k+2 ..
+            double xs = 0;
+            for (int r = k + 2 + tid; r < n; r += NT) xs += xb[r].x * xb[r].x + xb[r].y * xb[r].y;
+            xs = wsum(xs);
+            if (lane == 0) rs[w] = xs;
+            __syncthreads();
+            xs = 0;
+#pragma unroll
+            for (int i = 0; i < WT_WAVES; ++i) xs += rs[i];
+            const double2 alpha = xb[k + 1];
+            double2 tk = make_double2(0, 0), inv = make_double2(0, 0);
+            double beta = alpha.x;
+            if (!(xs == 0.0 && alpha.y == 0.0)) {
+                const double nr = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xs);
+                beta = alpha.x >= 0 ? -nr : nr;
+                tk = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+                const double2 den = make_double2(alpha.x - beta, alpha.y);
+                const double dn = den.x * den.x + den.y * den.y;
+                inv = make_double2(den.x / dn, -den.y / dn);
+            }
+            for (int r = tid; r < n; r += NT) {
+                double2 v = make_double2(0, 0);
+                if (r == k + 1) v = make_double2(1.0, 0.0);
+                else if (r > k + 1) v = c_mul(inv, xb[r]);
+                Vs[j][r] = v;
+                if (r > k) Am[(int64_t)k * n + r] = v;
+            }
+            if (tid == 0) {
+                dg[k] = xb[k].x;
+                eg[k] = beta;
+                taug[k] = tk;
+            }
+            __syncthreads();
+            // w_q^H v and v_q^H v of the panel's earlier columns (one wavefront each)
+            for (int u = w; u < 2 * j; u += WT_WAVES) {
+                const int q = u >> 1;
+                const double2 *a = (u & 1) ? Vs[q] : Ws[q];
+                double2 acc = make_double2(0, 0);
+                for (int r = k + 1 + lane; r < n; r += 64) {
+                    const double2 z = c_mulc(a[r], Vs[j][r]);
+                    acc.x += z.x;
+                    acc.y += z.y;
+                }
+                acc.x = wsum(acc.x);
+                acc.y = wsum(acc.y);
+                if (lane == 0) {
+                    if (u & 1) cv[q] = acc;
+                    else cw[q] = acc;
+                }
+            }
+            // A22 v with the panel-start matrix, one wavefront per row
+            const int m = n - k - 1;
+            for (int rr = w; rr < m; rr += WT_WAVES) {
+                const double2 *arow = Am + (int64_t)(k + 1 + rr) * n + k + 1;
+                const double2 *vs = &Vs[j][k + 1];
+                double2 acc = make_double2(0, 0);
+                for (int c = lane; c < m; c += 64) {
+                    const double2 z = c_mul(arow[c], vs[c]);
+                    acc.x += z.x;
+                    acc.y += z.y;
+                }
+                acc.x = wsum(acc.x);
+                acc.y = wsum(acc.y);
+                if (lane == 0) pb[k + 1 + rr] = acc;
+            }
+            __syncthreads();
+            // p = tau (A22 v - V cw - W cv), and p^H v
+            double2 pvp = make_double2(0, 0);
+            for (int r = k + 1 + tid; r < n; r += NT) {
+                double2 y = pb[r];
+                for (int q = 0; q < j; ++q) {
+                    const double2 a = c_mul(Vs[q][r], cw[q]), b = c_mul(Ws[q][r], cv[q]);
+                    y.x -= a.x + b.x;
+                    y.y -= a.y + b.y;
+                }
+                y = c_mul(tk, y);
+                pb[r] = y;
+                const double2 z = c_mulc(y, Vs[j][r]);
+                pvp.x += z.x;
+                pvp.y += z.y;
+            }
+            pvp.x = wsum(pvp.x);
+            pvp.y = wsum(pvp.y);
+            if (lane == 0) rs2[w] = pvp;
+            __syncthreads();
+            double2 pv = make_double2(0, 0);
+#pragma unroll
+            for (int i = 0; i < WT_WAVES; ++i) {
+                pv.x += rs2[i].x;
+                pv.y += rs2[i].y;
+            }
+            const double2 al = c_mul(make_double2(-0.5 * tk.x, -0.5 * tk.y), pv);
+            for (int r = tid; r < n; r += NT) {
+                double2 wv = make_double2(0, 0);
+                if (r > k) {
+                    const double2 z = c_mul(al, Vs[j][r]);
+                    wv = make_double2(pb[r].x + z.x, pb[r].y + z.y);
+                }
+                Ws[j][r] = wv;
+            }
+            __syncthreads();
+        }
+        // the panel's rank-2 nbp update of the trailing square (both triangles)
+        const int k1 = k0 + nbp, m1 = n - k1;
+        // (the row's v and w of eight reflectors at a time in registers)
+        constexpr int QC = 8;
+        for (int rr = w; rr < m1; rr += WT_WAVES) {
+            const int r = k1 + rr;
+            double2 *arow = Am + (int64_t)r * n;
+            for (int q0 = 0; q0 < nbp; q0 += QC) {
+                double2 vr[QC], wr[QC];
+#pragma unroll
+                for (int q = 0; q < QC; ++q) {
+                    vr[q] = q0 + q < nbp ? Vs[q0 + q][r] : make_double2(0, 0);
+                    wr[q] = q0 + q < nbp ? Ws[q0 + q][r] : make_double2(0, 0);
+                }
+                const int qn = min(QC, nbp - q0);
+                for (int c = k1 + lane; c < n; c += 64) {
+                    double2 a = arow[c];
+#pragma unroll
+                    for (int q = 0; q < QC; ++q) {
+                        if (q >= qn) break;
+                        const double2 x1 = c_mul(vr[q], c_conj(Ws[q0 + q][c]));
+                        const double2 x2 = c_mul(wr[q], c_conj(Vs[q0 + q][c]));
+                        a.x -= x1.x + x2.x;
+                        a.y -= x1.y + x2.y;
+                    }
+                    arow[c] = a;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        dg[n - 1] = Am[(int64_t)(n - 1) * n + n - 1].x;
+        eg[n - 1] = 0.0;
+    }
+}
+
 // 3w. Kept eigenvectors of T by inverse iteration for the wide path: the
 // algorithm of stein_kernel with kw lanes (kw / 64 wavefronts, lane c = the
 // c-th largest eigenvalue) and X in global memory [row][kw]; the
@@ -1556,7 +1747,18 @@ bool heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, d
         YQ_HIP(hipGetLastError());
     } else if (wide) {
         const HeevLayW lay = heev_lay_wide(n, kw);
-        hipLaunchKernelGGL(hetrd_wide_kernel, dim3(B), dim3(WT_WAVES * 64), 0, s, G, sG, W, sW, n, kw, mask);
+        static const int blk = [] {
+            const char *e = std::getenv("YAQS_HETRD_BLOCKED");
+            return e ? std::atoi(e) : 1;
+        }();
+        if (blk && n <= 256)
+            hipLaunchKernelGGL((hetrd_wide_blk_kernel<16, 256>), dim3(B), dim3(WT_WAVES * 64), 0, s, G, sG, W, sW, n,
+                               kw, mask);
+        else if (blk)
+            hipLaunchKernelGGL((hetrd_wide_blk_kernel<8, WIDE_MAX_N>), dim3(B), dim3(WT_WAVES * 64), 0, s, G, sG, W,
+                               sW, n, kw, mask);
+        else
+            hipLaunchKernelGGL(hetrd_wide_kernel, dim3(B), dim3(WT_WAVES * 64), 0, s, G, sG, W, sW, n, kw, mask);
         YQ_HIP(hipGetLastError());
         hipLaunchKernelGGL(stebz_kernel<WIDE_MAX_N>, dim3(B), dim3(round64(n)), 0, s, W, sW, lay.d, lay.e, n, ns,
                            evals, perm, sE, keep, rule, thr, max_bond, mask);
