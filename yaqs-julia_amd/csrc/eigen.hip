@@ -1337,7 +1337,7 @@ __global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_kernel(const c64 *__
 // product) and the trailing matrix is written once per panel, where the
 // unblocked pass reads and writes it once per column.  Same outputs (rows
 // holding v, d, e, tau) as hetrd_wide_kernel.
-template <int NB, int NMX>
+template <int NB, int NMX, bool LOW>
 __global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_blk_kernel(const c64 *__restrict__ G, int64_t sG,
                                                                        char *__restrict__ Wk, int64_t sW, int n,
                                                                        int kw, const int *__restrict__ mask) {
@@ -1347,6 +1347,8 @@ __global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_blk_kernel(const c64
     const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     __shared__ double2 Vs[NB][NMX], Ws[NB][NMX];
     __shared__ double2 xb[NMX], pb[NMX];
+    // LOW: per-wavefront partials of the product's upper half (column sums)
+    __shared__ double2 tpart[LOW ? WT_WAVES : 1][LOW ? NMX : 1];
     __shared__ double2 cw[NB], cv[NB];  // w_q^H v, v_q^H v
     __shared__ double rs[WT_WAVES];
     __shared__ double2 rs2[WT_WAVES];
@@ -1366,7 +1368,9 @@ __global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_blk_kernel(const c64
             const int k = k0 + j;
             // corrected column k, rows k .. n-1 (row k of the full storage, conjugated)
             for (int r = k + tid; r < n; r += NT) {
-                double2 x = r == k ? Am[(int64_t)k * n + k] : c_conj(Am[(int64_t)k * n + r]);
+                double2 x;
+                if (LOW) x = Am[(int64_t)r * n + k];  // column k of the lower triangle
+                else x = r == k ? Am[(int64_t)k * n + k] : c_conj(Am[(int64_t)k * n + r]);
                 for (int q = 0; q < j; ++q) {
                     const double2 a = c_mul(Vs[q][r], c_conj(Ws[q][k])), b = c_mul(Ws[q][r], c_conj(Vs[q][k]));
                     x.x -= a.x + b.x;
@@ -1427,20 +1431,69 @@ __global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_blk_kernel(const c64
             }
             // A22 v with the panel-start matrix, one wavefront per row
             const int m = n - k - 1;
-            for (int rr = w; rr < m; rr += WT_WAVES) {
-                const double2 *arow = Am + (int64_t)(k + 1 + rr) * n + k + 1;
+            if (LOW) {
+                // lower triangle only: row rr gives sum_{c <= rr} A[rr][c] v_c to
+                // y_rr and conj(A[rr][c]) v_rr to y_c (c < rr), the latter kept per
+                // lane (lane owns columns c = lane + 64 q) and summed over the
+                // wavefronts below in a fixed order
                 const double2 *vs = &Vs[j][k + 1];
-                double2 acc = make_double2(0, 0);
-                for (int c = lane; c < m; c += 64) {
-                    const double2 z = c_mul(arow[c], vs[c]);
-                    acc.x += z.x;
-                    acc.y += z.y;
+                constexpr int NQ = NMX / 64;
+                double2 tacc[NQ];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) tacc[q] = make_double2(0, 0);
+                for (int rr = w; rr < m; rr += WT_WAVES) {
+                    const double2 *arow = Am + (int64_t)(k + 1 + rr) * n + k + 1;
+                    const double2 vr = vs[rr];
+                    double2 acc = make_double2(0, 0);
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        const int c = lane + 64 * q;
+                        if (c <= rr) {
+                            const double2 a = arow[c];
+                            const double2 z = c_mul(a, vs[c]);
+                            acc.x += z.x;
+                            acc.y += z.y;
+                            if (c < rr) {
+                                const double2 u = c_mulc(a, vr);
+                                tacc[q].x += u.x;
+                                tacc[q].y += u.y;
+                            }
+                        }
+                    }
+                    acc.x = wsum(acc.x);
+                    acc.y = wsum(acc.y);
+                    if (lane == 0) pb[k + 1 + rr] = acc;
                 }
-                acc.x = wsum(acc.x);
-                acc.y = wsum(acc.y);
-                if (lane == 0) pb[k + 1 + rr] = acc;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q)
+                    if (lane + 64 * q < m) tpart[w][lane + 64 * q] = tacc[q];
+                __syncthreads();
+                for (int c = tid; c < m; c += NT) {
+                    double2 y = pb[k + 1 + c];
+#pragma unroll
+                    for (int i = 0; i < WT_WAVES; ++i) {
+                        y.x += tpart[i][c].x;
+                        y.y += tpart[i][c].y;
+                    }
+                    pb[k + 1 + c] = y;
+                }
+                __syncthreads();
+            } else {
+                for (int rr = w; rr < m; rr += WT_WAVES) {
+                    const double2 *arow = Am + (int64_t)(k + 1 + rr) * n + k + 1;
+                    const double2 *vs = &Vs[j][k + 1];
+                    double2 acc = make_double2(0, 0);
+                    for (int c = lane; c < m; c += 64) {
+                        const double2 z = c_mul(arow[c], vs[c]);
+                        acc.x += z.x;
+                        acc.y += z.y;
+                    }
+                    acc.x = wsum(acc.x);
+                    acc.y = wsum(acc.y);
+                    if (lane == 0) pb[k + 1 + rr] = acc;
+                }
+                __syncthreads();
             }
-            __syncthreads();
             // p = tau (A22 v - V cw - W cv), and p^H v
             double2 pvp = make_double2(0, 0);
             for (int r = k + 1 + tid; r < n; r += NT) {
@@ -1477,7 +1530,8 @@ __global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_blk_kernel(const c64
             }
             __syncthreads();
         }
-        // the panel's rank-2 nbp update of the trailing square (both triangles)
+        // the panel's rank-2 nbp update of the trailing square (both triangles,
+        // or with LOW the lower one)
         const int k1 = k0 + nbp, m1 = n - k1;
         // (the row's v and w of eight reflectors at a time in registers)
         constexpr int QC = 8;
@@ -1492,7 +1546,8 @@ __global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_blk_kernel(const c64
                     wr[q] = q0 + q < nbp ? Ws[q0 + q][r] : make_double2(0, 0);
                 }
                 const int qn = min(QC, nbp - q0);
-                for (int c = k1 + lane; c < n; c += 64) {
+                const int cend = LOW ? r + 1 : n;  // LOW: the lower triangle only
+                for (int c = k1 + lane; c < cend; c += 64) {
                     double2 a = arow[c];
 #pragma unroll
                     for (int q = 0; q < QC; ++q) {
@@ -1751,12 +1806,15 @@ bool heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, d
             const char *e = std::getenv("YAQS_HETRD_BLOCKED");
             return e ? std::atoi(e) : 1;
         }();
-        if (blk && n <= 256)
-            hipLaunchKernelGGL((hetrd_wide_blk_kernel<16, 256>), dim3(B), dim3(WT_WAVES * 64), 0, s, G, sG, W, sW, n,
-                               kw, mask);
-        else if (blk)
-            hipLaunchKernelGGL((hetrd_wide_blk_kernel<8, WIDE_MAX_N>), dim3(B), dim3(WT_WAVES * 64), 0, s, G, sG, W,
+        if (blk == 2 && n <= 256)
+            hipLaunchKernelGGL((hetrd_wide_blk_kernel<16, 256, false>), dim3(B), dim3(WT_WAVES * 64), 0, s, G, sG, W,
                                sW, n, kw, mask);
+        else if (blk && n <= 256)
+            hipLaunchKernelGGL((hetrd_wide_blk_kernel<8, 256, true>), dim3(B), dim3(WT_WAVES * 64), 0, s, G, sG, W,
+                               sW, n, kw, mask);
+        else if (blk)
+            hipLaunchKernelGGL((hetrd_wide_blk_kernel<8, WIDE_MAX_N, false>), dim3(B), dim3(WT_WAVES * 64), 0, s, G,
+                               sG, W, sW, n, kw, mask);
         else
             hipLaunchKernelGGL(hetrd_wide_kernel, dim3(B), dim3(WT_WAVES * 64), 0, s, G, sG, W, sW, n, kw, mask);
         YQ_HIP(hipGetLastError());
