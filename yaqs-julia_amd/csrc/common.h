@@ -52,6 +52,7 @@ struct GemmDesc {
     int outer;       // outer batch count (trajectories)
     int accumulate;  // 0: C = AB, 1: C += AB
     const int *active;  // optional per-outer mask
+    int lower = 0;      // generic kernel: C tiles strictly above the diagonal are not computed
 };
 
 }  // namespace yq

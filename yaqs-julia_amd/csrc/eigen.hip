@@ -1770,6 +1770,19 @@ void heev_fixup(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t
     YQ_HIP(hipGetLastError());
 }
 
+// Wide-path tridiagonalisation: 1 = blocked, lower triangle only up to
+// n = 256 (the default), 2 = blocked with full rows, 0 = unblocked
+// (YAQS_HETRD_BLOCKED overrides).
+static int hetrd_wide_mode() {
+    static const int m = [] {
+        const char *e = std::getenv("YAQS_HETRD_BLOCKED");
+        return e ? std::atoi(e) : 1;
+    }();
+    return m;
+}
+
+bool heev_gram_lower_only(int n) { return n > LDS_MAX_N && n <= 256 && hetrd_wide_mode() == 1; }
+
 bool heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, double2 *Pg, int64_t sP, void *work,
           int64_t sW, int *fallback, int n, int ns, double *evals, int *perm, int64_t sE, int *keep, int rule,
           double thr, int max_bond, const int *mask, int B, hipStream_t s, bool fixup) {
@@ -1802,10 +1815,7 @@ bool heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, d
         YQ_HIP(hipGetLastError());
     } else if (wide) {
         const HeevLayW lay = heev_lay_wide(n, kw);
-        static const int blk = [] {
-            const char *e = std::getenv("YAQS_HETRD_BLOCKED");
-            return e ? std::atoi(e) : 1;
-        }();
+        const int blk = hetrd_wide_mode();
         if (blk == 2 && n <= 256)
             hipLaunchKernelGGL((hetrd_wide_blk_kernel<16, 256, false>), dim3(B), dim3(WT_WAVES * 64), 0, s, G, sG, W,
                                sW, n, kw, mask);

@@ -79,6 +79,10 @@ int64_t heev_work_bytes(int n, int kmax);
 // Returns true when the multi-kernel path ran; with fixup = false the caller
 // then runs heev_fixup for the matrices flagged in `fallback` itself (for
 // instance only when its readback shows one).
+// True when heev reads only the lower triangle of its n x n matrices (the
+// wide path up to n = 256 and the single-workgroup solver): a Gram matrix
+// for it may skip the tiles above the diagonal.
+bool heev_gram_lower_only(int n);
 bool heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, double2 *Pg, int64_t sP, void *work,
           int64_t sW, int *fallback, int n, int ns, double *evals, int *perm, int64_t sE, int *keep, int rule,
           double thr, int max_bond, const int *mask, int B, hipStream_t s, bool fixup = true);

@@ -71,6 +71,7 @@ __global__ __launch_bounds__(256) void gemm_generic_kernel(GemmDesc d, const cpl
     if (d.active && !d.active[t]) return;
     const int tilesN = (d.N + BN - 1) / BN;
     const int m0 = (blockIdx.x / tilesN) * BM, n0 = (blockIdx.x % tilesN) * BN;
+    if (d.lower && n0 >= m0 + BM) return;  // the tile lies above the diagonal
     const cplx<TA> *Ab = A + t * d.sA_outer + jb * d.sA_inner;
     const cplx<TB> *Bb = B + t * d.sB_outer + jb * d.sB_inner;
     cplx<TC> *Cb = C + t * d.sC_outer + jb * d.sC_inner;
