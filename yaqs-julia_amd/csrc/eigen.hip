@@ -1112,6 +1112,7 @@ __global__ __launch_bounds__(NWV * 64) void ormtr_kernel(char *__restrict__ Wk, 
     const int c = blockIdx.y * CW + cl;
     __shared__ double2 vb[3][NMAX];
     __shared__ double2 part[2][NR][CW];
+    __shared__ double2 taus[NMAX];
     const char *base = Wk + t * sW;
     const double2 *__restrict__ Vg = (const double2 *)(base + ov);
     const double2 *__restrict__ taug = (const double2 *)(base + otau);
@@ -1122,12 +1123,20 @@ __global__ __launch_bounds__(NWV * 64) void ormtr_kernel(char *__restrict__ Wk, 
         const int r = rres + NR * j;
         u[j] = make_double2(r < n && c < ldx ? Xg[(int64_t)r * ldx + c] : 0.0, 0.0);
     }
+    // Every global load of the reflector loop is issued one step before its
+    // value is needed: the scalars tau in LDS up front, row k - 1 of V held
+    // in a register while step k + 1 runs (a load consumed in the step that
+    // issues it would stall every wave at the step's barrier).
+    for (int i = tid; i + 1 < n; i += blockDim.x) taus[i] = taug[i];
     if (n >= 2 && tid < n) vb[(n - 2) % 3][tid] = tid >= n - 1 ? Vg[(int64_t)(n - 2) * n + tid] : make_double2(0, 0);
+    double2 vnext = make_double2(0, 0);  // row k - 1 at the top of step k
+    if (n >= 3 && tid < n && tid >= n - 2) vnext = Vg[(int64_t)(n - 3) * n + tid];
     __syncthreads();
     for (int k = n - 2; k >= 0; --k) {
         const int cb = k % 3;
-        if (k >= 1 && tid < n) vb[(k - 1) % 3][tid] = tid >= k ? Vg[(int64_t)(k - 1) * n + tid] : make_double2(0, 0);
-        const double2 tk = taug[k];
+        if (k >= 1 && tid < n) vb[(k - 1) % 3][tid] = vnext;
+        if (k >= 2 && tid < n) vnext = tid >= k - 1 ? Vg[(int64_t)(k - 2) * n + tid] : make_double2(0, 0);
+        const double2 tk = taus[k];
         const double2 *v = vb[cb];
         double2 acc = make_double2(0, 0);
 #pragma unroll
@@ -1441,15 +1450,28 @@ __global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_blk_kernel(const c64
                 double2 tacc[NQ];
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) tacc[q] = make_double2(0, 0);
-                for (int rr = w; rr < m; rr += WT_WAVES) {
+                // two rows per wavefront and pass (rr and rr + WT_WAVES), both
+                // rows' loads issued before either is used; the per-lane
+                // column sums still take the rows in increasing order
+                for (int rr = w; rr < m; rr += 2 * WT_WAVES) {
+                    const int r2 = rr + WT_WAVES;
+                    const bool two = r2 < m;
                     const double2 *arow = Am + (int64_t)(k + 1 + rr) * n + k + 1;
-                    const double2 vr = vs[rr];
-                    double2 acc = make_double2(0, 0);
+                    const double2 *arow2 = Am + (int64_t)(k + 1 + (two ? r2 : rr)) * n + k + 1;
+                    double2 a1[NQ], a2[NQ];
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        const int c = lane + 64 * q;
+                        a1[q] = c <= rr ? arow[c] : make_double2(0, 0);
+                        a2[q] = two && c <= r2 ? arow2[c] : make_double2(0, 0);
+                    }
+                    const double2 vr = vs[rr], vr2 = vs[two ? r2 : rr];
+                    double2 acc = make_double2(0, 0), acc2 = make_double2(0, 0);
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) {
                         const int c = lane + 64 * q;
                         if (c <= rr) {
-                            const double2 a = arow[c];
+                            const double2 a = a1[q];
                             const double2 z = c_mul(a, vs[c]);
                             acc.x += z.x;
                             acc.y += z.y;
@@ -1460,9 +1482,31 @@ __global__ __launch_bounds__(WT_WAVES * 64) void hetrd_wide_blk_kernel(const c64
                             }
                         }
                     }
+                    if (two) {
+#pragma unroll
+                        for (int q = 0; q < NQ; ++q) {
+                            const int c = lane + 64 * q;
+                            if (c <= r2) {
+                                const double2 a = a2[q];
+                                const double2 z = c_mul(a, vs[c]);
+                                acc2.x += z.x;
+                                acc2.y += z.y;
+                                if (c < r2) {
+                                    const double2 u = c_mulc(a, vr2);
+                                    tacc[q].x += u.x;
+                                    tacc[q].y += u.y;
+                                }
+                            }
+                        }
+                    }
                     acc.x = wsum(acc.x);
                     acc.y = wsum(acc.y);
-                    if (lane == 0) pb[k + 1 + rr] = acc;
+                    acc2.x = wsum(acc2.x);
+                    acc2.y = wsum(acc2.y);
+                    if (lane == 0) {
+                        pb[k + 1 + rr] = acc;
+                        if (two) pb[k + 1 + r2] = acc2;
+                    }
                 }
 #pragma unroll
                 for (int q = 0; q < NQ; ++q)
@@ -1648,27 +1692,64 @@ __global__ __launch_bounds__(WIDE_MAX_K) void stein_wide_kernel(char *__restrict
                 const uint32_t h = (uint32_t)(i * 2654435761u) ^ (uint32_t)((j + 1) * 2246822519u);
                 x[(int64_t)i * kw] = 0.5 + (double)((h >> 8) & 0xffff) * (1.0 / 65536.0);
             }
+            // X sits in global memory beside the factors, so the compiler
+            // may not move a row's loads above the previous row's store:
+            // the solves load the operands of SCH rows before using them
+            // (same arithmetic, same order)
+            constexpr int SCH = 8;
             for (int it = 0; it < 3; ++it) {
                 double r0 = x[0];
-                for (int i = 0; i + 1 < n; ++i) {
-                    const double l = ml[(int64_t)i * kw + c];
-                    double r1 = x[(int64_t)(i + 1) * kw];
-                    if ((piv[i >> 6] >> (i & 63)) & 1ull) {
-                        const double tmp = r0;
-                        r0 = r1;
-                        r1 = tmp;
+                for (int i0 = 0; i0 + 1 < n; i0 += SCH) {
+                    double lv[SCH], xv[SCH];
+#pragma unroll
+                    for (int u = 0; u < SCH; ++u) {
+                        const int i = i0 + u;
+                        lv[u] = xv[u] = 0.0;
+                        if (i + 1 < n) {
+                            lv[u] = ml[(int64_t)i * kw + c];
+                            xv[u] = x[(int64_t)(i + 1) * kw];
+                        }
                     }
-                    x[(int64_t)i * kw] = r0;
-                    r0 = r1 - l * r0;
+#pragma unroll
+                    for (int u = 0; u < SCH; ++u) {
+                        const int i = i0 + u;
+                        if (i + 1 < n) {
+                            double r1 = xv[u];
+                            if ((piv[i >> 6] >> (i & 63)) & 1ull) {
+                                const double tmp = r0;
+                                r0 = r1;
+                                r1 = tmp;
+                            }
+                            x[(int64_t)i * kw] = r0;
+                            r0 = r1 - lv[u] * r0;
+                        }
+                    }
                 }
                 x[(int64_t)(n - 1) * kw] = r0;
                 double y2 = 0.0, y1 = 0.0;
-                for (int i = n - 1; i >= 0; --i) {
-                    const int64_t o = (int64_t)i * kw + c;
-                    const double y = (x[(int64_t)i * kw] - ub[o] * y1 - uc[o] * y2) * ui[o];
-                    x[(int64_t)i * kw] = y;
-                    y2 = y1;
-                    y1 = y;
+                for (int i0 = n - 1; i0 >= 0; i0 -= SCH) {
+                    double xv[SCH], bv[SCH], cv[SCH], dv[SCH];
+#pragma unroll
+                    for (int u = 0; u < SCH; ++u) {
+                        const int i = i0 - u;
+                        xv[u] = bv[u] = cv[u] = dv[u] = 0.0;
+                        if (i >= 0) {
+                            const int64_t o = (int64_t)i * kw + c;
+                            xv[u] = x[(int64_t)i * kw];
+                            bv[u] = ub[o];
+                            cv[u] = uc[o];
+                            dv[u] = ui[o];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < SCH; ++u) {
+                        if (i0 - u >= 0) {
+                            const double y = (xv[u] - bv[u] * y1 - cv[u] * y2) * dv[u];
+                            x[(int64_t)(i0 - u) * kw] = y;
+                            y2 = y1;
+                            y1 = y;
+                        }
+                    }
                 }
                 for (int p = c; p < j; ++p) {
                     double sp = 0;
@@ -1744,6 +1825,134 @@ __global__ __launch_bounds__(WIDE_MAX_K) void cgs_wide_kernel(char *__restrict__
     const double inv = (c < kc && nr > 0.0) ? 1.0 / sqrt(nr) : 0.0;
     __syncthreads();  // every lane has read the original X
     for (int i = 0; i < n; ++i) X[(int64_t)i * kw + c] = X2[(int64_t)i * kw + c] * inv;
+}
+
+// 3w''. The pass of cgs_wide_kernel for n <= 256, kw <= 128 as two
+// register-blocked products staged through LDS by 1024 threads (the
+// per-column loop above runs one dependent global load per row):
+//   S = X^T X: thread (ta, tb) = (tid / 32, tid % 32) forms the 4 x 4 block
+//   S[4 ta + u][4 tb + v] over chunks of 32 rows of X in LDS; the entries
+//   with p >= c are dropped (stored as 0) and S goes to the LU area;
+//   Y = X - X S: thread (ti, tc) keeps Y[8 ti + r][4 tc + v] in registers
+//   while p runs over chunks of 16 (X[:, p0 .. p0 + 16) and
+//   S[p0 .. p0 + 16][:] in LDS);
+//   column norms: per-thread partials over its 8 rows, summed over ti in a
+//   fixed order; unit columns (zero beyond the kept rank) written to X.
+constexpr int CGW_THREADS = 1024;
+__global__ __launch_bounds__(CGW_THREADS) void cgs_wide2_kernel(char *__restrict__ Wk, int64_t sW, int n, int kw,
+                                                                const int *__restrict__ keep,
+                                                                const int *__restrict__ mask,
+                                                                const int *__restrict__ fallback) {
+    constexpr int R = 32, PC = 16, KM = 128, NM = 256;
+    // phase 1: rows of X [R][KM]; phase 2: X[:, p0 ..) [NM][PC] and
+    // S[p0 ..)[:] [PC][KM]; then the norm partials [32][KM]
+    __shared__ double lds[NM * PC + PC * KM + 32 * KM];
+    const int t = blockIdx.x;
+    if (mask && !mask[t]) return;
+    if (fallback[t]) return;
+    const int tid = threadIdx.x;
+    const int kc = min(keep[t], kw);
+    const HeevLayW lay = heev_lay_wide(n, kw);
+    char *base = Wk + t * sW;
+    double *S = (double *)(base + lay.lu);  // [p][kw]
+    double *X = (double *)(base + lay.X);   // [i][kw]
+    const int ta = tid >> 5, tb = tid & 31;
+    {
+        double acc[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+        const bool act = 4 * ta < kw && 4 * tb < kw;
+        for (int i0 = 0; i0 < n; i0 += R) {
+            __syncthreads();
+            for (int e = tid; e < R * kw; e += CGW_THREADS) {
+                const int r = e / kw, c = e - r * kw;
+                lds[r * KM + c] = i0 + r < n ? X[(int64_t)(i0 + r) * kw + c] : 0.0;
+            }
+            __syncthreads();
+            if (act)
+#pragma unroll 2
+                for (int r = 0; r < R; ++r) {
+                    const double *row = lds + r * KM;
+                    double a[4], b[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        a[u] = row[4 * ta + u];
+                        b[u] = row[4 * tb + u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+                }
+        }
+        if (act)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int p = 4 * ta + u, c = 4 * tb + v;
+                    S[(int64_t)p * kw + c] = p < c ? acc[u][v] : 0.0;
+                }
+    }
+    const int ti = ta, tc = tb;
+    const bool act = 4 * tc < kw;
+    double y[8][4];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 8 * ti + r;
+            y[r][v] = act && i < n ? X[(int64_t)i * kw + 4 * tc + v] : 0.0;
+        }
+    double *xp = lds, *sp = lds + NM * PC, *nq = lds + NM * PC + PC * KM;
+    for (int p0 = 0; p0 < kw; p0 += PC) {
+        __syncthreads();  // S complete; the previous chunk consumed
+        for (int e = tid; e < NM * PC; e += CGW_THREADS) {
+            const int i = e / PC, q = e % PC;
+            xp[e] = i < n ? X[(int64_t)i * kw + p0 + q] : 0.0;
+        }
+        for (int e = tid; e < PC * kw; e += CGW_THREADS) {
+            const int q = e / kw, c = e - q * kw;
+            sp[q * KM + c] = S[(int64_t)(p0 + q) * kw + c];
+        }
+        __syncthreads();
+        if (act && 8 * ti < n)
+#pragma unroll 1
+            for (int q = 0; q < PC; ++q) {
+                double s4[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) s4[v] = sp[q * KM + 4 * tc + v];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const double xv = xp[(8 * ti + r) * PC + q];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) y[r][v] = fma(-s4[v], xv, y[r][v]);
+                }
+            }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        double sq = 0.0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) sq += y[r][v] * y[r][v];
+        nq[ti * KM + 4 * tc + v] = sq;
+    }
+    __syncthreads();  // every thread has staged its last chunk of X
+    if (!act) return;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int c = 4 * tc + v;
+        double sq = 0.0;
+        for (int w = 0; w < 32; ++w) sq += nq[w * KM + c];
+        const double inv = (c < kc && sq > 0.0) ? 1.0 / sqrt(sq) : 0.0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int i = 8 * ti + r;
+            if (i < n) X[(int64_t)i * kw + c] = y[r][v] * inv;
+        }
+    }
 }
 
 int64_t heev_work_bytes(int n, int kmax) {
@@ -1834,8 +2043,12 @@ bool heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, d
         hipLaunchKernelGGL(stein_wide_kernel, dim3(B), dim3(kw), 0, s, W, sW, n, kw, (const double *)evals, sE,
                            (const int *)keep, mask, fallback);
         YQ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(cgs_wide_kernel, dim3(B), dim3(kw), 0, s, W, sW, n, kw, (const int *)keep, mask,
-                           (const int *)fallback);
+        if (n <= 256 && kw <= 128)
+            hipLaunchKernelGGL(cgs_wide2_kernel, dim3(B), dim3(CGW_THREADS), 0, s, W, sW, n, kw, (const int *)keep, mask,
+                               (const int *)fallback);
+        else
+            hipLaunchKernelGGL(cgs_wide_kernel, dim3(B), dim3(kw), 0, s, W, sW, n, kw, (const int *)keep, mask,
+                               (const int *)fallback);
         YQ_HIP(hipGetLastError());
         if (n <= 256)  // 16 row residues per 64 columns
             hipLaunchKernelGGL((ormtr_kernel<256, 16, 64>), dim3(B, kw / 64), dim3(1024), 0, s, W, sW, lay.A,

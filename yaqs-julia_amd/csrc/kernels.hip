@@ -785,31 +785,45 @@ __global__ __launch_bounds__(256) void cgs_coef_kernel(cplx<T> *S, int64_t sS, c
 }
 
 // CGS2 of the vectors [p0, p0+pw) among themselves (they are already
-// orthogonal to the earlier ones), panel in LDS as FP64 [pw][len]; writes
-// Cf[k][j] for p0 <= j <= k and zeroes Cf[k][j] for j > k.
+// orthogonal to the earlier ones), panel in LDS as FP64 [pw][ld] with
+// ld = len + 1 (consecutive vectors one 16-byte slot apart, so the four
+// lanes of an element quad, which read four consecutive vectors, hit
+// different banks); writes Cf[k][j] for p0 <= j <= k and zeroes Cf[k][j]
+// for j > k.  Normalisation is deferred: vector j stays unscaled in LDS,
+// q_j = s_j u_j with s_j its inverse norm (0 for a dead direction), and the
+// panel is scaled as it is written back.  Per vector q and pass: wavefront
+// w forms <u_j, v> for j = w (mod waves), giving the coefficient
+// h_j = s_j <u_j, v> against q_j; barrier; element quad i (lane & 3 = part)
+// sums s_j h_j u_j[i] over j = part (mod 4), adds the four sums by DPP and
+// subtracts them from v[i]; barrier.  The second pass also forms the
+// norm's per-wave partials, so a vector takes four barriers.  Blocks of up
+// to 1024 threads (four per element, in multiples of 64).
 template <typename T>
-__global__ __launch_bounds__(256) void cgs2_panel_kernel(cplx<T> *A, int64_t sA, int nvec, int len, int64_t vs,
-                                                         int64_t es, int p0, int pw, cplx<T> *Cf, int64_t sC,
-                                                         const double *fro, const int *mask) {
-    extern __shared__ double2 pan[];  // [pw][len]
-    __shared__ double sh[16];
-    __shared__ double2 h[64];
+__global__ __launch_bounds__(1024) void cgs2_panel_kernel(cplx<T> *A, int64_t sA, int nvec, int len, int64_t vs,
+                                                          int64_t es, int p0, int pw, cplx<T> *Cf, int64_t sC,
+                                                          const double *fro, const int *mask) {
+    extern __shared__ double2 pan[];  // [pw][ld]
+    __shared__ double2 h[32], hs[32];  // h_j and s_j h_j of the current pass
+    __shared__ double sc[32];          // s_j of the finished vectors
+    __shared__ double npart[16];       // per-wave partials of the norm
     const int t = blockIdx.x;
     if (mask && !mask[t]) return;
     cplx<T> *M = A + t * sA;
     cplx<T> *C = Cf + t * sC;
     const double dead = 1e-12 * fro[t];
+    const int ld = len + 1;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int part = threadIdx.x & 3, nquad = blockDim.x >> 2;
     // load, coalesced along whichever index is contiguous
     if (vs == 1) {
         for (int e = threadIdx.x; e < pw * len; e += blockDim.x) {
             const int i = e / pw, q = e % pw;
-            pan[q * len + i] = ld2(M[(int64_t)(p0 + q) + (int64_t)i * es]);
+            pan[q * ld + i] = ld2(M[(int64_t)(p0 + q) + (int64_t)i * es]);
         }
     } else {
         for (int e = threadIdx.x; e < pw * len; e += blockDim.x) {
             const int q = e / len, i = e % len;
-            pan[q * len + i] = ld2(M[(int64_t)(p0 + q) * vs + (int64_t)i * es]);
+            pan[q * ld + i] = ld2(M[(int64_t)(p0 + q) * vs + (int64_t)i * es]);
         }
     }
     for (int e = threadIdx.x; e < pw * nvec; e += blockDim.x) {
@@ -817,54 +831,76 @@ __global__ __launch_bounds__(256) void cgs2_panel_kernel(cplx<T> *A, int64_t sA,
         if (j >= p0 + q) C[(int64_t)(p0 + q) * nvec + j] = cmake<T>((T)0, (T)0);  // diagonal set below
     }
     __syncthreads();
+    double s_prev = 0.0;  // s_{q-1}, held by every thread (sc[q-1] is not yet visible)
     for (int q = 0; q < pw; ++q) {
-        double2 *v = pan + q * len;
-        for (int pass = 0; pass < 2; ++pass) {
-            if (q > 0) {
-                for (int j = wid; j < q; j += nw) {
-                    const double2 *u = pan + j * len;
-                    double2 acc = make_double2(0, 0);
-                    for (int i = lane; i < len; i += 64) acc = cadd(acc, cmulc(u[i], v[i]));
-                    acc.x = wave_sum(acc.x);
-                    acc.y = wave_sum(acc.y);
-                    if (lane == 0) h[j] = acc;
+        double2 *v = pan + q * ld;
+        double nrm = 0.0;
+        for (int pass = (q == 0 ? 1 : 0); pass < 2; ++pass) {
+            for (int j = wid; j < q; j += nw) {
+                const double2 *u = pan + j * ld;
+                double2 acc = make_double2(0, 0);
+                for (int i = lane; i < len; i += 64) acc = cadd(acc, cmulc(u[i], v[i]));
+                acc.x = wave_sum(acc.x);
+                acc.y = wave_sum(acc.y);
+                const double sj = j == q - 1 ? s_prev : sc[j];
+                const double2 hj = cscale(acc, sj);
+                if (lane == 0) {
+                    h[j] = hj;
+                    hs[j] = cscale(hj, sj);
                 }
-                __syncthreads();
-                for (int i = threadIdx.x; i < len; i += blockDim.x) {
-                    double2 x = v[i];
-                    for (int j = 0; j < q; ++j) x = csub(x, cmul(h[j], pan[j * len + i]));
-                    v[i] = x;
-                }
-                for (int j = threadIdx.x; j < q; j += blockDim.x) {
-                    cplx<T> &c = C[(int64_t)(p0 + q) * nvec + p0 + j];
-                    st2(c, pass ? cadd(ld2(c), h[j]) : h[j]);
-                }
-                __syncthreads();
             }
+            __syncthreads();
+            for (int j = threadIdx.x; j < q; j += blockDim.x) {
+                cplx<T> &c = C[(int64_t)(p0 + q) * nvec + p0 + j];
+                st2(c, pass ? cadd(ld2(c), h[j]) : h[j]);
+            }
+            for (int i0 = 0; i0 < len; i0 += nquad) {
+                const int i = i0 + (threadIdx.x >> 2);
+                double2 acc = make_double2(0, 0);
+                if (i < len)
+                    for (int j = part; j < q; j += 4) acc = cadd(acc, cmul(hs[j], pan[j * ld + i]));
+                acc.x += dpp_move<0xb1>(acc.x);  // lanes xor 1
+                acc.y += dpp_move<0xb1>(acc.y);
+                acc.x += dpp_move<0x4e>(acc.x);  // lanes xor 2
+                acc.y += dpp_move<0x4e>(acc.y);
+                if (part == 0 && i < len) {
+                    const double2 x = csub(v[i], acc);
+                    v[i] = x;
+                    if (pass == 1) nrm += x.x * x.x + x.y * x.y;
+                }
+            }
+            if (pass == 1) {
+                nrm = wave_sum(nrm);
+                if (lane == 0) npart[wid] = nrm;
+            }
+            __syncthreads();
         }
-        double nr = 0;
-        for (int i = threadIdx.x; i < len; i += blockDim.x) nr += v[i].x * v[i].x + v[i].y * v[i].y;
-        nr = sqrt(block_sum(nr, sh));
+        double nr = 0.0;
+        for (int w = 0; w < nw; ++w) nr += npart[w];
+        nr = sqrt(nr);
         const bool live = nr > dead && nr > 0.0;
-        const double inv = live ? 1.0 / nr : 0.0;
-        for (int i = threadIdx.x; i < len; i += blockDim.x) v[i] = cscale(v[i], inv);
-        if (threadIdx.x == 0) C[(int64_t)(p0 + q) * nvec + p0 + q] = cmake<T>((T)(live ? nr : 0.0), (T)0);
-        __syncthreads();
+        s_prev = live ? 1.0 / nr : 0.0;
+        if (threadIdx.x == 0) {
+            sc[q] = s_prev;
+            C[(int64_t)(p0 + q) * nvec + p0 + q] = cmake<T>((T)(live ? nr : 0.0), (T)0);
+        }
     }
+    __syncthreads();
     if (vs == 1) {
         for (int e = threadIdx.x; e < pw * len; e += blockDim.x) {
             const int i = e / pw, q = e % pw;
-            st2(M[(int64_t)(p0 + q) + (int64_t)i * es], pan[q * len + i]);
+            st2(M[(int64_t)(p0 + q) + (int64_t)i * es], cscale(pan[q * ld + i], sc[q]));
         }
     } else {
         for (int e = threadIdx.x; e < pw * len; e += blockDim.x) {
             const int q = e / len, i = e % len;
-            st2(M[(int64_t)(p0 + q) * vs + (int64_t)i * es], pan[q * len + i]);
+            st2(M[(int64_t)(p0 + q) * vs + (int64_t)i * es], cscale(pan[q * ld + i], sc[q]));
         }
     }
 }
 
-// Panel width: the panel (pw x len FP64 complex) fits in 128 KB of LDS.
+// Panel width: the panel (pw x len FP64 complex) fits in 128 KB of LDS (its
+// padded rows add pw slots of 16 bytes).
 inline int cgs_panel_width(int len) {
     int pw = 32;
     while (pw > 1 && (int64_t)pw * len * 16 > 128 * 1024) pw /= 2;
@@ -880,6 +916,7 @@ void cgs2(cplx<T> *A, int64_t sA, int nvec, int len, int64_t vs, int64_t es, cpl
     if (vs != 1 && es != 1) throw HipError("cgs2: one of the strides must be 1");
     const bool lq = vs != 1;
     const int PB = cgs_panel_width(len);
+    const int panel_threads = std::min(1024, std::max(64, (4 * len + 63) / 64 * 64));
     hipLaunchKernelGGL(cgs_fro_kernel<T>, dim3(B), dim3(256), 0, s, A, sA, nvec, len, vs, es, fro, mask);
     YQ_HIP(hipGetLastError());
     for (int p0 = 0; p0 < nvec; p0 += PB) {
@@ -925,8 +962,8 @@ void cgs2(cplx<T> *A, int64_t sA, int nvec, int len, int64_t vs, int64_t es, cpl
                 if (!gemm_mfma<T>(g2, g2a, g2b, P, s, 0)) gemm<T, T, T, T>(g2, g2a, g2b, P, s);
             }
         }
-        hipLaunchKernelGGL(cgs2_panel_kernel<T>, dim3(B), dim3(256), (size_t)pw * len * sizeof(double2), s, A, sA, nvec,
-                           len, vs, es, p0, pw, Cf, sC, (const double *)fro, mask);
+        hipLaunchKernelGGL(cgs2_panel_kernel<T>, dim3(B), dim3(panel_threads), (size_t)pw * (len + 1) * sizeof(double2),
+                           s, A, sA, nvec, len, vs, es, p0, pw, Cf, sC, (const double *)fro, mask);
         YQ_HIP(hipGetLastError());
     }
 }
