@@ -131,3 +131,41 @@ def test_fused_mpo_step_matches_unfused(prec, tol):
         got = yf.view(np.complex128).reshape(P, J, Q)
         assert np.abs(got - ref).max() / max(1.0, np.abs(ref).max()) < tol, (P, Dl, dph, Dr, Q, K)
     assert n_dl == (5 if prec == 1 else 0)
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_gram_fp64_matrix_cores(form):
+    """FP64 Gram matrices of complex-FP32 operands on the FP64 matrix cores
+    (the bond split's G = Y Y^H or Y^H Y): against numpy on the FP32-rounded
+    operands (products of FP32 values are exact in FP64, so only the sum
+    order differs: 1e-13 of |Y|^2) and against the FP64 generic kernel;
+    with `lower` the tiles above the diagonal stay untouched (zero) and the
+    lower triangle is the same."""
+    print(f"case: gram form={form}", flush=True)
+    lib = _abi.load_library()
+    f = lib.yaqs_debug_gram
+    f.argtypes = [ct.c_int] * 5 + [c_dp, c_dp, ct.c_int]
+    rng = np.random.default_rng(11 + form)
+    for M, K, cnt in [(256, 256, 3), (128, 128, 2), (100, 33, 2), (64, 200, 1), (17, 5, 2)]:
+        shape = (cnt, M, K) if form == 0 else (cnt, K, M)
+        Y = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64).astype(np.complex128)
+        y = np.ascontiguousarray(Y).view(np.float64).ravel()
+        ref = Y @ np.conj(np.transpose(Y, (0, 2, 1))) if form == 0 else np.conj(np.transpose(Y, (0, 2, 1))) @ Y
+        scale = np.abs(Y).max() ** 2 * K
+        out = {}
+        for lower in (0, 1):
+            for mf in (0, 1):
+                g = np.zeros(2 * cnt * M * M)
+                rc = f(M, K, form, lower, cnt, y.ctypes.data_as(c_dp), g.ctypes.data_as(c_dp), mf)
+                assert rc == 0
+                out[lower, mf] = g.view(np.complex128).reshape(cnt, M, M)
+        full = out[0, 1]
+        assert np.abs(full - ref).max() < 1e-13 * scale, (M, K)
+        assert np.abs(full - out[0, 0]).max() < 1e-13 * scale, (M, K)
+        low = out[1, 1]
+        tril = np.tril(np.ones((M, M), bool))
+        assert np.abs(low[:, tril] - full[:, tril]).max() == 0.0, (M, K)
+        # tiles strictly above the diagonal (64 x 64) are not written
+        r, c = np.indices((M, M))
+        above = (c // 64) > (r // 64)
+        assert np.abs(low[:, above]).max(initial=0.0) == 0.0, (M, K)

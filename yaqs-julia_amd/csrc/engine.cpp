@@ -1509,7 +1509,12 @@ void Engine<T>::split(State &S, int i, bool right, int rule, double thr, const i
     GemmDesc gg = right ? gd(m, m, n, OP_N, OP_C, n, n, m, nmax, nmax, slotG, nb, msk)
                         : gd(n, n, m, OP_C, OP_N, n, n, n, nmax, nmax, slotG, nb, msk);
     gg.lower = heev_gram_lower_only(gn) ? 1 : 0;  // the solver reads the lower triangle only
-    gemm<T, T, double, double>(gg, theta.as<CT>(), theta.as<CT>(), G.as<c64>(), s);
+    bool gram_mfma;
+    if constexpr (std::is_same<T, float>::value)
+        gram_mfma = gemm_mfma_gram(gg, theta.as<CT>(), theta.as<CT>(), G.as<c64>(), s);
+    else
+        gram_mfma = gemm_mfma<double>(gg, theta.as<CT>(), theta.as<CT>(), G.as<c64>(), s, 0);
+    if (!gram_mfma) gemm<T, T, double, double>(gg, theta.as<CT>(), theta.as<CT>(), G.as<c64>(), s);
     guard_gram(G.as<c64>(), slotG, gn, nfail.as<int>(), msk, nb, s);
     if (msk) fill_int(keep.as<int>(), 0, nb, s);  // masked-out slots report rank 0
     const int mb = max_keep > 0 ? max_keep : (int)std::min<int64_t>(J.max_bond, 1 << 30);

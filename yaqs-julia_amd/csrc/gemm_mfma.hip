@@ -96,9 +96,13 @@ inline unsigned gemm_blocks(int ntile, int inner, int outer) {
     return (unsigned)ntile * inner * ((outer + GEMM_XCDS - 1) / GEMM_XCDS * GEMM_XCDS);
 }
 
-template <typename T, int OPA, int OPB, int BM, int BN, int BK, bool FW, bool DL = false>
-__global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T> *__restrict__ A,
-                                                        const cplx<T> *__restrict__ B, cplx<T> *__restrict__ C,
+// TI: storage type of A and B (float with T = double: complex-FP32 operands
+// converted to FP64 as they are staged, FP64 products and sums, as the
+// Gram matrices of the bond splits take them).  With d.lower set, the C
+// tiles strictly above the diagonal are not computed.
+template <typename T, int OPA, int OPB, int BM, int BN, int BK, bool FW, bool DL = false, typename TI = T>
+__global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<TI> *__restrict__ A,
+                                                        const cplx<TI> *__restrict__ B, cplx<T> *__restrict__ C,
                                                         WFuse w, int ntile) {
     constexpr int LDK = BK + 4, KQ = BK / 4;  // k values per lane and tile
     // 16 x 16 blocks per wave: 2 x 2 wavefronts of (BM/2) x (BN/2), or for
@@ -128,8 +132,10 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
         m0 = (tile / tilesN) * BM;
         n0 = (tile % tilesN) * BN;
     }
-    const cplx<T> *Ab = A + t * d.sA_outer + jb * d.sA_inner;
-    const cplx<T> *Bb = B + t * d.sB_outer + jb * d.sB_inner;
+    static_assert(std::is_same<TI, T>::value || (!FW && !DL), "converted operands: register-staged, unfused");
+    if (!FW && d.lower && n0 >= m0 + BM) return;
+    const cplx<TI> *Ab = A + t * d.sA_outer + jb * d.sA_inner;
+    const cplx<TI> *Bb = B + t * d.sB_outer + jb * d.sB_inner;
     constexpr int SM_AB = 2 * 2 * (BM + BN) * LDK;
     // operand buffers; the fused epilogue reuses them for T1 and appends W
     constexpr int SM_W = FW ? ((65 + W_NNZ_MAX) * (int)sizeof(int) + W_NNZ_MAX * (int)sizeof(c64)) / (int)sizeof(T) + 4 : 0;
@@ -193,7 +199,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
     }
 #pragma unroll
     for (int q = 0; q < NLA; ++q) arow_ok[q] = ar_[q] < rows_ok && m0 + ar_[q] < d.M;
-    cplx<T> ra[NLA], rb[NLB];
+    cplx<TI> ra[NLA], rb[NLB];
     unsigned oka = 0, okb = 0;  // in-bounds flags of the pending loads, applied at the LDS store
     auto load = [&](int k0) {
         oka = okb = 0;
@@ -217,15 +223,15 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
         for (int q = 0; q < NLA; ++q) {
             const bool ok = (oka >> q) & 1u;
             const int ka = (ak_[q] & 3) * KQ + (ak_[q] >> 2);
-            As[buf][0][ar_[q]][ka] = ok ? ra[q].re : (T)0;
-            As[buf][1][ar_[q]][ka] = ok ? (OPA == OP_C ? -ra[q].im : ra[q].im) : (T)0;
+            As[buf][0][ar_[q]][ka] = ok ? (T)ra[q].re : (T)0;
+            As[buf][1][ar_[q]][ka] = ok ? (T)(OPA == OP_C ? -ra[q].im : ra[q].im) : (T)0;
         }
 #pragma unroll
         for (int q = 0; q < NLB; ++q) {
             const bool ok2 = (okb >> q) & 1u;
             const int kb = (bk_[q] & 3) * KQ + (bk_[q] >> 2);
-            Bs[buf][0][br_[q]][kb] = ok2 ? rb[q].re : (T)0;
-            Bs[buf][1][br_[q]][kb] = ok2 ? (OPB == OP_C ? -rb[q].im : rb[q].im) : (T)0;
+            Bs[buf][0][br_[q]][kb] = ok2 ? (T)rb[q].re : (T)0;
+            Bs[buf][1][br_[q]][kb] = ok2 ? (T)(OPB == OP_C ? -rb[q].im : rb[q].im) : (T)0;
         }
     };
     // complex-FP32: three real products per complex one (P1 = Re A Re B,
@@ -280,7 +286,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
         // LDS address (the compiler keeps nothing in M0 in this kernel: LDS
         // instructions on gfx950 do not read it); the instruction writes no
         // register.
-        auto dma = [&](const cplx<T> *g, const char *l) {
+        auto dma = [&](const cplx<TI> *g, const char *l) {
             const unsigned la = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)l);
             asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(la), "v"(g)
                          : "memory");
@@ -588,6 +594,25 @@ bool gemm_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C
 #undef YQ_GEMM_CASE
     return false;
 }
+// FP64 Gram matrices of complex-FP32 operands (A Y^H or Y^H A of the bond
+// split): FP64 matrix cores on operands converted as they are staged.
+bool gemm_mfma_gram(const GemmDesc &d, const c32 *A, const c32 *B, c64 *C, hipStream_t s) {
+    if (!mfma_mode()) return false;
+    if (d.M <= 0 || d.N <= 0 || d.K <= 0 || d.outer <= 0 || d.inner <= 0) return false;
+    const int tiles = ((d.M + 63) / 64) * ((d.N + 63) / 64);
+    const dim3 grid(gemm_blocks(tiles, d.inner, d.outer));
+    if (d.opA == OP_N && d.opB == OP_C)
+        hipLaunchKernelGGL((gemm_mfma_kernel<double, OP_N, OP_C, 64, 64, 16, false, false, float>), grid, dim3(256), 0,
+                           s, d, A, B, C, WFuse{}, tiles);
+    else if (d.opA == OP_C && d.opB == OP_N)
+        hipLaunchKernelGGL((gemm_mfma_kernel<double, OP_C, OP_N, 64, 64, 16, false, false, float>), grid, dim3(256), 0,
+                           s, d, A, B, C, WFuse{}, tiles);
+    else
+        return false;
+    YQ_HIP(hipGetLastError());
+    return true;
+}
+
 bool gemm_w_covers(const GemmDesc &d, int P, int Dl, int dph, int Dr, int Q, int nnz) {
     if (!mfma_mode()) return false;
     if (nnz > W_NNZ_MAX) return false;  // W staged in LDS
@@ -719,6 +744,48 @@ extern "C" int yaqs_debug_gemm_tile(int prec, int M, int N, int K, int opA, int 
 extern "C" int yaqs_debug_gemm(int prec, int M, int N, int K, int opA, int opB, const double *A, const double *B,
                                double *Cout, int use_mfma) {
     return yaqs_debug_gemm_tile(prec, M, N, K, opA, opB, A, B, Cout, use_mfma ? 0 : -1);
+}
+
+// yaqs_debug_gram: FP64 Gram matrices of `count` complex-FP32 matrices Y
+// (M x K row-major, given in FP64 and rounded to FP32): G = Y Y^H (form 0)
+// or Y^H Y (form 1; Y is then K x M), through the FP64 matrix cores
+// (mfma = 1) or the FP64 generic kernel (mfma = 0).  With lower set the
+// tiles above the diagonal are skipped; G is cleared first.  Returns 0, or
+// 1 when the matrix-core kernel did not launch.
+extern "C" int yaqs_debug_gram(int M, int K, int form, int lower, int count, const double *Y, double *Gout,
+                               int mfma) {
+    using namespace yq;
+    try {
+        const int64_t ny = (int64_t)M * K, ng = (int64_t)M * M;
+        GemmDesc d;
+        d.M = M; d.N = M; d.K = K;
+        d.opA = form == 0 ? OP_N : OP_C;
+        d.opB = form == 0 ? OP_C : OP_N;
+        d.lda = d.ldb = form == 0 ? K : M;
+        d.ldc = M;
+        d.sA_outer = d.sB_outer = ny;
+        d.sC_outer = ng;
+        d.sA_inner = d.sB_inner = d.sC_inner = 0;
+        d.inner = 1; d.outer = count; d.accumulate = 0; d.active = nullptr; d.lower = lower;
+        std::vector<c32> hy(ny * count);
+        for (int64_t i = 0; i < ny * count; ++i) hy[i] = cmake<float>((float)Y[2 * i], (float)Y[2 * i + 1]);
+        c32 *dy;
+        c64 *dg;
+        YQ_HIP(hipMalloc(&dy, ny * count * sizeof(c32)));
+        YQ_HIP(hipMalloc(&dg, ng * count * sizeof(c64)));
+        YQ_HIP(hipMemcpy(dy, hy.data(), ny * count * sizeof(c32), hipMemcpyHostToDevice));
+        YQ_HIP(hipMemset(dg, 0, ng * count * sizeof(c64)));
+        bool ok = true;
+        if (mfma) ok = gemm_mfma_gram(d, dy, dy, dg, 0);
+        else gemm<float, float, double, double>(d, dy, dy, dg, 0);
+        YQ_HIP(hipDeviceSynchronize());
+        YQ_HIP(hipMemcpy(Gout, dg, ng * count * sizeof(c64), hipMemcpyDeviceToHost));
+        (void)hipFree(dy);
+        (void)hipFree(dg);
+        return ok ? 0 : 1;
+    } catch (const std::exception &) {
+        return 3;
+    }
 }
 
 // yaqs_debug_gemm_w: T1 = A B (A: (P Dl) x K, B: K x (dph Q), row-major)

@@ -22,6 +22,9 @@ constexpr int GEMM_TILE_BM[GEMM_TILES] = {64, 32, 64, 32, 64, 64};
 constexpr int GEMM_TILE_BN[GEMM_TILES] = {64, 64, 32, 32, 64, 64};
 template <typename T>
 bool gemm_mfma(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C, hipStream_t s, int tile = 0);
+// FP64 Gram matrix of complex-FP32 operands on the FP64 matrix cores
+// (op pairs (N, C) and (C, N); honours d.lower); false if not launched.
+bool gemm_mfma_gram(const GemmDesc &d, const c32 *A, const c32 *B, c64 *C, hipStream_t s);
 // True when tile 4 (k tiles straight into LDS) applies to the descriptor.
 template <typename T>
 bool gemm_dl_fits(const GemmDesc &d, const cplx<T> *A, const cplx<T> *B);
