@@ -104,11 +104,12 @@ def test_batched_fast_path(n, max_bond):
             assert np.abs(Pk - P1).max() < 1e-9
 
 
-@pytest.mark.parametrize("n,max_bond", [(130, 64), (200, 100), (256, 128), (256, 256), (384, 128), (512, 256)])
+@pytest.mark.parametrize("n,max_bond", [(130, 64), (200, 100), (256, 128), (256, 256), (384, 128), (512, 256),
+                                        (128, 128), (100, 90), (70, 65)])
 def test_batched_wide_path(n, max_bond):
-    """The global-memory path (128 < n <= 512, up to 256 kept columns) on a
-    batch of 10 matrices of every spectrum kind, against numpy and against
-    the single-workgroup solver."""
+    """The global-memory path (128 < n <= 512, up to 256 kept columns, and
+    n <= 128 with more than 64 kept) on a batch of 10 matrices of every
+    spectrum kind, against numpy and against the single-workgroup solver."""
     print(f"case: heev wide batch n={n} max_bond={max_bond}", flush=True)
     Gs = _gram_batch(n, 10, n + 1)
     ev, U, keep, fb = _heev_batch(Gs, max_bond=max_bond)

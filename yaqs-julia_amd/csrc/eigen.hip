@@ -1958,9 +1958,9 @@ __global__ __launch_bounds__(CGW_THREADS) void cgs_wide2_kernel(char *__restrict
 int64_t heev_work_bytes(int n, int kmax) {
     // calls with any n' <= n use the same slots
     const int64_t narrow = heev_lay(std::min(n, LDS_MAX_N)).total;
-    if (n <= LDS_MAX_N) return narrow;
     if (n > WIDE_MAX_N) return narrow;
     const int kw = round64(std::min(std::min(n, kmax), WIDE_MAX_K));
+    if (n <= LDS_MAX_N && kw <= 64) return narrow;
     return std::max(narrow, heev_lay_wide(n, kw).total);
 }
 
@@ -2000,7 +2000,9 @@ bool heev(const c64 *G, int64_t sG, c64 *U, int64_t sU, double *Z, int64_t sZ, d
     if (kout > n) kout = n;
     const bool fast = n <= LDS_MAX_N && kout <= 64 && work && fallback && heev_lay(n).total <= sW;
     const int kw = round64(kout);
-    const bool wide = n > LDS_MAX_N && n <= WIDE_MAX_N && kw <= WIDE_MAX_K && work && fallback &&
+    // the global-memory kernels also take n <= 128 when more than 64
+    // columns are kept (the LDS kernels hold 64)
+    const bool wide = !fast && n <= WIDE_MAX_N && kw <= WIDE_MAX_K && work && fallback &&
                       heev_lay_wide(n, kw).total <= sW;
     const int np = n * (n + 1) / 2;
     if (n > LDS_MAX_N && !Pg) throw HipError("heev: no global scratch for n > 128");
