@@ -25,6 +25,11 @@ __host__ __device__ inline cplx<T> cmake(T r, T i) {
     return z;
 }
 
+// Fused multiply-add in the operands' own precision (no promotion of
+// float arguments to double).
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 struct HipError : std::runtime_error {
     explicit HipError(const std::string &s) : std::runtime_error(s) {}
 };

@@ -52,8 +52,8 @@ struct Mfma<double> {
 
 // Arguments of the MPO step fused into a GEMM epilogue; W's non-zeros are
 // staged in LDS (at most W_NNZ_MAX, else the unfused kernels run).  The
-// epilogue accumulates in FP64 with FP64 W like wstep_kernel, so fused and
-// unfused results are identical.
+// epilogue accumulates in the contraction precision T with W rounded to T,
+// in wstep_kernel's order, so fused and unfused results are identical.
 constexpr int W_NNZ_MAX = 256;
 struct WFuse {
     int P = 0, Dl = 1, dph = 1, Dr = 1, Q = 0, pb = 0, qb = 0;
@@ -475,20 +475,21 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmDesc d, const cplx<T
             for (int u = slot; u < ngp * J; u += nslot) {
                 const int pg = u / J, jj = u - pg * J;
                 const cplx<T> *base = T1f + pg * PG * rs + qq;
-                double ax[PG], ay[PG];
+                T ax[PG], ay[PG];  // the contraction precision (as wstep_kernel)
 #pragma unroll
-                for (int r = 0; r < PG; ++r) ax[r] = ay[r] = 0;
+                for (int r = 0; r < PG; ++r) ax[r] = ay[r] = (T)0;
                 for (int z = wrp[jj]; z < wrp[jj + 1]; ++z) {
                     const c64 wv = wval[z];
+                    const T wr = (T)wv.re, wi = (T)wv.im;
                     const int off = wcol[z];
 #pragma unroll
                     for (int r = 0; r < PG; ++r) {
                         const cplx<T> x = base[r * rs + off];
                         // same operation order as wstep_kernel
-                        ax[r] = fma(wv.re, (double)x.re, ax[r]);
-                        ax[r] = fma(-wv.im, (double)x.im, ax[r]);
-                        ay[r] = fma(wv.re, (double)x.im, ay[r]);
-                        ay[r] = fma(wv.im, (double)x.re, ay[r]);
+                        ax[r] = fma_t(wr, x.re, ax[r]);
+                        ax[r] = fma_t(-wi, x.im, ax[r]);
+                        ay[r] = fma_t(wr, x.im, ay[r]);
+                        ay[r] = fma_t(wi, x.re, ay[r]);
                     }
                 }
 #pragma unroll

@@ -174,6 +174,8 @@ template void gemm<double, double, float, double>(const GemmDesc &, const c64 *,
 // ------------------------------------------------------------- W step ------
 // Y[t][p][j][q] = Σ_i W[j][i] X[t][p][i][q] with W in CSR (rows j).  The MPO
 // contraction of the effective Hamiltonian; memory-bound, coalesced along q.
+// Accumulated in the contraction precision T (W rounded to T), the order the
+// fused GEMM epilogue repeats.
 template <typename T>
 __global__ __launch_bounds__(256) void wstep_kernel(const cplx<T> *__restrict__ X, cplx<T> *__restrict__ Y, int P, int I,
                                                     int J, int Q, int64_t sX, int64_t sY, const int *__restrict__ rowptr,
@@ -187,17 +189,18 @@ __global__ __launch_bounds__(256) void wstep_kernel(const cplx<T> *__restrict__ 
     const cplx<T> *x = X + t * sX + (int64_t)p * I * Q + q;
     cplx<T> *y = Y + t * sY + (int64_t)p * J * Q + q;
     for (int j = 0; j < J; ++j) {
-        double2 acc = make_double2(0, 0);
+        T ax = (T)0, ay = (T)0;
         for (int z = rowptr[j]; z < rowptr[j + 1]; ++z) {
             const c64 w = val[z];
-            const double2 xv = ld2(x[(int64_t)col[z] * Q]);
+            const T wr = (T)w.re, wi = (T)w.im;
+            const cplx<T> xv = x[(int64_t)col[z] * Q];
             // explicit FMAs: the fused GEMM epilogue repeats this order exactly
-            acc.x = fma(w.re, xv.x, acc.x);
-            acc.x = fma(-w.im, xv.y, acc.x);
-            acc.y = fma(w.re, xv.y, acc.y);
-            acc.y = fma(w.im, xv.x, acc.y);
+            ax = fma_t(wr, xv.re, ax);
+            ax = fma_t(-wi, xv.im, ax);
+            ay = fma_t(wr, xv.im, ay);
+            ay = fma_t(wi, xv.re, ay);
         }
-        st2(y[(int64_t)j * Q], acc);
+        y[(int64_t)j * Q] = cmake<T>(ax, ay);
     }
 }
 
