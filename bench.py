@@ -19,9 +19,10 @@ path (one trajectory per batch, on a trajectory subset), the ensemble
 trajectories, and the FP64 CPU oracle (a bounded trajectory subset, one
 trajectory per thread, OpenBLAS zgemm / LAPACK zgesdd as the reference
 links them), plus the roofline of the dominant kernel (the complex-FP32
-MFMA GEMM) from HIP events around every launch in the window (in a second
-run of the job as one wave when the context ran it as concurrent waves,
-whose kernels share the device).
+MFMA GEMM) from HIP events around every launch in the window, in a run of
+the job as one wave before the timed run (the timed run's concurrent waves
+share the device; the one-wave run also lets the launch tuner time its
+variants alone, and the timed run takes the choices it cached).
 """
 import argparse
 import ctypes as ct
@@ -424,6 +425,19 @@ def main():
                           "gemm_flop": rrep["gemm_flop"],
                           "achieved_tflops": rrep["gemm_flop"] / (rrep["gemm_ms"] * 1e-3) / 1e12}), flush=True)
         return
+    # The roofline comes from a window where the GEMMs have the device to
+    # their wave: with concurrent waves, the HIP events around a launch also
+    # span kernels of the other waves' streams.  So the job runs first as one
+    # wave (max_batch = ntraj) over the first Kaux steps of the window, for
+    # the roofline; this also lets the launch tuner time its variants with
+    # the device to itself, and the timed run below (several concurrent
+    # waves) takes the choices it cached for the process.
+    rctx = Y.Context(0, max_batch=args.ntraj)
+    print(f"[bench] roofline window: steps {win0 + 1}..{win0 + Kaux} as one wave", flush=True)
+    rrep, _ = gpu_window(Y, lib, rctx, args.ntraj, win0, Kaux, args.precision)
+    rctx.close()
+    rv = args.ntraj * L * Kaux / rrep["timed_s"]
+    print(f"[bench] one wave: {rrep['timed_s']:.2f}s -> {rv:.4g} traj-site-steps/s", flush=True)
     ctx = Y.Context(0)
     print(f"[bench] C2 window: steps {win0 + 1}..{win0 + K} (warm-up {win0}, asked {args.warmup}) "
           f"ntraj={args.ntraj} {args.precision}", flush=True)
@@ -431,22 +445,9 @@ def main():
     value = args.ntraj * L * K / rep["timed_s"]
     print(f"[bench] GPU window {rep['timed_s']:.2f}s -> {value:.4g} traj-site-steps/s, max bond {rep['max_bond']}",
           flush=True)
-    # The roofline comes from a window where the GEMMs have the device to
-    # their wave: with concurrent waves, the HIP events around a launch also
-    # span kernels of the other waves' streams.  So a job the context ran as
-    # several concurrent waves is run once more as one wave (max_batch =
-    # ntraj), over the first Kaux steps of the window, for the roofline.
-    rrep, roof_note = rep, "the timed window above"
-    if rep["concurrent_waves"] > 1:
-        rctx = Y.Context(0, max_batch=args.ntraj)
-        print(f"[bench] roofline window: steps {win0 + 1}..{win0 + Kaux} as one wave", flush=True)
-        rrep, _ = gpu_window(Y, lib, rctx, args.ntraj, win0, Kaux, args.precision)
-        rctx.close()
-        rv = args.ntraj * L * Kaux / rrep["timed_s"]
-        print(f"[bench] one wave: {rrep['timed_s']:.2f}s -> {rv:.4g} traj-site-steps/s", flush=True)
-        roof_note = (f"steps {win0 + 1}..{win0 + Kaux} run as one wave of {args.ntraj} trajectories "
-                     f"({rv:.0f} traj-site-steps/s; the timed run used {int(rep['concurrent_waves'])} concurrent "
-                     f"waves); reproduce with: python bench.py --only roofline --steps {Kaux}")
+    roof_note = (f"steps {win0 + 1}..{win0 + Kaux} run as one wave of {args.ntraj} trajectories before the timed "
+                 f"run ({rv:.0f} traj-site-steps/s; the timed run used {int(rep['concurrent_waves'])} concurrent "
+                 f"wave(s)); reproduce with: python bench.py --only roofline --steps {Kaux}")
     achieved = rrep["gemm_flop"] / (rrep["gemm_ms"] * 1e-3) if rrep["gemm_ms"] > 0 else 0.0
     out = {
         "metric": METRIC,

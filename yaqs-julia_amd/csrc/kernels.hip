@@ -957,12 +957,23 @@ void cgs2(cplx<T> *A, int64_t sA, int nvec, int len, int64_t vs, int64_t es, cpl
                 g2.lda = p0; g2.ldb = vs; g2.ldc = vs;
                 g2a = S; g2b = A;
             }
+            // C tiles matched to the panel width (32 columns or rows: the
+            // 64 x 32 / 32 x 64 tiles, bit-identical to the 64 x 64 one;
+            // they exist for op N right operands)
+            auto tile_for = [](const GemmDesc &g) {
+                if (g.opB != OP_N) return 0;
+                if (g.M <= 32 && g.N <= 32) return 3;
+                if (g.N <= 32) return 2;
+                if (g.M <= 32) return 1;
+                return 0;
+            };
+            const int t1 = tile_for(g1), t2 = tile_for(g2);
             for (int pass = 0; pass < 2; ++pass) {
-                if (!gemm_mfma<T>(g1, g1a, g1b, S, s, 0)) gemm<T, T, T, T>(g1, g1a, g1b, S, s);
+                if (!gemm_mfma<T>(g1, g1a, g1b, S, s, t1)) gemm<T, T, T, T>(g1, g1a, g1b, S, s);
                 hipLaunchKernelGGL(cgs_coef_kernel<T>, dim3((p0 * pw + 255) / 256, B), dim3(256), 0, s, S, sS, Cf, sC,
                                    nvec, p0, pw, lq ? 1 : 0, pass, mask);
                 YQ_HIP(hipGetLastError());
-                if (!gemm_mfma<T>(g2, g2a, g2b, P, s, 0)) gemm<T, T, T, T>(g2, g2a, g2b, P, s);
+                if (!gemm_mfma<T>(g2, g2a, g2b, P, s, t2)) gemm<T, T, T, T>(g2, g2a, g2b, P, s);
             }
         }
         hipLaunchKernelGGL(cgs2_panel_kernel<T>, dim3(B), dim3(panel_threads), (size_t)pw * (len + 1) * sizeof(double2),
