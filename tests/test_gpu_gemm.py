@@ -43,9 +43,9 @@ def test_gemm_forms(prec, tol, mfma):
             assert err < tol, (opA, opB, M, N, K, err)
 
 
-def _tile_run(lib, prec, opA, M, N, K, tile, a, b):
+def _tile_run(lib, prec, opA, M, N, K, tile, a, b, opB=0):
     out = np.zeros(2 * M * N)
-    rc = lib.yaqs_debug_gemm_tile(prec, M, N, K, opA, 0, a.ctypes.data_as(c_dp), b.ctypes.data_as(c_dp),
+    rc = lib.yaqs_debug_gemm_tile(prec, M, N, K, opA, opB, a.ctypes.data_as(c_dp), b.ctypes.data_as(c_dp),
                                   out.ctypes.data_as(c_dp), tile)
     assert rc == 0
     return out
@@ -79,6 +79,17 @@ def test_tile_shapes_bit_identical(prec, tol):
             for tile in range(1, 6):
                 other = _tile_run(lib, prec, opA, M, N, K, tile, a, b)
                 assert np.array_equal(other, base), (opA, M, N, K, tile)
+            # a conjugated right operand (the LQ side of the gauge steps'
+            # projections): the 32 x 64, 64 x 32 and 32 x 32 tiles too
+            Bc = rng.standard_normal((N, K)) + 1j * rng.standard_normal((N, K))
+            bc = np.ascontiguousarray(Bc).view(np.float64).ravel()
+            base_c = _tile_run(lib, prec, opA, M, N, K, 0, a, bc, opB=2)
+            ref_c = op[opA](A) @ Bc.conj().T
+            got_c = base_c.view(np.complex128).reshape(M, N)
+            assert np.abs(got_c - ref_c).max() / max(1.0, np.abs(ref_c).max()) < tol
+            for tile in range(1, 4):
+                other = _tile_run(lib, prec, opA, M, N, K, tile, a, bc, opB=2)
+                assert np.array_equal(other, base_c), (opA, M, N, K, tile, "opB=C")
 
 
 def _csr(Wd):

@@ -554,14 +554,14 @@ static bool dl_fits(const GemmDesc &d, const void *A, const void *B, int BM) {
 template <typename T, int OA, int OB>
 static void launch_shape(int tile, const GemmDesc &d, const cplx<T> *A, const cplx<T> *B, cplx<T> *C,
                          hipStream_t s) {
-    if constexpr (OB == OP_N) {
+    if constexpr (OB == OP_N || OB == OP_C) {
         switch (tile) {
             case 1: return launch_tile<T, OA, OB, 32, 64>(d, A, B, C, s);
             case 2: return launch_tile<T, OA, OB, 64, 32>(d, A, B, C, s);
             case 3: return launch_tile<T, OA, OB, 32, 32>(d, A, B, C, s);
             case 4:
             case 5:
-                if constexpr (std::is_same<T, float>::value && OA == OP_N) {
+                if constexpr (std::is_same<T, float>::value && OA == OP_N && OB == OP_N) {
                     if (dl_fits<T>(d, A, B, 64)) {
                         if (tile == 4) return launch_tile<T, OA, OB, 64, 64, true>(d, A, B, C, s);
                         return launch_tile<T, OA, OB, 64, 64, true, 8>(d, A, B, C, s);

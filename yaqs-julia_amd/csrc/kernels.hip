@@ -959,9 +959,9 @@ void cgs2(cplx<T> *A, int64_t sA, int nvec, int len, int64_t vs, int64_t es, cpl
             }
             // C tiles matched to the panel width (32 columns or rows: the
             // 64 x 32 / 32 x 64 tiles, bit-identical to the 64 x 64 one;
-            // they exist for op N right operands)
+            // they exist for op N and op C right operands)
             auto tile_for = [](const GemmDesc &g) {
-                if (g.opB != OP_N) return 0;
+                if (g.opB == OP_T) return 0;
                 if (g.M <= 32 && g.N <= 32) return 3;
                 if (g.N <= 32) return 2;
                 if (g.M <= 32) return 1;
